@@ -1,0 +1,153 @@
+/*
+ * vp8g — C ABI of the gfx950 (MI355X) implementation of oxideav-vp8's
+ * encoder-side data-parallel path: ARNR altref synthesis, VP8 motion
+ * compensation and sub-pel SAD scoring.
+ *
+ * Every entry point replaces one reference function (paths relative to the
+ * oxideav-vp8 crate root) and keeps its argument meaning, output bytes and
+ * error order.  Plain pointers and sizes only.
+ *
+ * Memory: entry points without a suffix take HOST pointers (the library
+ * copies in, runs, copies out, and returns after the result is in place).
+ * `_dev` entry points take DEVICE pointers, enqueue on the context's stream
+ * and return without waiting; call vp8g_ctx_synchronize() before reading.
+ *
+ * Threading: a context is not internally locked; use one per host thread.
+ * Results never depend on launch shapes, batch composition or thread count.
+ */
+#ifndef VP8G_H
+#define VP8G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define VP8G_OK 0
+#define VP8G_E_INVALID_DIMENSIONS 1   /* EncodeError::InvalidDimensions   (src/encoder.rs:72) */
+#define VP8G_E_REF_DIMS_MISMATCH 2    /* EncodeError::ReferenceDimensionsMismatch (src/encoder.rs:99-104) */
+#define VP8G_E_CENTER_OUT_OF_RANGE 3  /* the reference panics (src/arnr.rs:494-498) */
+#define VP8G_E_ARG 4                  /* null pointer, short stride, limit exceeded */
+#define VP8G_E_DEVICE 5               /* HIP runtime error; no output was written */
+#define VP8G_E_NO_DEVICE 6            /* no usable gfx950 device */
+
+/* Largest group the ARNR entry points accept in one call. */
+#define VP8G_MAX_FRAMES 64
+
+typedef struct vp8g_ctx vp8g_ctx;
+
+/* I420Frame, src/encoder.rs:5419-5434: visible dims, planes, strides. */
+typedef struct {
+    uint32_t width, height;
+    const uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride;
+} vp8g_i420;
+
+/* EncodeError payloads: InvalidDimensions{width,height} -> src_w/src_h;
+ * ReferenceDimensionsMismatch{source, reference} -> all four. */
+typedef struct {
+    uint32_t src_w, src_h, ref_w, ref_h;
+} vp8g_err;
+
+/* Mv, src/motion_vector.rs:64-70 (quarter-pel). */
+typedef struct {
+    int16_t row, col;
+} vp8g_mv;
+
+/* ReferencePlanes, src/motion_comp.rs:131-146 (MB-aligned planes). */
+typedef struct {
+    const uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride, mb_cols, mb_rows;
+} vp8g_ref_planes;
+
+/* LumaRef, src/motion_search.rs:230-239 (width/height = clamp bounds). */
+typedef struct {
+    const uint8_t *plane;
+    size_t stride, width, height;
+} vp8g_luma_ref;
+
+/* SearchResult, src/motion_search.rs:248-255. */
+typedef struct {
+    vp8g_mv mv;
+    uint32_t sad;
+} vp8g_search_result;
+
+/* ---- context ---- */
+int vp8g_ctx_create(int device, vp8g_ctx **out);
+void vp8g_ctx_destroy(vp8g_ctx *ctx);
+/* Run later work on `stream` (a hipStream_t; NULL restores the context's
+ * own stream). */
+int vp8g_ctx_set_stream(vp8g_ctx *ctx, void *stream);
+void *vp8g_ctx_stream(vp8g_ctx *ctx);
+int vp8g_ctx_synchronize(vp8g_ctx *ctx);
+/* Waves per ARNR workgroup (0 = one per neighbour frame, capped at 16). */
+int vp8g_ctx_set_arnr_waves(vp8g_ctx *ctx, int waves);
+const char *vp8g_status_str(int status);
+
+/* ---- ARNR altref synthesis ----
+ * Replaces build_arnr_altref (src/arnr.rs:476-482) and
+ * build_arnr_altref_threaded (src/arnr.rs:488-650; the thread budget has no
+ * meaning here and the output never depended on it).
+ * Check order as the reference: centre index, centre dims (0 or > 0x3FFF),
+ * then the first frame whose dims differ from the centre's.  Output planes
+ * are tightly packed: y[w*h], u[cw*ch], v[cw*ch], cw = ceil(w/2),
+ * ch = ceil(h/2) (ArnrPicture, src/arnr.rs:90-101).  `strength` is the raw
+ * ArnrConfig::strength (src/arnr.rs:52-56): 0 copies the centre, values
+ * above 6 act as 6 (src/arnr.rs:533).  `err` may be NULL. */
+int vp8g_build_arnr_altref(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames,
+                           size_t center, uint8_t strength, uint8_t *out_y, uint8_t *out_u,
+                           uint8_t *out_v, vp8g_err *err);
+/* Same with device-resident planes and outputs.  `mv_out` (device, may be
+ * NULL) receives the per-block alignment table
+ * n_frames * ceil(h/16) * ceil(w/16) * {mvx, mvy, sad} (centre rows are
+ * left untouched). */
+int vp8g_build_arnr_altref_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames,
+                               size_t center, uint8_t strength, uint8_t *out_y, uint8_t *out_u,
+                               uint8_t *out_v, vp8g_err *err, int32_t *mv_out);
+
+/* ---- motion compensation ----
+ * Replaces predict_inter_mb (src/motion_comp.rs:1986-2078) over a batch:
+ * MB i sits at (mb_xy[2i], mb_xy[2i+1]) = (mb_col, mb_row) with quarter-pel
+ * `mv[i]`; `filters` is the 8x6 tap table (SIXTAP_FILTERS or
+ * BILINEAR_FILTERS, src/motion_comp.rs:294-337, or any other table);
+ * `out` receives n * 384 bytes, each ReconstructedMb's y[256] u[64] v[64]
+ * (src/reconstruct.rs:176-195).  n == 0 is a no-op. */
+int vp8g_predict_inter_mb_batch(vp8g_ctx *ctx, const vp8g_ref_planes *ref,
+                                const uint32_t *mb_xy, const vp8g_mv *mv, size_t n,
+                                int full_pixel, const int32_t filters[8][6], uint8_t *out);
+int vp8g_predict_inter_mb_batch_dev(vp8g_ctx *ctx, const vp8g_ref_planes *ref,
+                                    const uint32_t *mb_xy, const vp8g_mv *mv, size_t n,
+                                    int full_pixel, const int32_t filters[8][6], uint8_t *out);
+
+/* ---- sub-pel SAD scoring ----
+ * Replaces mb_luma_sad_at_mv (src/motion_search.rs:470-529) over a batch:
+ * sad[i] = SAD of src[256*i .. +256] against the six-tap prediction of MB
+ * (mb_xy[2i], mb_xy[2i+1]) at quarter-pel mv[i].  Whole-pel vectors give
+ * mb_luma_sad_at_whole_mv (src/motion_search.rs:291-349). */
+int vp8g_mb_luma_sad_at_mv_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref,
+                                 const uint32_t *mb_xy, const uint8_t *src, const vp8g_mv *mv,
+                                 size_t n, uint32_t *sad);
+int vp8g_mb_luma_sad_at_mv_batch_dev(vp8g_ctx *ctx, const vp8g_luma_ref *ref,
+                                     const uint32_t *mb_xy, const uint8_t *src,
+                                     const vp8g_mv *mv, size_t n, uint32_t *sad);
+
+/* Replaces half_pixel_refine_luma (src/motion_search.rs:564-627) and
+ * quarter_pixel_refine_luma (src/motion_search.rs:666-728) over a batch.
+ * Centre components must lie in [-1023, 1023] (VP8G_E_ARG otherwise; the
+ * reference debug-asserts it). */
+int vp8g_half_pixel_refine_luma_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref,
+                                      const uint32_t *mb_xy, const uint8_t *src,
+                                      const vp8g_mv *centers, size_t n,
+                                      vp8g_search_result *out);
+int vp8g_quarter_pixel_refine_luma_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref,
+                                         const uint32_t *mb_xy, const uint8_t *src,
+                                         const vp8g_mv *centers, size_t n,
+                                         vp8g_search_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,156 @@
+"""ctypes loader for the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker / the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+class I420(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("y", C.c_void_p),
+                ("u", C.c_void_p), ("v", C.c_void_p), ("y_stride", C.c_size_t),
+                ("uv_stride", C.c_size_t)]
+
+
+class Err(C.Structure):
+    _fields_ = [("src_w", C.c_uint32), ("src_h", C.c_uint32), ("ref_w", C.c_uint32),
+                ("ref_h", C.c_uint32)]
+
+
+class MvS(C.Structure):
+    _fields_ = [("row", C.c_int16), ("col", C.c_int16)]
+
+
+class RefPlanes(C.Structure):
+    _fields_ = [("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("y_stride", C.c_size_t), ("uv_stride", C.c_size_t),
+                ("mb_cols", C.c_size_t), ("mb_rows", C.c_size_t)]
+
+
+class LumaRef(C.Structure):
+    _fields_ = [("plane", C.c_void_p), ("stride", C.c_size_t), ("width", C.c_size_t),
+                ("height", C.c_size_t)]
+
+
+class SearchResult(C.Structure):
+    _fields_ = [("mv", MvS), ("sad", C.c_uint32)]
+
+
+class HalfPlanes(C.Structure):
+    _fields_ = [("h", C.c_uint8 * (22 * 17)), ("v", C.c_uint8 * (17 * 16)),
+                ("d", C.c_uint8 * (17 * 17))]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Oracle:
+    def __init__(self, path: Path | None = None):
+        p = Path(path) if path else LIB
+        if not p.exists():
+            build()
+        self.lib = L = C.CDLL(str(p))
+        vp, sz, i32, u32, u8 = C.c_void_p, C.c_size_t, C.c_int, C.c_uint32, C.c_uint8
+        L.orc_build_arnr_altref.argtypes = [C.POINTER(I420), sz, sz, u8, sz, vp, vp, vp,
+                                            C.POINTER(Err), vp]
+        L.orc_build_arnr_altref.restype = i32
+        L.orc_block_sad.argtypes = [C.POINTER(I420), C.POINTER(I420)] + [i32] * 6 + [u32]
+        L.orc_block_sad.restype = u32
+        L.orc_block_sad_generic.argtypes = L.orc_block_sad.argtypes
+        L.orc_block_sad_generic.restype = u32
+        L.orc_refine_search.argtypes = [C.POINTER(I420), C.POINTER(I420)] + [i32] * 4 + [vp]
+        L.orc_refine_search.restype = None
+        L.orc_pixel_weight.argtypes = [C.c_int32, u32]
+        L.orc_pixel_weight.restype = u32
+        L.orc_weight_table.argtypes = [u32, vp]
+        L.orc_accumulate_luma_block.argtypes = [vp, vp, sz] + [i32] * 8 + [vp, vp, vp]
+        L.orc_accumulate_chroma_block.argtypes = [vp, vp, sz] + [i32] * 8 + [vp, vp, vp]
+        L.orc_interp.argtypes = [vp, vp]
+        L.orc_interp.restype = u8
+        L.orc_avg.argtypes = [C.c_int32] * 4
+        L.orc_avg.restype = C.c_int32
+        L.orc_stored_luma_mv.argtypes = [MvS]
+        L.orc_stored_luma_mv.restype = MvS
+        L.orc_chroma_mv.argtypes = [MvS]
+        L.orc_chroma_mv.restype = MvS
+        L.orc_apply_full_pixel.argtypes = [MvS]
+        L.orc_apply_full_pixel.restype = MvS
+        L.orc_fetch_block_halo.argtypes = [vp, sz, sz, sz, sz, sz, MvS, vp]
+        L.orc_sixtap_2d.argtypes = [vp, i32, i32, vp, vp]
+        L.orc_fetch_luma_mb_halo.argtypes = [vp, sz, sz, sz, sz, sz, MvS, vp]
+        L.orc_sixtap_mb_luma.argtypes = [vp, i32, i32, vp, vp]
+        L.orc_fetch_chroma_mb_halo.argtypes = [vp, sz, sz, sz, sz, sz, MvS, vp]
+        L.orc_sixtap_mb_chroma.argtypes = [vp, i32, i32, vp, vp]
+        L.orc_fetch_luma_mb_whole_pixel.argtypes = [vp, sz, sz, sz, sz, sz, MvS, vp]
+        L.orc_fetch_chroma_mb_whole_pixel.argtypes = [vp, sz, sz, sz, sz, sz, MvS, vp]
+        L.orc_predict_inter_mb.argtypes = [C.POINTER(RefPlanes), sz, sz, MvS, i32, vp, vp]
+        L.orc_predict_inter_mb_batch.argtypes = [C.POINTER(RefPlanes), vp, vp, sz, i32, vp, vp]
+        L.orc_block_sad_16x16.argtypes = [vp, vp]
+        L.orc_block_sad_16x16.restype = u32
+        L.orc_mb_luma_sad_at_whole_mv.argtypes = [C.POINTER(LumaRef), sz, sz, vp, MvS]
+        L.orc_mb_luma_sad_at_whole_mv.restype = u32
+        L.orc_mb_luma_sad_at_mv.argtypes = [C.POINTER(LumaRef), sz, sz, vp, MvS]
+        L.orc_mb_luma_sad_at_mv.restype = u32
+        L.orc_mb_luma_sad_at_mv_batch.argtypes = [C.POINTER(LumaRef), vp, vp, vp, sz, vp]
+        L.orc_half_pixel_refine_luma.argtypes = [C.POINTER(LumaRef), sz, sz, vp, MvS]
+        L.orc_half_pixel_refine_luma.restype = SearchResult
+        L.orc_quarter_pixel_refine_luma.argtypes = [C.POINTER(LumaRef), sz, sz, vp, MvS]
+        L.orc_quarter_pixel_refine_luma.restype = SearchResult
+        L.orc_fetch_window.argtypes = [C.POINTER(LumaRef), i32, i32, vp]
+        L.orc_half_planes_new.argtypes = [vp, vp, i32, C.POINTER(HalfPlanes)]
+        L.orc_half_planes_sad.argtypes = [C.POINTER(HalfPlanes), vp, i32, i32]
+        L.orc_half_planes_sad.restype = u32
+        self.SIXTAP = np.ctypeslib.as_array((C.c_int32 * 48).in_dll(L, "ORC_SIXTAP_FILTERS")).reshape(8, 6).copy()
+        self.BILINEAR = np.ctypeslib.as_array((C.c_int32 * 48).in_dll(L, "ORC_BILINEAR_FILTERS")).reshape(8, 6).copy()
+
+    # ---- ARNR
+    @staticmethod
+    def i420(width, height, y, u, v, y_stride, uv_stride, keep: list) -> I420:
+        y = np.ascontiguousarray(y, np.uint8).reshape(-1)
+        u = np.ascontiguousarray(u, np.uint8).reshape(-1)
+        v = np.ascontiguousarray(v, np.uint8).reshape(-1)
+        keep += [y, u, v]
+        return I420(width, height, _p(y) if y.size else None, _p(u) if u.size else None,
+                    _p(v) if v.size else None, y_stride, uv_stride)
+
+    def build_arnr_altref(self, frames, center, strength=3, threads=1, want_mvs=False):
+        """frames: list of (w, h, y, u, v, y_stride, uv_stride).  Returns
+        (status, (y, u, v) or None, err, mvs or None)."""
+        keep: list = []
+        n = len(frames)
+        arr = (I420 * max(n, 1))()
+        for i, f in enumerate(frames):
+            arr[i] = self.i420(*f, keep=keep)
+        err = Err()
+        if center >= n:
+            st = self.lib.orc_build_arnr_altref(arr, n, center, strength, threads, None, None,
+                                                None, C.byref(err), None)
+            return st, None, err, None
+        w, h = frames[center][0], frames[center][1]
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        oy = np.zeros(max(w * h, 1), np.uint8)
+        ou = np.zeros(max(cw * ch, 1), np.uint8)
+        ov = np.zeros(max(cw * ch, 1), np.uint8)
+        mvs = None
+        if want_mvs:
+            mvs = np.zeros((n, (h + 15) // 16, (w + 15) // 16, 3), np.int32)
+        st = self.lib.orc_build_arnr_altref(arr, n, center, strength, threads, _p(oy), _p(ou),
+                                            _p(ov), C.byref(err), _p(mvs) if want_mvs else None)
+        if st != 0:
+            return st, None, err, None
+        return st, (oy[:w * h], ou[:cw * ch], ov[:cw * ch]), err, mvs

@@ -1,0 +1,654 @@
+/*
+ * CPU oracle for the oxideav-vp8 ARNR + motion-compensation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline).  The
+ * product library never links or calls this file.  Each function restates
+ * the reference crate's integer arithmetic and cites the file:line it
+ * follows; the output-invariant fast paths the reference keeps (in-bounds
+ * row slices, monotone SAD early exit, the 256-entry weight table) are kept
+ * too, so the timed baseline does the same work the reference does.
+ */
+#define _GNU_SOURCE
+#include "vp8_oracle.h"
+
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define W_MAX 16u                       /* src/arnr.rs:37 */
+#define BLOCK_SAD_PER_PIXEL_CUTOFF 12u  /* src/arnr.rs:42 */
+#define MAX_STRENGTH 6                  /* src/arnr.rs:69 */
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static inline uint32_t absdiff(int a, int b) { return (uint32_t)(a > b ? a - b : b - a); }
+
+/* ======================================================================
+ * ARNR altref synthesis — src/arnr.rs
+ * ====================================================================== */
+
+/* pel: src/arnr.rs:149-153 (visible-dim clamp). */
+static inline uint8_t pel(const uint8_t *plane, size_t stride, int w, int h, int x, int y)
+{
+    return plane[(size_t)clampi(y, 0, h - 1) * stride + (size_t)clampi(x, 0, w - 1)];
+}
+
+/* block_sad_generic: src/arnr.rs:229-258. */
+uint32_t orc_block_sad_generic(const orc_i420 *src, const orc_i420 *ref,
+                               int bx, int by, int bw, int bh, int mvx, int mvy,
+                               uint32_t early_exit_at)
+{
+    int w = (int)src->width, h = (int)src->height;
+    uint32_t sad = 0;
+    for (int r = 0; r < bh; r++) {
+        int sy = by + r;
+        const uint8_t *srow = src->y + (size_t)sy * src->y_stride + bx;
+        for (int c = 0; c < bw; c++) {
+            uint8_t rp = pel(ref->y, ref->y_stride, w, h, bx + c + mvx, sy + mvy);
+            sad += absdiff(srow[c], rp);
+        }
+        if (sad >= early_exit_at) return sad;
+    }
+    return sad;
+}
+
+/* block_sad: src/arnr.rs:181-222 (in-bounds fast path + row early exit). */
+uint32_t orc_block_sad(const orc_i420 *src, const orc_i420 *ref,
+                       int bx, int by, int bw, int bh, int mvx, int mvy,
+                       uint32_t early_exit_at)
+{
+    int w = (int)src->width, h = (int)src->height;
+    int x0 = bx + mvx, y0 = by + mvy;
+    if (!(x0 >= 0 && y0 >= 0 && x0 + bw <= w && y0 + bh <= h))
+        return orc_block_sad_generic(src, ref, bx, by, bw, bh, mvx, mvy, early_exit_at);
+    uint32_t sad = 0;
+    for (int r = 0; r < bh; r++) {
+        const uint8_t *a = src->y + (size_t)(by + r) * src->y_stride + bx;
+        const uint8_t *b = ref->y + (size_t)(y0 + r) * ref->y_stride + x0;
+        uint32_t run = 0;
+        for (int c = 0; c < bw; c++) run += absdiff(a[c], b[c]);
+        sad += run;
+        if (sad >= early_exit_at) return sad;
+    }
+    return sad;
+}
+
+/* refine_search: src/arnr.rs:264-300.  8 -> 4 -> 2 -> 1 descent, fixed
+ * probe order, strict '<'. */
+void orc_refine_search(const orc_i420 *src, const orc_i420 *ref,
+                       int bx, int by, int bw, int bh, int32_t out3[3])
+{
+    int bestx = 0, besty = 0;
+    uint32_t best_sad = orc_block_sad(src, ref, bx, by, bw, bh, 0, 0, UINT32_MAX);
+    for (int step = 8; step >= 1; step /= 2) {
+        const int cx = bestx, cy = besty;
+        const int d[8][2] = {{-step, -step}, {0, -step}, {step, -step}, {-step, 0},
+                             {step, 0},      {-step, step}, {0, step},  {step, step}};
+        for (int k = 0; k < 8; k++) {
+            int px = cx + d[k][0], py = cy + d[k][1];
+            if (abs(px) > 15 || abs(py) > 15) continue;
+            uint32_t sad = orc_block_sad(src, ref, bx, by, bw, bh, px, py, best_sad);
+            if (sad < best_sad) {
+                best_sad = sad;
+                bestx = px;
+                besty = py;
+            }
+        }
+    }
+    out3[0] = bestx;
+    out3[1] = besty;
+    out3[2] = (int32_t)best_sad;
+}
+
+/* pixel_weight: src/arnr.rs:307-309. */
+uint32_t orc_pixel_weight(int32_t d, uint32_t s_scale)
+{
+    return (W_MAX * s_scale) / (s_scale + (uint32_t)(d * d));
+}
+
+/* weight_table: src/arnr.rs:317-323. */
+void orc_weight_table(uint32_t s_scale, uint32_t table[256])
+{
+    for (int d = 0; d < 256; d++) table[d] = orc_pixel_weight(d, s_scale);
+}
+
+/* accumulate_luma_block: src/arnr.rs:337-388. */
+void orc_accumulate_luma_block(const uint8_t *ctr_y, const uint8_t *f_y, size_t f_stride,
+                               int w, int h, int bx, int by, int bw, int bh,
+                               int mvx, int mvy, const uint32_t wt[256],
+                               uint32_t *acc, uint32_t *wsum)
+{
+    int x0 = bx + mvx, y0 = by + mvy;
+    if (x0 >= 0 && y0 >= 0 && x0 + bw <= w && y0 + bh <= h) {
+        for (int r = 0; r < bh; r++) {
+            size_t ob = (size_t)(by + r) * (size_t)w + (size_t)bx;
+            const uint8_t *rr = f_y + (size_t)(y0 + r) * f_stride + x0;
+            for (int c = 0; c < bw; c++) {
+                int rp = rr[c];
+                uint32_t wgt = wt[absdiff(rp, ctr_y[ob + c])];
+                acc[ob + c] += wgt * (uint32_t)rp;
+                wsum[ob + c] += wgt;
+            }
+        }
+        return;
+    }
+    for (int r = 0; r < bh; r++) {
+        int yy = by + r;
+        for (int c = 0; c < bw; c++) {
+            int xx = bx + c;
+            size_t o = (size_t)yy * (size_t)w + (size_t)xx;
+            int rp = pel(f_y, f_stride, w, h, xx + mvx, yy + mvy);
+            uint32_t wgt = wt[absdiff(rp, ctr_y[o])];
+            acc[o] += wgt * (uint32_t)rp;
+            wsum[o] += wgt;
+        }
+    }
+}
+
+/* accumulate_chroma_block: src/arnr.rs:402-459 (output coordinates are
+ * clamped on the generic path, double-accumulating overhang). */
+void orc_accumulate_chroma_block(const uint8_t *ctr, const uint8_t *f, size_t f_stride,
+                                 int cw, int ch, int cbx, int cby, int cbw, int cbh,
+                                 int cmvx, int cmvy, const uint32_t wt[256],
+                                 uint32_t *acc, uint32_t *wsum)
+{
+    int x0 = cbx + cmvx, y0 = cby + cmvy;
+    int out_in = cbx + cbw <= cw && cby + cbh <= ch;
+    if (out_in && x0 >= 0 && y0 >= 0 && x0 + cbw <= cw && y0 + cbh <= ch) {
+        for (int r = 0; r < cbh; r++) {
+            size_t ob = (size_t)(cby + r) * (size_t)cw + (size_t)cbx;
+            const uint8_t *rr = f + (size_t)(y0 + r) * f_stride + x0;
+            for (int c = 0; c < cbw; c++) {
+                int rp = rr[c];
+                uint32_t wgt = wt[absdiff(rp, ctr[ob + c])];
+                acc[ob + c] += wgt * (uint32_t)rp;
+                wsum[ob + c] += wgt;
+            }
+        }
+        return;
+    }
+    for (int r = 0; r < cbh; r++) {
+        int yy = cby + r < ch - 1 ? cby + r : ch - 1;
+        for (int c = 0; c < cbw; c++) {
+            int xx = cbx + c < cw - 1 ? cbx + c : cw - 1;
+            size_t o = (size_t)yy * (size_t)cw + (size_t)xx;
+            int rp = pel(f, f_stride, cw, ch, xx + cmvx, yy + cmvy);
+            uint32_t wgt = wt[absdiff(rp, ctr[o])];
+            acc[o] += wgt * (uint32_t)rp;
+            wsum[o] += wgt;
+        }
+    }
+}
+
+/* Search work queue: one unit per (frame, block row), src/arnr.rs:551-590. */
+typedef struct {
+    const orc_i420 *frames;
+    size_t n, center;
+    int w, h, bcols, brows;
+    int32_t *aligned;
+    atomic_size_t next;
+} search_job;
+
+static void *search_worker(void *arg)
+{
+    search_job *j = (search_job *)arg;
+    const size_t units = j->n * (size_t)j->brows;
+    for (;;) {
+        size_t unit = atomic_fetch_add(&j->next, 1);
+        if (unit >= units) break;
+        size_t fi = unit / (size_t)j->brows;
+        int brow = (int)(unit % (size_t)j->brows);
+        if (fi == j->center) continue;
+        int by = brow * 16, bh = j->h - by < 16 ? j->h - by : 16;
+        for (int bcol = 0; bcol < j->bcols; bcol++) {
+            int bx = bcol * 16, bw = j->w - bx < 16 ? j->w - bx : 16;
+            int32_t *slot = j->aligned + 3 * (unit * (size_t)j->bcols + (size_t)bcol);
+            orc_refine_search(&j->frames[j->center], &j->frames[fi], bx, by, bw, bh, slot);
+        }
+    }
+    return NULL;
+}
+
+/* copy_plane: src/arnr.rs:653-659. */
+static void copy_plane(const uint8_t *plane, size_t stride, size_t w, size_t h, uint8_t *out)
+{
+    for (size_t r = 0; r < h; r++) memcpy(out + r * w, plane + r * stride, w);
+}
+
+/* build_arnr_altref / build_arnr_altref_threaded: src/arnr.rs:476-650. */
+int orc_build_arnr_altref(const orc_i420 *frames, size_t n, size_t center,
+                          uint8_t strength, size_t threads,
+                          uint8_t *out_y, uint8_t *out_u, uint8_t *out_v,
+                          orc_err *err, int32_t *aligned_out)
+{
+    if (center >= n) return ORC_E_CENTER_OUT_OF_RANGE; /* the reference panics */
+    const orc_i420 *ctr = &frames[center];
+    if (ctr->width == 0 || ctr->height == 0 || ctr->width > 0x3FFF || ctr->height > 0x3FFF) {
+        if (err) {
+            err->src_w = ctr->width;
+            err->src_h = ctr->height;
+            err->ref_w = err->ref_h = 0;
+        }
+        return ORC_E_INVALID_DIMENSIONS;
+    }
+    for (size_t i = 0; i < n; i++) {
+        if (frames[i].width != ctr->width || frames[i].height != ctr->height) {
+            if (err) {
+                err->src_w = ctr->width;
+                err->src_h = ctr->height;
+                err->ref_w = frames[i].width;
+                err->ref_h = frames[i].height;
+            }
+            return ORC_E_REF_DIMS_MISMATCH;
+        }
+    }
+    const int w = (int)ctr->width, h = (int)ctr->height;
+    const int cw = (w + 1) / 2, ch = (h + 1) / 2;
+    copy_plane(ctr->y, ctr->y_stride, (size_t)w, (size_t)h, out_y);
+    copy_plane(ctr->u, ctr->uv_stride, (size_t)cw, (size_t)ch, out_u);
+    copy_plane(ctr->v, ctr->uv_stride, (size_t)cw, (size_t)ch, out_v);
+    if (strength == 0 || n < 2) return ORC_OK;
+
+    uint32_t wt[256];
+    orc_weight_table(8u << (strength < MAX_STRENGTH ? strength : MAX_STRENGTH), wt);
+
+    const size_t ny = (size_t)w * (size_t)h, nc = (size_t)cw * (size_t)ch;
+    uint32_t *acc_y = malloc(ny * 4), *ws_y = malloc(ny * 4);
+    uint32_t *acc_u = malloc(nc * 4), *ws_u = malloc(nc * 4);
+    uint32_t *acc_v = malloc(nc * 4), *ws_v = malloc(nc * 4);
+    /* accumulate_center: src/arnr.rs:662-667. */
+    for (size_t i = 0; i < ny; i++) { acc_y[i] = W_MAX * out_y[i]; ws_y[i] = W_MAX; }
+    for (size_t i = 0; i < nc; i++) {
+        acc_u[i] = W_MAX * out_u[i]; ws_u[i] = W_MAX;
+        acc_v[i] = W_MAX * out_v[i]; ws_v[i] = W_MAX;
+    }
+
+    const int bcols = (w + 15) / 16, brows = (h + 15) / 16;
+    const size_t nal = n * (size_t)brows * (size_t)bcols;
+    int32_t *aligned = aligned_out ? aligned_out : malloc(nal * 3 * sizeof(int32_t));
+    memset(aligned, 0, nal * 3 * sizeof(int32_t));
+    search_job job = {frames, n, center, w, h, bcols, brows, aligned, 0};
+    size_t units = n * (size_t)brows;
+    size_t workers = threads < 1 ? 1 : threads;
+    if (workers > units) workers = units;
+    if (workers <= 1) {
+        search_worker(&job);
+    } else {
+        pthread_t *tid = malloc(sizeof(pthread_t) * (workers - 1));
+        for (size_t t = 0; t + 1 < workers; t++) pthread_create(&tid[t], NULL, search_worker, &job);
+        search_worker(&job);
+        for (size_t t = 0; t + 1 < workers; t++) pthread_join(tid[t], NULL);
+        free(tid);
+    }
+
+    /* Blend, frame-major, block raster: src/arnr.rs:592-644. */
+    for (size_t fi = 0; fi < n; fi++) {
+        if (fi == center) continue;
+        const orc_i420 *f = &frames[fi];
+        for (int by = 0; by < h; by += 16) {
+            int bh = h - by < 16 ? h - by : 16;
+            for (int bx = 0; bx < w; bx += 16) {
+                int bw = w - bx < 16 ? w - bx : 16;
+                const int32_t *a = aligned + 3 * ((fi * (size_t)brows + (size_t)(by / 16)) * (size_t)bcols + (size_t)(bx / 16));
+                int mvx = a[0], mvy = a[1];
+                uint32_t sad = (uint32_t)a[2];
+                if (sad / (uint32_t)(bw * bh) > BLOCK_SAD_PER_PIXEL_CUTOFF) continue;
+                orc_accumulate_luma_block(out_y, f->y, f->y_stride, w, h, bx, by, bw, bh,
+                                          mvx, mvy, wt, acc_y, ws_y);
+                int cmvx = mvx / 2, cmvy = mvy / 2; /* C '/' truncates toward zero */
+                int cbx = bx / 2, cby = by / 2, cbw = (bw + 1) / 2, cbh = (bh + 1) / 2;
+                orc_accumulate_chroma_block(out_u, f->u, f->uv_stride, cw, ch, cbx, cby, cbw,
+                                            cbh, cmvx, cmvy, wt, acc_u, ws_u);
+                orc_accumulate_chroma_block(out_v, f->v, f->uv_stride, cw, ch, cbx, cby, cbw,
+                                            cbh, cmvx, cmvy, wt, acc_v, ws_v);
+            }
+        }
+    }
+    /* resolve_plane: src/arnr.rs:670-675. */
+    for (size_t i = 0; i < ny; i++) out_y[i] = (uint8_t)((acc_y[i] + ws_y[i] / 2) / ws_y[i]);
+    for (size_t i = 0; i < nc; i++) {
+        out_u[i] = (uint8_t)((acc_u[i] + ws_u[i] / 2) / ws_u[i]);
+        out_v[i] = (uint8_t)((acc_v[i] + ws_v[i] / 2) / ws_v[i]);
+    }
+    if (!aligned_out) free(aligned);
+    free(acc_y); free(ws_y); free(acc_u); free(ws_u); free(acc_v); free(ws_v);
+    return ORC_OK;
+}
+
+/* ======================================================================
+ * Motion compensation — src/motion_comp.rs
+ * ====================================================================== */
+
+/* SIXTAP_FILTERS / BILINEAR_FILTERS: src/motion_comp.rs:294-303, 328-337
+ * (RFC 6386 section 18.3 tables). */
+const int32_t ORC_SIXTAP_FILTERS[8][6] = {
+    {0, 0, 128, 0, 0, 0},     {0, -6, 123, 12, -1, 0}, {2, -11, 108, 36, -8, 1},
+    {0, -9, 93, 50, -6, 0},   {3, -16, 77, 77, -16, 3}, {0, -6, 50, 93, -9, 0},
+    {1, -8, 36, 108, -11, 2}, {0, -1, 12, 123, -6, 0}};
+const int32_t ORC_BILINEAR_FILTERS[8][6] = {
+    {0, 0, 128, 0, 0, 0}, {0, 0, 112, 16, 0, 0}, {0, 0, 96, 32, 0, 0}, {0, 0, 80, 48, 0, 0},
+    {0, 0, 64, 64, 0, 0}, {0, 0, 48, 80, 0, 0},  {0, 0, 32, 96, 0, 0}, {0, 0, 16, 112, 0, 0}};
+
+/* stored_luma_mv: src/motion_comp.rs:187-192 (wrapping x2 in i16). */
+orc_mv orc_stored_luma_mv(orc_mv mv)
+{
+    orc_mv o = {(int16_t)(uint16_t)((uint16_t)mv.row * 2u), (int16_t)(uint16_t)((uint16_t)mv.col * 2u)};
+    return o;
+}
+
+/* avg: src/motion_comp.rs:209-216. */
+int32_t orc_avg(int32_t c1, int32_t c2, int32_t c3, int32_t c4)
+{
+    int32_t s = c1 + c2 + c3 + c4;
+    return s >= 0 ? (s + 4) >> 3 : -((-s + 4) >> 3);
+}
+
+/* chroma_mv: src/motion_comp.rs:232-239. */
+orc_mv orc_chroma_mv(orc_mv l)
+{
+    orc_mv o = {(int16_t)orc_avg(l.row, l.row, l.row, l.row),
+                (int16_t)orc_avg(l.col, l.col, l.col, l.col)};
+    return o;
+}
+
+/* apply_full_pixel: src/motion_comp.rs:256-261. */
+orc_mv orc_apply_full_pixel(orc_mv mv)
+{
+    orc_mv o = {(int16_t)(mv.row & ~7), (int16_t)(mv.col & ~7)};
+    return o;
+}
+
+/* clamp255: src/inverse_transform.rs:155-157. */
+static inline uint8_t clamp255(int32_t v) { return (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v)); }
+
+/* interp: src/motion_comp.rs:396-402.  The dot product wraps like Rust's
+ * release-mode i32 arithmetic; >> is arithmetic. */
+uint8_t orc_interp(const int32_t f[6], const uint8_t s[6])
+{
+    uint32_t a = 0;
+    for (int i = 0; i < 6; i++) a += (uint32_t)s[i] * (uint32_t)f[i];
+    return clamp255(((int32_t)(a + 64u)) >> 7);
+}
+
+/* Generic edge-clamped halo gather shared by the fetch functions below. */
+static void gather(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                   long x0, long y0, int rows, int cols, uint8_t *out)
+{
+    long wi = (long)w, hi = (long)h;
+    if (x0 >= 0 && y0 >= 0 && x0 + cols <= wi && y0 + rows <= hi) {
+        for (int r = 0; r < rows; r++)
+            memcpy(out + r * cols, plane + (size_t)(y0 + r) * stride + (size_t)x0, (size_t)cols);
+        return;
+    }
+    for (int r = 0; r < rows; r++) {
+        long sy = y0 + r;
+        sy = sy < 0 ? 0 : (sy > hi - 1 ? hi - 1 : sy);
+        for (int c = 0; c < cols; c++) {
+            long sx = x0 + c;
+            sx = sx < 0 ? 0 : (sx > wi - 1 ? wi - 1 : sx);
+            out[r * cols + c] = plane[(size_t)sy * stride + (size_t)sx];
+        }
+    }
+}
+
+/* fetch_block_halo: src/motion_comp.rs:1633-1677 (9x9 at origin-2). */
+void orc_fetch_block_halo(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                          size_t blk_x, size_t blk_y, orc_mv mv, uint8_t out[81])
+{
+    gather(plane, stride, w, h, (long)blk_x + (mv.col >> 3) - 2, (long)blk_y + (mv.row >> 3) - 2,
+           9, 9, out);
+}
+
+/* Two-pass section 20.14 filter over an OW x OW block from its (OW+5)^2
+ * halo: sixtap_mb_luma_scalar / sixtap_mb_chroma_scalar / sixtap_2d_scalar,
+ * src/motion_comp.rs:1018-1057, 1311-1350, 680-736.  Running both passes
+ * always is exact: an identity tap row {0,0,128,0,0,0} returns its input
+ * (src/motion_comp.rs:919-1007). */
+static void sixtap_block(const uint8_t *halo, int ow, int mx, int my,
+                         const int32_t filters[8][6], uint8_t *out)
+{
+    const int hw = ow + 5;
+    uint8_t temp[21 * 16];
+    for (int r = 0; r < hw; r++)
+        for (int c = 0; c < ow; c++) temp[r * ow + c] = orc_interp(filters[mx], halo + r * hw + c);
+    for (int r = 0; r < ow; r++)
+        for (int c = 0; c < ow; c++) {
+            uint8_t s[6];
+            for (int k = 0; k < 6; k++) s[k] = temp[(r + k) * ow + c];
+            out[r * ow + c] = orc_interp(filters[my], s);
+        }
+}
+
+/* sixtap_2d: src/motion_comp.rs:680-736. */
+void orc_sixtap_2d(const uint8_t halo[81], int mx, int my, const int32_t filters[8][6],
+                   uint8_t out[16])
+{
+    sixtap_block(halo, 4, mx, my, filters, out);
+}
+
+/* fetch_luma_mb_halo: src/motion_comp.rs:837-880. */
+void orc_fetch_luma_mb_halo(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                            size_t mb_x, size_t mb_y, orc_mv mv, uint8_t out[441])
+{
+    gather(plane, stride, w, h, (long)mb_x + (mv.col >> 3) - 2, (long)mb_y + (mv.row >> 3) - 2,
+           21, 21, out);
+}
+
+/* sixtap_mb_luma: src/motion_comp.rs:902-917. */
+void orc_sixtap_mb_luma(const uint8_t halo[441], int mx, int my,
+                        const int32_t filters[8][6], uint8_t out[256])
+{
+    sixtap_block(halo, 16, mx, my, filters, out);
+}
+
+/* fetch_chroma_mb_halo: src/motion_comp.rs:1153-1196. */
+void orc_fetch_chroma_mb_halo(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                              size_t mb_x, size_t mb_y, orc_mv mv, uint8_t out[169])
+{
+    gather(plane, stride, w, h, (long)mb_x + (mv.col >> 3) - 2, (long)mb_y + (mv.row >> 3) - 2,
+           13, 13, out);
+}
+
+/* sixtap_mb_chroma: src/motion_comp.rs:1218-1233. */
+void orc_sixtap_mb_chroma(const uint8_t halo[169], int mx, int my,
+                          const int32_t filters[8][6], uint8_t out[64])
+{
+    sixtap_block(halo, 8, mx, my, filters, out);
+}
+
+/* fetch_luma_mb_whole_pixel: src/motion_comp.rs:1513-1560. */
+void orc_fetch_luma_mb_whole_pixel(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                                   size_t mb_x, size_t mb_y, orc_mv mv, uint8_t out[256])
+{
+    gather(plane, stride, w, h, (long)mb_x + (mv.col >> 3), (long)mb_y + (mv.row >> 3), 16, 16,
+           out);
+}
+
+/* fetch_chroma_mb_whole_pixel: src/motion_comp.rs:1577-1613. */
+void orc_fetch_chroma_mb_whole_pixel(const uint8_t *plane, size_t stride, size_t w,
+                                     size_t h, size_t mb_x, size_t mb_y, orc_mv mv,
+                                     uint8_t out[64])
+{
+    gather(plane, stride, w, h, (long)mb_x + (mv.col >> 3), (long)mb_y + (mv.row >> 3), 8, 8, out);
+}
+
+/* predict_inter_mb: src/motion_comp.rs:1986-2078. */
+void orc_predict_inter_mb(const orc_ref_planes *ref, size_t mb_col, size_t mb_row,
+                          orc_mv luma_mv, int full_pixel, const int32_t filters[8][6],
+                          uint8_t out[384])
+{
+    orc_mv ymv = orc_stored_luma_mv(luma_mv);
+    orc_mv uvmv = orc_chroma_mv(ymv);
+    if (full_pixel) {
+        ymv = orc_apply_full_pixel(ymv);
+        uvmv = orc_apply_full_pixel(uvmv);
+    }
+    size_t lw = ref->mb_cols * 16, lh = ref->mb_rows * 16;
+    size_t cw = ref->mb_cols * 8, ch = ref->mb_rows * 8;
+    int mx = ymv.col & 7, my = ymv.row & 7;
+    if (mx == 0 && my == 0) {
+        orc_fetch_luma_mb_whole_pixel(ref->y, ref->y_stride, lw, lh, mb_col * 16, mb_row * 16,
+                                      ymv, out);
+    } else {
+        uint8_t halo[441];
+        orc_fetch_luma_mb_halo(ref->y, ref->y_stride, lw, lh, mb_col * 16, mb_row * 16, ymv, halo);
+        orc_sixtap_mb_luma(halo, mx, my, filters, out);
+    }
+    int cmx = uvmv.col & 7, cmy = uvmv.row & 7;
+    if (cmx == 0 && cmy == 0) {
+        orc_fetch_chroma_mb_whole_pixel(ref->u, ref->uv_stride, cw, ch, mb_col * 8, mb_row * 8,
+                                        uvmv, out + 256);
+        orc_fetch_chroma_mb_whole_pixel(ref->v, ref->uv_stride, cw, ch, mb_col * 8, mb_row * 8,
+                                        uvmv, out + 320);
+    } else {
+        uint8_t uh[169], vh[169];
+        orc_fetch_chroma_mb_halo(ref->u, ref->uv_stride, cw, ch, mb_col * 8, mb_row * 8, uvmv, uh);
+        orc_fetch_chroma_mb_halo(ref->v, ref->uv_stride, cw, ch, mb_col * 8, mb_row * 8, uvmv, vh);
+        orc_sixtap_mb_chroma(uh, cmx, cmy, filters, out + 256);
+        orc_sixtap_mb_chroma(vh, cmx, cmy, filters, out + 320);
+    }
+}
+
+/* ======================================================================
+ * SAD scoring — src/motion_search.rs
+ * ====================================================================== */
+
+/* block_sad_16x16_scalar: src/motion_search.rs:153-161. */
+uint32_t orc_block_sad_16x16(const uint8_t src[256], const uint8_t pred[256])
+{
+    uint32_t acc = 0;
+    for (int i = 0; i < 256; i++) acc += absdiff(src[i], pred[i]);
+    return acc;
+}
+
+/* mb_luma_sad_at_whole_mv: src/motion_search.rs:291-349. */
+uint32_t orc_mb_luma_sad_at_whole_mv(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
+                                     const uint8_t src[256], orc_mv mv)
+{
+    uint8_t pred[256];
+    orc_fetch_luma_mb_whole_pixel(ref->plane, ref->stride, ref->width, ref->height, mb_col * 16,
+                                  mb_row * 16, orc_stored_luma_mv(mv), pred);
+    return orc_block_sad_16x16(src, pred);
+}
+
+/* mb_luma_sad_at_mv: src/motion_search.rs:470-529 (six-tap set always). */
+uint32_t orc_mb_luma_sad_at_mv(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
+                               const uint8_t src[256], orc_mv mv)
+{
+    orc_mv m8 = orc_stored_luma_mv(mv);
+    int mx = m8.col & 7, my = m8.row & 7;
+    uint8_t pred[256];
+    if (mx == 0 && my == 0) {
+        orc_fetch_luma_mb_whole_pixel(ref->plane, ref->stride, ref->width, ref->height,
+                                      mb_col * 16, mb_row * 16, m8, pred);
+    } else {
+        uint8_t halo[441];
+        orc_fetch_luma_mb_halo(ref->plane, ref->stride, ref->width, ref->height, mb_col * 16,
+                               mb_row * 16, m8, halo);
+        orc_sixtap_mb_luma(halo, mx, my, ORC_SIXTAP_FILTERS, pred);
+    }
+    return orc_block_sad_16x16(src, pred);
+}
+
+/* clamp_component: src/motion_search.rs:353-355 (MV_MIN/MV_MAX :79-82). */
+static inline int16_t clamp_component(int v) { return (int16_t)clampi(v, -1023, 1023); }
+
+/* Shared body of half_pixel_refine_luma / quarter_pixel_refine_luma,
+ * src/motion_search.rs:564-627 and 666-728: centre plus 8 offsets in
+ * (row, col) order, collapse-skip, strict '<'. */
+static orc_search_result refine_ladder(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
+                                       const uint8_t src[256], orc_mv center, int step)
+{
+    orc_search_result best = {center, orc_mb_luma_sad_at_mv(ref, mb_col, mb_row, src, center)};
+    const int off[8][2] = {{-step, -step}, {-step, 0}, {-step, step}, {0, -step},
+                           {0, step},      {step, -step}, {step, 0}, {step, step}};
+    for (int k = 0; k < 8; k++) {
+        orc_mv c = {clamp_component(center.row + off[k][0]), clamp_component(center.col + off[k][1])};
+        if (c.row == best.mv.row && c.col == best.mv.col) continue;
+        uint32_t s = orc_mb_luma_sad_at_mv(ref, mb_col, mb_row, src, c);
+        if (s < best.sad) {
+            best.sad = s;
+            best.mv = c;
+        }
+    }
+    return best;
+}
+
+orc_search_result orc_half_pixel_refine_luma(const orc_luma_ref *ref, size_t mb_col,
+                                             size_t mb_row, const uint8_t src[256],
+                                             orc_mv center)
+{
+    return refine_ladder(ref, mb_col, mb_row, src, center, 2);
+}
+
+orc_search_result orc_quarter_pixel_refine_luma(const orc_luma_ref *ref, size_t mb_col,
+                                                size_t mb_row, const uint8_t src[256],
+                                                orc_mv center)
+{
+    return refine_ladder(ref, mb_col, mb_row, src, center, 1);
+}
+
+/* fetch_window: src/encoder/inter_search.rs:266-287 (22x22 at (x-3, y-3)). */
+void orc_fetch_window(const orc_luma_ref *ref, int x, int y, uint8_t out[22 * 22])
+{
+    gather(ref->plane, ref->stride, ref->width, ref->height, x - 3, y - 3, 22, 22, out);
+}
+
+/* HalfPlanes::new: src/encoder/inter_search.rs:303-330. */
+void orc_half_planes_new(const uint8_t win[22 * 22], const int32_t taps[6], int diagonals,
+                         orc_half_planes *p)
+{
+    memset(p, 0, sizeof(*p));
+    int r0 = diagonals ? 0 : 3, r1 = diagonals ? 22 : 19;
+    for (int r = r0; r < r1; r++)
+        for (int c = 0; c < 17; c++) p->h[r * 17 + c] = orc_interp(taps, win + r * 22 + c);
+    for (int r = 0; r < 17; r++)
+        for (int c = 0; c < 16; c++) {
+            uint8_t s[6];
+            for (int k = 0; k < 6; k++) s[k] = win[(r + k) * 22 + 3 + c];
+            p->v[r * 16 + c] = orc_interp(taps, s);
+        }
+    if (diagonals)
+        for (int r = 0; r < 17; r++)
+            for (int c = 0; c < 17; c++) {
+                uint8_t s[6];
+                for (int k = 0; k < 6; k++) s[k] = p->h[(r + k) * 17 + c];
+                p->d[r * 17 + c] = orc_interp(taps, s);
+            }
+}
+
+/* window_sad: src/encoder/inter_search.rs:211-217. */
+static uint32_t window_sad(const uint8_t src[256], const uint8_t *plane, int stride, int offset)
+{
+    uint32_t acc = 0;
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) acc += absdiff(src[r * 16 + c], plane[offset + r * stride + c]);
+    return acc;
+}
+
+/* HalfPlanes::sad: src/encoder/inter_search.rs:334-345. */
+uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], int dx, int dy)
+{
+    int cx = dx > 0, cy = dy > 0;
+    if (dy == 0) return window_sad(src, p->h, 17, 3 * 17 + cx);
+    if (dx == 0) return window_sad(src, p->v, 16, cy * 16);
+    return window_sad(src, p->d, 17, cy * 17 + cx);
+}
+
+/* ---------------- batch helpers ---------------- */
+void orc_predict_inter_mb_batch(const orc_ref_planes *ref, const uint32_t *mb_xy,
+                                const orc_mv *mv, size_t n, int full_pixel,
+                                const int32_t filters[8][6], uint8_t *out)
+{
+    for (size_t i = 0; i < n; i++)
+        orc_predict_inter_mb(ref, mb_xy[2 * i], mb_xy[2 * i + 1], mv[i], full_pixel, filters,
+                             out + 384 * i);
+}
+
+void orc_mb_luma_sad_at_mv_batch(const orc_luma_ref *ref, const uint32_t *mb_xy,
+                                 const uint8_t *src, const orc_mv *mv, size_t n,
+                                 uint32_t *sad)
+{
+    for (size_t i = 0; i < n; i++)
+        sad[i] = orc_mb_luma_sad_at_mv(ref, mb_xy[2 * i], mb_xy[2 * i + 1], src + 256 * i, mv[i]);
+}

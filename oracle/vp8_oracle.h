@@ -1,0 +1,161 @@
+/*
+ * CPU oracle for the oxideav-vp8 ARNR + motion-compensation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference crate's integer algorithms, used as the checker for the HIP
+ * path and as the timed CPU baseline.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product library never
+ * links, calls or falls back to it.
+ *
+ * Every function cites the reference file:line it follows (paths relative
+ * to the oxideav-vp8 crate root).  Pinning of each function (which
+ * known-answer test, invariant or brute-force check holds it) is listed in
+ * tests/test_oracle_*.py; no function is "parity unpinned".
+ */
+#ifndef VP8_ORACLE_H
+#define VP8_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes shared with include/vp8g.h (same numeric values). */
+enum {
+    ORC_OK = 0,
+    ORC_E_INVALID_DIMENSIONS = 1,
+    ORC_E_REF_DIMS_MISMATCH = 2,
+    ORC_E_CENTER_OUT_OF_RANGE = 3,
+    ORC_E_ARG = 4
+};
+
+/* I420Frame, src/encoder.rs:5419-5434. */
+typedef struct {
+    uint32_t width, height;
+    const uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride;
+} orc_i420;
+
+/* EncodeError payloads, src/encoder.rs:69-104. */
+typedef struct {
+    uint32_t src_w, src_h, ref_w, ref_h;
+} orc_err;
+
+/* Mv, src/motion_vector.rs:64-70 (quarter-pel, row then col). */
+typedef struct {
+    int16_t row, col;
+} orc_mv;
+
+/* ---------------- ARNR (src/arnr.rs) ---------------- */
+uint32_t orc_pixel_weight(int32_t d, uint32_t s_scale);
+void orc_weight_table(uint32_t s_scale, uint32_t table[256]);
+uint32_t orc_block_sad(const orc_i420 *src, const orc_i420 *ref,
+                       int bx, int by, int bw, int bh, int mvx, int mvy,
+                       uint32_t early_exit_at);
+uint32_t orc_block_sad_generic(const orc_i420 *src, const orc_i420 *ref,
+                               int bx, int by, int bw, int bh, int mvx, int mvy,
+                               uint32_t early_exit_at);
+/* out3 = {mvx, mvy, sad} */
+void orc_refine_search(const orc_i420 *src, const orc_i420 *ref,
+                       int bx, int by, int bw, int bh, int32_t out3[3]);
+void orc_accumulate_luma_block(const uint8_t *ctr_y, const uint8_t *f_y, size_t f_stride,
+                               int w, int h, int bx, int by, int bw, int bh,
+                               int mvx, int mvy, const uint32_t wt[256],
+                               uint32_t *acc, uint32_t *wsum);
+void orc_accumulate_chroma_block(const uint8_t *ctr, const uint8_t *f, size_t f_stride,
+                                 int cw, int ch, int cbx, int cby, int cbw, int cbh,
+                                 int cmvx, int cmvy, const uint32_t wt[256],
+                                 uint32_t *acc, uint32_t *wsum);
+/*
+ * build_arnr_altref_threaded.  Output planes are tightly packed
+ * (w*h, cw*ch, cw*ch).  `aligned_out` (optional, may be NULL) receives the
+ * search table, n * brows * bcols entries of {mvx, mvy, sad} (centre rows
+ * left zero).  Returns an ORC_* status.
+ */
+int orc_build_arnr_altref(const orc_i420 *frames, size_t n, size_t center,
+                          uint8_t strength, size_t threads,
+                          uint8_t *out_y, uint8_t *out_u, uint8_t *out_v,
+                          orc_err *err, int32_t *aligned_out);
+
+/* ---------------- motion compensation (src/motion_comp.rs) ---------------- */
+extern const int32_t ORC_SIXTAP_FILTERS[8][6];
+extern const int32_t ORC_BILINEAR_FILTERS[8][6];
+orc_mv orc_stored_luma_mv(orc_mv mv);
+int32_t orc_avg(int32_t c1, int32_t c2, int32_t c3, int32_t c4);
+orc_mv orc_chroma_mv(orc_mv luma_mv);
+orc_mv orc_apply_full_pixel(orc_mv mv);
+uint8_t orc_interp(const int32_t filter[6], const uint8_t support[6]);
+void orc_fetch_block_halo(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                          size_t blk_x, size_t blk_y, orc_mv mv, uint8_t out[81]);
+void orc_sixtap_2d(const uint8_t halo[81], int mx, int my, const int32_t filters[8][6],
+                   uint8_t out[16]);
+void orc_fetch_luma_mb_halo(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                            size_t mb_x, size_t mb_y, orc_mv mv, uint8_t out[441]);
+void orc_sixtap_mb_luma(const uint8_t halo[441], int mx, int my,
+                        const int32_t filters[8][6], uint8_t out[256]);
+void orc_fetch_chroma_mb_halo(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                              size_t mb_x, size_t mb_y, orc_mv mv, uint8_t out[169]);
+void orc_sixtap_mb_chroma(const uint8_t halo[169], int mx, int my,
+                          const int32_t filters[8][6], uint8_t out[64]);
+void orc_fetch_luma_mb_whole_pixel(const uint8_t *plane, size_t stride, size_t w, size_t h,
+                                   size_t mb_x, size_t mb_y, orc_mv mv, uint8_t out[256]);
+void orc_fetch_chroma_mb_whole_pixel(const uint8_t *plane, size_t stride, size_t w,
+                                     size_t h, size_t mb_x, size_t mb_y, orc_mv mv,
+                                     uint8_t out[64]);
+/* ReferencePlanes, src/motion_comp.rs:131-146. */
+typedef struct {
+    const uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride, mb_cols, mb_rows;
+} orc_ref_planes;
+/* predict_inter_mb; out = y[256] | u[64] | v[64] (ReconstructedMb). */
+void orc_predict_inter_mb(const orc_ref_planes *ref, size_t mb_col, size_t mb_row,
+                          orc_mv luma_mv, int full_pixel, const int32_t filters[8][6],
+                          uint8_t out[384]);
+
+/* ---------------- SAD scoring (src/motion_search.rs) ---------------- */
+/* LumaRef, src/motion_search.rs:230-239. */
+typedef struct {
+    const uint8_t *plane;
+    size_t stride, width, height;
+} orc_luma_ref;
+typedef struct {
+    orc_mv mv;
+    uint32_t sad;
+} orc_search_result;
+uint32_t orc_block_sad_16x16(const uint8_t src[256], const uint8_t pred[256]);
+uint32_t orc_mb_luma_sad_at_whole_mv(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
+                                     const uint8_t src[256], orc_mv mv);
+uint32_t orc_mb_luma_sad_at_mv(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
+                               const uint8_t src[256], orc_mv mv);
+orc_search_result orc_half_pixel_refine_luma(const orc_luma_ref *ref, size_t mb_col,
+                                             size_t mb_row, const uint8_t src[256],
+                                             orc_mv center);
+orc_search_result orc_quarter_pixel_refine_luma(const orc_luma_ref *ref, size_t mb_col,
+                                                size_t mb_row, const uint8_t src[256],
+                                                orc_mv center);
+
+/* HalfPlanes, src/encoder/inter_search.rs:256-346. */
+void orc_fetch_window(const orc_luma_ref *ref, int x, int y, uint8_t out[22 * 22]);
+typedef struct {
+    uint8_t h[22 * 17];
+    uint8_t v[17 * 16];
+    uint8_t d[17 * 17];
+} orc_half_planes;
+void orc_half_planes_new(const uint8_t window[22 * 22], const int32_t taps[6], int diagonals,
+                         orc_half_planes *out);
+uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], int dx, int dy);
+
+/* ---------------- batch helpers (one call over many MBs) ---------------- */
+void orc_predict_inter_mb_batch(const orc_ref_planes *ref, const uint32_t *mb_xy,
+                                const orc_mv *mv, size_t n, int full_pixel,
+                                const int32_t filters[8][6], uint8_t *out);
+void orc_mb_luma_sad_at_mv_batch(const orc_luma_ref *ref, const uint32_t *mb_xy,
+                                 const uint8_t *src, const orc_mv *mv, size_t n,
+                                 uint32_t *sad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

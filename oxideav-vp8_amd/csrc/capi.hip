@@ -1,0 +1,460 @@
+// C ABI (include/vp8g.h): argument validation in the reference's order,
+// device staging for the host-pointer entry points, kernel dispatch.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/vp8g.h"
+#include "vp8g_internal.h"
+
+using namespace vp8g;
+
+struct vp8g_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  int arnr_waves = 0;
+  // Device arena for the host-pointer entry points (grown, never shrunk).
+  uint8_t* arena = nullptr;
+  size_t arena_bytes = 0;
+};
+
+namespace {
+
+bool hip_ok(hipError_t e) { return e == hipSuccess; }
+
+int ensure_arena(vp8g_ctx* c, size_t bytes) {
+  if (bytes <= c->arena_bytes) return VP8G_OK;
+  if (c->arena) {
+    if (!hip_ok(hipStreamSynchronize(c->stream))) return VP8G_E_DEVICE;
+    (void)hipFree(c->arena);
+    c->arena = nullptr;
+    c->arena_bytes = 0;
+  }
+  size_t want = bytes + (bytes >> 2);
+  if (!hip_ok(hipMalloc(&c->arena, want))) return VP8G_E_DEVICE;
+  c->arena_bytes = want;
+  return VP8G_OK;
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Reference check order, src/arnr.rs:494-515.
+int validate_arnr(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* err) {
+  if (!frames && n) return VP8G_E_ARG;
+  if (center >= n) return VP8G_E_CENTER_OUT_OF_RANGE;
+  const vp8g_i420& c = frames[center];
+  if (c.width == 0 || c.height == 0 || c.width > 0x3FFF || c.height > 0x3FFF) {
+    if (err) *err = vp8g_err{c.width, c.height, 0, 0};
+    return VP8G_E_INVALID_DIMENSIONS;
+  }
+  for (size_t i = 0; i < n; i++) {
+    if (frames[i].width != c.width || frames[i].height != c.height) {
+      if (err) *err = vp8g_err{c.width, c.height, frames[i].width, frames[i].height};
+      return VP8G_E_REF_DIMS_MISMATCH;
+    }
+  }
+  if (n > (size_t)VP8G_MAX_FRAMES) return VP8G_E_ARG;
+  const size_t cw = (c.width + 1) / 2;
+  for (size_t i = 0; i < n; i++) {
+    const vp8g_i420& f = frames[i];
+    if (!f.y || !f.u || !f.v || f.y_stride < c.width || f.uv_stride < cw) return VP8G_E_ARG;
+    if (f.y_stride > 0xFFFFFFFFu || f.uv_stride > 0xFFFFFFFFu) return VP8G_E_ARG;
+  }
+  return VP8G_OK;
+}
+
+int pick_waves(const vp8g_ctx* c, int nnb) {
+  int nw = c->arnr_waves > 0 ? c->arnr_waves : nnb;
+  if (nw < 1) nw = 1;
+  if (nw > 16) nw = 16;
+  return nw;
+}
+
+// Enqueue the fused ARNR kernel over device-resident frames.
+int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, uint8_t strength,
+                 uint8_t* oy, uint8_t* ou, uint8_t* ov, int32_t* mv_out) {
+  ArnrArgs a;
+  memset(&a, 0, sizeof(a));
+  const vp8g_i420& ctr = frames[center];
+  bool aligned4 = true;
+  for (size_t i = 0; i < n; i++) {
+    a.frames[i] = FrameDesc{frames[i].y, frames[i].u, frames[i].v, (uint32_t)frames[i].y_stride,
+                            (uint32_t)frames[i].uv_stride};
+    if (((uintptr_t)frames[i].y & 3) || (frames[i].y_stride & 3)) aligned4 = false;
+  }
+  a.out_y = oy;
+  a.out_u = ou;
+  a.out_v = ov;
+  a.mv_out = mv_out;
+  a.w = (int)ctr.width;
+  a.h = (int)ctr.height;
+  a.cw = (a.w + 1) / 2;
+  a.ch = (a.h + 1) / 2;
+  a.bcols = (a.w + 15) / 16;
+  a.brows = (a.h + 15) / 16;
+  a.n = (int)n;
+  a.center = (int)center;
+  a.filter_on = (strength != 0 && n >= 2) ? 1 : 0;
+  a.aligned4 = aligned4 ? 1 : 0;
+  // weight_table(8 << min(strength, 6)), src/arnr.rs:307-323, 533-536.
+  const uint32_t s = 8u << (strength < kMaxStrength ? strength : kMaxStrength);
+  for (uint32_t d = 0; d < 256; d++) a.wt[d] = (uint8_t)((kWMax * s) / (s + d * d));
+  const int nw = pick_waves(c, a.filter_on ? (int)n - 1 : 1);
+  return hip_ok(launch_arnr_fused(a, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int check_ref_planes(const vp8g_ref_planes* r) {
+  if (!r || !r->y || !r->u || !r->v || r->mb_cols == 0 || r->mb_rows == 0) return VP8G_E_ARG;
+  if (r->y_stride < r->mb_cols * 16 || r->uv_stride < r->mb_cols * 8) return VP8G_E_ARG;
+  if (r->mb_cols * 16 > 0x7FFFFFF || r->mb_rows * 16 > 0x7FFFFFF) return VP8G_E_ARG;
+  if (r->y_stride > 0xFFFFFFFFu || r->uv_stride > 0xFFFFFFFFu) return VP8G_E_ARG;
+  return VP8G_OK;
+}
+
+int check_luma_ref(const vp8g_luma_ref* r) {
+  if (!r || !r->plane || r->width == 0 || r->height == 0 || r->stride < r->width) return VP8G_E_ARG;
+  if (r->width > 0x7FFFFFF || r->height > 0x7FFFFFF || r->stride > 0xFFFFFFFFu) return VP8G_E_ARG;
+  return VP8G_OK;
+}
+
+McArgs make_mc_args(const vp8g_ref_planes* r, const uint32_t* mb_xy, const vp8g_mv* mv, size_t n,
+                    int full_pixel, const int32_t filters[8][6], uint8_t* out) {
+  McArgs a;
+  memset(&a, 0, sizeof(a));
+  a.ref = McPlanes{r->y, r->u, r->v, (uint32_t)r->y_stride, (uint32_t)r->uv_stride,
+                   (int)(r->mb_cols * 16), (int)(r->mb_rows * 16), (int)(r->mb_cols * 8),
+                   (int)(r->mb_rows * 8)};
+  a.mb_xy = mb_xy;
+  a.mv = reinterpret_cast<const int16_t*>(mv);
+  a.out = out;
+  a.n = (int)n;
+  a.full_pixel = full_pixel ? 1 : 0;
+  memcpy(a.filters, filters, sizeof(a.filters));
+  return a;
+}
+
+SadArgs make_sad_args(const vp8g_luma_ref* r, const uint32_t* mb_xy, const uint8_t* src,
+                      const vp8g_mv* mv, size_t n, uint32_t* sad) {
+  SadArgs a;
+  a.plane = r->plane;
+  a.stride = (uint32_t)r->stride;
+  a.w = (int)r->width;
+  a.h = (int)r->height;
+  a.mb_xy = mb_xy;
+  a.src = src;
+  a.mv = reinterpret_cast<const int16_t*>(mv);
+  a.sad = sad;
+  a.n = (int)n;
+  return a;
+}
+
+// Macroblock positions past the plane are legal (every fetch clamps, as in
+// the reference); the kernels compute pixel origins in 32-bit, so positions
+// are limited to 2^20 macroblocks per axis.
+bool mb_xy_in_range(const uint32_t* mb_xy, size_t n) {
+  for (size_t i = 0; i < 2 * n; i++)
+    if (mb_xy[i] > 0xFFFFFu) return false;
+  return true;
+}
+
+// Bytes of a strided plane that the kernels may read.
+size_t plane_span(size_t stride, size_t width, size_t rows) { return rows ? (rows - 1) * stride + width : 0; }
+
+int copy_out(vp8g_ctx* c, void* host, const void* dev, size_t bytes) {
+  if (!bytes) return VP8G_OK;
+  return hip_ok(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream)) ? VP8G_OK
+                                                                                     : VP8G_E_DEVICE;
+}
+
+int copy_in(vp8g_ctx* c, void* dev, const void* host, size_t bytes) {
+  if (!bytes) return VP8G_OK;
+  return hip_ok(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream)) ? VP8G_OK
+                                                                                    : VP8G_E_DEVICE;
+}
+
+int finish(vp8g_ctx* c, int st) {
+  if (st != VP8G_OK) {
+    (void)hipStreamSynchronize(c->stream);
+    return st;
+  }
+  return hip_ok(hipStreamSynchronize(c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vp8g_status_str(int s) {
+  switch (s) {
+    case VP8G_OK: return "ok";
+    case VP8G_E_INVALID_DIMENSIONS: return "invalid dimensions";
+    case VP8G_E_REF_DIMS_MISMATCH: return "reference dimensions mismatch";
+    case VP8G_E_CENTER_OUT_OF_RANGE: return "center index out of range";
+    case VP8G_E_ARG: return "invalid argument";
+    case VP8G_E_DEVICE: return "HIP device error";
+    case VP8G_E_NO_DEVICE: return "no usable device";
+    default: return "unknown status";
+  }
+}
+
+int vp8g_ctx_create(int device, vp8g_ctx** out) {
+  if (!out) return VP8G_E_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (!hip_ok(hipGetDeviceCount(&count)) || count <= 0) return VP8G_E_NO_DEVICE;
+  if (device < 0 || device >= count) return VP8G_E_NO_DEVICE;
+  hipDeviceProp_t prop;
+  if (!hip_ok(hipGetDeviceProperties(&prop, device))) return VP8G_E_NO_DEVICE;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return VP8G_E_NO_DEVICE;
+  if (!hip_ok(hipSetDevice(device))) return VP8G_E_DEVICE;
+  vp8g_ctx* c = new vp8g_ctx();
+  c->device = device;
+  if (!hip_ok(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking))) {
+    delete c;
+    return VP8G_E_DEVICE;
+  }
+  c->stream = c->own;
+  *out = c;
+  return VP8G_OK;
+}
+
+void vp8g_ctx_destroy(vp8g_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
+int vp8g_ctx_set_stream(vp8g_ctx* c, void* stream) {
+  if (!c) return VP8G_E_ARG;
+  c->stream = stream ? (hipStream_t)stream : c->own;
+  return VP8G_OK;
+}
+
+void* vp8g_ctx_stream(vp8g_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int vp8g_ctx_synchronize(vp8g_ctx* c) {
+  if (!c) return VP8G_E_ARG;
+  return hip_ok(hipStreamSynchronize(c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int vp8g_ctx_set_arnr_waves(vp8g_ctx* c, int waves) {
+  if (!c || waves < 0 || waves > 16) return VP8G_E_ARG;
+  c->arnr_waves = waves;
+  return VP8G_OK;
+}
+
+int vp8g_build_arnr_altref_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
+                               uint8_t strength, uint8_t* oy, uint8_t* ou, uint8_t* ov,
+                               vp8g_err* err, int32_t* mv_out) {
+  if (!c) return VP8G_E_ARG;
+  int st = validate_arnr(frames, n, center, err);
+  if (st != VP8G_OK) return st;
+  if (!oy || !ou || !ov) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  return enqueue_arnr(c, frames, n, center, strength, oy, ou, ov, mv_out);
+}
+
+int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
+                           uint8_t strength, uint8_t* oy, uint8_t* ou, uint8_t* ov,
+                           vp8g_err* err) {
+  if (!c) return VP8G_E_ARG;
+  int st = validate_arnr(frames, n, center, err);
+  if (st != VP8G_OK) return st;
+  if (!oy || !ou || !ov) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const size_t w = frames[center].width, h = frames[center].height;
+  const size_t cw = (w + 1) / 2, ch = (h + 1) / 2;
+  // Device layout: each frame's planes at a 256-byte aligned offset with
+  // pitch rounded to 64 bytes, then the packed outputs.
+  const size_t yp = align_up(w, 64), cp = align_up(cw, 64);
+  const size_t ysz = align_up(yp * h, 256), csz = align_up(cp * ch, 256);
+  const size_t per = ysz + 2 * csz;
+  const size_t out_bytes = align_up(w * h, 256) + 2 * align_up(cw * ch, 256);
+  st = ensure_arena(c, per * n + out_bytes);
+  if (st != VP8G_OK) return st;
+  std::vector<vp8g_i420> dev(n);
+  for (size_t i = 0; i < n; i++) {
+    uint8_t* base = c->arena + per * i;
+    const vp8g_i420& f = frames[i];
+    if (!hip_ok(hipMemcpy2DAsync(base, yp, f.y, f.y_stride, w, h, hipMemcpyHostToDevice, c->stream)) ||
+        !hip_ok(hipMemcpy2DAsync(base + ysz, cp, f.u, f.uv_stride, cw, ch, hipMemcpyHostToDevice,
+                                 c->stream)) ||
+        !hip_ok(hipMemcpy2DAsync(base + ysz + csz, cp, f.v, f.uv_stride, cw, ch,
+                                 hipMemcpyHostToDevice, c->stream)))
+      return finish(c, VP8G_E_DEVICE);
+    dev[i] = vp8g_i420{f.width, f.height, base, base + ysz, base + ysz + csz, yp, cp};
+  }
+  uint8_t* dy = c->arena + per * n;
+  uint8_t* du = dy + align_up(w * h, 256);
+  uint8_t* dv = du + align_up(cw * ch, 256);
+  st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr);
+  if (st == VP8G_OK) st = copy_out(c, oy, dy, w * h);
+  if (st == VP8G_OK) st = copy_out(c, ou, du, cw * ch);
+  if (st == VP8G_OK) st = copy_out(c, ov, dv, cw * ch);
+  return finish(c, st);
+}
+
+int vp8g_predict_inter_mb_batch_dev(vp8g_ctx* c, const vp8g_ref_planes* ref,
+                                    const uint32_t* mb_xy, const vp8g_mv* mv, size_t n,
+                                    int full_pixel, const int32_t filters[8][6], uint8_t* out) {
+  if (!c || !filters) return VP8G_E_ARG;
+  int st = check_ref_planes(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !mv || !out || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  McArgs a = make_mc_args(ref, mb_xy, mv, n, full_pixel, filters, out);
+  return hip_ok(launch_predict_inter_mb(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int vp8g_predict_inter_mb_batch(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy,
+                                const vp8g_mv* mv, size_t n, int full_pixel,
+                                const int32_t filters[8][6], uint8_t* out) {
+  if (!c || !filters) return VP8G_E_ARG;
+  int st = check_ref_planes(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !mv || !out || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (!mb_xy_in_range(mb_xy, n)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const size_t ys = plane_span(ref->y_stride, ref->mb_cols * 16, ref->mb_rows * 16);
+  const size_t cs = plane_span(ref->uv_stride, ref->mb_cols * 8, ref->mb_rows * 8);
+  const size_t o_y = 0, o_u = align_up(ys, 256), o_v = o_u + align_up(cs, 256);
+  const size_t o_xy = o_v + align_up(cs, 256), o_mv = o_xy + align_up(8 * n, 256);
+  const size_t o_out = o_mv + align_up(4 * n, 256), total = o_out + 384 * n;
+  st = ensure_arena(c, total);
+  if (st != VP8G_OK) return st;
+  uint8_t* b = c->arena;
+  st = copy_in(c, b + o_y, ref->y, ys);
+  if (st == VP8G_OK) st = copy_in(c, b + o_u, ref->u, cs);
+  if (st == VP8G_OK) st = copy_in(c, b + o_v, ref->v, cs);
+  if (st == VP8G_OK) st = copy_in(c, b + o_xy, mb_xy, 8 * n);
+  if (st == VP8G_OK) st = copy_in(c, b + o_mv, mv, 4 * n);
+  if (st != VP8G_OK) return finish(c, st);
+  vp8g_ref_planes d = *ref;
+  d.y = b + o_y;
+  d.u = b + o_u;
+  d.v = b + o_v;
+  McArgs a = make_mc_args(&d, reinterpret_cast<const uint32_t*>(b + o_xy),
+                          reinterpret_cast<const vp8g_mv*>(b + o_mv), n, full_pixel, filters,
+                          b + o_out);
+  st = hip_ok(launch_predict_inter_mb(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  if (st == VP8G_OK) st = copy_out(c, out, b + o_out, 384 * n);
+  return finish(c, st);
+}
+
+int vp8g_mb_luma_sad_at_mv_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref,
+                                     const uint32_t* mb_xy, const uint8_t* src,
+                                     const vp8g_mv* mv, size_t n, uint32_t* sad) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !src || !mv || !sad || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  SadArgs a = make_sad_args(ref, mb_xy, src, mv, n, sad);
+  return hip_ok(launch_mb_luma_sad_at_mv(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+// Host staging shared by the SAD-family entry points: plane, mb_xy, src,
+// per-MB vectors; returns device pointers through the out-params.
+static int stage_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                            const uint8_t* src, const vp8g_mv* mv, size_t n, size_t out_bytes,
+                            vp8g_luma_ref* dref, uint32_t** dxy, uint8_t** dsrc, vp8g_mv** dmv,
+                            uint8_t** dout) {
+  const size_t ps = plane_span(ref->stride, ref->width, ref->height);
+  const size_t o_xy = align_up(ps, 256), o_src = o_xy + align_up(8 * n, 256);
+  const size_t o_mv = o_src + align_up(256 * n, 256), o_out = o_mv + align_up(4 * n, 256);
+  int st = ensure_arena(c, o_out + out_bytes);
+  if (st != VP8G_OK) return st;
+  uint8_t* b = c->arena;
+  st = copy_in(c, b, ref->plane, ps);
+  if (st == VP8G_OK) st = copy_in(c, b + o_xy, mb_xy, 8 * n);
+  if (st == VP8G_OK) st = copy_in(c, b + o_src, src, 256 * n);
+  if (st == VP8G_OK) st = copy_in(c, b + o_mv, mv, 4 * n);
+  *dref = *ref;
+  dref->plane = b;
+  *dxy = reinterpret_cast<uint32_t*>(b + o_xy);
+  *dsrc = b + o_src;
+  *dmv = reinterpret_cast<vp8g_mv*>(b + o_mv);
+  *dout = b + o_out;
+  return st;
+}
+
+int vp8g_mb_luma_sad_at_mv_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                                 const uint8_t* src, const vp8g_mv* mv, size_t n, uint32_t* sad) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !src || !mv || !sad || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (!mb_xy_in_range(mb_xy, n)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  vp8g_luma_ref d;
+  uint32_t* dxy;
+  uint8_t *dsrc, *dout;
+  vp8g_mv* dmv;
+  st = stage_luma_batch(c, ref, mb_xy, src, mv, n, 4 * n, &d, &dxy, &dsrc, &dmv, &dout);
+  if (st != VP8G_OK) return finish(c, st);
+  SadArgs a = make_sad_args(&d, dxy, dsrc, dmv, n, reinterpret_cast<uint32_t*>(dout));
+  st = hip_ok(launch_mb_luma_sad_at_mv(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  if (st == VP8G_OK) st = copy_out(c, sad, dout, 4 * n);
+  return finish(c, st);
+}
+
+static int refine_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                        const uint8_t* src, const vp8g_mv* centers, size_t n,
+                        vp8g_search_result* out, int step) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !src || !centers || !out || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (!mb_xy_in_range(mb_xy, n)) return VP8G_E_ARG;
+  for (size_t i = 0; i < n; i++)
+    if (centers[i].row < -1023 || centers[i].row > 1023 || centers[i].col < -1023 ||
+        centers[i].col > 1023)
+      return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  static_assert(sizeof(vp8g_search_result) == 8, "search result layout");
+  vp8g_luma_ref d;
+  uint32_t* dxy;
+  uint8_t *dsrc, *dout;
+  vp8g_mv* dmv;
+  st = stage_luma_batch(c, ref, mb_xy, src, centers, n, 8 * n, &d, &dxy, &dsrc, &dmv, &dout);
+  if (st != VP8G_OK) return finish(c, st);
+  RefineArgs a;
+  a.plane = d.plane;
+  a.stride = (uint32_t)d.stride;
+  a.w = (int)d.width;
+  a.h = (int)d.height;
+  a.mb_xy = dxy;
+  a.src = dsrc;
+  a.center = reinterpret_cast<const int16_t*>(dmv);
+  a.out = reinterpret_cast<uint32_t*>(dout);
+  a.n = (int)n;
+  a.step = step;
+  st = hip_ok(launch_refine_ladder(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  if (st == VP8G_OK) st = copy_out(c, out, dout, 8 * n);
+  return finish(c, st);
+}
+
+int vp8g_half_pixel_refine_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref,
+                                      const uint32_t* mb_xy, const uint8_t* src,
+                                      const vp8g_mv* centers, size_t n, vp8g_search_result* out) {
+  return refine_batch(c, ref, mb_xy, src, centers, n, out, 2);
+}
+
+int vp8g_quarter_pixel_refine_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref,
+                                         const uint32_t* mb_xy, const uint8_t* src,
+                                         const vp8g_mv* centers, size_t n,
+                                         vp8g_search_result* out) {
+  return refine_batch(c, ref, mb_xy, src, centers, n, out, 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// Device-side helpers shared by the gfx950 kernels: packed-u8 SAD,
+// byte funnel shifts, DPP wave reductions.  CDNA4 (wave64) only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vp8g {
+
+// Unaligned 32-bit window of the byte stream (lo:hi little-endian) starting
+// `shift` bytes into `lo` (v_alignbyte_b32).
+__device__ __forceinline__ uint32_t funnel(uint32_t lo, uint32_t hi, uint32_t shift) {
+  return __builtin_amdgcn_alignbyte(hi, lo, shift);
+}
+
+// v_sad_u8: acc + sum over the 4 byte lanes of |a - b|.
+__device__ __forceinline__ uint32_t sad4(uint32_t a, uint32_t b, uint32_t acc) {
+  return __builtin_amdgcn_sad_u8(a, b, acc);
+}
+
+// DPP controls (gfx9 encoding).
+constexpr int kDppQuadXor1 = 0xB1;       // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int kDppRowHalfMirror = 0x141; // lane i <- lane 7-i within 8
+constexpr int kDppRowMirror = 0x140;     // lane i <- lane 15-i within 16
+
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, 0xF, 0xF, false);
+}
+
+// Sum over each aligned group of 8 lanes; every lane of the group gets it.
+__device__ __forceinline__ uint32_t sum8(uint32_t v) {
+  v += dpp<kDppQuadXor1>(v);
+  v += dpp<kDppQuadXor2>(v);
+  v += dpp<kDppRowHalfMirror>(v);
+  return v;
+}
+
+// Sum over each aligned group of 16 lanes; every lane of the group gets it.
+__device__ __forceinline__ uint32_t sum16(uint32_t v) {
+  v = sum8(v);
+  v += dpp<kDppRowMirror>(v);
+  return v;
+}
+
+// Full wave64 sum, valid in every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  v = sum16(v);
+  v += (uint32_t)__shfl_xor((int)v, 16, 64);
+  v += (uint32_t)__shfl_xor((int)v, 32, 64);
+  return v;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+}  // namespace vp8g

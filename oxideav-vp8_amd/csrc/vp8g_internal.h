@@ -1,0 +1,87 @@
+// Kernel argument blocks and launchers shared between the kernel sources
+// and the C-ABI layer.  Host- and device-visible.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace vp8g {
+
+constexpr uint32_t kWMax = 16;              // centre weight, src/arnr.rs:37
+constexpr uint32_t kSadPerPixelCutoff = 12; // src/arnr.rs:42
+constexpr int kMaxStrength = 6;             // src/arnr.rs:69
+constexpr int kMaxFrames = 64;              // frames per group carried in kernel arguments
+
+// One I420 frame as the kernels see it (device pointers).
+struct FrameDesc {
+  const uint8_t* y;
+  const uint8_t* u;
+  const uint8_t* v;
+  uint32_t y_stride;
+  uint32_t uv_stride;
+};
+
+struct ArnrArgs {
+  FrameDesc frames[kMaxFrames];
+  uint8_t* out_y;   // packed w*h
+  uint8_t* out_u;   // packed cw*ch
+  uint8_t* out_v;
+  int32_t* mv_out;  // optional n*brows*bcols*{mvx,mvy,sad}
+  int w, h, cw, ch, bcols, brows;
+  int n, center;
+  int filter_on;    // 0: strength 0 or a single frame -> exact centre copy
+  int aligned4;     // every luma plane pointer and stride is a multiple of 4
+  uint8_t wt[256];  // blend weight per |ref - ctr| (values <= 16)
+};
+
+size_t arnr_fused_lds_bytes(int nw);
+hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream);
+
+// Motion compensation / SAD scoring.
+struct McPlanes {
+  const uint8_t* y;
+  const uint8_t* u;
+  const uint8_t* v;
+  uint32_t y_stride, uv_stride;
+  int lw, lh;  // luma clamp bounds
+  int cw, ch;  // chroma clamp bounds
+};
+
+struct McArgs {
+  McPlanes ref;
+  const uint32_t* mb_xy;   // n x {mb_col, mb_row}
+  const int16_t* mv;       // n x {row, col} quarter-pel
+  uint8_t* out;            // n x 384 (y256 | u64 | v64)
+  int n;
+  int full_pixel;
+  int32_t filters[8][6];
+};
+
+struct SadArgs {
+  const uint8_t* plane;
+  uint32_t stride;
+  int w, h;                // clamp bounds (LumaRef width/height)
+  const uint32_t* mb_xy;
+  const uint8_t* src;      // n x 256
+  const int16_t* mv;       // n x {row, col}
+  uint32_t* sad;           // n
+  int n;
+};
+
+struct RefineArgs {
+  const uint8_t* plane;
+  uint32_t stride;
+  int w, h;
+  const uint32_t* mb_xy;
+  const uint8_t* src;      // n x 256
+  const int16_t* center;   // n x {row, col}
+  uint32_t* out;           // n x {row | col << 16, sad} (vp8g_search_result layout)
+  int n;
+  int step;                // 2: half-pel ladder, 1: quarter-pel ladder
+};
+
+hipError_t launch_predict_inter_mb(const McArgs& a, hipStream_t stream);
+hipError_t launch_mb_luma_sad_at_mv(const SadArgs& a, hipStream_t stream);
+hipError_t launch_refine_ladder(const RefineArgs& a, hipStream_t stream);
+
+}  // namespace vp8g

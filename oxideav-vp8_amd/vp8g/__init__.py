@@ -1,0 +1,450 @@
+"""Python host-side mirror of oxideav-vp8's ARNR / MC / SAD interface over
+the gfx950 C ABI (include/vp8g.h).
+
+Names, argument meaning and error behaviour follow the reference crate:
+
+* ``build_arnr_altref(frames, center, cfg)`` — src/arnr.rs:476-482; raises
+  ``InvalidDimensions`` / ``ReferenceDimensionsMismatch`` (the two
+  ``EncodeError`` variants of src/encoder.rs:69-104 it can return) and
+  ``IndexError`` where the reference panics (src/arnr.rs:494-498).
+* ``predict_inter_mb(reference, mb_col, mb_row, luma_mv, full_pixel,
+  filters)`` — src/motion_comp.rs:1986-2078, returns ``ReconstructedMb``.
+* ``mb_luma_sad_at_mv(reference, mb_col, mb_row, src_y, mv)`` —
+  src/motion_search.rs:470-529.
+* ``half_pixel_refine_luma`` / ``quarter_pixel_refine_luma`` —
+  src/motion_search.rs:564-728.
+
+Every call runs on the GPU through libvp8g.so; there is no CPU path.  If
+the library is missing or no gfx950 device is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent.parent / "libvp8g.so"
+
+VP8G_OK = 0
+VP8G_E_INVALID_DIMENSIONS = 1
+VP8G_E_REF_DIMS_MISMATCH = 2
+VP8G_E_CENTER_OUT_OF_RANGE = 3
+VP8G_E_ARG = 4
+VP8G_E_DEVICE = 5
+VP8G_E_NO_DEVICE = 6
+MAX_FRAMES = 64
+
+# RFC 6386 section 18.3 tap tables, src/motion_comp.rs:294-303, 328-337.
+SIXTAP_FILTERS = np.array(
+    [[0, 0, 128, 0, 0, 0], [0, -6, 123, 12, -1, 0], [2, -11, 108, 36, -8, 1],
+     [0, -9, 93, 50, -6, 0], [3, -16, 77, 77, -16, 3], [0, -6, 50, 93, -9, 0],
+     [1, -8, 36, 108, -11, 2], [0, -1, 12, 123, -6, 0]], dtype=np.int32)
+BILINEAR_FILTERS = np.array(
+    [[0, 0, 128, 0, 0, 0], [0, 0, 112, 16, 0, 0], [0, 0, 96, 32, 0, 0], [0, 0, 80, 48, 0, 0],
+     [0, 0, 64, 64, 0, 0], [0, 0, 48, 80, 0, 0], [0, 0, 32, 96, 0, 0], [0, 0, 16, 112, 0, 0]],
+    dtype=np.int32)
+
+
+def filter_set_for_version(version: int) -> np.ndarray:
+    """src/motion_comp.rs:370-376: version 0 -> six-tap, else bilinear."""
+    return SIXTAP_FILTERS if version == 0 else BILINEAR_FILTERS
+
+
+class _I420(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("y", C.c_void_p),
+                ("u", C.c_void_p), ("v", C.c_void_p), ("y_stride", C.c_size_t),
+                ("uv_stride", C.c_size_t)]
+
+
+class _Err(C.Structure):
+    _fields_ = [("src_w", C.c_uint32), ("src_h", C.c_uint32), ("ref_w", C.c_uint32),
+                ("ref_h", C.c_uint32)]
+
+
+class _RefPlanes(C.Structure):
+    _fields_ = [("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("y_stride", C.c_size_t), ("uv_stride", C.c_size_t),
+                ("mb_cols", C.c_size_t), ("mb_rows", C.c_size_t)]
+
+
+class _LumaRef(C.Structure):
+    _fields_ = [("plane", C.c_void_p), ("stride", C.c_size_t), ("width", C.c_size_t),
+                ("height", C.c_size_t)]
+
+
+EXPORTS = (
+    "vp8g_ctx_create", "vp8g_ctx_destroy", "vp8g_ctx_set_stream", "vp8g_ctx_stream",
+    "vp8g_ctx_synchronize", "vp8g_ctx_set_arnr_waves", "vp8g_status_str",
+    "vp8g_build_arnr_altref", "vp8g_build_arnr_altref_dev",
+    "vp8g_predict_inter_mb_batch", "vp8g_predict_inter_mb_batch_dev",
+    "vp8g_mb_luma_sad_at_mv_batch", "vp8g_mb_luma_sad_at_mv_batch_dev",
+    "vp8g_half_pixel_refine_luma_batch", "vp8g_quarter_pixel_refine_luma_batch",
+)
+
+
+def load_library(path: Path | None = None) -> C.CDLL:
+    p = Path(path) if path else _LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"libvp8g.so not built at {p}; run oxideav-vp8_amd/build.py")
+    lib = C.CDLL(str(p))
+    vp, sz, i32, u8 = C.c_void_p, C.c_size_t, C.c_int, C.c_uint8
+    sig = {
+        "vp8g_ctx_create": (i32, [i32, C.POINTER(vp)]),
+        "vp8g_ctx_destroy": (None, [vp]),
+        "vp8g_ctx_set_stream": (i32, [vp, vp]),
+        "vp8g_ctx_stream": (vp, [vp]),
+        "vp8g_ctx_synchronize": (i32, [vp]),
+        "vp8g_ctx_set_arnr_waves": (i32, [vp, i32]),
+        "vp8g_status_str": (C.c_char_p, [i32]),
+        "vp8g_build_arnr_altref": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, C.POINTER(_Err)]),
+        "vp8g_build_arnr_altref_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp,
+                                             C.POINTER(_Err), vp]),
+        "vp8g_predict_inter_mb_batch": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp]),
+        "vp8g_predict_inter_mb_batch_dev": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp]),
+        "vp8g_mb_luma_sad_at_mv_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
+        "vp8g_mb_luma_sad_at_mv_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
+        "vp8g_half_pixel_refine_luma_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
+        "vp8g_quarter_pixel_refine_luma_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+# ---------------------------------------------------------------- errors
+class EncodeError(Exception):
+    """src/encoder.rs:69-135 (the variants build_arnr_altref can return)."""
+
+
+class InvalidDimensions(EncodeError):
+    def __init__(self, width: int, height: int):
+        super().__init__(f"InvalidDimensions {{ width: {width}, height: {height} }}")
+        self.width, self.height = width, height
+
+
+class ReferenceDimensionsMismatch(EncodeError):
+    def __init__(self, source: tuple, reference: tuple):
+        super().__init__(f"ReferenceDimensionsMismatch {{ source: {source}, reference: {reference} }}")
+        self.source, self.reference = source, reference
+
+
+class Vp8gError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: status {status}")
+        self.status = status
+
+
+# ---------------------------------------------------------------- types
+@dataclass
+class I420Frame:
+    """src/encoder.rs:5419-5434.  Planes are 1-D uint8 arrays."""
+    width: int
+    height: int
+    y: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    y_stride: int
+    uv_stride: int
+
+    @staticmethod
+    def packed(width: int, height: int, y, u, v) -> "I420Frame":
+        """I420Frame::packed, src/encoder.rs:5510-5520."""
+        return I420Frame(width, height, np.ascontiguousarray(y, np.uint8).reshape(-1),
+                         np.ascontiguousarray(u, np.uint8).reshape(-1),
+                         np.ascontiguousarray(v, np.uint8).reshape(-1),
+                         width, (width + 1) // 2)
+
+
+@dataclass(frozen=True)
+class ArnrConfig:
+    """src/arnr.rs:52-83."""
+    strength: int = 3
+    MAX_STRENGTH = 6
+
+    @staticmethod
+    def new(strength: int) -> "ArnrConfig":
+        return ArnrConfig(min(int(strength), ArnrConfig.MAX_STRENGTH))
+
+    def with_strength(self, v: int) -> "ArnrConfig":
+        return ArnrConfig(int(v))
+
+
+@dataclass
+class ArnrPicture:
+    """src/arnr.rs:90-145: tightly packed output planes."""
+    width: int
+    height: int
+    y: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+
+    def as_i420(self) -> I420Frame:
+        return I420Frame.packed(self.width, self.height, self.y, self.u, self.v)
+
+
+@dataclass(frozen=True)
+class Mv:
+    """src/motion_vector.rs:64-70 (quarter-pel)."""
+    row: int = 0
+    col: int = 0
+
+
+@dataclass
+class ReferencePlanes:
+    """src/motion_comp.rs:131-146 (MB-aligned planes)."""
+    y: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    y_stride: int
+    uv_stride: int
+    mb_cols: int
+    mb_rows: int
+
+
+@dataclass
+class LumaRef:
+    """src/motion_search.rs:230-239."""
+    plane: np.ndarray
+    stride: int
+    width: int
+    height: int
+
+
+@dataclass
+class ReconstructedMb:
+    """src/reconstruct.rs:176-195."""
+    y: np.ndarray = field(default_factory=lambda: np.zeros(256, np.uint8))
+    u: np.ndarray = field(default_factory=lambda: np.zeros(64, np.uint8))
+    v: np.ndarray = field(default_factory=lambda: np.zeros(64, np.uint8))
+
+
+@dataclass(frozen=True)
+class SearchResult:
+    """src/motion_search.rs:248-255."""
+    mv: Mv
+    sad: int
+
+
+# ---------------------------------------------------------------- context
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _check_plane(name: str, a: np.ndarray, need: int) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint8).reshape(-1)
+    if a.size < need:
+        raise ValueError(f"{name} plane holds {a.size} bytes, needs {need}")
+    return a
+
+
+class Context:
+    """One device context (a HIP stream plus a staging arena).  Not locked:
+    use one per thread."""
+
+    def __init__(self, device: int = 0, lib_path: Path | None = None):
+        self.lib = load_library(lib_path)
+        h = C.c_void_p()
+        st = self.lib.vp8g_ctx_create(device, C.byref(h))
+        if st != VP8G_OK:
+            raise Vp8gError(st, f"vp8g_ctx_create(device={device}) failed "
+                                f"({self.lib.vp8g_status_str(st).decode()})")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.vp8g_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _raise(self, st: int, what: str):
+        if st != VP8G_OK:
+            raise Vp8gError(st, f"{what} ({self.lib.vp8g_status_str(st).decode()})")
+
+    def set_stream(self, stream: int | None) -> None:
+        self._raise(self.lib.vp8g_ctx_set_stream(self.h, stream), "set_stream")
+
+    def stream(self) -> int:
+        return self.lib.vp8g_ctx_stream(self.h) or 0
+
+    def synchronize(self) -> None:
+        self._raise(self.lib.vp8g_ctx_synchronize(self.h), "synchronize")
+
+    def set_arnr_waves(self, waves: int) -> None:
+        self._raise(self.lib.vp8g_ctx_set_arnr_waves(self.h, waves), "set_arnr_waves")
+
+    # ---- ARNR
+    def build_arnr_altref(self, frames, center: int, cfg: ArnrConfig | None = None) -> ArnrPicture:
+        cfg = cfg or ArnrConfig()
+        frames = list(frames)
+        n = len(frames)
+        if center >= n:
+            raise IndexError(f"build_arnr_altref: center index {center} out of range ({n} frames)")
+        keep, descs = [], (_I420 * max(n, 1))()
+        ctr = frames[center]
+        cw0 = (ctr.width + 1) // 2
+        for i, f in enumerate(frames):
+            y = np.ascontiguousarray(f.y, np.uint8).reshape(-1)
+            u = np.ascontiguousarray(f.u, np.uint8).reshape(-1)
+            v = np.ascontiguousarray(f.v, np.uint8).reshape(-1)
+            keep += [y, u, v]
+            # Only well-formed planes are dereferenced; the dimension checks
+            # below run first, as in the reference.
+            if f.width == ctr.width and f.height == ctr.height and 0 < f.width <= 0x3FFF \
+                    and 0 < f.height <= 0x3FFF:
+                need_y = (f.height - 1) * f.y_stride + f.width
+                need_c = ((f.height + 1) // 2 - 1) * f.uv_stride + cw0
+                if y.size < need_y or u.size < need_c or v.size < need_c:
+                    raise ValueError(f"frame {i}: plane buffers shorter than stride x height")
+            descs[i] = _I420(f.width, f.height, _ptr(y) if y.size else None,
+                             _ptr(u) if u.size else None, _ptr(v) if v.size else None,
+                             f.y_stride, f.uv_stride)
+        w, h = ctr.width, ctr.height
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        oy = np.empty(max(w * h, 1), np.uint8)
+        ou = np.empty(max(cw * ch, 1), np.uint8)
+        ov = np.empty(max(cw * ch, 1), np.uint8)
+        err = _Err()
+        st = self.lib.vp8g_build_arnr_altref(self.h, descs, n, center, int(cfg.strength) & 0xFF,
+                                             _ptr(oy), _ptr(ou), _ptr(ov), C.byref(err))
+        if st == VP8G_E_INVALID_DIMENSIONS:
+            raise InvalidDimensions(err.src_w, err.src_h)
+        if st == VP8G_E_REF_DIMS_MISMATCH:
+            raise ReferenceDimensionsMismatch((err.src_w, err.src_h), (err.ref_w, err.ref_h))
+        if st == VP8G_E_CENTER_OUT_OF_RANGE:
+            raise IndexError(f"build_arnr_altref: center index {center} out of range ({n} frames)")
+        self._raise(st, "vp8g_build_arnr_altref")
+        return ArnrPicture(w, h, oy[:w * h], ou[:cw * ch], ov[:cw * ch])
+
+    def build_arnr_altref_dev(self, descs, n: int, center: int, strength: int,
+                              oy: int, ou: int, ov: int, mv_out: int | None = None) -> None:
+        """Device-resident form: ``descs`` is a ctypes ``_I420`` array holding
+        device pointers; outputs are device pointers.  Enqueues only."""
+        err = _Err()
+        st = self.lib.vp8g_build_arnr_altref_dev(self.h, descs, n, center, strength, oy, ou, ov,
+                                                 C.byref(err), mv_out)
+        self._raise(st, "vp8g_build_arnr_altref_dev")
+
+    # ---- motion compensation
+    def predict_inter_mb_batch(self, reference: ReferencePlanes, mb_xy, mvs, full_pixel: bool,
+                               filters) -> np.ndarray:
+        mb_xy = np.ascontiguousarray(mb_xy, np.uint32).reshape(-1, 2)
+        mvs = np.ascontiguousarray(mvs, np.int16).reshape(-1, 2)
+        n = mb_xy.shape[0]
+        if mvs.shape[0] != n:
+            raise ValueError("one MV per macroblock")
+        filt = np.ascontiguousarray(filters, np.int32).reshape(8, 6)
+        ys = reference.y_stride
+        y = _check_plane("y", reference.y, (reference.mb_rows * 16 - 1) * ys + reference.mb_cols * 16)
+        cneed = (reference.mb_rows * 8 - 1) * reference.uv_stride + reference.mb_cols * 8
+        u = _check_plane("u", reference.u, cneed)
+        v = _check_plane("v", reference.v, cneed)
+        rp = _RefPlanes(_ptr(y), _ptr(u), _ptr(v), ys, reference.uv_stride, reference.mb_cols,
+                        reference.mb_rows)
+        out = np.empty((max(n, 1), 384), np.uint8)
+        st = self.lib.vp8g_predict_inter_mb_batch(self.h, C.byref(rp), _ptr(mb_xy), _ptr(mvs), n,
+                                                  1 if full_pixel else 0, _ptr(filt), _ptr(out))
+        self._raise(st, "vp8g_predict_inter_mb_batch")
+        return out[:n]
+
+    def predict_inter_mb(self, reference: ReferencePlanes, mb_col: int, mb_row: int, luma_mv: Mv,
+                         full_pixel: bool, filters) -> ReconstructedMb:
+        o = self.predict_inter_mb_batch(reference, [[mb_col, mb_row]], [[luma_mv.row, luma_mv.col]],
+                                        full_pixel, filters)[0]
+        return ReconstructedMb(o[:256].copy(), o[256:320].copy(), o[320:].copy())
+
+    # ---- SAD scoring
+    def _luma_ref(self, reference: LumaRef):
+        p = _check_plane("luma", reference.plane,
+                         (reference.height - 1) * reference.stride + reference.width)
+        return p, _LumaRef(_ptr(p), reference.stride, reference.width, reference.height)
+
+    def mb_luma_sad_at_mv_batch(self, reference: LumaRef, mb_xy, src, mvs) -> np.ndarray:
+        mb_xy = np.ascontiguousarray(mb_xy, np.uint32).reshape(-1, 2)
+        n = mb_xy.shape[0]
+        src = np.ascontiguousarray(src, np.uint8).reshape(n, 256)
+        mvs = np.ascontiguousarray(mvs, np.int16).reshape(n, 2)
+        p, lr = self._luma_ref(reference)
+        out = np.empty(max(n, 1), np.uint32)
+        st = self.lib.vp8g_mb_luma_sad_at_mv_batch(self.h, C.byref(lr), _ptr(mb_xy), _ptr(src),
+                                                   _ptr(mvs), n, _ptr(out))
+        self._raise(st, "vp8g_mb_luma_sad_at_mv_batch")
+        return out[:n]
+
+    def mb_luma_sad_at_mv(self, reference: LumaRef, mb_col: int, mb_row: int, src_y, mv: Mv) -> int:
+        return int(self.mb_luma_sad_at_mv_batch(reference, [[mb_col, mb_row]], src_y,
+                                                [[mv.row, mv.col]])[0])
+
+    def _refine(self, fn, reference: LumaRef, mb_xy, src, centers) -> np.ndarray:
+        mb_xy = np.ascontiguousarray(mb_xy, np.uint32).reshape(-1, 2)
+        n = mb_xy.shape[0]
+        src = np.ascontiguousarray(src, np.uint8).reshape(n, 256)
+        centers = np.ascontiguousarray(centers, np.int16).reshape(n, 2)
+        p, lr = self._luma_ref(reference)
+        out = np.empty((max(n, 1), 4), np.uint16)  # row, col, sad lo, sad hi
+        st = fn(self.h, C.byref(lr), _ptr(mb_xy), _ptr(src), _ptr(centers), n, _ptr(out))
+        self._raise(st, "refine ladder")
+        res = np.empty((n, 3), np.int64)
+        res[:, 0] = out[:n, 0].view(np.int16)
+        res[:, 1] = out[:n, 1].view(np.int16)
+        res[:, 2] = out[:n, 2].astype(np.int64) | (out[:n, 3].astype(np.int64) << 16)
+        return res
+
+    def half_pixel_refine_luma_batch(self, reference, mb_xy, src, centers) -> np.ndarray:
+        """Rows of (mv.row, mv.col, sad)."""
+        return self._refine(self.lib.vp8g_half_pixel_refine_luma_batch, reference, mb_xy, src, centers)
+
+    def quarter_pixel_refine_luma_batch(self, reference, mb_xy, src, centers) -> np.ndarray:
+        return self._refine(self.lib.vp8g_quarter_pixel_refine_luma_batch, reference, mb_xy, src,
+                            centers)
+
+    def half_pixel_refine_luma(self, reference, mb_col, mb_row, src_y, center: Mv) -> SearchResult:
+        r = self.half_pixel_refine_luma_batch(reference, [[mb_col, mb_row]], src_y,
+                                              [[center.row, center.col]])[0]
+        return SearchResult(Mv(int(r[0]), int(r[1])), int(r[2]))
+
+    def quarter_pixel_refine_luma(self, reference, mb_col, mb_row, src_y, center: Mv) -> SearchResult:
+        r = self.quarter_pixel_refine_luma_batch(reference, [[mb_col, mb_row]], src_y,
+                                                 [[center.row, center.col]])[0]
+        return SearchResult(Mv(int(r[0]), int(r[1])), int(r[2]))
+
+
+# ------------------------------------------------ module-level reference API
+_default = threading.local()
+
+
+def default_context() -> Context:
+    ctx = getattr(_default, "ctx", None)
+    if ctx is None:
+        ctx = Context(0)
+        _default.ctx = ctx
+    return ctx
+
+
+def build_arnr_altref(frames, center: int, cfg: ArnrConfig | None = None) -> ArnrPicture:
+    return default_context().build_arnr_altref(frames, center, cfg)
+
+
+def predict_inter_mb(reference, mb_col, mb_row, luma_mv, full_pixel, filters) -> ReconstructedMb:
+    return default_context().predict_inter_mb(reference, mb_col, mb_row, luma_mv, full_pixel, filters)
+
+
+def mb_luma_sad_at_mv(reference, mb_col, mb_row, src_y, mv) -> int:
+    return default_context().mb_luma_sad_at_mv(reference, mb_col, mb_row, src_y, mv)
+
+
+def half_pixel_refine_luma(reference, mb_col, mb_row, src_y, center) -> SearchResult:
+    return default_context().half_pixel_refine_luma(reference, mb_col, mb_row, src_y, center)
+
+
+def quarter_pixel_refine_luma(reference, mb_col, mb_row, src_y, center) -> SearchResult:
+    return default_context().quarter_pixel_refine_luma(reference, mb_col, mb_row, src_y, center)
