@@ -1,0 +1,284 @@
+"""Throughput of the ARNR + MC + sub-pel SAD path on one MI355X.
+
+One step = one pass of the hot path over one 1920x1080 I420 lookahead group
+of 7 frames (8x8 region motion, velocities in [-12, 12] px/frame, 20
+occluding rectangles per frame, sigma-4 noise):
+
+  1. build_arnr_altref of the whole group (centre 3, strength 3),
+  2. predict_inter_mb for all 8160 MBs under random quarter-pel MVs in
+     [-64, 64] (six-tap) against frame 4, and
+  3. mb_luma_sad_at_mv for all 8160 MBs at the same MVs.
+
+Inputs are resident in HBM before timing starts.  `value` = anchor MBs
+(16x16 luma + 8x8 U + 8x8 V of the altref) per second over the whole step.
+The `roofline` object describes the dominant kernel (the fused ARNR kernel)
+timed with HIP events on the stream it runs on; `cpu_baseline` times the CPU
+oracle on the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "oxideav-vp8_amd"))
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_OPS = 256 * 64 * 2.4e9   # 256 CUs x 64 lanes x 2.4 GHz, 32-bit integer VALU
+METRIC = "ARNR altref MB/s @1080p (exact vs learned-seeded) + % HBM/VALU roofline"
+
+
+def arnr_algorithmic(w: int, h: int, n: int):
+    """Algorithmic HBM bytes and integer VALU ops of one group:
+    bytes: every frame's planes read once + the altref written once;
+    ops: (n-1)*33 probes of bw*bh abs-diffs counted 4 per v_sad_u8, blend
+    4 ops/pixel/neighbour, resolve 4 ops/pixel."""
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+    frame_bytes = w * h + 2 * cw * ch
+    ops = 0
+    for by in range(0, h, 16):
+        bh = min(16, h - by)
+        for bx in range(0, w, 16):
+            bw = min(16, w - bx)
+            px = bw * bh + 2 * ((bw + 1) // 2) * ((bh + 1) // 2)
+            ops += (n - 1) * 33 * bw * bh // 4 + (n - 1) * px * 4 + px * 4
+    return (n + 1) * frame_bytes, ops
+
+
+def cpu_baseline(group, mc, seconds: float, cores: int):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle_ctypes import LumaRef, Oracle, RefPlanes
+
+    orc = Oracle()
+    frames = group.frames()
+    n_mb = len(mc.mb_xy)
+    rp = RefPlanes(mc.ref_y.ctypes.data, mc.ref_u.ctypes.data, mc.ref_v.ctypes.data,
+                   mc.mb_cols * 16, mc.mb_cols * 8, mc.mb_cols, mc.mb_rows)
+    lr = LumaRef(mc.ref_y.ctypes.data, mc.mb_cols * 16, mc.mb_cols * 16, mc.mb_rows * 16)
+    pred = np.zeros((n_mb, 384), np.uint8)
+    sad = np.zeros(n_mb, np.uint32)
+    filt = np.ascontiguousarray(orc.SIXTAP)
+    pool = ThreadPoolExecutor(cores)
+    chunks = np.array_split(np.arange(n_mb), cores)
+
+    def mc_chunk(idx):
+        if len(idx) == 0:
+            return
+        s = int(idx[0])
+        orc.lib.orc_predict_inter_mb_batch(rp, mc.mb_xy[s:].ctypes.data, mc.mvs[s:].ctypes.data,
+                                           len(idx), 0, filt.ctypes.data, pred[s:].ctypes.data)
+        orc.lib.orc_mb_luma_sad_at_mv_batch(lr, mc.mb_xy[s:].ctypes.data, mc.src[s:].ctypes.data,
+                                            mc.mvs[s:].ctypes.data, len(idx), sad[s:].ctypes.data)
+
+    def one(threads):
+        st, _, _, _ = orc.build_arnr_altref(frames, group.center, 3, threads=threads)
+        assert st == 0
+        if threads == 1:
+            mc_chunk(np.arange(n_mb))
+        else:
+            list(pool.map(mc_chunk, chunks))
+
+    res = {}
+    for threads in (cores, 1):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            one(threads)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds / 2:
+                break
+        res[threads] = (reps, el)
+    pool.shutdown()
+    anchors = ((group.width + 15) // 16) * ((group.height + 15) // 16)
+    reps, el = res[cores]
+    r1, e1 = res[1]
+    return {
+        "value": anchors * reps / el,
+        "unit": "anchor-MB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"{reps} full steps (1080p 7-frame ARNR build + 8160-MB six-tap MC + SAD) "
+                   f"in {el:.1f} s on {cores} threads; 1 thread: {anchors * r1 / e1:.0f} anchor-MB/s "
+                   f"({r1} steps in {e1:.1f} s)"),
+        "value_1thread": anchors * r1 / e1,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--arnr-waves", type=int, default=0)
+    ap.add_argument("--no-xcd-remap", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards per clip/group with no exchange; see the design notes)",
+              file=sys.stderr, flush=True)
+        return 2
+
+    import torch
+
+    import vp8g
+    import vp8g.synth as synth
+
+    t0 = time.perf_counter()
+    group = synth.arnr_group_regions()
+    mc = synth.mc_workload(group)
+    print(f"[bench] workload generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    # torch is loaded first, so libvp8g.so binds to the same HIP runtime and
+    # the library's own stream can be wrapped for event timing.
+    ctx = vp8g.Context(0)
+    stream = torch.cuda.ExternalStream(ctx.stream(), device=dev)
+    if args.arnr_waves:
+        ctx.set_arnr_waves(args.arnr_waves)
+    if args.no_xcd_remap:
+        ctx.set_xcd_remap(False)
+
+    w, h, n = group.width, group.height, group.n
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+    planes = [(torch.from_numpy(y.reshape(-1).copy()).to(dev), torch.from_numpy(u.reshape(-1).copy()).to(dev),
+               torch.from_numpy(v.reshape(-1).copy()).to(dev)) for y, u, v in zip(group.y, group.u, group.v)]
+    descs = (vp8g._I420 * n)()
+    for i, (y, u, v) in enumerate(planes):
+        descs[i] = vp8g._I420(w, h, y.data_ptr(), u.data_ptr(), v.data_ptr(), w, cw)
+    oy = torch.empty(w * h, dtype=torch.uint8, device=dev)
+    ou = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
+    ov = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
+
+    n_mb = len(mc.mb_xy)
+    ry = torch.from_numpy(mc.ref_y.reshape(-1).copy()).to(dev)
+    ru = torch.from_numpy(mc.ref_u.reshape(-1).copy()).to(dev)
+    rv = torch.from_numpy(mc.ref_v.reshape(-1).copy()).to(dev)
+    xy = torch.from_numpy(mc.mb_xy.reshape(-1).astype(np.int32)).to(dev)
+    mvs = torch.from_numpy(mc.mvs.reshape(-1).copy()).to(dev)
+    src = torch.from_numpy(mc.src.reshape(-1).copy()).to(dev)
+    pred = torch.empty(n_mb * 384, dtype=torch.uint8, device=dev)
+    sad = torch.empty(n_mb, dtype=torch.int32, device=dev)
+    rp = vp8g._RefPlanes(ry.data_ptr(), ru.data_ptr(), rv.data_ptr(), mc.mb_cols * 16, mc.mb_cols * 8,
+                         mc.mb_cols, mc.mb_rows)
+    lr = vp8g._LumaRef(ry.data_ptr(), mc.mb_cols * 16, mc.mb_cols * 16, mc.mb_rows * 16)
+    filt = np.ascontiguousarray(vp8g.SIXTAP_FILTERS, np.int32)
+    lib = ctx.lib
+    torch.cuda.synchronize()
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+
+    def step(marks=None):
+        if marks:
+            marks[0].record(stream)
+        st = lib.vp8g_build_arnr_altref_dev(ctx.h, descs, n, group.center, 3, oy.data_ptr(),
+                                            ou.data_ptr(), ov.data_ptr(), None, None)
+        if marks:
+            marks[1].record(stream)
+        st |= lib.vp8g_predict_inter_mb_batch_dev(ctx.h, rp, xy.data_ptr(), mvs.data_ptr(), n_mb, 0,
+                                                  filt.ctypes.data, pred.data_ptr())
+        if marks:
+            marks[2].record(stream)
+        st |= lib.vp8g_mb_luma_sad_at_mv_batch_dev(ctx.h, lr, xy.data_ptr(), src.data_ptr(),
+                                                   mvs.data_ptr(), n_mb, sad.data_ptr())
+        if marks:
+            marks[3].record(stream)
+        if st:
+            raise RuntimeError(f"vp8g launch failed: {st}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ref_out = oy.clone()
+    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    start.record(stream)
+    for k in range(args.steps):
+        step(ev[k])
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms_total = start.elapsed_time(end)
+    ms_step = ms_total / args.steps
+    arnr_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    mc_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    sad_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in ev]))
+    if not torch.equal(ref_out, oy):
+        raise RuntimeError("altref output changed between steps")
+    print(f"[bench] timed {args.steps} steps: {ms_step:.4f} ms/step (wall {wall:.3f}s)",
+          file=sys.stderr, flush=True)
+
+    anchors = ((w + 15) // 16) * ((h + 15) // 16)
+    bytes_alg, ops_alg = arnr_algorithmic(w, h, n)
+    achieved = bytes_alg / (arnr_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = ROOT / "profiles" / "arnr_fused_1080p_traffic.json"
+    if tpath.exists():
+        traffic = json.loads(tpath.read_text()).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": METRIC,
+        "value": anchors / (ms_step * 1e-3),
+        "unit": "anchor-MB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "1920x1080 I420, 7-frame group (centre 3, strength 3), 8x8 region motion "
+                        "[-12,12] px/frame, 20 occluders/frame, sigma 4 noise; + predict_inter_mb "
+                        "(six-tap) and mb_luma_sad_at_mv for all 8160 MBs at quarter-pel MVs in [-64,64]",
+            "anchor_mbs_per_step": anchors,
+            "group_frames": n,
+            "parallelism": "single GPU",
+        },
+        "kernels_ms": {"arnr_fused": arnr_ms, "predict_inter_mb": mc_ms, "mb_luma_sad_at_mv": sad_ms},
+        "arnr_only_anchor_mb_per_s": anchors / (arnr_ms * 1e-3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "arnr_fused_kernel",
+            "algorithmic_bytes_per_launch": bytes_alg,
+        },
+        "valu": {
+            "ops_per_launch": ops_alg,
+            "achieved_tops": ops_alg / (arnr_ms * 1e-3) / 1e12,
+            "peak_tops": VALU_PEAK_OPS / 1e12,
+            "frac": ops_alg / (arnr_ms * 1e-3) / VALU_PEAK_OPS,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        cores = max(1, min(os.cpu_count() or 1, 16))
+        print(f"[bench] timing CPU oracle on {cores} threads", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(group, mc, args.cpu_seconds, cores)
+        result["gpu_over_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

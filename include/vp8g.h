@@ -11,6 +11,7 @@
  * copies in, runs, copies out, and returns after the result is in place).
  * `_dev` entry points take DEVICE pointers, enqueue on the context's stream
  * and return without waiting; call vp8g_ctx_synchronize() before reading.
+ * Their per-MB arrays (mb_xy, mv, src, out, sad) must be 4-byte aligned.
  *
  * Threading: a context is not internally locked; use one per host thread.
  * Results never depend on launch shapes, batch composition or thread count.
@@ -83,8 +84,14 @@ void vp8g_ctx_destroy(vp8g_ctx *ctx);
 int vp8g_ctx_set_stream(vp8g_ctx *ctx, void *stream);
 void *vp8g_ctx_stream(vp8g_ctx *ctx);
 int vp8g_ctx_synchronize(vp8g_ctx *ctx);
-/* Waves per ARNR workgroup (0 = one per neighbour frame, capped at 16). */
-int vp8g_ctx_set_arnr_waves(vp8g_ctx *ctx, int waves);
+/* Launch-shape options; none of them changes any output byte.
+ * VP8G_OPT_ARNR_WAVES: waves per ARNR workgroup (0 = one per neighbour
+ *   frame, at most 16).
+ * VP8G_OPT_XCD_REMAP: 1 (default) gives each XCD a contiguous band of
+ *   macroblocks, 0 uses the launch order. */
+#define VP8G_OPT_ARNR_WAVES 1
+#define VP8G_OPT_XCD_REMAP 2
+int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);
 
 /* ---- ARNR altref synthesis ----

@@ -15,6 +15,7 @@ struct vp8g_ctx {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   int arnr_waves = 0;
+  int xcd_remap = 1;
   // Device arena for the host-pointer entry points (grown, never shrunk).
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
@@ -78,11 +79,13 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   ArnrArgs a;
   memset(&a, 0, sizeof(a));
   const vp8g_i420& ctr = frames[center];
-  bool aligned4 = true;
+  bool aligned4 = true, aligned16 = true, aligned4c = true;
   for (size_t i = 0; i < n; i++) {
-    a.frames[i] = FrameDesc{frames[i].y, frames[i].u, frames[i].v, (uint32_t)frames[i].y_stride,
-                            (uint32_t)frames[i].uv_stride};
-    if (((uintptr_t)frames[i].y & 3) || (frames[i].y_stride & 3)) aligned4 = false;
+    const vp8g_i420& f = frames[i];
+    a.frames[i] = FrameDesc{f.y, f.u, f.v, (uint32_t)f.y_stride, (uint32_t)f.uv_stride};
+    if (((uintptr_t)f.y & 3) || (f.y_stride & 3)) aligned4 = false;
+    if (((uintptr_t)f.y & 15) || (f.y_stride & 15)) aligned16 = false;
+    if (((uintptr_t)f.u & 3) || ((uintptr_t)f.v & 3) || (f.uv_stride & 3)) aligned4c = false;
   }
   a.out_y = oy;
   a.out_u = ou;
@@ -98,9 +101,14 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   a.center = (int)center;
   a.filter_on = (strength != 0 && n >= 2) ? 1 : 0;
   a.aligned4 = aligned4 ? 1 : 0;
+  a.aligned16 = aligned16 ? 1 : 0;
+  a.aligned4c = aligned4c ? 1 : 0;
+  a.xcd_remap = c->xcd_remap;
   // weight_table(8 << min(strength, 6)), src/arnr.rs:307-323, 533-536.
   const uint32_t s = 8u << (strength < kMaxStrength ? strength : kMaxStrength);
-  for (uint32_t d = 0; d < 256; d++) a.wt[d] = (uint8_t)((kWMax * s) / (s + d * d));
+  uint8_t wt[256];
+  for (uint32_t d = 0; d < 256; d++) wt[d] = (uint8_t)((kWMax * s) / (s + d * d));
+  memcpy(a.wt4, wt, sizeof(wt));
   const int nw = pick_waves(c, a.filter_on ? (int)n - 1 : 1);
   return hip_ok(launch_arnr_fused(a, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
@@ -242,10 +250,20 @@ int vp8g_ctx_synchronize(vp8g_ctx* c) {
   return hip_ok(hipStreamSynchronize(c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
-int vp8g_ctx_set_arnr_waves(vp8g_ctx* c, int waves) {
-  if (!c || waves < 0 || waves > 16) return VP8G_E_ARG;
-  c->arnr_waves = waves;
-  return VP8G_OK;
+int vp8g_ctx_set_option(vp8g_ctx* c, int option, int value) {
+  if (!c) return VP8G_E_ARG;
+  switch (option) {
+    case VP8G_OPT_ARNR_WAVES:
+      if (value < 0 || value > 16) return VP8G_E_ARG;
+      c->arnr_waves = value;
+      return VP8G_OK;
+    case VP8G_OPT_XCD_REMAP:
+      if (value < 0 || value > 1) return VP8G_E_ARG;
+      c->xcd_remap = value;
+      return VP8G_OK;
+    default:
+      return VP8G_E_ARG;
+  }
 }
 
 int vp8g_build_arnr_altref_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
@@ -307,6 +325,7 @@ int vp8g_predict_inter_mb_batch_dev(vp8g_ctx* c, const vp8g_ref_planes* ref,
   if (st != VP8G_OK) return st;
   if (n == 0) return VP8G_OK;
   if (!mb_xy || !mv || !out || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (((uintptr_t)mb_xy | (uintptr_t)mv | (uintptr_t)out) & 3) return VP8G_E_ARG;
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
   McArgs a = make_mc_args(ref, mb_xy, mv, n, full_pixel, filters, out);
   return hip_ok(launch_predict_inter_mb(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
@@ -356,6 +375,7 @@ int vp8g_mb_luma_sad_at_mv_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref,
   if (st != VP8G_OK) return st;
   if (n == 0) return VP8G_OK;
   if (!mb_xy || !src || !mv || !sad || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (((uintptr_t)mb_xy | (uintptr_t)src | (uintptr_t)mv | (uintptr_t)sad) & 3) return VP8G_E_ARG;
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
   SadArgs a = make_sad_args(ref, mb_xy, src, mv, n, sad);
   return hip_ok(launch_mb_luma_sad_at_mv(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;

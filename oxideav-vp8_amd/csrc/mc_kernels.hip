@@ -52,18 +52,39 @@ __device__ __forceinline__ uint32_t interp6(const Taps& k, int s0, int s1, int s
   return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
 }
 
-// Clamped gather of a rows x cols block at (x0, y0) into LDS (row pitch
-// `pitch`), spread over the wave.
-__device__ __forceinline__ void gather_lds(uint8_t* dst, int pitch, const uint8_t* plane,
-                                           uint32_t stride, int w, int h, int x0, int y0, int rows,
-                                           int cols, int lane) {
-  const int total = rows * cols;
-  for (int s = lane; s < total; s += 64) {
-    const int r = s / cols, c = s - r * cols;
-    const int y = clampi(y0 + r, 0, h - 1), x = clampi(x0 + c, 0, w - 1);
-    dst[r * pitch + c] = plane[(size_t)y * stride + x];
+// Clamped gather of a ROWS x COLS block at (x0, y0), spread over the wave:
+// load() issues every byte load into registers, store() writes them to LDS
+// (row pitch `pitch`), so all loads of a macroblock are in flight at once.
+template <int ROWS, int COLS>
+struct Gather {
+  static constexpr int kTotal = ROWS * COLS;
+  static constexpr int kIters = (kTotal + 63) / 64;
+  uint8_t v[kIters];
+
+  __device__ __forceinline__ void load(const uint8_t* plane, uint32_t stride, int w, int h, int x0,
+                                       int y0, int lane) {
+#pragma unroll
+    for (int it = 0; it < kIters; it++) {
+      const int s = lane + 64 * it;
+      if (s < kTotal) {
+        const int r = s / COLS, c = s - r * COLS;
+        const int y = clampi(y0 + r, 0, h - 1), x = clampi(x0 + c, 0, w - 1);
+        v[it] = plane[(size_t)y * stride + x];
+      }
+    }
   }
-}
+
+  __device__ __forceinline__ void store(uint8_t* dst, int pitch, int lane) const {
+#pragma unroll
+    for (int it = 0; it < kIters; it++) {
+      const int s = lane + 64 * it;
+      if (s < kTotal) {
+        const int r = s / COLS, c = s - r * COLS;
+        dst[r * pitch + c] = v[it];
+      }
+    }
+  }
+};
 
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -149,17 +170,25 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
     yr = (int16_t)(yr & ~7); yc = (int16_t)(yc & ~7);
     ur = (int16_t)(ur & ~7); uc = (int16_t)(uc & ~7);
   }
-  const Taps fh = load_taps(a.filters[yc & 7]);
-  const Taps fv = load_taps(a.filters[yr & 7]);
-  const Taps cfh = load_taps(a.filters[uc & 7]);
-  const Taps cfv = load_taps(a.filters[ur & 7]);
+  // A whole-pixel vector is a plain copy whatever row 0 of the caller's
+  // table holds (src/motion_comp.rs:2022-2028, 2044-2067).
+  Taps fh = load_taps(a.filters[yc & 7]);
+  Taps fv = load_taps(a.filters[yr & 7]);
+  Taps cfh = load_taps(a.filters[uc & 7]);
+  Taps cfv = load_taps(a.filters[ur & 7]);
+  if ((yc & 7) == 0 && (yr & 7) == 0) fh.ident = fv.ident = true;
+  if ((uc & 7) == 0 && (ur & 7) == 0) cfh.ident = cfv.ident = true;
 
-  gather_lds(lh, 24, a.ref.y, a.ref.y_stride, a.ref.lw, a.ref.lh, mb_col * 16 + (yc >> 3) - 2,
-             mb_row * 16 + (yr >> 3) - 2, 21, 21, lane);
+  Gather<21, 21> gy;
+  Gather<13, 13> gu, gv;
+  gy.load(a.ref.y, a.ref.y_stride, a.ref.lw, a.ref.lh, mb_col * 16 + (yc >> 3) - 2,
+          mb_row * 16 + (yr >> 3) - 2, lane);
   const int cx0 = mb_col * 8 + (uc >> 3) - 2, cy0 = mb_row * 8 + (ur >> 3) - 2;
-  gather_lds(chalo, 16, a.ref.u, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, 13, 13, lane);
-  gather_lds(chalo + 13 * 16, 16, a.ref.v, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, 13, 13,
-             lane);
+  gu.load(a.ref.u, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, lane);
+  gv.load(a.ref.v, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, lane);
+  gy.store(lh, 24, lane);
+  gu.store(chalo, 16, lane);
+  gv.store(chalo + 13 * 16, 16, lane);
   wave_lds_sync();
   hpass<16>(lh, 24, lt, fh, lane);
   if (lane < 26) {
@@ -200,11 +229,13 @@ __global__ void __launch_bounds__(256) mb_luma_sad_at_mv_kernel(const SadArgs a)
   uint8_t* lt = lh + 21 * 24;
   const int mb_col = (int)a.mb_xy[2 * i], mb_row = (int)a.mb_xy[2 * i + 1];
   const int16_t r8 = dbl16(a.mv[2 * i]), c8 = dbl16(a.mv[2 * i + 1]);
-  gather_lds(lh, 24, a.plane, a.stride, a.w, a.h, mb_col * 16 + (c8 >> 3) - 2,
-             mb_row * 16 + (r8 >> 3) - 2, 21, 21, lane);
+  Gather<21, 21> gy;
+  gy.load(a.plane, a.stride, a.w, a.h, mb_col * 16 + (c8 >> 3) - 2, mb_row * 16 + (r8 >> 3) - 2,
+          lane);
+  const uint32_t s = *reinterpret_cast<const uint32_t*>(a.src + (size_t)i * 256 + 4 * lane);
+  gy.store(lh, 24, lane);
   wave_lds_sync();
   const uint32_t p = luma_pred_from_window(lh, 0, 0, r8, c8, lt, lane);
-  const uint32_t s = *reinterpret_cast<const uint32_t*>(a.src + (size_t)i * 256 + 4 * lane);
   const uint32_t tot = wave_sum(sad4(s, p, 0));
   if (lane == 0) a.sad[i] = tot;
 }
@@ -237,10 +268,11 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
     minr = min(minr, (int)(dbl16((int16_t)cr[k]) >> 3));
     minc = min(minc, (int)(dbl16((int16_t)ccl[k]) >> 3));
   }
-  gather_lds(win, 24, a.plane, a.stride, a.w, a.h, mb_col * 16 + minc - 2, mb_row * 16 + minr - 2,
-             24, 24, lane);
-  wave_lds_sync();
+  Gather<24, 24> gw;
+  gw.load(a.plane, a.stride, a.w, a.h, mb_col * 16 + minc - 2, mb_row * 16 + minr - 2, lane);
   const uint32_t s = *reinterpret_cast<const uint32_t*>(a.src + (size_t)i * 256 + 4 * lane);
+  gw.store(win, 24, lane);
+  wave_lds_sync();
   int best_r = cr[0], best_c = ccl[0];
   uint32_t best_sad = 0;
 #pragma unroll 1

@@ -31,7 +31,10 @@ struct ArnrArgs {
   int n, center;
   int filter_on;    // 0: strength 0 or a single frame -> exact centre copy
   int aligned4;     // every luma plane pointer and stride is a multiple of 4
-  uint8_t wt[256];  // blend weight per |ref - ctr| (values <= 16)
+  int aligned16;    // ... of 16
+  int aligned4c;    // every chroma plane pointer and stride is a multiple of 4
+  int xcd_remap;    // map workgroups to contiguous MB bands per XCD
+  uint32_t wt4[64]; // blend weight per |ref - ctr| as 256 bytes (values <= 16)
 };
 
 size_t arnr_fused_lds_bytes(int nw);

@@ -36,6 +36,8 @@ VP8G_E_ARG = 4
 VP8G_E_DEVICE = 5
 VP8G_E_NO_DEVICE = 6
 MAX_FRAMES = 64
+OPT_ARNR_WAVES = 1
+OPT_XCD_REMAP = 2
 
 # RFC 6386 section 18.3 tap tables, src/motion_comp.rs:294-303, 328-337.
 SIXTAP_FILTERS = np.array(
@@ -77,7 +79,7 @@ class _LumaRef(C.Structure):
 
 EXPORTS = (
     "vp8g_ctx_create", "vp8g_ctx_destroy", "vp8g_ctx_set_stream", "vp8g_ctx_stream",
-    "vp8g_ctx_synchronize", "vp8g_ctx_set_arnr_waves", "vp8g_status_str",
+    "vp8g_ctx_synchronize", "vp8g_ctx_set_option", "vp8g_status_str",
     "vp8g_build_arnr_altref", "vp8g_build_arnr_altref_dev",
     "vp8g_predict_inter_mb_batch", "vp8g_predict_inter_mb_batch_dev",
     "vp8g_mb_luma_sad_at_mv_batch", "vp8g_mb_luma_sad_at_mv_batch_dev",
@@ -97,7 +99,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_ctx_set_stream": (i32, [vp, vp]),
         "vp8g_ctx_stream": (vp, [vp]),
         "vp8g_ctx_synchronize": (i32, [vp]),
-        "vp8g_ctx_set_arnr_waves": (i32, [vp, i32]),
+        "vp8g_ctx_set_option": (i32, [vp, i32, i32]),
         "vp8g_status_str": (C.c_char_p, [i32]),
         "vp8g_build_arnr_altref": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, C.POINTER(_Err)]),
         "vp8g_build_arnr_altref_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp,
@@ -280,7 +282,10 @@ class Context:
         self._raise(self.lib.vp8g_ctx_synchronize(self.h), "synchronize")
 
     def set_arnr_waves(self, waves: int) -> None:
-        self._raise(self.lib.vp8g_ctx_set_arnr_waves(self.h, waves), "set_arnr_waves")
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_ARNR_WAVES, waves), "set_arnr_waves")
+
+    def set_xcd_remap(self, on: bool) -> None:
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_XCD_REMAP, int(on)), "set_xcd_remap")
 
     # ---- ARNR
     def build_arnr_altref(self, frames, center: int, cfg: ArnrConfig | None = None) -> ArnrPicture:

@@ -4,6 +4,10 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+# torch ships its own HIP runtime with the same soname as the system one;
+# loading it before libvp8g.so makes both share one runtime in-process.
+import torch  # noqa: F401
+
 ROOT = Path(__file__).resolve().parent.parent
 for p in (ROOT / "oxideav-vp8_amd", ROOT / "oracle", ROOT):
     if str(p) not in sys.path:
