@@ -121,7 +121,21 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--arnr-waves", type=int, default=0)
     ap.add_argument("--no-xcd-remap", action="store_true")
+    ap.add_argument("--arnr-mbs", type=int, default=0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (5 times), the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings")
     args = ap.parse_args()
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib << 20
+        src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "launches": 5}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the path shards per clip/group with no exchange; see the design notes)",
@@ -148,6 +162,8 @@ def main() -> int:
         ctx.set_arnr_waves(args.arnr_waves)
     if args.no_xcd_remap:
         ctx.set_xcd_remap(False)
+    if args.arnr_mbs:
+        ctx.set_arnr_mbs(args.arnr_mbs)
 
     w, h, n = group.width, group.height, group.n
     cw, ch = (w + 1) // 2, (h + 1) // 2

@@ -88,9 +88,12 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
  * VP8G_OPT_ARNR_WAVES: waves per ARNR workgroup (0 = one per neighbour
  *   frame, at most 16).
  * VP8G_OPT_XCD_REMAP: 1 (default) gives each XCD a contiguous band of
- *   macroblocks, 0 uses the launch order. */
+ *   macroblocks, 0 uses the launch order.
+ * VP8G_OPT_ARNR_MBS: anchor macroblocks per ARNR workgroup, 1..8
+ *   (0 = default 4; reduced automatically to fit 64 KiB of LDS). */
 #define VP8G_OPT_ARNR_WAVES 1
 #define VP8G_OPT_XCD_REMAP 2
+#define VP8G_OPT_ARNR_MBS 3
 int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);
 

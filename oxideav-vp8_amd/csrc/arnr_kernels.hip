@@ -1,6 +1,13 @@
-// ARNR altref synthesis on gfx950: one workgroup per anchor macroblock,
-// one wave per neighbour frame, search + occlusion guard + blend + resolve
-// fused into a single launch.
+// ARNR altref synthesis on gfx950, search + occlusion guard + blend + resolve
+// fused into one launch.
+//
+// Work decomposition: one workgroup per strip of M horizontally adjacent
+// anchor macroblocks of one MB row; one wave per neighbour frame.  Each wave
+// stages its neighbour's clamped luma and chroma windows covering the whole
+// strip (+-15 px luma, +-7 px chroma) into LDS once, then runs the M block
+// searches and blends out of LDS.  The centre strip is staged once per
+// workgroup.  Per-pixel sums from all waves meet in LDS accumulators and the
+// workgroup resolves the strip.
 //
 // Semantics follow src/arnr.rs:476-675 of the reference crate:
 //   * search: refine_search (src/arnr.rs:264-300) — SAD at (0,0), then for
@@ -9,18 +16,16 @@
 //     strictly smaller SAD, so the first minimum in that order wins.  The
 //     reference's row early exit (src/arnr.rs:217-219) never changes the
 //     winner, so full SADs are computed.
-//   * edge clamping uses the visible dims (pel, src/arnr.rs:149-153); every
-//     sample a search or blend can touch (|mv| <= 15, chroma |mv/2| <= 7) is
-//     staged once per (block, neighbour) into clamped LDS windows (luma
-//     46x48, chroma 22x24 per plane), after which all reads are LDS reads.
+//   * edge clamping uses the visible dims (pel, src/arnr.rs:149-153): the
+//     staged windows hold the clamped samples, so every later read is plain.
 //   * occlusion guard: skip the block when sad / (bw*bh) > 12
 //     (src/arnr.rs:603-608).
 //   * blend: acc += wt[|ref - ctr|] * ref, wsum += wt[..] with the 256-entry
 //     table of src/arnr.rs:307-323; chroma uses (mv/2 truncated toward zero)
-//     and the chroma-plane clamp (src/arnr.rs:402-459, 620-639).  The two
-//     sums are packed into one u32 (acc < 2^20, wsum < 2^12 for <= 64
-//     frames) and added with LDS atomics: integer addition is exact and
-//     commutative, so the result does not depend on wave scheduling.
+//     and the chroma-plane clamp (src/arnr.rs:402-459, 620-639).  Both sums
+//     are packed into one u32 (acc < 2^20, wsum < 2^12 for <= 64 frames)
+//     and added with LDS atomics: integer addition is exact and commutative,
+//     so the result does not depend on wave scheduling.
 //   * resolve: (acc + ws/2) / ws with the centre's weight-16 seed
 //     (src/arnr.rs:662-675).
 // Chroma output clamping (src/arnr.rs:442-444) never fires from the block
@@ -30,14 +35,22 @@
 
 namespace vp8g {
 
-constexpr int kWinRows = 46;                         // by-15 .. by+30
-constexpr int kWinCols = 48;                         // bx-16 .. bx+31
-constexpr int kWinDwords = kWinRows * kWinCols / 4;  // 552
-constexpr int kCWinRows = 22;                        // cby-7 .. cby+14
-constexpr int kCWinCols = 24;                        // cbx-8 .. cbx+15
-constexpr int kCWinDwords = kCWinRows * kCWinCols / 4;  // 132 per plane
-constexpr int kWaveLdsDwords = kWinDwords + 2 * kCWinDwords;  // 816
-constexpr int kAccShift = 20;                        // packed: wsum << 20 | acc
+typedef uint4 __attribute__((aligned(1))) uint4_u;     // unaligned LDS reads
+typedef uint32_t __attribute__((aligned(1))) uint32_u;
+
+constexpr int kWinRows = 46;   // by-15 .. by+30
+constexpr int kCWinRows = 22;  // cby-7 .. cby+14
+constexpr int kAccShift = 20;  // packed accumulator: wsum << 20 | acc
+
+// Strip geometry for M MBs per workgroup; compile-time so that every
+// index division in the staging and resolve loops is a multiply-shift.
+template <int M_>
+struct StripGeom {
+  static constexpr int m = M_;                 // MBs per strip
+  static constexpr int lp = 16 * M_ + 32;      // luma window pitch (cols bx0-16 .. bx0+16M+15)
+  static constexpr int cp = 8 * M_ + 16;       // chroma window pitch (cols cbx0-8 .. cbx0+8M+7)
+  static constexpr int wave_bytes = (kWinRows * lp + 2 * kCWinRows * cp + 15) & ~15;
+};
 
 __device__ __forceinline__ uint32_t load_clamped4(const uint8_t* row, int x0, int w, bool aligned4) {
   if (aligned4 && x0 >= 0 && x0 + 4 <= w) return *reinterpret_cast<const uint32_t*>(row + x0);
@@ -57,278 +70,325 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Stage the neighbour's clamped luma and chroma windows into this wave's
-// LDS region.  All global loads are issued before the first LDS store.
-__device__ __forceinline__ void stage_windows(uint32_t* win, const FrameDesc& f, const ArnrArgs& a,
-                                              int bx, int by, bool interior16, int lane) {
-  const int cbx = bx >> 1, cby = by >> 1;
-  if (interior16) {
-    // 46 rows x 3 aligned 16-byte pieces, no clamping needed.
-    uint4 v[3];
-#pragma unroll
-    for (int it = 0; it < 3; it++) {
-      const int s = lane + 64 * it;
-      if (s < kWinRows * 3) {
-        const int wy = s / 3, part = s - wy * 3;
-        v[it] = *reinterpret_cast<const uint4*>(f.y + (size_t)(by - 15 + wy) * f.y_stride + bx - 16 +
-                                                16 * part);
+// Stage frame f's clamped windows for the strip starting at (bx0, by):
+// luma rows by-15..by+30, cols bx0-16..bx0+16M+15; chroma rows
+// cby-7..cby+14, cols cbx0-8..cbx0+8M+7 (U then V).
+template <class G>
+__device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrArgs& a, const G g,
+                              int bx0, int by, int lane) {
+  const int cbx0 = bx0 >> 1, cby = by >> 1;
+  const ptrdiff_t v_minus_u = f.v - f.u;  // plane choice as integer arithmetic
+  uint8_t* cwin = win + kWinRows * g.lp;
+  constexpr int lpieces = G::lp >> 4;  // 16-byte pieces per luma row
+  constexpr int cdw = G::cp >> 2;      // dwords per chroma row
+  const bool interior = a.aligned16 && bx0 >= 16 && bx0 + 16 * G::m + 16 <= a.w && by >= 15 &&
+                        by + 31 <= a.h;
+  if (interior) {
+    // Luma: whole aligned 16-byte pieces; chroma (then interior as well:
+    // cbx0 >= 8, cbx0 + 8M + 8 <= cw, cby >= 8, cby + 15 <= ch): dwords.
+    // Loads go out in batches of 4 per lane before their LDS stores.
+    const uint8_t* ybase = f.y + (size_t)(by - 15) * f.y_stride + bx0 - 16;
+    constexpr int total = kWinRows * lpieces;
+    auto yld = [&](int s) -> uint4 {
+      s = min(s, total - 1);
+      const int wy = s / lpieces, pc = s - wy * lpieces;
+      return *reinterpret_cast<const uint4*>(ybase + (size_t)wy * f.y_stride + 16 * pc);
+    };
+    auto yst = [&](int s, const uint4& v) {
+      if (s < total) {
+        const int wy = s / lpieces, pc = s - wy * lpieces;
+        *reinterpret_cast<uint4*>(win + wy * G::lp + 16 * pc) = v;
       }
+    };
+    for (int s0 = lane; s0 < total; s0 += 256) {
+      const uint4 v0 = yld(s0), v1 = yld(s0 + 64), v2 = yld(s0 + 128), v3 = yld(s0 + 192);
+      yst(s0, v0);
+      yst(s0 + 64, v1);
+      yst(s0 + 128, v2);
+      yst(s0 + 192, v3);
     }
-    // The luma window being interior implies the chroma windows are too
-    // (cbx >= 8, cbx + 16 <= cw, cby >= 8, cby + 15 <= ch).
-    uint32_t c[5];
-#pragma unroll
-    for (int it = 0; it < 5; it++) {
-      const int s = lane + 64 * it;
-      if (s < 2 * kCWinDwords) {
-        const int pl = s >= kCWinDwords;
-        const int t = s - pl * kCWinDwords;
-        const int wy = t / 6, dq = t - wy * 6;
-        const uint8_t* row = (pl ? f.v : f.u) + (size_t)(cby - 7 + wy) * f.uv_stride + cbx - 8 + 4 * dq;
-        c[it] = a.aligned4c ? *reinterpret_cast<const uint32_t*>(row)
-                            : (uint32_t)row[0] | ((uint32_t)row[1] << 8) | ((uint32_t)row[2] << 16) |
-                                  ((uint32_t)row[3] << 24);
-      }
-    }
-#pragma unroll
-    for (int it = 0; it < 3; it++) {
-      const int s = lane + 64 * it;
-      if (s < kWinRows * 3) reinterpret_cast<uint4*>(win)[s] = v[it];
-    }
-#pragma unroll
-    for (int it = 0; it < 5; it++) {
-      const int s = lane + 64 * it;
-      if (s < 2 * kCWinDwords) win[kWinDwords + s] = c[it];
+    const size_t coff = (size_t)(cby - 7) * f.uv_stride + cbx0 - 8;
+    constexpr int ctotal = 2 * kCWinRows * cdw;
+    const bool al4 = a.aligned4c != 0;
+    auto cld = [&](int s) -> uint32_t {
+      s = min(s, ctotal - 1);
+      const int pl = s >= kCWinRows * cdw;
+      const int t = s - pl * kCWinRows * cdw;
+      const int wy = t / cdw, dq = t - wy * cdw;
+      const uint8_t* p = f.u + (pl ? v_minus_u : 0) + coff + (size_t)wy * f.uv_stride + 4 * dq;
+      return al4 ? *reinterpret_cast<const uint32_t*>(p)
+                 : (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+    };
+    uint32_t* cw32 = reinterpret_cast<uint32_t*>(cwin);
+    for (int s0 = lane; s0 < ctotal; s0 += 256) {
+      const uint32_t c0 = cld(s0), c1 = cld(s0 + 64), c2 = cld(s0 + 128), c3 = cld(s0 + 192);
+      cw32[s0] = c0;
+      if (s0 + 64 < ctotal) cw32[s0 + 64] = c1;
+      if (s0 + 128 < ctotal) cw32[s0 + 128] = c2;
+      if (s0 + 192 < ctotal) cw32[s0 + 192] = c3;
     }
     return;
   }
-  // Frame-edge blocks (a few percent of a frame): clamped dword gathers.
-#pragma unroll 1
-  for (int s = lane; s < kWinDwords; s += 64) {
-    const int wy = s / 12, dq = s - wy * 12;
+  // Frame-edge strips: clamped dword gathers.
+  const int ldw = g.lp >> 2;
+  for (int s = lane; s < kWinRows * ldw; s += 64) {
+    const int wy = s / ldw, dq = s - wy * ldw;
     const int y = clampi(by - 15 + wy, 0, a.h - 1);
-    win[s] = load_clamped4(f.y + (size_t)y * f.y_stride, bx - 16 + 4 * dq, a.w, a.aligned4 != 0);
+    reinterpret_cast<uint32_t*>(win)[s] =
+        load_clamped4(f.y + (size_t)y * f.y_stride, bx0 - 16 + 4 * dq, a.w, a.aligned4 != 0);
   }
-#pragma unroll 1
-  for (int s = lane; s < 2 * kCWinDwords; s += 64) {
-    const int pl = s >= kCWinDwords;
-    const int t = s - pl * kCWinDwords;
-    const int wy = t / 6, dq = t - wy * 6;
+  for (int s = lane; s < 2 * kCWinRows * cdw; s += 64) {
+    const int pl = s >= kCWinRows * cdw;
+    const int t = s - pl * kCWinRows * cdw;
+    const int wy = t / cdw, dq = t - wy * cdw;
     const int y = clampi(cby - 7 + wy, 0, a.ch - 1);
-    win[kWinDwords + s] = load_clamped4((pl ? f.v : f.u) + (size_t)y * f.uv_stride, cbx - 8 + 4 * dq,
-                                        a.cw, a.aligned4c != 0);
+    reinterpret_cast<uint32_t*>(cwin)[s] = load_clamped4(f.u + (pl ? v_minus_u : 0) + (size_t)y * f.uv_stride,
+                                                         cbx0 - 8 + 4 * dq, a.cw, a.aligned4c != 0);
   }
 }
 
-// Maps the launch index onto the MB grid so that each of the 8 XCDs (which
-// receive workgroups round-robin) works on one contiguous band of MBs and
-// keeps that band's frame rows in its own L2.
+// Maps the launch index onto strips so that each of the 8 XCDs (which
+// receive workgroups round-robin) works on one contiguous band and keeps
+// that band's frame rows in its own L2.
 __device__ __forceinline__ int xcd_band(int i, int total) {
   const int full = (total >> 3) << 3;
   if (i >= full) return i;
   return (i & 7) * (total >> 3) + (i >> 3);
 }
 
+template <int M>
 __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
-  extern __shared__ __align__(16) uint32_t smem[];
-  uint32_t* s_acc = smem;                                   // 256 luma + 64 U + 64 V, packed
-  uint8_t* s_lut = reinterpret_cast<uint8_t*>(smem + 384);  // 256 bytes
-  uint32_t* s_win = smem + 384 + 64;                        // per wave: luma | U | V windows
+  extern __shared__ __align__(16) uint8_t smem[];
+  const StripGeom<M> g;
+  // LDS: packed accumulators M*384 u32 | weight LUT 256 B | centre luma
+  // 16 x 16M | centre U, V 8 x 8M each | per-wave windows.
+  uint32_t* s_acc = reinterpret_cast<uint32_t*>(smem);
+  uint8_t* s_lut = smem + M * 384 * 4;
+  uint8_t* s_cy = s_lut + 256;
+  uint8_t* s_cu = s_cy + 256 * M;
+  uint8_t* s_cv = s_cu + 64 * M;
+  uint8_t* s_win = s_cv + 64 * M;
 
   const int nw = blockDim.x >> 6;
   const int tid = threadIdx.x;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
 
-  const int mb = a.xcd_remap ? xcd_band(blockIdx.x, a.bcols * a.brows) : (int)blockIdx.x;
-  const int bcol = mb % a.bcols;
-  const int brow = mb / a.bcols;
-  const int bx = bcol * 16, by = brow * 16;
-  const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
-  const int cbx = bx >> 1, cby = by >> 1;
-  const int cbw = (bw + 1) >> 1, cbh = (bh + 1) >> 1;
+  const int strip = a.xcd_remap ? xcd_band(blockIdx.x, a.strips_per_row * a.brows) : (int)blockIdx.x;
+  const int brow = strip / a.strips_per_row;
+  const int scol = strip - brow * a.strips_per_row;
+  const int bx0 = scol * 16 * M, by = brow * 16;
+  const int cbx0 = bx0 >> 1, cby = by >> 1;
+  const int bh = min(16, a.h - by), cbh = (bh + 1) >> 1;
   const FrameDesc C = a.frames[a.center];
-  const bool full_block = bw == 16 && bh == 16;
-  const bool interior16 = a.aligned16 && bx >= 16 && bx + 32 <= a.w && by >= 15 && by + 31 <= a.h;
-
   const int nnb = a.filter_on ? a.n - 1 : 0;
-  uint32_t* win = s_win + wave * kWaveLdsDwords;
-  const int g = lane >> 3, kk = lane & 7;   // search: probe group, row pair
-  const int lr = lane >> 2, lq = lane & 3;  // blend: luma row, dword
-  const int cr = lane >> 3, cc = lane & 7;  // blend: chroma pixel
+  uint8_t* win = s_win + wave * g.wave_bytes;
+  uint8_t* cwin = win + kWinRows * g.lp;
 
-  // First neighbour's windows are issued before anything else.
+  // The first neighbour's windows are staged before anything else.
   int k = wave;
-  if (k < nnb) stage_windows(win, a.frames[k < a.center ? k : k + 1], a, bx, by, interior16, lane);
+  if (k < nnb) stage_windows(win, a.frames[k < a.center ? k : k + 1], a, g, bx0, by, lane);
 
-  // Centre samples this lane needs: 2 source rows for the search, one
-  // dword for the luma blend, one U and one V pixel.  Zero outside bw x bh.
-  uint32_t src[8];
-  uint32_t ctr_l = 0;
-  int cu8 = 0, cv8 = 0;
-  const bool chroma_in = cr < cbh && cc < cbw;
-  if (full_block && a.aligned16) {
-#pragma unroll
-    for (int rr = 0; rr < 2; rr++) {
-      const uint4 q = *reinterpret_cast<const uint4*>(C.y + (size_t)(by + 2 * kk + rr) * C.y_stride + bx);
-      src[rr * 4 + 0] = q.x; src[rr * 4 + 1] = q.y; src[rr * 4 + 2] = q.z; src[rr * 4 + 3] = q.w;
-    }
-    ctr_l = *reinterpret_cast<const uint32_t*>(C.y + (size_t)(by + lr) * C.y_stride + bx + 4 * lq);
-  } else {
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int r = 2 * kk + (i >> 2), q = i & 3;
-      uint32_t v = 0;
-      if (r < bh) {
-#pragma unroll
+  // Centre strip (zero outside the frame), accumulators, LUT.
+  for (int i = tid; i < 64 * M; i += blockDim.x) {  // luma: 16 rows x 4M dwords
+    const int r = i / (4 * M), q = i - r * 4 * M;
+    const int x = bx0 + 4 * q;
+    uint32_t v = 0;
+    if (r < bh && x < a.w) {
+      const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride;
+      if (a.aligned4 && x + 4 <= a.w) {
+        v = *reinterpret_cast<const uint32_t*>(row + x);
+      } else {
         for (int j = 0; j < 4; j++)
-          if (4 * q + j < bw) v |= (uint32_t)C.y[(size_t)(by + r) * C.y_stride + bx + 4 * q + j] << (8 * j);
+          if (x + j < a.w) v |= (uint32_t)row[x + j] << (8 * j);
       }
-      src[i] = v;
     }
-    if (lr < bh) {
-#pragma unroll
+    reinterpret_cast<uint32_t*>(s_cy)[i] = v;
+  }
+  for (int i = tid; i < 32 * M; i += blockDim.x) {  // chroma: 2 planes x 8 rows x 2M dwords
+    const int pl = i >= 16 * M;
+    const int t = i - pl * 16 * M;
+    const int r = t / (2 * M), q = t - r * 2 * M;
+    const int x = cbx0 + 4 * q;
+    uint32_t v = 0;
+    if (r < cbh && x < a.cw) {
+      const uint8_t* row = (pl ? C.v : C.u) + (size_t)(cby + r) * C.uv_stride;
       for (int j = 0; j < 4; j++)
-        if (4 * lq + j < bw) ctr_l |= (uint32_t)C.y[(size_t)(by + lr) * C.y_stride + bx + 4 * lq + j] << (8 * j);
+        if (x + j < a.cw) v |= (uint32_t)row[x + j] << (8 * j);
     }
+    reinterpret_cast<uint32_t*>(pl ? s_cv : s_cu)[t] = v;
   }
-  if (chroma_in) {
-    cu8 = C.u[(size_t)(cby + cr) * C.uv_stride + cbx + cc];
-    cv8 = C.v[(size_t)(cby + cr) * C.uv_stride + cbx + cc];
-  }
-  for (int i = tid; i < 384; i += blockDim.x) s_acc[i] = 0;
+  for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
   for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = a.wt4[i];
   __syncthreads();
 
-  uint32_t colmask[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-      if (4 * q + j < bw) m |= 0xFFu << (8 * j);
-    colmask[q] = m;
-  }
-  const int gdx = (g == 0 || g == 3 || g == 5) ? -1 : ((g == 1 || g == 6) ? 0 : 1);
-  const int gdy = g < 3 ? -1 : (g < 5 ? 0 : 1);
-  const uint32_t area = (uint32_t)(bw * bh);
+  const int gq = lane >> 3, kk = lane & 7;   // search: probe group, row pair
+  const int gdx = (gq == 0 || gq == 3 || gq == 5) ? -1 : ((gq == 1 || gq == 6) ? 0 : 1);
+  const int gdy = gq < 3 ? -1 : (gq < 5 ? 0 : 1);
+  const int lr = lane >> 2, lq = lane & 3;   // blend: luma row, dword
+  const int cr = lane >> 3, cc = lane & 7;   // blend: chroma pixel
 
   for (; k < nnb; k += nw) {
     const int fi = k < a.center ? k : k + 1;
-    if (k != wave) stage_windows(win, a.frames[fi], a, bx, by, interior16, lane);
+    if (k != wave) stage_windows(win, a.frames[fi], a, g, bx0, by, lane);
     wave_lds_sync();
 
-    // SAD of probe (cx, cy) summed over this lane's 8-lane group.
-    auto probe = [&](int cx, int cy) -> uint32_t {
-      uint32_t acc = 0;
-#pragma unroll
-      for (int rr = 0; rr < 2; rr++) {
-        const int r = 2 * kk + rr;
-        if (r < bh) {
-          const int off = (r + cy + 15) * kWinCols + cx + 16;
-          const uint32_t* p = win + (off >> 2);
-          const uint32_t sh = off & 3;
-          const uint32_t w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
-          uint32_t a0 = funnel(w0, w1, sh), a1 = funnel(w1, w2, sh);
-          uint32_t a2 = funnel(w2, w3, sh), a3 = funnel(w3, w4, sh);
-          if (bw < 16) {
-            a0 &= colmask[0]; a1 &= colmask[1]; a2 &= colmask[2]; a3 &= colmask[3];
-          }
-          acc = sad4(src[rr * 4 + 0], a0, acc);
-          acc = sad4(src[rr * 4 + 1], a1, acc);
-          acc = sad4(src[rr * 4 + 2], a2, acc);
-          acc = sad4(src[rr * 4 + 3], a3, acc);
-        }
-      }
-      return sum8(acc);
-    };
-
-    int bestx = 0, besty = 0;
-    uint32_t best_sad = (uint32_t)__builtin_amdgcn_readfirstlane((int)probe(0, 0));
 #pragma unroll 1
-    for (int step = 8; step >= 1; step >>= 1) {
-      const int px = bestx + gdx * step, py = besty + gdy * step;
-      const bool valid = px >= -15 && px <= 15 && py >= -15 && py <= 15;
-      uint32_t s = probe(valid ? px : 0, valid ? py : 0);
-      if (!valid) s = 0xFFFFFFFFu;
-      const int cx0 = bestx, cy0 = besty;
+    for (int m = 0; m < M; m++) {
+      const int bx = bx0 + 16 * m;
+      if (bx >= a.w) break;
+      const int bw = min(16, a.w - bx);
+      const int cbw = (bw + 1) >> 1;
+      uint32_t colmask[4];
 #pragma unroll
-      for (int gg = 0; gg < 8; gg++) {
-        const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)s, gg * 8);
-        if (v < best_sad) {
-          const int ddx = (gg == 0 || gg == 3 || gg == 5) ? -1 : ((gg == 1 || gg == 6) ? 0 : 1);
-          const int ddy = gg < 3 ? -1 : (gg < 5 ? 0 : 1);
-          best_sad = v;
-          bestx = cx0 + ddx * step;
-          besty = cy0 + ddy * step;
-        }
+      for (int q = 0; q < 4; q++) {
+        const int left = bw - 4 * q;
+        colmask[q] = left >= 4 ? 0xFFFFFFFFu : (left <= 0 ? 0u : (0xFFFFFFFFu >> (32 - 8 * left)));
       }
-    }
-    if (a.mv_out && lane == 0) {
-      int32_t* o = a.mv_out + 3 * ((size_t)fi * a.brows * a.bcols + mb);
-      o[0] = bestx; o[1] = besty; o[2] = (int32_t)best_sad;
-    }
-
-    if (best_sad / area <= kSadPerPixelCutoff) {
-      if (lr < bh) {
-        const int off = (lr + besty + 15) * kWinCols + 4 * lq + bestx + 16;
-        const uint32_t ref4 = funnel(win[off >> 2], win[(off >> 2) + 1], off & 3);
+      uint4 src[2];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (4 * lq + j < bw) {
-            const int rp = (ref4 >> (8 * j)) & 255;
-            const int cp = (ctr_l >> (8 * j)) & 255;
-            const uint32_t wgt = s_lut[rp > cp ? rp - cp : cp - rp];
-            lds_add(&s_acc[lr * 16 + 4 * lq + j], (wgt << kAccShift) | (wgt * (uint32_t)rp));
+      for (int rr = 0; rr < 2; rr++)
+        src[rr] = *reinterpret_cast<const uint4*>(s_cy + (2 * kk + rr) * 16 * M + 16 * m);
+
+      // SAD of probe (cx, cy) summed over this lane's 8-lane group.
+      auto probe = [&](int cx, int cy) -> uint32_t {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int rr = 0; rr < 2; rr++) {
+          const int r = 2 * kk + rr;
+          if (r < bh) {
+            uint4 q = *reinterpret_cast<const uint4_u*>(win + (r + cy + 15) * g.lp + 16 * m + 16 + cx);
+            if (bw < 16) {
+              q.x &= colmask[0]; q.y &= colmask[1]; q.z &= colmask[2]; q.w &= colmask[3];
+            }
+            acc = sad4(src[rr].x, q.x, acc);
+            acc = sad4(src[rr].y, q.y, acc);
+            acc = sad4(src[rr].z, q.z, acc);
+            acc = sad4(src[rr].w, q.w, acc);
           }
         }
+        return sum8(acc);
+      };
+
+      int bestx = 0, besty = 0;
+      uint32_t best_sad = (uint32_t)__builtin_amdgcn_readfirstlane((int)probe(0, 0));
+#pragma unroll 1
+      for (int step = 8; step >= 1; step >>= 1) {
+        const int px = bestx + gdx * step, py = besty + gdy * step;
+        const bool valid = px >= -15 && px <= 15 && py >= -15 && py <= 15;
+        const uint32_t s = probe(valid ? px : 0, valid ? py : 0);
+        // Probe order is the group index; the incumbent keeps ties, so the
+        // key (sad << 4 | group + 1) against (best_sad << 4) picks exactly
+        // the reference's winner with one unsigned minimum.
+        uint32_t key = valid ? ((s << 4) | (uint32_t)(gq + 1)) : 0xFFFFFFFFu;
+        key = min(key, dpp<0x128>(key));  // row_ror:8 -> the other group of the row
+        uint32_t kmin = (uint32_t)__builtin_amdgcn_readlane((int)key, 0);
+        kmin = min(kmin, (uint32_t)__builtin_amdgcn_readlane((int)key, 16));
+        kmin = min(kmin, (uint32_t)__builtin_amdgcn_readlane((int)key, 32));
+        kmin = min(kmin, (uint32_t)__builtin_amdgcn_readlane((int)key, 48));
+        if (kmin < (best_sad << 4)) {
+          const int w8 = (int)(kmin & 15) - 1;
+          best_sad = kmin >> 4;
+          bestx += ((w8 == 0 || w8 == 3 || w8 == 5) ? -1 : ((w8 == 1 || w8 == 6) ? 0 : 1)) * step;
+          besty += (w8 < 3 ? -1 : (w8 < 5 ? 0 : 1)) * step;
+        }
       }
-      if (chroma_in) {
-        const int cmvx = bestx / 2, cmvy = besty / 2;  // truncation toward zero
-        const uint8_t* cw8 = reinterpret_cast<const uint8_t*>(win + kWinDwords);
-        const int o = (cr + cmvy + 7) * kCWinCols + cc + cmvx + 8;
-        const int ru = cw8[o], rv = cw8[kCWinDwords * 4 + o];
-        const uint32_t wu = s_lut[ru > cu8 ? ru - cu8 : cu8 - ru];
-        const uint32_t wv = s_lut[rv > cv8 ? rv - cv8 : cv8 - rv];
-        lds_add(&s_acc[256 + cr * 8 + cc], (wu << kAccShift) | (wu * (uint32_t)ru));
-        lds_add(&s_acc[320 + cr * 8 + cc], (wv << kAccShift) | (wv * (uint32_t)rv));
+      if (a.mv_out && lane == 0) {
+        int32_t* o = a.mv_out + 3 * (((size_t)fi * a.brows + brow) * a.bcols + (bx >> 4));
+        o[0] = bestx; o[1] = besty; o[2] = (int32_t)best_sad;
+      }
+
+      // sad / (bw*bh) <= 12  <=>  sad < 13 * bw * bh (integer division).
+      if (best_sad < (kSadPerPixelCutoff + 1) * (uint32_t)(bw * bh)) {
+        uint32_t* acc = s_acc + 384 * m;
+        if (lr < bh) {
+          const uint32_t ref4 = *reinterpret_cast<const uint32_u*>(
+              win + (lr + besty + 15) * g.lp + 16 * m + 16 + 4 * lq + bestx);
+          const uint32_t ctr4 = *reinterpret_cast<const uint32_t*>(s_cy + lr * 16 * M + 16 * m + 4 * lq);
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            if (4 * lq + j < bw) {
+              const int rp = (ref4 >> (8 * j)) & 255;
+              const int cp = (ctr4 >> (8 * j)) & 255;
+              const uint32_t wgt = s_lut[rp > cp ? rp - cp : cp - rp];
+              lds_add(&acc[lr * 16 + 4 * lq + j], (wgt << kAccShift) | (wgt * (uint32_t)rp));
+            }
+          }
+        }
+        if (cr < cbh && cc < cbw) {
+          const int cmvx = bestx / 2, cmvy = besty / 2;  // truncation toward zero
+          const int o = (cr + cmvy + 7) * g.cp + 8 * m + 8 + cc + cmvx;
+          const int ru = cwin[o], rv = cwin[kCWinRows * g.cp + o];
+          const int cu = s_cu[cr * 8 * M + 8 * m + cc], cv = s_cv[cr * 8 * M + 8 * m + cc];
+          const uint32_t wu = s_lut[ru > cu ? ru - cu : cu - ru];
+          const uint32_t wv = s_lut[rv > cv ? rv - cv : cv - rv];
+          lds_add(&acc[256 + cr * 8 + cc], (wu << kAccShift) | (wu * (uint32_t)ru));
+          lds_add(&acc[320 + cr * 8 + cc], (wv << kAccShift) | (wv * (uint32_t)rv));
+        }
       }
     }
     wave_lds_sync();  // window reads done before the next neighbour restages it
   }
   __syncthreads();
 
-  // Resolve: lane-parallel over the block's pixels, dword stores where the
-  // row is whole and aligned.
-  for (int i = tid; i < 384; i += blockDim.x) {
-    uint32_t ctr;
-    uint8_t* dst;
-    bool in;
-    if (i < 256) {
-      const int r = i >> 4, c = i & 15;
-      in = r < bh && c < bw;
-      ctr = in ? C.y[(size_t)(by + r) * C.y_stride + bx + c] : 0;
-      dst = a.out_y + (size_t)(by + r) * a.w + bx + c;
-    } else {
-      const int j = (i - 256) & 63, r = j >> 3, c = j & 7;
-      in = r < cbh && c < cbw;
-      const uint8_t* pl = i < 320 ? C.u : C.v;
-      ctr = in ? pl[(size_t)(cby + r) * C.uv_stride + cbx + c] : 0;
-      dst = (i < 320 ? a.out_u : a.out_v) + (size_t)(cby + r) * a.cw + cbx + c;
+  // Resolve the strip: each thread handles 4 horizontally adjacent pixels.
+  for (int i = tid; i < 96 * M; i += blockDim.x) {
+    const int m = i / 96, j = i - m * 96;
+    const int bx = bx0 + 16 * m;
+    if (bx >= a.w) continue;
+    const uint32_t* acc = s_acc + 384 * m;
+    int x, y, plane_w, base;
+    const uint8_t* ctrp;
+    uint8_t* out;
+    if (j < 64) {  // luma: 16 rows x 4 quads
+      const int r = j >> 2, q = j & 3;
+      x = bx + 4 * q; y = by + r; plane_w = a.w; base = r * 16 + 4 * q;
+      ctrp = s_cy + r * 16 * M + 16 * m + 4 * q;
+      out = a.out_y;
+      if (r >= bh) continue;
+    } else {  // chroma: 2 planes x 8 rows x 2 quads
+      const int t = j - 64, pl = t >= 16, u = t & 15, r = u >> 1, q = u & 1;
+      x = (bx >> 1) + 4 * q; y = cby + r; plane_w = a.cw; base = 256 + 64 * pl + r * 8 + 4 * q;
+      ctrp = (pl ? s_cv : s_cu) + r * 8 * M + 8 * m + 4 * q;
+      out = pl ? a.out_v : a.out_u;
+      if (r >= cbh) continue;
     }
-    const uint32_t packed = s_acc[i];
-    const uint32_t acc = kWMax * ctr + (packed & ((1u << kAccShift) - 1));
-    const uint32_t ws = kWMax + (packed >> kAccShift);
-    if (in) *dst = (uint8_t)((acc + (ws >> 1)) / ws);
+    uint32_t px4 = 0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const uint32_t packed = acc[base + e];
+      const uint32_t sum = kWMax * ctrp[e] + (packed & ((1u << kAccShift) - 1));
+      const uint32_t ws = kWMax + (packed >> kAccShift);
+      px4 |= ((sum + (ws >> 1)) / ws) << (8 * e);
+    }
+    uint8_t* dst = out + (size_t)y * plane_w + x;
+    if (x + 4 <= plane_w && ((((uintptr_t)dst) & 3) == 0)) {
+      *reinterpret_cast<uint32_t*>(dst) = px4;
+    } else {
+      for (int e = 0; e < 4; e++)
+        if (x + e < plane_w) dst[e] = (uint8_t)(px4 >> (8 * e));
+    }
   }
 }
 
-size_t arnr_fused_lds_bytes(int nw) { return (size_t)(384 + 64 + nw * kWaveLdsDwords) * 4; }
+size_t arnr_fused_lds_bytes(int nw, int m) {
+  const int lp = 16 * m + 32, cp = 8 * m + 16;
+  const int wave_bytes = (kWinRows * lp + 2 * kCWinRows * cp + 15) & ~15;
+  return (size_t)m * 384 * 4 + 256 + 256 * m + 128 * m + (size_t)nw * wave_bytes;
+}
+
+int arnr_supported_mbs(int m) { return m >= 8 ? 8 : (m >= 4 ? 4 : (m >= 2 ? 2 : 1)); }
 
 hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream) {
-  const int blocks = a.bcols * a.brows;
-  hipLaunchKernelGGL(arnr_fused_kernel, dim3(blocks), dim3(64 * nw), arnr_fused_lds_bytes(nw),
-                     stream, a);
+  const int blocks = a.strips_per_row * a.brows;
+  const size_t lds = arnr_fused_lds_bytes(nw, a.mbs_per_wg);
+  switch (a.mbs_per_wg) {
+    case 1: hipLaunchKernelGGL(arnr_fused_kernel<1>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
+    case 2: hipLaunchKernelGGL(arnr_fused_kernel<2>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
+    case 4: hipLaunchKernelGGL(arnr_fused_kernel<4>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
+    case 8: hipLaunchKernelGGL(arnr_fused_kernel<8>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@ struct vp8g_ctx {
   hipStream_t stream = nullptr;
   int arnr_waves = 0;
   int xcd_remap = 1;
+  int arnr_mbs = 0;
   // Device arena for the host-pointer entry points (grown, never shrunk).
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
@@ -66,11 +67,17 @@ int validate_arnr(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* er
   return VP8G_OK;
 }
 
-int pick_waves(const vp8g_ctx* c, int nnb) {
-  int nw = c->arnr_waves > 0 ? c->arnr_waves : nnb;
+// Launch shape: waves per workgroup (one neighbour frame each, looping when
+// fewer) and MBs per workgroup strip, shrunk until the LDS fits.
+void pick_arnr_shape(const vp8g_ctx* c, int nnb, int* nw_out, int* m_out) {
+  int nw = c->arnr_waves > 0 ? c->arnr_waves : (nnb < 8 ? nnb : 8);
   if (nw < 1) nw = 1;
   if (nw > 16) nw = 16;
-  return nw;
+  int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : 4);
+  while (m > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) m = arnr_supported_mbs(m - 1);
+  while (nw > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) nw--;
+  *nw_out = nw;
+  *m_out = m;
 }
 
 // Enqueue the fused ARNR kernel over device-resident frames.
@@ -109,7 +116,10 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   uint8_t wt[256];
   for (uint32_t d = 0; d < 256; d++) wt[d] = (uint8_t)((kWMax * s) / (s + d * d));
   memcpy(a.wt4, wt, sizeof(wt));
-  const int nw = pick_waves(c, a.filter_on ? (int)n - 1 : 1);
+  int nw, m;
+  pick_arnr_shape(c, a.filter_on ? (int)n - 1 : 1, &nw, &m);
+  a.mbs_per_wg = m;
+  a.strips_per_row = (a.bcols + m - 1) / m;
   return hip_ok(launch_arnr_fused(a, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
@@ -260,6 +270,10 @@ int vp8g_ctx_set_option(vp8g_ctx* c, int option, int value) {
     case VP8G_OPT_XCD_REMAP:
       if (value < 0 || value > 1) return VP8G_E_ARG;
       c->xcd_remap = value;
+      return VP8G_OK;
+    case VP8G_OPT_ARNR_MBS:
+      if (value < 0 || value > 8) return VP8G_E_ARG;
+      c->arnr_mbs = value;
       return VP8G_OK;
     default:
       return VP8G_E_ARG;

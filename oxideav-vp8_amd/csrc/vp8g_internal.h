@@ -34,10 +34,14 @@ struct ArnrArgs {
   int aligned16;    // ... of 16
   int aligned4c;    // every chroma plane pointer and stride is a multiple of 4
   int xcd_remap;    // map workgroups to contiguous MB bands per XCD
+  int mbs_per_wg;   // M: anchor MBs per workgroup (one strip of an MB row)
+  int strips_per_row;
   uint32_t wt4[64]; // blend weight per |ref - ctr| as 256 bytes (values <= 16)
 };
 
-size_t arnr_fused_lds_bytes(int nw);
+constexpr size_t kMaxLdsPerWorkgroup = 65536;
+size_t arnr_fused_lds_bytes(int nw, int mbs_per_wg);
+int arnr_supported_mbs(int m);  // largest compiled strip width <= m (1, 2, 4, 8)
 hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream);
 
 // Motion compensation / SAD scoring.

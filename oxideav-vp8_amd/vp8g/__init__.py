@@ -38,6 +38,7 @@ VP8G_E_NO_DEVICE = 6
 MAX_FRAMES = 64
 OPT_ARNR_WAVES = 1
 OPT_XCD_REMAP = 2
+OPT_ARNR_MBS = 3
 
 # RFC 6386 section 18.3 tap tables, src/motion_comp.rs:294-303, 328-337.
 SIXTAP_FILTERS = np.array(
@@ -283,6 +284,9 @@ class Context:
 
     def set_arnr_waves(self, waves: int) -> None:
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_ARNR_WAVES, waves), "set_arnr_waves")
+
+    def set_arnr_mbs(self, mbs: int) -> None:
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_ARNR_MBS, mbs), "set_arnr_mbs")
 
     def set_xcd_remap(self, on: bool) -> None:
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_XCD_REMAP, int(on)), "set_xcd_remap")

@@ -71,6 +71,22 @@ def packed(w, h, y, u, v):
     return (w, h, y, u, v, w, (w + 1) // 2)
 
 
+def assert_same(got, want, what="", context=None):
+    """Byte-exact comparison with a short report (first differing rows)."""
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, f"{what}: shape {got.shape} != {want.shape}"
+    if np.array_equal(got, want):
+        return
+    bad = np.argwhere(got != want)
+    lines = [f"{what}: {len(bad)} of {got.size} elements differ"]
+    for idx in bad[:6]:
+        idx = tuple(int(i) for i in idx)
+        extra = f" ctx={context(idx)}" if context else ""
+        lines.append(f"  at {idx}: got {got[idx]} want {want[idx]}{extra}")
+    raise AssertionError("\n".join(lines))
+
+
 def mse(a, b) -> float:
     d = a.astype(np.int64) - b.astype(np.int64)
     return float((d * d).sum()) / len(a)

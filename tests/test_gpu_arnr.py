@@ -92,14 +92,17 @@ def test_cif_group_golden(gpu, oracle):
     g = synth.arnr_group_cif()
     pic = check(gpu, oracle, g.frames(), g.center, 3)
     assert hashlib.sha256(pic.y.tobytes() + pic.u.tobytes() + pic.v.tobytes()).hexdigest() == gold["output_sha256"]
-    for waves, remap in ((1, 1), (2, 0), (3, 1), (4, 0)):  # launch shape never changes bytes
+    # launch shape never changes bytes
+    for waves, remap, mbs in ((1, 1, 1), (2, 0, 3), (3, 1, 8), (4, 0, 2), (16, 1, 5)):
         gpu.set_arnr_waves(waves)
         gpu.set_xcd_remap(remap)
+        gpu.set_arnr_mbs(mbs)
         try:
             p2 = gpu.build_arnr_altref(as_frames(vp8g, g.frames()), g.center, vp8g.ArnrConfig(3))
         finally:
             gpu.set_arnr_waves(0)
             gpu.set_xcd_remap(1)
+            gpu.set_arnr_mbs(0)
         assert (p2.y == pic.y).all() and (p2.u == pic.u).all() and (p2.v == pic.v).all()
 
 

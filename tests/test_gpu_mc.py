@@ -9,6 +9,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from conftest import assert_same
+
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -45,7 +47,7 @@ def test_predict_cif_golden(gpu, oracle, cif_mc, table, full):
     filt = vp8g.SIXTAP_FILTERS if table == "sixtap" else vp8g.BILINEAR_FILTERS
     got = gpu.predict_inter_mb_batch(ref_planes(vp8g, cif_mc), cif_mc.mb_xy, cif_mc.mvs, full, filt)
     want = oracle_predict(oracle, cif_mc, cif_mc.mb_xy, cif_mc.mvs, full, filt)
-    assert np.array_equal(got, want)
+    assert_same(got, want, f"{table} full_pixel={full}")
     gold = json.loads((GOLDEN / "mc_cif.json").read_text())
     assert hashlib.sha256(got.tobytes()).hexdigest() == gold[f"predict_{table}_full{int(full)}_sha256"]
 
@@ -61,7 +63,8 @@ def test_predict_extreme_vectors_and_positions(gpu, oracle, cif_mc):
     for filt in (vp8g.SIXTAP_FILTERS, vp8g.BILINEAR_FILTERS):
         for full in (False, True):
             got = gpu.predict_inter_mb_batch(ref_planes(vp8g, cif_mc), mb_xy, mvs, full, filt)
-            assert np.array_equal(got, oracle_predict(oracle, cif_mc, mb_xy, mvs, full, filt))
+            assert_same(got, oracle_predict(oracle, cif_mc, mb_xy, mvs, full, filt), f"extreme full={full}",
+                        lambda i: (tuple(mb_xy[i[0]]), tuple(mvs[i[0]])))
 
 
 def test_predict_custom_tap_table(gpu, oracle, cif_mc):
@@ -73,9 +76,12 @@ def test_predict_custom_tap_table(gpu, oracle, cif_mc):
     # row 0 is not the identity here: whole-pixel vectors must still copy
     mvs = cif_mc.mvs.copy()
     mvs[::3] = mvs[::3] // 4 * 4
+    y8 = (mvs.astype(np.int64) * 2) & 7
     for full in (False, True):
         got = gpu.predict_inter_mb_batch(ref_planes(vp8g, cif_mc), cif_mc.mb_xy, mvs, full, filt)
-        assert np.array_equal(got, oracle_predict(oracle, cif_mc, cif_mc.mb_xy, mvs, full, filt))
+        want = oracle_predict(oracle, cif_mc, cif_mc.mb_xy, mvs, full, filt)
+        assert_same(got, want, f"custom taps full_pixel={full}",
+                    lambda i: (tuple(mvs[i[0]]), "luma(mx,my)", (int(y8[i[0], 1]), int(y8[i[0], 0]))))
 
 
 def test_predict_single_mb_api(gpu, oracle, cif_mc):
@@ -92,7 +98,7 @@ def test_predict_1080p_all_mbs(gpu, oracle):
     mc = synth.mc_workload(g)
     for filt in (vp8g.SIXTAP_FILTERS, vp8g.BILINEAR_FILTERS):
         got = gpu.predict_inter_mb_batch(ref_planes(vp8g, mc), mc.mb_xy, mc.mvs, False, filt)
-        assert np.array_equal(got, oracle_predict(oracle, mc, mc.mb_xy, mc.mvs, False, filt))
+        assert_same(got, oracle_predict(oracle, mc, mc.mb_xy, mc.mvs, False, filt), "1080p predict")
 
 
 def oracle_sad(oracle, plane, stride, w, h, mb_xy, src, mvs):
@@ -116,14 +122,14 @@ def test_sad_1080p_and_visible_dims(gpu, oracle):
     plane = mc.ref_y.reshape(-1)
     lw, lh = mc.mb_cols * 16, mc.mb_rows * 16
     got = gpu.mb_luma_sad_at_mv_batch(vp8g.LumaRef(plane, lw, lw, lh), mc.mb_xy, mc.src, mc.mvs)
-    assert np.array_equal(got, oracle_sad(oracle, plane, lw, lw, lh, mc.mb_xy, mc.src, mc.mvs))
+    assert_same(got, oracle_sad(oracle, plane, lw, lw, lh, mc.mb_xy, mc.src, mc.mvs), "sad")
     # visible-dim clamp bounds narrower than the stride (encoder tests' LumaRef)
     got = gpu.mb_luma_sad_at_mv_batch(vp8g.LumaRef(plane, lw, 1917, 1079), mc.mb_xy, mc.src, mc.mvs)
-    assert np.array_equal(got, oracle_sad(oracle, plane, lw, 1917, 1079, mc.mb_xy, mc.src, mc.mvs))
+    assert_same(got, oracle_sad(oracle, plane, lw, 1917, 1079, mc.mb_xy, mc.src, mc.mvs), "sad visible")
     # whole-pel vectors: mb_luma_sad_at_whole_mv
     whole = (mc.mvs // 4 * 4).astype(np.int16)
     got = gpu.mb_luma_sad_at_mv_batch(vp8g.LumaRef(plane, lw, lw, lh), mc.mb_xy, mc.src, whole)
-    assert np.array_equal(got, oracle_sad(oracle, plane, lw, lw, lh, mc.mb_xy, mc.src, whole))
+    assert_same(got, oracle_sad(oracle, plane, lw, lw, lh, mc.mb_xy, mc.src, whole), "sad whole")
 
 
 def test_sad_cif_golden(gpu, cif_mc):
