@@ -35,8 +35,6 @@
 
 namespace vp8g {
 
-typedef uint4 __attribute__((aligned(1))) uint4_u;     // unaligned LDS reads
-typedef uint32_t __attribute__((aligned(1))) uint32_u;
 
 constexpr int kWinRows = 46;   // by-15 .. by+30
 constexpr int kCWinRows = 22;  // cby-7 .. cby+14
@@ -47,7 +45,11 @@ constexpr int kAccShift = 20;  // packed accumulator: wsum << 20 | acc
 template <int M_>
 struct StripGeom {
   static constexpr int m = M_;                 // MBs per strip
-  static constexpr int lp = 16 * M_ + 32;      // luma window pitch (cols bx0-16 .. bx0+16M+15)
+  static constexpr int lw = 16 * M_ + 32;      // luma window width (cols bx0-16 .. bx0+16M+15)
+  // Row pitch = width + 8 bytes: two rows apart (the row pair of one search
+  // lane) then land 4 banks apart per lane, so the 8 lanes of a probe group
+  // read disjoint LDS banks.
+  static constexpr int lp = lw + 8;
   static constexpr int cp = 8 * M_ + 16;       // chroma window pitch (cols cbx0-8 .. cbx0+8M+7)
   static constexpr int wave_bytes = (kWinRows * lp + 2 * kCWinRows * cp + 15) & ~15;
 };
@@ -79,7 +81,7 @@ __device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrArgs& a
   const int cbx0 = bx0 >> 1, cby = by >> 1;
   const ptrdiff_t v_minus_u = f.v - f.u;  // plane choice as integer arithmetic
   uint8_t* cwin = win + kWinRows * g.lp;
-  constexpr int lpieces = G::lp >> 4;  // 16-byte pieces per luma row
+  constexpr int lpieces = G::lw >> 4;  // 16-byte pieces per luma row
   constexpr int cdw = G::cp >> 2;      // dwords per chroma row
   const bool interior = a.aligned16 && bx0 >= 16 && bx0 + 16 * G::m + 16 <= a.w && by >= 15 &&
                         by + 31 <= a.h;
@@ -97,7 +99,9 @@ __device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrArgs& a
     auto yst = [&](int s, const uint4& v) {
       if (s < total) {
         const int wy = s / lpieces, pc = s - wy * lpieces;
-        *reinterpret_cast<uint4*>(win + wy * G::lp + 16 * pc) = v;
+        uint2* d = reinterpret_cast<uint2*>(win + wy * G::lp + 16 * pc);  // 8-byte aligned
+        d[0] = make_uint2(v.x, v.y);
+        d[1] = make_uint2(v.z, v.w);
       }
     };
     for (int s0 = lane; s0 < total; s0 += 256) {
@@ -130,11 +134,11 @@ __device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrArgs& a
     return;
   }
   // Frame-edge strips: clamped dword gathers.
-  const int ldw = g.lp >> 2;
+  constexpr int ldw = G::lw >> 2;
   for (int s = lane; s < kWinRows * ldw; s += 64) {
     const int wy = s / ldw, dq = s - wy * ldw;
     const int y = clampi(by - 15 + wy, 0, a.h - 1);
-    reinterpret_cast<uint32_t*>(win)[s] =
+    reinterpret_cast<uint32_t*>(win)[wy * (G::lp >> 2) + dq] =
         load_clamped4(f.y + (size_t)y * f.y_stride, bx0 - 16 + 4 * dq, a.w, a.aligned4 != 0);
   }
   for (int s = lane; s < 2 * kCWinRows * cdw; s += 64) {
@@ -145,6 +149,91 @@ __device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrArgs& a
     reinterpret_cast<uint32_t*>(cwin)[s] = load_clamped4(f.u + (pl ? v_minus_u : 0) + (size_t)y * f.uv_stride,
                                                          cbx0 - 8 + 4 * dq, a.cw, a.aligned4c != 0);
   }
+}
+
+// Probe g of a step sits at centre + s*(dx_g, dy_g) with the reference's
+// order (-1,-1),(0,-1),(1,-1),(-1,0),(1,0),(-1,1),(0,1),(1,1)
+// (src/arnr.rs:274-283); (d + 1) is packed 2 bits per probe.
+constexpr uint32_t kProbeDx1 = (0u << 0) | (1u << 2) | (2u << 4) | (0u << 6) | (2u << 8) | (0u << 10) |
+                               (1u << 12) | (2u << 14);
+constexpr uint32_t kProbeDy1 = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) | (1u << 8) | (2u << 10) |
+                               (2u << 12) | (2u << 14);
+
+// refine_search (src/arnr.rs:264-300) for one block against the staged
+// window.  Lanes are 8 probe groups x 8 row pairs; `row0` points at this
+// lane's first source row's (0,0)-displaced reference row.  The probes of
+// a step never leave +-15 (8+4+2+1), so the reference's range check never
+// fires and is not evaluated.  FULL: 16x16 block; otherwise columns >= bw
+// and rows >= bh are masked out of every SAD (the centre strip holds zeros
+// there, the reference row is masked to zero).
+template <bool FULL, class G>
+__device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, const uint4 src1,
+                                        int gd_off, int gq, int kk, int bw, int bh, int& bestx,
+                                        int& besty, uint32_t& best_sad) {
+  uint32_t colmask[4] = {~0u, ~0u, ~0u, ~0u};
+  uint32_t rowmask0 = ~0u, rowmask1 = ~0u;
+  if (!FULL) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int left = bw - 4 * q;
+      colmask[q] = left >= 4 ? 0xFFFFFFFFu : (left <= 0 ? 0u : (0xFFFFFFFFu >> (32 - 8 * left)));
+    }
+    rowmask0 = 2 * kk < bh ? ~0u : 0u;
+    rowmask1 = 2 * kk + 1 < bh ? ~0u : 0u;
+  }
+  // 16 bytes at an arbitrary byte offset: five naturally aligned dword
+  // reads and byte funnel shifts (unaligned LDS reads stall the LDS pipe).
+  auto row16 = [&](const uint8_t* p) -> uint4 {
+    const uint32_t sh = (uint32_t)(uintptr_t)p & 3;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p - sh);
+    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
+    return make_uint4(funnel(w0, w1, sh), funnel(w1, w2, sh), funnel(w2, w3, sh), funnel(w3, w4, sh));
+  };
+  auto sad_at = [&](int off) -> uint32_t {
+    uint4 q0 = row16(row0 + off);
+    uint4 q1 = row16(row0 + off + G::lp);
+    if (!FULL) {
+      q0.x &= colmask[0] & rowmask0; q0.y &= colmask[1] & rowmask0;
+      q0.z &= colmask[2] & rowmask0; q0.w &= colmask[3] & rowmask0;
+      q1.x &= colmask[0] & rowmask1; q1.y &= colmask[1] & rowmask1;
+      q1.z &= colmask[2] & rowmask1; q1.w &= colmask[3] & rowmask1;
+    }
+    uint32_t acc = sad4(src0.x, q0.x, 0);
+    acc = sad4(src0.y, q0.y, acc);
+    acc = sad4(src0.z, q0.z, acc);
+    acc = sad4(src0.w, q0.w, acc);
+    acc = sad4(src1.x, q1.x, acc);
+    acc = sad4(src1.y, q1.y, acc);
+    acc = sad4(src1.z, q1.z, acc);
+    acc = sad4(src1.w, q1.w, acc);
+    return sum8(acc);
+  };
+  int bx = 0, by = 0, boff = 0;
+  uint32_t bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0));
+#pragma unroll
+  for (int sh = 3; sh >= 0; sh--) {
+    const uint32_t s = sad_at(boff + gd_off * (1 << sh));
+    // Probe order is the group index and the incumbent keeps ties, so the
+    // key (sad << 4 | group + 1) against (best << 4) yields exactly the
+    // reference's winner as one unsigned minimum over the wave.
+    uint32_t key = (s << 4) | (uint32_t)(gq + 1);
+    key = min(key, dpp<0x128>(key));  // row_ror:8: the row's other group
+    key = min(key, (uint32_t)__builtin_amdgcn_update_dpp((int)key, (int)key, 0x142, 0xA, 0xF, false));
+    key = min(key, (uint32_t)__builtin_amdgcn_update_dpp((int)key, (int)key, 0x143, 0xC, 0xF, false));
+    const uint32_t kmin = (uint32_t)__builtin_amdgcn_readlane((int)key, 63);
+    if (kmin < (bsad << 4)) {
+      const int w = (int)(kmin & 15) - 1;
+      const int dx = (int)((kProbeDx1 >> (2 * w)) & 3) - 1;
+      const int dy = (int)((kProbeDy1 >> (2 * w)) & 3) - 1;
+      bsad = kmin >> 4;
+      bx += dx * (1 << sh);
+      by += dy * (1 << sh);
+      boff += (dy * G::lp + dx) * (1 << sh);
+    }
+  }
+  bestx = bx;
+  besty = by;
+  best_sad = bsad;
 }
 
 // Maps the launch index onto strips so that each of the 8 XCDs (which
@@ -159,7 +248,8 @@ __device__ __forceinline__ int xcd_band(int i, int total) {
 template <int M>
 __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
-  const StripGeom<M> g;
+  using G = StripGeom<M>;
+  const G g;
   // LDS: packed accumulators M*384 u32 | weight LUT 256 B | centre luma
   // 16 x 16M | centre U, V 8 x 8M each | per-wave windows.
   uint32_t* s_acc = reinterpret_cast<uint32_t*>(smem);
@@ -223,8 +313,8 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
   __syncthreads();
 
   const int gq = lane >> 3, kk = lane & 7;   // search: probe group, row pair
-  const int gdx = (gq == 0 || gq == 3 || gq == 5) ? -1 : ((gq == 1 || gq == 6) ? 0 : 1);
-  const int gdy = gq < 3 ? -1 : (gq < 5 ? 0 : 1);
+  // This lane's probe direction as a window offset per unit step.
+  const int gd_off = (((kProbeDy1 >> (2 * gq)) & 3) - 1) * G::lp + (((kProbeDx1 >> (2 * gq)) & 3) - 1);
   const int lr = lane >> 2, lq = lane & 3;   // blend: luma row, dword
   const int cr = lane >> 3, cc = lane & 7;   // blend: chroma pixel
 
@@ -239,59 +329,16 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
       if (bx >= a.w) break;
       const int bw = min(16, a.w - bx);
       const int cbw = (bw + 1) >> 1;
-      uint32_t colmask[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int left = bw - 4 * q;
-        colmask[q] = left >= 4 ? 0xFFFFFFFFu : (left <= 0 ? 0u : (0xFFFFFFFFu >> (32 - 8 * left)));
-      }
-      uint4 src[2];
-#pragma unroll
-      for (int rr = 0; rr < 2; rr++)
-        src[rr] = *reinterpret_cast<const uint4*>(s_cy + (2 * kk + rr) * 16 * M + 16 * m);
-
-      // SAD of probe (cx, cy) summed over this lane's 8-lane group.
-      auto probe = [&](int cx, int cy) -> uint32_t {
-        uint32_t acc = 0;
-#pragma unroll
-        for (int rr = 0; rr < 2; rr++) {
-          const int r = 2 * kk + rr;
-          if (r < bh) {
-            uint4 q = *reinterpret_cast<const uint4_u*>(win + (r + cy + 15) * g.lp + 16 * m + 16 + cx);
-            if (bw < 16) {
-              q.x &= colmask[0]; q.y &= colmask[1]; q.z &= colmask[2]; q.w &= colmask[3];
-            }
-            acc = sad4(src[rr].x, q.x, acc);
-            acc = sad4(src[rr].y, q.y, acc);
-            acc = sad4(src[rr].z, q.z, acc);
-            acc = sad4(src[rr].w, q.w, acc);
-          }
-        }
-        return sum8(acc);
-      };
-
-      int bestx = 0, besty = 0;
-      uint32_t best_sad = (uint32_t)__builtin_amdgcn_readfirstlane((int)probe(0, 0));
-#pragma unroll 1
-      for (int step = 8; step >= 1; step >>= 1) {
-        const int px = bestx + gdx * step, py = besty + gdy * step;
-        const bool valid = px >= -15 && px <= 15 && py >= -15 && py <= 15;
-        const uint32_t s = probe(valid ? px : 0, valid ? py : 0);
-        // Probe order is the group index; the incumbent keeps ties, so the
-        // key (sad << 4 | group + 1) against (best_sad << 4) picks exactly
-        // the reference's winner with one unsigned minimum.
-        uint32_t key = valid ? ((s << 4) | (uint32_t)(gq + 1)) : 0xFFFFFFFFu;
-        key = min(key, dpp<0x128>(key));  // row_ror:8 -> the other group of the row
-        uint32_t kmin = (uint32_t)__builtin_amdgcn_readlane((int)key, 0);
-        kmin = min(kmin, (uint32_t)__builtin_amdgcn_readlane((int)key, 16));
-        kmin = min(kmin, (uint32_t)__builtin_amdgcn_readlane((int)key, 32));
-        kmin = min(kmin, (uint32_t)__builtin_amdgcn_readlane((int)key, 48));
-        if (kmin < (best_sad << 4)) {
-          const int w8 = (int)(kmin & 15) - 1;
-          best_sad = kmin >> 4;
-          bestx += ((w8 == 0 || w8 == 3 || w8 == 5) ? -1 : ((w8 == 1 || w8 == 6) ? 0 : 1)) * step;
-          besty += (w8 < 3 ? -1 : (w8 < 5 ? 0 : 1)) * step;
-        }
+      const uint8_t* s_src = s_cy + 2 * kk * 16 * M + 16 * m;
+      const uint4 src0 = *reinterpret_cast<const uint4*>(s_src);
+      const uint4 src1 = *reinterpret_cast<const uint4*>(s_src + 16 * M);
+      const uint8_t* row0 = win + (2 * kk + 15) * G::lp + 16 * m + 16;
+      int bestx, besty;
+      uint32_t best_sad;
+      if (bw == 16 && bh == 16) {
+        descend<true, G>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad);
+      } else {
+        descend<false, G>(row0, src0, src1, gd_off, gq, kk, bw, bh, bestx, besty, best_sad);
       }
       if (a.mv_out && lane == 0) {
         int32_t* o = a.mv_out + 3 * (((size_t)fi * a.brows + brow) * a.bcols + (bx >> 4));
@@ -302,8 +349,10 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
       if (best_sad < (kSadPerPixelCutoff + 1) * (uint32_t)(bw * bh)) {
         uint32_t* acc = s_acc + 384 * m;
         if (lr < bh) {
-          const uint32_t ref4 = *reinterpret_cast<const uint32_u*>(
-              win + (lr + besty + 15) * g.lp + 16 * m + 16 + 4 * lq + bestx);
+          const uint8_t* rp4 = win + (lr + besty + 15) * g.lp + 16 * m + 16 + 4 * lq + bestx;
+          const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
+          const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
+          const uint32_t ref4 = funnel(rw[0], rw[1], rsh);
           const uint32_t ctr4 = *reinterpret_cast<const uint32_t*>(s_cy + lr * 16 * M + 16 * m + 4 * lq);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
@@ -372,7 +421,7 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
 }
 
 size_t arnr_fused_lds_bytes(int nw, int m) {
-  const int lp = 16 * m + 32, cp = 8 * m + 16;
+  const int lp = 16 * m + 40, cp = 8 * m + 16;  // StripGeom<m>::lp, ::cp
   const int wave_bytes = (kWinRows * lp + 2 * kCWinRows * cp + 15) & ~15;
   return (size_t)m * 384 * 4 + 256 + 256 * m + 128 * m + (size_t)nw * wave_bytes;
 }

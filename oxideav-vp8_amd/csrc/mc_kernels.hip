@@ -48,8 +48,12 @@ __device__ __forceinline__ uint32_t interp6(const Taps& k, int s0, int s1, int s
   uint32_t a = (uint32_t)s0 * (uint32_t)k.t[0] + (uint32_t)s1 * (uint32_t)k.t[1] +
                (uint32_t)s2 * (uint32_t)k.t[2] + (uint32_t)s3 * (uint32_t)k.t[3] +
                (uint32_t)s4 * (uint32_t)k.t[4] + (uint32_t)s5 * (uint32_t)k.t[5];
-  int v = ((int)(a + 64u)) >> 7;
-  return (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+  // clamp255(x >> 7) written as (clamp(x, 0, 32767)) >> 7 (equal for every
+  // x): the shift-then-saturate form gets selected to v_ashr_pk_u8_i32,
+  // whose result's upper half is then assumed zero by the code that packs
+  // the bytes, which corrupts bytes 2-3 once a tap sum reaches 2^16.
+  const int x = (int)(a + 64u);
+  return (uint32_t)min(max(x, 0), 32767) >> 7;
 }
 
 // Clamped gather of a ROWS x COLS block at (x0, y0), spread over the wave:
@@ -104,14 +108,21 @@ __device__ __forceinline__ void hpass(const uint8_t* halo, int hp, uint8_t* tmp,
   constexpr int UNITS = HW * UPR;
   for (int u = lane; u < UNITS; u += 64) {
     const int r = u / UPR, c0 = 4 * (u % UPR);
-    const uint8_t* s = halo + r * hp + c0;
+    // Three naturally aligned dwords hold the 9 support samples (unaligned
+    // LDS reads stall the LDS pipe).
+    const uint32_t* s32 = reinterpret_cast<const uint32_t*>(halo + r * hp + c0);
+    const uint32_t w0 = s32[0], w1 = s32[1], w2 = s32[2];
     uint32_t packed;
     if (fh.ident) {
-      packed = (uint32_t)s[2] | ((uint32_t)s[3] << 8) | ((uint32_t)s[4] << 16) | ((uint32_t)s[5] << 24);
+      packed = funnel(w0, w1, 2);
     } else {
       int v[9];
 #pragma unroll
-      for (int j = 0; j < 9; j++) v[j] = s[j];
+      for (int j = 0; j < 4; j++) {
+        v[j] = (w0 >> (8 * j)) & 255;
+        v[j + 4] = (w1 >> (8 * j)) & 255;
+      }
+      v[8] = w2 & 255;
       packed = 0;
 #pragma unroll
       for (int j = 0; j < 4; j++)
