@@ -90,10 +90,13 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
  * VP8G_OPT_XCD_REMAP: 1 (default) gives each XCD a contiguous band of
  *   macroblocks, 0 uses the launch order.
  * VP8G_OPT_ARNR_MBS: anchor macroblocks per ARNR workgroup, 1..8
- *   (0 = default 4; reduced automatically to fit 64 KiB of LDS). */
+ *   (0 = default 4; reduced automatically to fit 64 KiB of LDS).
+ * VP8G_OPT_BATCH_WGS_PER_CU: persistent workgroups per CU of a batch,
+ *   0..32 (0 = as many as fit). */
 #define VP8G_OPT_ARNR_WAVES 1
 #define VP8G_OPT_XCD_REMAP 2
 #define VP8G_OPT_ARNR_MBS 3
+#define VP8G_OPT_BATCH_WGS_PER_CU 4
 int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);
 
@@ -117,6 +120,41 @@ int vp8g_build_arnr_altref(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_fram
 int vp8g_build_arnr_altref_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames,
                                size_t center, uint8_t strength, uint8_t *out_y, uint8_t *out_u,
                                uint8_t *out_v, vp8g_err *err, int32_t *mv_out);
+
+/* ---- batched ARNR ----
+ * Many groups (clips of any resolution and length) in one persistent launch,
+ * captured once into a HIP graph and replayed by vp8g_arnr_batch_run.
+ * Replaces a loop of build_arnr_altref_threaded calls
+ * (src/arnr.rs:488-650), one per job, as the stream driver's flush_group
+ * (src/stream.rs:2806-2957) issues them; every job's output bytes equal its
+ * own single call's.  Each job is validated as vp8g_build_arnr_altref_dev
+ * validates its arguments, in job order; on failure *failed_job names the
+ * job and `err` carries its payload.  Planes, outputs and mv_out are device
+ * pointers fixed at creation: a run reads whatever the frame buffers hold
+ * when it executes.  Launch shape comes from the context's options at
+ * creation (VP8G_OPT_ARNR_MBS, VP8G_OPT_ARNR_WAVES — default 2 for a batch —
+ * and VP8G_OPT_BATCH_WGS_PER_CU).  At most 65535 jobs. */
+typedef struct {
+    const vp8g_i420 *frames;   /* host array of n_frames device-plane frames */
+    size_t n_frames, center;
+    uint8_t strength;
+    uint8_t *out_y, *out_u, *out_v;  /* device, packed as vp8g_build_arnr_altref */
+    int32_t *mv_out;                 /* device, may be NULL */
+} vp8g_arnr_job;
+
+typedef struct vp8g_arnr_batch vp8g_arnr_batch;
+
+int vp8g_arnr_batch_create(vp8g_ctx *ctx, const vp8g_arnr_job *jobs, size_t n_jobs,
+                           vp8g_arnr_batch **out, size_t *failed_job, vp8g_err *err);
+/* Enqueue one replay on the context's current stream; returns without
+ * waiting.  Replays on one stream are serialised; a batch must not run on
+ * two streams at once. */
+int vp8g_arnr_batch_run(vp8g_arnr_batch *batch);
+/* Launch shape chosen at creation: MBs per workgroup, waves per workgroup,
+ * persistent workgroups, work units (strips). Any pointer may be NULL. */
+int vp8g_arnr_batch_shape(const vp8g_arnr_batch *batch, int *mbs, int *waves, int *workgroups,
+                          size_t *units);
+void vp8g_arnr_batch_destroy(vp8g_arnr_batch *batch);
 
 /* ---- motion compensation ----
  * Replaces predict_inter_mb (src/motion_comp.rs:1986-2078) over a batch:
@@ -156,6 +194,19 @@ int vp8g_quarter_pixel_refine_luma_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref
                                          const uint32_t *mb_xy, const uint8_t *src,
                                          const vp8g_mv *centers, size_t n,
                                          vp8g_search_result *out);
+
+/* Replaces the encoder's half-pel probe set, fetch_window + HalfPlanes::new
+ * + HalfPlanes::sad (src/encoder/inter_search.rs:266-346, used by
+ * sub_pixel_refine :509-536), over a batch: block i's whole-pel position is
+ * (xy[2i], xy[2i+1]) (MB origin + whole MV, any value, clamped reads), its
+ * source src[256*i..]; `taps` is the caller's filters[4]; sad[8*i + k]
+ * holds HalfPlanes::sad for (dx, dy) in (dy, dx) raster order without
+ * (0, 0): (-1,-1) (0,-1) (1,-1) (-1,0) (1,0) (-1,1) (0,1) (1,1).  With
+ * `diagonals` = 0 the four diagonal entries are 0xFFFFFFFF (not computed,
+ * as HalfPlanes::new(.., false)). */
+int vp8g_half_plane_sads_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const int32_t *xy,
+                               const uint8_t *src, size_t n, const int32_t taps[6],
+                               int diagonals, uint32_t *sad);
 
 #ifdef __cplusplus
 }

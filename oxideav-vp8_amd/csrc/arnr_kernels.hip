@@ -76,7 +76,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // luma rows by-15..by+30, cols bx0-16..bx0+16M+15; chroma rows
 // cby-7..cby+14, cols cbx0-8..cbx0+8M+7 (U then V).
 template <class G>
-__device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrArgs& a, const G g,
+__device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrGroup& a, const G g,
                               int bx0, int by, int lane) {
   const int cbx0 = bx0 >> 1, cby = by >> 1;
   const ptrdiff_t v_minus_u = f.v - f.u;  // plane choice as integer arithmetic
@@ -245,9 +245,14 @@ __device__ __forceinline__ int xcd_band(int i, int total) {
   return (i & 7) * (total >> 3) + (i >> 3);
 }
 
-template <int M>
-__global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
-  extern __shared__ __align__(16) uint8_t smem[];
+// One strip of group `a`: M anchor MBs of MB row `brow`, strip column
+// `scol`.  `frame_at(i)` returns frame i of the group, `wt4` the weight LUT.
+// Ends with the strip resolved into the output planes; the caller
+// synchronises the workgroup before reusing LDS.
+template <int M, class FrameAt>
+__device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t* wt4,
+                                              const FrameAt& frame_at, int brow, int scol,
+                                              uint8_t* smem) {
   using G = StripGeom<M>;
   const G g;
   // LDS: packed accumulators M*384 u32 | weight LUT 256 B | centre luma
@@ -264,20 +269,17 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
 
-  const int strip = a.xcd_remap ? xcd_band(blockIdx.x, a.strips_per_row * a.brows) : (int)blockIdx.x;
-  const int brow = strip / a.strips_per_row;
-  const int scol = strip - brow * a.strips_per_row;
   const int bx0 = scol * 16 * M, by = brow * 16;
   const int cbx0 = bx0 >> 1, cby = by >> 1;
   const int bh = min(16, a.h - by), cbh = (bh + 1) >> 1;
-  const FrameDesc C = a.frames[a.center];
+  const FrameDesc C = frame_at(a.center);
   const int nnb = a.filter_on ? a.n - 1 : 0;
   uint8_t* win = s_win + wave * g.wave_bytes;
   uint8_t* cwin = win + kWinRows * g.lp;
 
   // The first neighbour's windows are staged before anything else.
   int k = wave;
-  if (k < nnb) stage_windows(win, a.frames[k < a.center ? k : k + 1], a, g, bx0, by, lane);
+  if (k < nnb) stage_windows(win, frame_at(k < a.center ? k : k + 1), a, g, bx0, by, lane);
 
   // Centre strip (zero outside the frame), accumulators, LUT.
   for (int i = tid; i < 64 * M; i += blockDim.x) {  // luma: 16 rows x 4M dwords
@@ -309,7 +311,7 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
     reinterpret_cast<uint32_t*>(pl ? s_cv : s_cu)[t] = v;
   }
   for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
-  for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = a.wt4[i];
+  for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = wt4[i];
   __syncthreads();
 
   const int gq = lane >> 3, kk = lane & 7;   // search: probe group, row pair
@@ -320,7 +322,7 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
 
   for (; k < nnb; k += nw) {
     const int fi = k < a.center ? k : k + 1;
-    if (k != wave) stage_windows(win, a.frames[fi], a, g, bx0, by, lane);
+    if (k != wave) stage_windows(win, frame_at(fi), a, g, bx0, by, lane);
     wave_lds_sync();
 
 #pragma unroll 1
@@ -420,6 +422,103 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
   }
 }
 
+template <int M>
+__global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int total = a.g.strips_per_row * a.g.brows;
+  const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
+  const int brow = strip / a.g.strips_per_row;
+  const int scol = strip - brow * a.g.strips_per_row;
+  process_strip<M>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, smem);
+}
+
+// Descriptor fields read from global memory and made wave-uniform, so that
+// they live in scalar registers and drive uniform branches.
+__device__ __forceinline__ int uload(const int* p) { return __builtin_amdgcn_readfirstlane(*p); }
+__device__ __forceinline__ uint32_t uload(const uint32_t* p) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)*p);
+}
+template <class T>
+__device__ __forceinline__ T* uload(T* const* p) {
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)v.x);
+  const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)v.y);
+  return reinterpret_cast<T*>(lo | (hi << 32));
+}
+
+__device__ __forceinline__ FrameDesc load_frame(const FrameDesc* p) {
+  FrameDesc f;
+  f.y = uload(&p->y);
+  f.u = uload(&p->u);
+  f.v = uload(&p->v);
+  f.y_stride = uload(&p->y_stride);
+  f.uv_stride = uload(&p->uv_stride);
+  return f;
+}
+
+__device__ __forceinline__ ArnrGroup load_group(const ArnrGroup* p) {
+  ArnrGroup g;
+  g.out_y = uload(&p->out_y);
+  g.out_u = uload(&p->out_u);
+  g.out_v = uload(&p->out_v);
+  g.mv_out = uload(&p->mv_out);
+  g.w = uload(&p->w);
+  g.h = uload(&p->h);
+  g.cw = uload(&p->cw);
+  g.ch = uload(&p->ch);
+  g.bcols = uload(&p->bcols);
+  g.brows = uload(&p->brows);
+  g.n = uload(&p->n);
+  g.center = uload(&p->center);
+  g.filter_on = uload(&p->filter_on);
+  g.aligned4 = uload(&p->aligned4);
+  g.aligned16 = uload(&p->aligned16);
+  g.aligned4c = uload(&p->aligned4c);
+  g.strips_per_row = uload(&p->strips_per_row);
+  return g;
+}
+
+// Persistent batch kernel: every workgroup loops taking units (one strip of
+// one group) until all chunks are drained; a unit is processed exactly as
+// the single-group kernel processes that strip.
+template <int M>
+__global__ void __launch_bounds__(1024) arnr_batch_kernel(const ArnrBatchArgs b) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  __shared__ int s_unit;
+  const int first = blockIdx.x % kBatchChunks;
+  int c = 0;
+  while (c < kBatchChunks) {
+    const int chunk = (first + c) % kBatchChunks;
+    const int lo = (int)((long long)b.total_units * chunk / kBatchChunks);
+    const int hi = (int)((long long)b.total_units * (chunk + 1) / kBatchChunks);
+    if (threadIdx.x == 0) {
+      const int t = (int)atomicAdd(&b.ticket[chunk], 1u);
+      s_unit = t < hi - lo ? lo + t : -1;
+    }
+    __syncthreads();
+    const int unit = __builtin_amdgcn_readfirstlane(s_unit);
+    if (unit < 0) {
+      c++;  // chunk drained: move on (every workgroup ends after kBatchChunks misses)
+    } else {
+      const int j = __builtin_amdgcn_readfirstlane((int)b.unit_job[unit]);
+      const ArnrBatchJob* job = b.jobs + j;
+      const ArnrGroup g = load_group(&job->g);
+      const int fb = uload(&job->frame_base);
+      const int local = unit - uload(&job->unit_base);
+      const int brow = local / g.strips_per_row;
+      const int scol = local - brow * g.strips_per_row;
+      process_strip<M>(g, job->wt4, [&](int i) { return load_frame(b.frames + fb + i); }, brow, scol,
+                       smem);
+    }
+    __syncthreads();  // LDS and s_unit are free for the next unit
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 size_t arnr_fused_lds_bytes(int nw, int m) {
   const int lp = 16 * m + 40, cp = 8 * m + 16;  // StripGeom<m>::lp, ::cp
   const int wave_bytes = (kWinRows * lp + 2 * kCWinRows * cp + 15) & ~15;
@@ -429,7 +528,7 @@ size_t arnr_fused_lds_bytes(int nw, int m) {
 int arnr_supported_mbs(int m) { return m >= 8 ? 8 : (m >= 4 ? 4 : (m >= 2 ? 2 : 1)); }
 
 hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream) {
-  const int blocks = a.strips_per_row * a.brows;
+  const int blocks = a.g.strips_per_row * a.g.brows;
   const size_t lds = arnr_fused_lds_bytes(nw, a.mbs_per_wg);
   switch (a.mbs_per_wg) {
     case 1: hipLaunchKernelGGL(arnr_fused_kernel<1>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
@@ -438,6 +537,43 @@ hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream) {
     case 8: hipLaunchKernelGGL(arnr_fused_kernel<8>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+template <int M>
+static int batch_occupancy(int nw) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, arnr_batch_kernel<M>, 64 * nw,
+                                                   arnr_fused_lds_bytes(nw, M)) != hipSuccess)
+    return 0;
+  return per_cu;
+}
+
+int arnr_batch_workgroups_per_cu(int nw, int m) {
+  switch (m) {
+    case 1: return batch_occupancy<1>(nw);
+    case 2: return batch_occupancy<2>(nw);
+    case 4: return batch_occupancy<4>(nw);
+    case 8: return batch_occupancy<8>(nw);
+    default: return 0;
+  }
+}
+
+hipError_t launch_arnr_batch(const ArnrBatchArgs& b, int nw, int workgroups, hipStream_t stream) {
+  const size_t lds = arnr_fused_lds_bytes(nw, b.mbs_per_wg);
+  switch (b.mbs_per_wg) {
+    case 1: hipLaunchKernelGGL(arnr_batch_kernel<1>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
+    case 2: hipLaunchKernelGGL(arnr_batch_kernel<2>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
+    case 4: hipLaunchKernelGGL(arnr_batch_kernel<4>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
+    case 8: hipLaunchKernelGGL(arnr_batch_kernel<8>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
   return hipGetLastError();
 }
 

@@ -17,9 +17,20 @@ struct vp8g_ctx {
   int arnr_waves = 0;
   int xcd_remap = 1;
   int arnr_mbs = 0;
+  int batch_wgs_per_cu = 0;
+  int cus = 0;
   // Device arena for the host-pointer entry points (grown, never shrunk).
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
+};
+
+struct vp8g_arnr_batch {
+  vp8g_ctx* ctx = nullptr;
+  uint8_t* dev = nullptr;  // jobs | frames | unit_job | tickets
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int mbs = 0, waves = 0, workgroups = 0;
+  size_t units = 0;
 };
 
 namespace {
@@ -80,46 +91,73 @@ void pick_arnr_shape(const vp8g_ctx* c, int nnb, int* nw_out, int* m_out) {
   *m_out = m;
 }
 
+// Batch launch shape: 2 waves by default (every group length 3..11 leaves
+// an even number of neighbours), same LDS limit.
+void pick_batch_shape(const vp8g_ctx* c, int* nw_out, int* m_out) {
+  int nw = c->arnr_waves > 0 ? c->arnr_waves : 2;
+  int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : 4);
+  while (m > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) m = arnr_supported_mbs(m - 1);
+  while (nw > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) nw--;
+  *nw_out = nw;
+  *m_out = m;
+}
+
+void batch_free(vp8g_arnr_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->dev) (void)hipFree(b->dev);
+  delete b;
+}
+
+// Fill the kernel descriptor of one validated group: frame descriptors
+// into fd[0..n), the group geometry, the weight LUT.
+void fill_group(const vp8g_i420* frames, size_t n, size_t center, uint8_t strength, uint8_t* oy,
+                uint8_t* ou, uint8_t* ov, int32_t* mv_out, FrameDesc* fd, ArnrGroup* g,
+                uint32_t wt4[64]) {
+  memset(g, 0, sizeof(*g));
+  const vp8g_i420& ctr = frames[center];
+  bool aligned4 = true, aligned16 = true, aligned4c = true;
+  for (size_t i = 0; i < n; i++) {
+    const vp8g_i420& f = frames[i];
+    fd[i] = FrameDesc{f.y, f.u, f.v, (uint32_t)f.y_stride, (uint32_t)f.uv_stride};
+    if (((uintptr_t)f.y & 3) || (f.y_stride & 3)) aligned4 = false;
+    if (((uintptr_t)f.y & 15) || (f.y_stride & 15)) aligned16 = false;
+    if (((uintptr_t)f.u & 3) || ((uintptr_t)f.v & 3) || (f.uv_stride & 3)) aligned4c = false;
+  }
+  g->out_y = oy;
+  g->out_u = ou;
+  g->out_v = ov;
+  g->mv_out = mv_out;
+  g->w = (int)ctr.width;
+  g->h = (int)ctr.height;
+  g->cw = (g->w + 1) / 2;
+  g->ch = (g->h + 1) / 2;
+  g->bcols = (g->w + 15) / 16;
+  g->brows = (g->h + 15) / 16;
+  g->n = (int)n;
+  g->center = (int)center;
+  g->filter_on = (strength != 0 && n >= 2) ? 1 : 0;
+  g->aligned4 = aligned4 ? 1 : 0;
+  g->aligned16 = aligned16 ? 1 : 0;
+  g->aligned4c = aligned4c ? 1 : 0;
+  // weight_table(8 << min(strength, 6)), src/arnr.rs:307-323, 533-536.
+  const uint32_t s = 8u << (strength < kMaxStrength ? strength : kMaxStrength);
+  uint8_t wt[256];
+  for (uint32_t d = 0; d < 256; d++) wt[d] = (uint8_t)((kWMax * s) / (s + d * d));
+  memcpy(wt4, wt, sizeof(wt));
+}
+
 // Enqueue the fused ARNR kernel over device-resident frames.
 int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, uint8_t strength,
                  uint8_t* oy, uint8_t* ou, uint8_t* ov, int32_t* mv_out) {
   ArnrArgs a;
   memset(&a, 0, sizeof(a));
-  const vp8g_i420& ctr = frames[center];
-  bool aligned4 = true, aligned16 = true, aligned4c = true;
-  for (size_t i = 0; i < n; i++) {
-    const vp8g_i420& f = frames[i];
-    a.frames[i] = FrameDesc{f.y, f.u, f.v, (uint32_t)f.y_stride, (uint32_t)f.uv_stride};
-    if (((uintptr_t)f.y & 3) || (f.y_stride & 3)) aligned4 = false;
-    if (((uintptr_t)f.y & 15) || (f.y_stride & 15)) aligned16 = false;
-    if (((uintptr_t)f.u & 3) || ((uintptr_t)f.v & 3) || (f.uv_stride & 3)) aligned4c = false;
-  }
-  a.out_y = oy;
-  a.out_u = ou;
-  a.out_v = ov;
-  a.mv_out = mv_out;
-  a.w = (int)ctr.width;
-  a.h = (int)ctr.height;
-  a.cw = (a.w + 1) / 2;
-  a.ch = (a.h + 1) / 2;
-  a.bcols = (a.w + 15) / 16;
-  a.brows = (a.h + 15) / 16;
-  a.n = (int)n;
-  a.center = (int)center;
-  a.filter_on = (strength != 0 && n >= 2) ? 1 : 0;
-  a.aligned4 = aligned4 ? 1 : 0;
-  a.aligned16 = aligned16 ? 1 : 0;
-  a.aligned4c = aligned4c ? 1 : 0;
+  fill_group(frames, n, center, strength, oy, ou, ov, mv_out, a.frames, &a.g, a.wt4);
   a.xcd_remap = c->xcd_remap;
-  // weight_table(8 << min(strength, 6)), src/arnr.rs:307-323, 533-536.
-  const uint32_t s = 8u << (strength < kMaxStrength ? strength : kMaxStrength);
-  uint8_t wt[256];
-  for (uint32_t d = 0; d < 256; d++) wt[d] = (uint8_t)((kWMax * s) / (s + d * d));
-  memcpy(a.wt4, wt, sizeof(wt));
   int nw, m;
-  pick_arnr_shape(c, a.filter_on ? (int)n - 1 : 1, &nw, &m);
+  pick_arnr_shape(c, a.g.filter_on ? (int)n - 1 : 1, &nw, &m);
   a.mbs_per_wg = m;
-  a.strips_per_row = (a.bcols + m - 1) / m;
+  a.g.strips_per_row = (a.g.bcols + m - 1) / m;
   return hip_ok(launch_arnr_fused(a, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
@@ -229,6 +267,7 @@ int vp8g_ctx_create(int device, vp8g_ctx** out) {
   if (!hip_ok(hipSetDevice(device))) return VP8G_E_DEVICE;
   vp8g_ctx* c = new vp8g_ctx();
   c->device = device;
+  c->cus = prop.multiProcessorCount;
   if (!hip_ok(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking))) {
     delete c;
     return VP8G_E_DEVICE;
@@ -274,6 +313,10 @@ int vp8g_ctx_set_option(vp8g_ctx* c, int option, int value) {
     case VP8G_OPT_ARNR_MBS:
       if (value < 0 || value > 8) return VP8G_E_ARG;
       c->arnr_mbs = value;
+      return VP8G_OK;
+    case VP8G_OPT_BATCH_WGS_PER_CU:
+      if (value < 0 || value > 32) return VP8G_E_ARG;
+      c->batch_wgs_per_cu = value;
       return VP8G_OK;
     default:
       return VP8G_E_ARG;
@@ -329,6 +372,120 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   if (st == VP8G_OK) st = copy_out(c, ou, du, cw * ch);
   if (st == VP8G_OK) st = copy_out(c, ov, dv, cw * ch);
   return finish(c, st);
+}
+
+int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
+                           vp8g_arnr_batch** out, size_t* failed_job, vp8g_err* err) {
+  if (!c || !out || (!jobs && nj)) return VP8G_E_ARG;
+  *out = nullptr;
+  if (nj == 0 || nj > 0xFFFF) return VP8G_E_ARG;
+  size_t total_frames = 0;
+  for (size_t j = 0; j < nj; j++) {
+    const vp8g_arnr_job& jb = jobs[j];
+    int st = validate_arnr(jb.frames, jb.n_frames, jb.center, err);
+    if (st == VP8G_OK && (!jb.out_y || !jb.out_u || !jb.out_v)) st = VP8G_E_ARG;
+    if (st != VP8G_OK) {
+      if (failed_job) *failed_job = j;
+      return st;
+    }
+    total_frames += jb.n_frames;
+  }
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  int nw, m;
+  pick_batch_shape(c, &nw, &m);
+  std::vector<ArnrBatchJob> hj(nj);
+  std::vector<FrameDesc> hf(total_frames);
+  std::vector<uint16_t> hu;
+  size_t fb = 0, units = 0;
+  for (size_t j = 0; j < nj; j++) {
+    const vp8g_arnr_job& jb = jobs[j];
+    ArnrBatchJob& d = hj[j];
+    memset(&d, 0, sizeof(d));
+    fill_group(jb.frames, jb.n_frames, jb.center, jb.strength, jb.out_y, jb.out_u, jb.out_v,
+               jb.mv_out, hf.data() + fb, &d.g, d.wt4);
+    d.g.strips_per_row = (d.g.bcols + m - 1) / m;
+    d.frame_base = (int)fb;
+    d.unit_base = (int)units;
+    const size_t ju = (size_t)d.g.strips_per_row * d.g.brows;
+    hu.insert(hu.end(), ju, (uint16_t)j);
+    units += ju;
+    fb += jb.n_frames;
+  }
+  if (units > 0x7FFFFFFF) return VP8G_E_ARG;
+  int per_cu = arnr_batch_workgroups_per_cu(nw, m);
+  if (per_cu < 1) return VP8G_E_DEVICE;
+  if (c->batch_wgs_per_cu > 0 && c->batch_wgs_per_cu < per_cu) per_cu = c->batch_wgs_per_cu;
+  size_t wgs = (size_t)per_cu * (size_t)c->cus;
+  if (wgs > units) wgs = units;
+
+  vp8g_arnr_batch* b = new vp8g_arnr_batch();
+  b->ctx = c;
+  b->mbs = m;
+  b->waves = nw;
+  b->workgroups = (int)wgs;
+  b->units = units;
+  const size_t o_jobs = 0, o_frames = align_up(sizeof(ArnrBatchJob) * nj, 256);
+  const size_t o_units = o_frames + align_up(sizeof(FrameDesc) * total_frames, 256);
+  const size_t o_ticket = o_units + align_up(sizeof(uint16_t) * units, 256);
+  const size_t bytes = o_ticket + 256;
+  hipStream_t cs = nullptr;
+  bool ok = hip_ok(hipMalloc(&b->dev, bytes)) &&
+            hip_ok(hipMemcpyAsync(b->dev + o_jobs, hj.data(), sizeof(ArnrBatchJob) * nj,
+                                  hipMemcpyHostToDevice, c->stream)) &&
+            hip_ok(hipMemcpyAsync(b->dev + o_frames, hf.data(), sizeof(FrameDesc) * total_frames,
+                                  hipMemcpyHostToDevice, c->stream)) &&
+            hip_ok(hipMemcpyAsync(b->dev + o_units, hu.data(), sizeof(uint16_t) * units,
+                                  hipMemcpyHostToDevice, c->stream)) &&
+            hip_ok(hipStreamSynchronize(c->stream));
+  if (ok) {
+    ArnrBatchArgs a;
+    a.jobs = reinterpret_cast<const ArnrBatchJob*>(b->dev + o_jobs);
+    a.frames = reinterpret_cast<const FrameDesc*>(b->dev + o_frames);
+    a.unit_job = reinterpret_cast<const uint16_t*>(b->dev + o_units);
+    a.total_units = (int)units;
+    a.ticket = reinterpret_cast<uint32_t*>(b->dev + o_ticket);
+    a.mbs_per_wg = m;
+    // Captured on a private stream: the tickets are cleared by a kernel
+    // (not a memset node), then the persistent kernel runs.
+    ok = hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) &&
+         hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    if (ok) {
+      const bool launched = hip_ok(launch_zero_u32(a.ticket, kBatchChunks, cs)) &&
+                            hip_ok(launch_arnr_batch(a, nw, (int)wgs, cs));
+      ok = hip_ok(hipStreamEndCapture(cs, &b->graph)) && launched;
+    }
+    ok = ok && hip_ok(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  if (cs) (void)hipStreamDestroy(cs);
+  if (!ok) {
+    batch_free(b);
+    return VP8G_E_DEVICE;
+  }
+  *out = b;
+  return VP8G_OK;
+}
+
+int vp8g_arnr_batch_run(vp8g_arnr_batch* b) {
+  if (!b) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(b->ctx->device))) return VP8G_E_DEVICE;
+  return hip_ok(hipGraphLaunch(b->exec, b->ctx->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int vp8g_arnr_batch_shape(const vp8g_arnr_batch* b, int* mbs, int* waves, int* workgroups,
+                          size_t* units) {
+  if (!b) return VP8G_E_ARG;
+  if (mbs) *mbs = b->mbs;
+  if (waves) *waves = b->waves;
+  if (workgroups) *workgroups = b->workgroups;
+  if (units) *units = b->units;
+  return VP8G_OK;
+}
+
+void vp8g_arnr_batch_destroy(vp8g_arnr_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_free(b);
 }
 
 int vp8g_predict_inter_mb_batch_dev(vp8g_ctx* c, const vp8g_ref_planes* ref,
@@ -475,6 +632,43 @@ static int refine_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* m
   a.step = step;
   st = hip_ok(launch_refine_ladder(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
   if (st == VP8G_OK) st = copy_out(c, out, dout, 8 * n);
+  return finish(c, st);
+}
+
+int vp8g_half_plane_sads_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const int32_t* xy,
+                               const uint8_t* src, size_t n, const int32_t taps[6], int diagonals,
+                               uint32_t* sad) {
+  if (!c || !taps) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!xy || !src || !sad || n > 0x7FFFFFF) return VP8G_E_ARG;
+  for (size_t i = 0; i < 2 * n; i++)
+    if (xy[i] < -(1 << 24) || xy[i] > (1 << 24)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const size_t ps = plane_span(ref->stride, ref->width, ref->height);
+  const size_t o_xy = align_up(ps, 256), o_src = o_xy + align_up(8 * n, 256);
+  const size_t o_out = o_src + align_up(256 * n, 256);
+  st = ensure_arena(c, o_out + 32 * n);
+  if (st != VP8G_OK) return st;
+  uint8_t* b = c->arena;
+  st = copy_in(c, b, ref->plane, ps);
+  if (st == VP8G_OK) st = copy_in(c, b + o_xy, xy, 8 * n);
+  if (st == VP8G_OK) st = copy_in(c, b + o_src, src, 256 * n);
+  if (st != VP8G_OK) return finish(c, st);
+  HalfPlaneArgs a;
+  a.plane = b;
+  a.stride = (uint32_t)ref->stride;
+  a.w = (int)ref->width;
+  a.h = (int)ref->height;
+  a.xy = reinterpret_cast<const int32_t*>(b + o_xy);
+  a.src = b + o_src;
+  a.sad = reinterpret_cast<uint32_t*>(b + o_out);
+  a.n = (int)n;
+  a.diagonals = diagonals ? 1 : 0;
+  memcpy(a.taps, taps, sizeof(a.taps));
+  st = hip_ok(launch_half_plane_sads(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  if (st == VP8G_OK) st = copy_out(c, sad, b + o_out, 32 * n);
   return finish(c, st);
 }
 

@@ -316,6 +316,80 @@ hipError_t launch_mb_luma_sad_at_mv(const SadArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// HalfPlanes (src/encoder/inter_search.rs:256-346): the 22x22 window at
+// (x-3, y-3), edge-clamped (fetch_window :266-287); the horizontal plane
+// h[r][c] = interp(win[r][c..c+5]) over all 22 rows (or rows 3..18 without
+// diagonals), v[r][c] = interp(win[r..r+5][c+3]) 17x16, the diagonal plane
+// d[r][c] = interp(h[r..r+5][c]) 17x17; the SAD for half-pel offset
+// (dx, dy) reads plane position (dx > 0, dy > 0) (HalfPlanes::sad :334-345).
+// One wave per block; outputs the 8 SADs in (dy, dx) raster order, the
+// diagonal ones as 0xFFFFFFFF when `diagonals` is off.
+constexpr int kHpLds = 22 * 24 + 22 * 17 + 17 * 16 + 17 * 17;
+
+__global__ void __launch_bounds__(256) half_plane_sads_kernel(const HalfPlaneArgs a) {
+  __shared__ __align__(16) uint8_t smem[4 * ((kHpLds + 15) & ~15)];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + wave;
+  if (i >= a.n) return;
+  uint8_t* win = smem + wave * ((kHpLds + 15) & ~15);
+  uint8_t* hp = win + 22 * 24;
+  uint8_t* vp = hp + 22 * 17;
+  uint8_t* dp = vp + 17 * 16;
+  const int x = a.xy[2 * i], y = a.xy[2 * i + 1];
+  Gather<22, 22> gw;
+  gw.load(a.plane, a.stride, a.w, a.h, x - 3, y - 3, lane);
+  const uint32_t s4 = *reinterpret_cast<const uint32_t*>(a.src + (size_t)i * 256 + 4 * lane);
+  gw.store(win, 24, lane);
+  Taps t;
+#pragma unroll
+  for (int k = 0; k < 6; k++) t.t[k] = a.taps[k];
+  t.ident = false;
+  wave_lds_sync();
+  const int r0 = a.diagonals ? 0 : 3, r1 = a.diagonals ? 22 : 19;
+  for (int o = lane; o < (r1 - r0) * 17; o += 64) {
+    const int r = r0 + o / 17, c = o % 17;
+    const uint8_t* s = win + r * 24 + c;
+    hp[r * 17 + c] = (uint8_t)interp6(t, s[0], s[1], s[2], s[3], s[4], s[5]);
+  }
+  for (int o = lane; o < 17 * 16; o += 64) {
+    const int r = o >> 4, c = o & 15;
+    const uint8_t* s = win + r * 24 + c + 3;
+    vp[o] = (uint8_t)interp6(t, s[0], s[24], s[48], s[72], s[96], s[120]);
+  }
+  wave_lds_sync();
+  if (a.diagonals) {
+    for (int o = lane; o < 17 * 17; o += 64) {
+      const uint8_t* s = hp + o;
+      dp[o] = (uint8_t)interp6(t, s[0], s[17], s[34], s[51], s[68], s[85]);
+    }
+  }
+  wave_lds_sync();
+  const int pr = lane >> 2, pc = 4 * (lane & 3);
+  int k = 0;
+#pragma unroll
+  for (int dy = -1; dy <= 1; dy++) {
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+      if (dx == 0 && dy == 0) continue;
+      const int cx = dx > 0, cy = dy > 0;
+      const uint8_t* p;
+      if (dy == 0) p = hp + (3 + pr) * 17 + cx + pc;
+      else if (dx == 0) p = vp + (cy + pr) * 16 + pc;
+      else p = dp + (cy + pr) * 17 + cx + pc;
+      const uint32_t pred = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+      const uint32_t tot = wave_sum(sad4(s4, pred, 0));
+      if (lane == 0) a.sad[8 * i + k] = (dx != 0 && dy != 0 && !a.diagonals) ? 0xFFFFFFFFu : tot;
+      k++;
+    }
+  }
+}
+
+hipError_t launch_half_plane_sads(const HalfPlaneArgs& a, hipStream_t stream) {
+  if (a.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(half_plane_sads_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_refine_ladder(const RefineArgs& a, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(refine_ladder_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);

@@ -21,8 +21,8 @@ struct FrameDesc {
   uint32_t uv_stride;
 };
 
-struct ArnrArgs {
-  FrameDesc frames[kMaxFrames];
+// One ARNR group (lookahead window) as the kernels see it.
+struct ArnrGroup {
   uint8_t* out_y;   // packed w*h
   uint8_t* out_u;   // packed cw*ch
   uint8_t* out_v;
@@ -33,11 +33,44 @@ struct ArnrArgs {
   int aligned4;     // every luma plane pointer and stride is a multiple of 4
   int aligned16;    // ... of 16
   int aligned4c;    // every chroma plane pointer and stride is a multiple of 4
+  int strips_per_row;
+};
+
+// Single-group launch: frames travel in the kernel arguments.
+struct ArnrArgs {
+  FrameDesc frames[kMaxFrames];
+  ArnrGroup g;
+  uint32_t wt4[64]; // blend weight per |ref - ctr| as 256 bytes (values <= 16)
   int xcd_remap;    // map workgroups to contiguous MB bands per XCD
   int mbs_per_wg;   // M: anchor MBs per workgroup (one strip of an MB row)
-  int strips_per_row;
-  uint32_t wt4[64]; // blend weight per |ref - ctr| as 256 bytes (values <= 16)
 };
+
+// Batched launch: many groups of any size and length in one persistent
+// kernel; workgroups take (group, strip) units from an atomic ticket.
+// Units are split into kBatchChunks contiguous chunks with one ticket each;
+// workgroup i starts on chunk i % kBatchChunks (its XCD's share) and then
+// drains the others.
+constexpr int kBatchChunks = 8;
+
+struct ArnrBatchJob {
+  ArnrGroup g;
+  int frame_base;   // index of the group's first frame in ArnrBatchArgs::frames
+  int unit_base;    // first unit (strip) of this group
+  uint32_t wt4[64];
+};
+
+struct ArnrBatchArgs {
+  const ArnrBatchJob* jobs;
+  const FrameDesc* frames;
+  const uint16_t* unit_job;  // job index of every unit
+  int total_units;
+  uint32_t* ticket;          // kBatchChunks counters, zeroed before each launch
+  int mbs_per_wg;
+};
+
+hipError_t launch_arnr_batch(const ArnrBatchArgs& a, int nw, int workgroups, hipStream_t stream);
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
+int arnr_batch_workgroups_per_cu(int nw, int mbs_per_wg);
 
 constexpr size_t kMaxLdsPerWorkgroup = 65536;
 size_t arnr_fused_lds_bytes(int nw, int mbs_per_wg);
@@ -74,6 +107,20 @@ struct SadArgs {
   uint32_t* sad;           // n
   int n;
 };
+
+struct HalfPlaneArgs {
+  const uint8_t* plane;
+  uint32_t stride;
+  int w, h;
+  const int32_t* xy;       // n x {x, y}: whole-pel position of the 16x16 block
+  const uint8_t* src;      // n x 256
+  uint32_t* sad;           // n x 8, (dy, dx) raster order without (0, 0)
+  int n;
+  int diagonals;
+  int32_t taps[6];
+};
+
+hipError_t launch_half_plane_sads(const HalfPlaneArgs& a, hipStream_t stream);
 
 struct RefineArgs {
   const uint8_t* plane;

@@ -39,6 +39,7 @@ MAX_FRAMES = 64
 OPT_ARNR_WAVES = 1
 OPT_XCD_REMAP = 2
 OPT_ARNR_MBS = 3
+OPT_BATCH_WGS_PER_CU = 4
 
 # RFC 6386 section 18.3 tap tables, src/motion_comp.rs:294-303, 328-337.
 SIXTAP_FILTERS = np.array(
@@ -78,6 +79,12 @@ class _LumaRef(C.Structure):
                 ("height", C.c_size_t)]
 
 
+class _ArnrJob(C.Structure):
+    _fields_ = [("frames", C.POINTER(_I420)), ("n_frames", C.c_size_t), ("center", C.c_size_t),
+                ("strength", C.c_uint8), ("out_y", C.c_void_p), ("out_u", C.c_void_p),
+                ("out_v", C.c_void_p), ("mv_out", C.c_void_p)]
+
+
 EXPORTS = (
     "vp8g_ctx_create", "vp8g_ctx_destroy", "vp8g_ctx_set_stream", "vp8g_ctx_stream",
     "vp8g_ctx_synchronize", "vp8g_ctx_set_option", "vp8g_status_str",
@@ -85,6 +92,9 @@ EXPORTS = (
     "vp8g_predict_inter_mb_batch", "vp8g_predict_inter_mb_batch_dev",
     "vp8g_mb_luma_sad_at_mv_batch", "vp8g_mb_luma_sad_at_mv_batch_dev",
     "vp8g_half_pixel_refine_luma_batch", "vp8g_quarter_pixel_refine_luma_batch",
+    "vp8g_half_plane_sads_batch",
+    "vp8g_arnr_batch_create", "vp8g_arnr_batch_run", "vp8g_arnr_batch_shape",
+    "vp8g_arnr_batch_destroy",
 )
 
 
@@ -111,6 +121,13 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_mb_luma_sad_at_mv_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
         "vp8g_half_pixel_refine_luma_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
         "vp8g_quarter_pixel_refine_luma_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
+        "vp8g_half_plane_sads_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, sz, vp, i32, vp]),
+        "vp8g_arnr_batch_create": (i32, [vp, C.POINTER(_ArnrJob), sz, C.POINTER(vp),
+                                         C.POINTER(sz), C.POINTER(_Err)]),
+        "vp8g_arnr_batch_run": (i32, [vp]),
+        "vp8g_arnr_batch_shape": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                        C.POINTER(sz)]),
+        "vp8g_arnr_batch_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -291,6 +308,18 @@ class Context:
     def set_xcd_remap(self, on: bool) -> None:
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_XCD_REMAP, int(on)), "set_xcd_remap")
 
+    def set_batch_wgs_per_cu(self, wgs: int) -> None:
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_BATCH_WGS_PER_CU, wgs),
+                    "set_batch_wgs_per_cu")
+
+    def apply_tuning(self, entry: dict) -> None:
+        """Set the launch-shape options of one tuning-table entry
+        (keys ``mbs``, ``waves``, ``wgs_per_cu``; missing keys keep 0 =
+        library default)."""
+        self.set_arnr_mbs(int(entry.get("mbs", 0)))
+        self.set_arnr_waves(int(entry.get("waves", 0)))
+        self.set_batch_wgs_per_cu(int(entry.get("wgs_per_cu", 0)))
+
     # ---- ARNR
     def build_arnr_altref(self, frames, center: int, cfg: ArnrConfig | None = None) -> ArnrPicture:
         cfg = cfg or ArnrConfig()
@@ -342,6 +371,38 @@ class Context:
         st = self.lib.vp8g_build_arnr_altref_dev(self.h, descs, n, center, strength, oy, ou, ov,
                                                  C.byref(err), mv_out)
         self._raise(st, "vp8g_build_arnr_altref_dev")
+
+    def create_arnr_batch(self, jobs) -> "ArnrBatch":
+        """Batched ARNR over device-resident groups.  ``jobs`` is a sequence
+        of ``ArnrJob``; raises as ``build_arnr_altref`` does for the first
+        job that fails validation (the exception carries ``job``)."""
+        jobs = list(jobs)
+        arr = (_ArnrJob * max(len(jobs), 1))()
+        keep = []
+        for i, j in enumerate(jobs):
+            descs = (_I420 * max(len(j.frames), 1))()
+            for k, f in enumerate(j.frames):
+                descs[k] = _I420(*f)
+            keep.append(descs)
+            arr[i] = _ArnrJob(descs, len(j.frames), j.center, int(j.strength) & 0xFF,
+                              j.out_y, j.out_u, j.out_v, j.mv_out)
+        h = C.c_void_p()
+        failed = C.c_size_t(0)
+        err = _Err()
+        st = self.lib.vp8g_arnr_batch_create(self.h, arr, len(jobs), C.byref(h), C.byref(failed),
+                                             C.byref(err))
+        exc = None
+        if st == VP8G_E_INVALID_DIMENSIONS:
+            exc = InvalidDimensions(err.src_w, err.src_h)
+        elif st == VP8G_E_REF_DIMS_MISMATCH:
+            exc = ReferenceDimensionsMismatch((err.src_w, err.src_h), (err.ref_w, err.ref_h))
+        elif st == VP8G_E_CENTER_OUT_OF_RANGE:
+            exc = IndexError(f"arnr batch job {failed.value}: center index out of range")
+        if exc is not None:
+            exc.job = failed.value
+            raise exc
+        self._raise(st, "vp8g_arnr_batch_create")
+        return ArnrBatch(self, h)
 
     # ---- motion compensation
     def predict_inter_mb_batch(self, reference: ReferencePlanes, mb_xy, mvs, full_pixel: bool,
@@ -408,6 +469,20 @@ class Context:
         res[:, 2] = out[:n, 2].astype(np.int64) | (out[:n, 3].astype(np.int64) << 16)
         return res
 
+    def half_plane_sads_batch(self, reference: LumaRef, xy, src, taps, diagonals: bool = True) -> np.ndarray:
+        """fetch_window + HalfPlanes (src/encoder/inter_search.rs:266-346):
+        (n, 8) SADs in (dy, dx) raster order without (0, 0)."""
+        xy = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
+        n = xy.shape[0]
+        src = np.ascontiguousarray(src, np.uint8).reshape(n, 256)
+        taps = np.ascontiguousarray(taps, np.int32).reshape(6)
+        p, lr = self._luma_ref(reference)
+        out = np.empty((max(n, 1), 8), np.uint32)
+        st = self.lib.vp8g_half_plane_sads_batch(self.h, C.byref(lr), _ptr(xy), _ptr(src), n, _ptr(taps),
+                                                 1 if diagonals else 0, _ptr(out))
+        self._raise(st, "vp8g_half_plane_sads_batch")
+        return out[:n]
+
     def half_pixel_refine_luma_batch(self, reference, mb_xy, src, centers) -> np.ndarray:
         """Rows of (mv.row, mv.col, sad)."""
         return self._refine(self.lib.vp8g_half_pixel_refine_luma_batch, reference, mb_xy, src, centers)
@@ -429,6 +504,49 @@ class Context:
 
 # ------------------------------------------------ module-level reference API
 _default = threading.local()
+
+
+@dataclass
+class ArnrJob:
+    """One group of a batch: ``frames`` holds per-frame tuples
+    (width, height, y, u, v, y_stride, uv_stride) of device pointers;
+    outputs are device pointers (packed planes), ``mv_out`` optional."""
+    frames: list
+    center: int
+    strength: int
+    out_y: int
+    out_u: int
+    out_v: int
+    mv_out: int | None = None
+
+
+class ArnrBatch:
+    """A created batch (one persistent launch captured in a HIP graph)."""
+
+    def __init__(self, ctx: Context, h: C.c_void_p):
+        self.ctx, self.h = ctx, h
+
+    def run(self) -> None:
+        """Enqueue one replay on the context's stream."""
+        self.ctx._raise(self.ctx.lib.vp8g_arnr_batch_run(self.h), "vp8g_arnr_batch_run")
+
+    def shape(self) -> dict:
+        m, w, g, u = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
+        self.ctx._raise(self.ctx.lib.vp8g_arnr_batch_shape(self.h, C.byref(m), C.byref(w),
+                                                           C.byref(g), C.byref(u)),
+                        "vp8g_arnr_batch_shape")
+        return {"mbs": m.value, "waves": w.value, "workgroups": g.value, "units": u.value}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx.lib.vp8g_arnr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def default_context() -> Context:

@@ -16,6 +16,9 @@ Workloads
   moving with its own integer velocity in [-vmax, vmax]^2 px/frame
   relative to frame 0, plus `occluders` rectangles per frame (sides
   16..96) replaced by fresh blurred noise, plus sigma-4 noise.
+* ``batch_clips``: the uneven batch: `count` clips, resolution uniform
+  over BATCH_RESOLUTIONS, group length uniform over {3, 5, 7, 9, 11},
+  centre n // 2, content as ``arnr_group_regions``.
 * ``mc_workload``: per-MB quarter-pel MVs uniform in [-64, 64], MB-aligned
   edge-padded reference planes, source MBs from the centre frame.
 """
@@ -170,6 +173,30 @@ def arnr_group_regions(width: int = 1920, height: int = 1080, n: int = 7, vmax: 
         us.append(add_noise(fu, base + 101 + 3 * t))
         vs.append(add_noise(fv, base + 102 + 3 * t))
     return Group(width, height, n // 2, ys, us, vs, base)
+
+
+BATCH_RESOLUTIONS = ((176, 144), (352, 288), (640, 360), (1280, 720), (1920, 1080))
+BATCH_LENGTHS = (3, 5, 7, 9, 11)
+
+
+def batch_specs(count: int = 64, config: int = 2):
+    """(width, height, n) of each clip of the uneven batch."""
+    r = uniform_int(_seed(config, 0, 9), 2 * count, 0, 4).reshape(count, 2)
+    return [(*BATCH_RESOLUTIONS[int(a)], BATCH_LENGTHS[int(b)]) for a, b in r]
+
+
+def batch_clips(count: int = 64, config: int = 2, specs=None, workers: int = 8):
+    """The uneven batch as a list of Groups (clip i seeded by (config, i)).
+    Clips are generated on a thread pool; the bytes do not depend on it."""
+    from concurrent.futures import ThreadPoolExecutor
+    specs = specs if specs is not None else batch_specs(count, config)
+
+    def one(i):
+        w, h, n = specs[i]
+        return arnr_group_regions(w, h, n, vmax=12, occluders=20, clip=i, config=config)
+
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        return list(ex.map(one, range(len(specs))))
 
 
 def pad_to_mb(p: np.ndarray, mb: int) -> np.ndarray:

@@ -170,3 +170,41 @@ def test_refine_rejects_out_of_range_center(gpu, cif_mc):
     with pytest.raises(vp8g.Vp8gError):
         gpu.half_pixel_refine_luma_batch(vp8g.LumaRef(cif_mc.ref_y.reshape(-1), lw, lw, lh),
                                          [[0, 0]], cif_mc.src[:1], [[1024, 0]])
+
+
+@pytest.mark.parametrize("diagonals", [True, False])
+def test_half_plane_sads_match_oracle(gpu, oracle, diagonals):
+    """HalfPlanes over the 1080p reference at random whole-pel positions,
+    including positions far outside the plane (all reads clamp)."""
+    import vp8g
+    import vp8g.synth as synth
+    from oracle_ctypes import HalfPlanes, LumaRef
+    g = synth.arnr_group_cif()
+    mc = synth.mc_workload(g, config=0)
+    lw, lh = mc.mb_cols * 16, mc.mb_rows * 16
+    plane = mc.ref_y.reshape(-1)
+    rng = np.random.default_rng(33 + diagonals)
+    n = 600
+    xy = np.stack([rng.integers(-40, lw + 40, n), rng.integers(-40, lh + 40, n)], 1).astype(np.int32)
+    src = rng.integers(0, 256, (n, 256)).astype(np.uint8)
+    for taps in (vp8g.SIXTAP_FILTERS[4], vp8g.BILINEAR_FILTERS[4], np.array([5, -30, 90, 70, -12, 5], np.int32)):
+        got = gpu.half_plane_sads_batch(vp8g.LumaRef(plane, lw, lw, lh), xy, src, taps, diagonals)
+        lr = LumaRef(plane.ctypes.data, lw, lw, lh)
+        t = np.ascontiguousarray(taps, np.int32)
+        want = np.zeros((n, 8), np.uint32)
+        for i in range(n):
+            win = np.zeros(22 * 22, np.uint8)
+            oracle.lib.orc_fetch_window(C.byref(lr), int(xy[i, 0]), int(xy[i, 1]), win.ctypes.data)
+            hp = HalfPlanes()
+            oracle.lib.orc_half_planes_new(win.ctypes.data, t.ctypes.data, int(diagonals), C.byref(hp))
+            s = np.ascontiguousarray(src[i])
+            k = 0
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if dx == 0 and dy == 0:
+                        continue
+                    diag = dx != 0 and dy != 0
+                    want[i, k] = 0xFFFFFFFF if (diag and not diagonals) else \
+                        oracle.lib.orc_half_planes_sad(C.byref(hp), s.ctypes.data, dx, dy)
+                    k += 1
+        assert_same(got, want, f"half planes taps={list(taps)} diagonals={diagonals}")
