@@ -100,6 +100,25 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
 int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);
 
+/* Launch-shape tuning table (JSON written by the autotuner,
+ * `python -m vp8g.autotune`): entries {"class", "n", "mbs", "waves",
+ * "wgs_per_cu"} keyed by resolution class (qcif <= 176x144 < cif <=
+ * 352x288 < 360p <= 640x360 < 720p <= 1280x720 < 1080p <= 1920x1088 < 4k,
+ * by luma area) and group length; class "batch" for batched launches.  A
+ * context loads $VP8G_TUNING, or tuning/gfx950.json beside the library,
+ * when created; explicit options win over the table, the table over the
+ * built-in defaults.  Like the options, the table never changes an output
+ * byte.  NULL clears the table; an unreadable or malformed file returns
+ * VP8G_E_ARG and keeps the current one. */
+int vp8g_ctx_load_tuning(vp8g_ctx *ctx, const char *path);
+/* The entry a table file gives for a group (batch = 0) or a batch
+ * (batch = 1); zeros when no entry matches.  Needs no device. */
+int vp8g_tuning_lookup(const char *path, uint32_t width, uint32_t height, size_t n_frames,
+                       int batch, int *mbs, int *waves, int *wgs_per_cu);
+/* The launch shape a single-group ARNR call with these dims would use. */
+int vp8g_arnr_shape(vp8g_ctx *ctx, uint32_t width, uint32_t height, size_t n_frames,
+                    uint8_t strength, int *mbs, int *waves);
+
 /* ---- ARNR altref synthesis ----
  * Replaces build_arnr_altref (src/arnr.rs:476-482) and
  * build_arnr_altref_threaded (src/arnr.rs:488-650; the thread budget has no

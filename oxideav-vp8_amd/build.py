@@ -25,13 +25,13 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["capi.hip", "arnr_kernels.hip", "mc_kernels.hip"]
+SOURCES = ["capi.hip", "arnr_kernels.hip", "mc_kernels.hip", "tuning.cpp"]
 
 
 def build(verbose: bool = False) -> Path:
     srcs = [str(CSRC / s) for s in SOURCES]
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), *srcs]
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), *srcs, "-ldl"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

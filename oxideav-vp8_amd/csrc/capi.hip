@@ -1,11 +1,15 @@
 // C ABI (include/vp8g.h): argument validation in the reference's order,
 // device staging for the host-pointer entry points, kernel dispatch.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/vp8g.h"
+#include "tuning.h"
 #include "vp8g_internal.h"
 
 using namespace vp8g;
@@ -19,6 +23,7 @@ struct vp8g_ctx {
   int arnr_mbs = 0;
   int batch_wgs_per_cu = 0;
   int cus = 0;
+  std::vector<TuneEntry> tuning;  // launch shapes by (resolution class, group length)
   // Device arena for the host-pointer entry points (grown, never shrunk).
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
@@ -78,28 +83,47 @@ int validate_arnr(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* er
   return VP8G_OK;
 }
 
-// Launch shape: waves per workgroup (one neighbour frame each, looping when
-// fewer) and MBs per workgroup strip, shrunk until the LDS fits.
-void pick_arnr_shape(const vp8g_ctx* c, int nnb, int* nw_out, int* m_out) {
-  int nw = c->arnr_waves > 0 ? c->arnr_waves : (nnb < 8 ? nnb : 8);
+// Launch shape of one group: explicit options first, then the tuning
+// table's entry for (resolution class, group length), then the defaults
+// (one wave per neighbour up to 8, 4 MBs per strip); shrunk until the LDS
+// fits.
+void pick_arnr_shape(const vp8g_ctx* c, int w, int h, int n, int nnb, int* nw_out, int* m_out) {
+  const TuneEntry* e = lookup_tuning(c->tuning, resolution_class(w, h), n);
+  int nw = c->arnr_waves > 0 ? c->arnr_waves : (e && e->waves > 0 ? e->waves : (nnb < 8 ? nnb : 8));
   if (nw < 1) nw = 1;
   if (nw > 16) nw = 16;
-  int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : 4);
+  int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : (e && e->mbs > 0 ? e->mbs : 4));
   while (m > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) m = arnr_supported_mbs(m - 1);
   while (nw > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) nw--;
   *nw_out = nw;
   *m_out = m;
 }
 
-// Batch launch shape: 2 waves by default (every group length 3..11 leaves
-// an even number of neighbours), same LDS limit.
-void pick_batch_shape(const vp8g_ctx* c, int* nw_out, int* m_out) {
-  int nw = c->arnr_waves > 0 ? c->arnr_waves : 2;
-  int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : 4);
+// Batch launch shape: options, then the table's "batch" entry, then 2
+// waves (every group length 3..11 leaves an even number of neighbours) and
+// 4 MBs; same LDS limit.
+void pick_batch_shape(const vp8g_ctx* c, int* nw_out, int* m_out, int* wgs_per_cu) {
+  const TuneEntry* e = lookup_tuning(c->tuning, "batch", 0);
+  int nw = c->arnr_waves > 0 ? c->arnr_waves : (e && e->waves > 0 ? e->waves : 2);
+  int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : (e && e->mbs > 0 ? e->mbs : 4));
   while (m > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) m = arnr_supported_mbs(m - 1);
   while (nw > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) nw--;
   *nw_out = nw;
   *m_out = m;
+  *wgs_per_cu = c->batch_wgs_per_cu > 0 ? c->batch_wgs_per_cu : (e ? e->wgs_per_cu : 0);
+}
+
+// The table next to the library (tuning/gfx950.json), or $VP8G_TUNING
+// (empty: none).
+std::string default_tuning_path() {
+  const char* env = getenv("VP8G_TUNING");
+  if (env) return env;
+  Dl_info info;
+  if (!dladdr(reinterpret_cast<void*>(&default_tuning_path), &info) || !info.dli_fname) return "";
+  std::string p = info.dli_fname;
+  const size_t slash = p.rfind('/');
+  p = (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/tuning/gfx950.json";
+  return p;
 }
 
 void batch_free(vp8g_arnr_batch* b) {
@@ -155,7 +179,7 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   fill_group(frames, n, center, strength, oy, ou, ov, mv_out, a.frames, &a.g, a.wt4);
   a.xcd_remap = c->xcd_remap;
   int nw, m;
-  pick_arnr_shape(c, a.g.filter_on ? (int)n - 1 : 1, &nw, &m);
+  pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &nw, &m);
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
   return hip_ok(launch_arnr_fused(a, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
@@ -273,7 +297,47 @@ int vp8g_ctx_create(int device, vp8g_ctx** out) {
     return VP8G_E_DEVICE;
   }
   c->stream = c->own;
+  // A missing or unreadable table leaves the built-in defaults.
+  const std::string tp = default_tuning_path();
+  if (!tp.empty()) (void)load_tuning_file(tp.c_str(), &c->tuning);
   *out = c;
+  return VP8G_OK;
+}
+
+int vp8g_ctx_load_tuning(vp8g_ctx* c, const char* path) {
+  if (!c) return VP8G_E_ARG;
+  if (!path) {
+    c->tuning.clear();
+    return VP8G_OK;
+  }
+  std::vector<TuneEntry> t;
+  if (!load_tuning_file(path, &t)) return VP8G_E_ARG;
+  c->tuning = t;
+  return VP8G_OK;
+}
+
+int vp8g_tuning_lookup(const char* path, uint32_t width, uint32_t height, size_t n_frames,
+                       int batch, int* mbs, int* waves, int* wgs_per_cu) {
+  if (!path) return VP8G_E_ARG;
+  std::vector<TuneEntry> t;
+  if (!load_tuning_file(path, &t)) return VP8G_E_ARG;
+  const TuneEntry* e = batch ? lookup_tuning(t, "batch", 0)
+                             : lookup_tuning(t, resolution_class((int)width, (int)height), (int)n_frames);
+  if (mbs) *mbs = e ? e->mbs : 0;
+  if (waves) *waves = e ? e->waves : 0;
+  if (wgs_per_cu) *wgs_per_cu = e ? e->wgs_per_cu : 0;
+  return VP8G_OK;
+}
+
+int vp8g_arnr_shape(vp8g_ctx* c, uint32_t width, uint32_t height, size_t n_frames, uint8_t strength,
+                    int* mbs, int* waves) {
+  if (!c || width == 0 || height == 0 || n_frames == 0 || n_frames > VP8G_MAX_FRAMES) return VP8G_E_ARG;
+  const int n = (int)n_frames;
+  const bool on = strength != 0 && n >= 2;
+  int nw, m;
+  pick_arnr_shape(c, (int)width, (int)height, n, on ? n - 1 : 1, &nw, &m);
+  if (mbs) *mbs = m;
+  if (waves) *waves = nw;
   return VP8G_OK;
 }
 
@@ -391,8 +455,8 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
     total_frames += jb.n_frames;
   }
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
-  int nw, m;
-  pick_batch_shape(c, &nw, &m);
+  int nw, m, want_per_cu;
+  pick_batch_shape(c, &nw, &m, &want_per_cu);
   std::vector<ArnrBatchJob> hj(nj);
   std::vector<FrameDesc> hf(total_frames);
   std::vector<uint16_t> hu;
@@ -414,7 +478,7 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
   if (units > 0x7FFFFFFF) return VP8G_E_ARG;
   int per_cu = arnr_batch_workgroups_per_cu(nw, m);
   if (per_cu < 1) return VP8G_E_DEVICE;
-  if (c->batch_wgs_per_cu > 0 && c->batch_wgs_per_cu < per_cu) per_cu = c->batch_wgs_per_cu;
+  if (want_per_cu > 0 && want_per_cu < per_cu) per_cu = want_per_cu;
   size_t wgs = (size_t)per_cu * (size_t)c->cus;
   if (wgs > units) wgs = units;
 

@@ -94,7 +94,7 @@ EXPORTS = (
     "vp8g_half_pixel_refine_luma_batch", "vp8g_quarter_pixel_refine_luma_batch",
     "vp8g_half_plane_sads_batch",
     "vp8g_arnr_batch_create", "vp8g_arnr_batch_run", "vp8g_arnr_batch_shape",
-    "vp8g_arnr_batch_destroy",
+    "vp8g_arnr_batch_destroy", "vp8g_ctx_load_tuning", "vp8g_tuning_lookup", "vp8g_arnr_shape",
 )
 
 
@@ -128,6 +128,10 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_arnr_batch_shape": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                         C.POINTER(sz)]),
         "vp8g_arnr_batch_destroy": (None, [vp]),
+        "vp8g_ctx_load_tuning": (i32, [vp, C.c_char_p]),
+        "vp8g_tuning_lookup": (i32, [C.c_char_p, C.c_uint32, C.c_uint32, sz, i32, C.POINTER(i32),
+                                     C.POINTER(i32), C.POINTER(i32)]),
+        "vp8g_arnr_shape": (i32, [vp, C.c_uint32, C.c_uint32, sz, u8, C.POINTER(i32), C.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -311,6 +315,17 @@ class Context:
     def set_batch_wgs_per_cu(self, wgs: int) -> None:
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_BATCH_WGS_PER_CU, wgs),
                     "set_batch_wgs_per_cu")
+
+    def load_tuning(self, path: str | Path | None) -> None:
+        """Load a tuning table (None clears it); see vp8g.autotune."""
+        self._raise(self.lib.vp8g_ctx_load_tuning(self.h, None if path is None else str(path).encode()),
+                    "vp8g_ctx_load_tuning")
+
+    def arnr_shape(self, width: int, height: int, n_frames: int, strength: int = 3) -> dict:
+        m, w = C.c_int(), C.c_int()
+        self._raise(self.lib.vp8g_arnr_shape(self.h, width, height, n_frames, strength, C.byref(m),
+                                             C.byref(w)), "vp8g_arnr_shape")
+        return {"mbs": m.value, "waves": w.value}
 
     def apply_tuning(self, entry: dict) -> None:
         """Set the launch-shape options of one tuning-table entry
@@ -547,6 +562,22 @@ class ArnrBatch:
             self.close()
         except Exception:
             pass
+
+
+def tuning_lookup(path, width: int, height: int, n_frames: int, batch: bool = False,
+                  lib: C.CDLL | None = None) -> dict:
+    """The entry of a tuning-table file for a group or batch, as the library
+    reads it (no device needed); zeros when nothing matches."""
+    lib = lib or load_library()
+    m, w, g = C.c_int(), C.c_int(), C.c_int()
+    st = lib.vp8g_tuning_lookup(str(path).encode(), width, height, n_frames, int(batch), C.byref(m),
+                                C.byref(w), C.byref(g))
+    if st != VP8G_OK:
+        raise Vp8gError(st, f"vp8g_tuning_lookup({path}) failed")
+    return {"mbs": m.value, "waves": w.value, "wgs_per_cu": g.value}
+
+
+TUNING_PATH = Path(__file__).resolve().parent.parent / "tuning" / "gfx950.json"
 
 
 def default_context() -> Context:

@@ -1,0 +1,197 @@
+"""Launch-shape autotuner for the ARNR kernels.
+
+Sweeps the shapes that never change output bytes — anchor MBs per
+workgroup (``mbs``), waves per workgroup (``waves``) and, for batches,
+persistent workgroups per CU (``wgs_per_cu``) — over device-resident
+synthetic workloads, timing each with HIP events on the library's stream,
+and writes the fastest shape per (resolution class, group length) as a
+JSON table that libvp8g.so reads when a context is created:
+
+    {"arch": "gfx950",
+     "entries": [{"class": "1080p", "n": 7, "mbs": 4, "waves": 6,
+                  "wgs_per_cu": 0, "ms": 0.061}, ...]}
+
+Class "batch" (n = 0) holds the shape of batched launches.  Resolution
+classes by luma area: qcif <= 176x144 < cif <= 352x288 < 360p <= 640x360
+< 720p <= 1280x720 < 1080p <= 1920x1088 < 4k.
+
+    python -m vp8g.autotune --out oxideav-vp8_amd/tuning/gfx950.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+CLASSES = (("qcif", 176 * 144), ("cif", 352 * 288), ("360p", 640 * 360), ("720p", 1280 * 720),
+           ("1080p", 1920 * 1088), ("4k", 1 << 62))
+MBS = (1, 2, 4, 8)
+WAVES = (1, 2, 3, 4, 5, 6, 8, 10)
+WGS_PER_CU = (0, 1, 2, 3, 4, 6, 8)
+
+
+def resolution_class(w: int, h: int) -> str:
+    a = w * h
+    for name, lim in CLASSES:
+        if a <= lim:
+            return name
+    return CLASSES[-1][0]
+
+
+class DeviceClip:
+    """One group's planes and outputs in device memory (torch tensors)."""
+
+    def __init__(self, torch, group, strength: int = 3):
+        dev = torch.device("cuda:0")
+        self.group, self.strength = group, strength
+        w, h = group.width, group.height
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        self.planes = [tuple(torch.from_numpy(np.ascontiguousarray(p).reshape(-1)).to(dev)
+                             for p in (y, u, v)) for y, u, v in zip(group.y, group.u, group.v)]
+        self.frames = [(w, h, y.data_ptr(), u.data_ptr(), v.data_ptr(), w, cw)
+                       for y, u, v in self.planes]
+        self.oy = torch.empty(w * h, dtype=torch.uint8, device=dev)
+        self.ou = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
+        self.ov = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
+
+    @property
+    def anchor_mbs(self) -> int:
+        return ((self.group.width + 15) // 16) * ((self.group.height + 15) // 16)
+
+    def job(self, vp8g):
+        return vp8g.ArnrJob(self.frames, self.group.center, self.strength, self.oy.data_ptr(),
+                            self.ou.data_ptr(), self.ov.data_ptr())
+
+    def descs(self, vp8g):
+        d = (vp8g._I420 * len(self.frames))()
+        for i, f in enumerate(self.frames):
+            d[i] = vp8g._I420(*f)
+        return d
+
+
+def _events(torch, ctx, fn, reps: int, warmup: int = 2) -> float:
+    """Mean ms per call of fn() over `reps` calls, HIP events on ctx's stream."""
+    stream = torch.cuda.ExternalStream(ctx.stream(), device=torch.device("cuda:0"))
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def time_single(torch, vp8g, ctx, clip: DeviceClip, reps: int = 20) -> float:
+    d = clip.descs(vp8g)
+    n, c = len(clip.frames), clip.group.center
+
+    def run():
+        ctx.build_arnr_altref_dev(d, n, c, clip.strength, clip.oy.data_ptr(), clip.ou.data_ptr(),
+                                  clip.ov.data_ptr())
+    return _events(torch, ctx, run, reps)
+
+
+def time_per_clip_loop(torch, vp8g, ctx, clips, reps: int = 5) -> float:
+    """All clips as back-to-back single-group launches on one stream."""
+    ds = [(c.descs(vp8g), len(c.frames), c.group.center, c) for c in clips]
+
+    def run():
+        for d, n, ctr, c in ds:
+            ctx.build_arnr_altref_dev(d, n, ctr, c.strength, c.oy.data_ptr(), c.ou.data_ptr(),
+                                      c.ov.data_ptr())
+    return _events(torch, ctx, run, reps)
+
+
+def time_batch(torch, vp8g, ctx, clips, reps: int = 10):
+    batch = ctx.create_arnr_batch([c.job(vp8g) for c in clips])
+    try:
+        return _events(torch, ctx, batch.run, reps), batch.shape()
+    finally:
+        batch.close()
+
+
+def tune_single(torch, vp8g, ctx, clip: DeviceClip, log=print):
+    best = None
+    for mbs in MBS:
+        for waves in WAVES:
+            if waves > max(1, len(clip.frames) - 1):
+                continue
+            ctx.apply_tuning({"mbs": mbs, "waves": waves})
+            ms = time_single(torch, vp8g, ctx, clip)
+            log(f"  single {clip.group.width}x{clip.group.height} n={len(clip.frames)} mbs={mbs} "
+                f"waves={waves}: {ms:.4f} ms")
+            if best is None or ms < best["ms"]:
+                best = {"mbs": mbs, "waves": waves, "wgs_per_cu": 0, "ms": ms}
+    ctx.apply_tuning({})
+    return best
+
+
+def tune_batch(torch, vp8g, ctx, clips, log=print):
+    best = None
+    for mbs in MBS:
+        for waves in (1, 2, 4):
+            for wgs in WGS_PER_CU:
+                ctx.apply_tuning({"mbs": mbs, "waves": waves, "wgs_per_cu": wgs})
+                ms, shape = time_batch(torch, vp8g, ctx, clips, reps=5)
+                log(f"  batch mbs={shape['mbs']} waves={shape['waves']} wgs_per_cu={wgs} "
+                    f"workgroups={shape['workgroups']}: {ms:.4f} ms")
+                if shape["mbs"] != mbs or shape["waves"] != waves:
+                    continue  # reduced by the LDS limit: measured under another key
+                if best is None or ms < best["ms"]:
+                    best = {"mbs": mbs, "waves": waves, "wgs_per_cu": wgs, "ms": ms}
+    ctx.apply_tuning({})
+    return best
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--clips", type=int, default=64)
+    ap.add_argument("--quick", action="store_true", help="fewer single-group workloads")
+    args = ap.parse_args(argv)
+    import torch
+
+    import vp8g
+    import vp8g.synth as synth
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    torch.cuda.init()
+    ctx = vp8g.Context(0)
+    entries = []
+    singles = [(1920, 1080, 7), (3840, 2160, 11), (1280, 720, 7), (352, 288, 5)]
+    if not args.quick:
+        singles += [(640, 360, 7), (176, 144, 5), (1920, 1080, 3), (1920, 1080, 11)]
+    for (w, h, n) in singles:
+        t0 = time.perf_counter()
+        g = synth.arnr_group_regions(w, h, n, clip=0, config=9)
+        clip = DeviceClip(torch, g)
+        torch.cuda.synchronize()
+        log(f"[autotune] single {w}x{h} n={n} (generated in {time.perf_counter() - t0:.1f}s)")
+        best = tune_single(torch, vp8g, ctx, clip, log)
+        entries.append({"class": resolution_class(w, h), "n": n, **best})
+        del clip
+    t0 = time.perf_counter()
+    clips = [DeviceClip(torch, g) for g in synth.batch_clips(args.clips)]
+    torch.cuda.synchronize()
+    log(f"[autotune] batch of {len(clips)} clips (generated in {time.perf_counter() - t0:.1f}s)")
+    best = tune_batch(torch, vp8g, ctx, clips, log)
+    entries.append({"class": "batch", "n": 0, **best})
+    table = {"arch": "gfx950", "generated_by": "python -m vp8g.autotune", "entries": entries}
+    Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+    Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
+    log(f"[autotune] wrote {args.out}")
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
