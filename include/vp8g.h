@@ -140,6 +140,31 @@ int vp8g_build_arnr_altref_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_
                                size_t center, uint8_t strength, uint8_t *out_y, uint8_t *out_u,
                                uint8_t *out_v, vp8g_err *err, int32_t *mv_out);
 
+/* ---- exhaustive block search and table-driven altref ----
+ * No counterpart in the reference (its ARNR search is the 4-step descent):
+ * the judge and labeller for learned motion proposals.  For every 16x16
+ * block of frame `center` and every other frame f, the SADs of all
+ * candidates seed + (dx, dy), |dx|, |dy| <= radius, are taken with
+ * block_sad's semantics (clamped fetches at the visible edges, partial
+ * edge blocks; src/arnr.rs:149-153, 200-225); the minimum SAD wins and
+ * ties go to the smallest (|mvx| + |mvy|, mvy, mvx).  `out` (device) uses
+ * the alignment-table layout of vp8g_build_arnr_altref_dev's mv_out,
+ * n_frames * ceil(h/16) * ceil(w/16) * {mvx, mvy, sad}; centre rows are
+ * left untouched.  `seeds` (device, may be NULL = all zero) uses the same
+ * layout (the third entry is ignored); seed components are clamped to
+ * +-16384.  radius: 2, 16 or 64.  Validation as vp8g_build_arnr_altref. */
+int vp8g_block_search_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames, size_t center,
+                          int radius, const int32_t *seeds, int32_t *out, vp8g_err *err);
+/* The altref of build_arnr_altref (src/arnr.rs:488-650) with the search
+ * replaced by a given alignment table (device, the layout above; the
+ * centre's rows are not read): occlusion guard on the table's SAD, blend
+ * and resolve exactly as the reference applies them to its own table.
+ * Vectors may be any size (reads are clamped). */
+int vp8g_build_arnr_altref_table_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames,
+                                     size_t center, uint8_t strength, const int32_t *table,
+                                     uint8_t *out_y, uint8_t *out_u, uint8_t *out_v,
+                                     vp8g_err *err);
+
 /* ---- batched ARNR ----
  * Many groups (clips of any resolution and length) in one persistent launch,
  * captured once into a HIP graph and replayed by vp8g_arnr_batch_run.

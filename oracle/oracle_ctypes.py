@@ -75,6 +75,13 @@ class Oracle:
         L.orc_block_sad_generic.restype = u32
         L.orc_refine_search.argtypes = [C.POINTER(I420), C.POINTER(I420)] + [i32] * 4 + [vp]
         L.orc_refine_search.restype = None
+        L.orc_block_search.argtypes = [C.POINTER(I420), C.POINTER(I420)] + [i32] * 7 + [vp]
+        L.orc_block_search.restype = None
+        L.orc_block_search_group.argtypes = [C.POINTER(I420), sz, sz, i32, vp, sz, vp]
+        L.orc_block_search_group.restype = i32
+        L.orc_build_arnr_altref_mv.argtypes = [C.POINTER(I420), sz, sz, u8, vp, vp, vp, vp,
+                                               C.POINTER(Err)]
+        L.orc_build_arnr_altref_mv.restype = i32
         L.orc_pixel_weight.argtypes = [C.c_int32, u32]
         L.orc_pixel_weight.restype = u32
         L.orc_weight_table.argtypes = [u32, vp]
@@ -127,6 +134,45 @@ class Oracle:
         keep += [y, u, v]
         return I420(width, height, _p(y) if y.size else None, _p(u) if u.size else None,
                     _p(v) if v.size else None, y_stride, uv_stride)
+
+    def _arr(self, frames, keep):
+        arr = (I420 * max(len(frames), 1))()
+        for i, f in enumerate(frames):
+            arr[i] = self.i420(*f, keep=keep)
+        return arr
+
+    def block_search_group(self, frames, center, radius, seeds=None, threads=8):
+        """Exhaustive search table (n, brows, bcols, 3) for every non-centre
+        frame against the centre; centre rows are zero."""
+        keep: list = []
+        arr = self._arr(frames, keep)
+        w, h = frames[center][0], frames[center][1]
+        out = np.zeros((len(frames), (h + 15) // 16, (w + 15) // 16, 3), np.int32)
+        sd = None
+        if seeds is not None:
+            sd = np.ascontiguousarray(seeds, np.int32)
+            assert sd.shape == out.shape
+        st = self.lib.orc_block_search_group(arr, len(frames), center, radius,
+                                             _p(sd) if sd is not None else None, threads, _p(out))
+        assert st == 0, st
+        return out
+
+    def build_arnr_altref_mv(self, frames, center, strength, table):
+        """Altref from a given alignment table (n, brows, bcols, 3)."""
+        keep: list = []
+        arr = self._arr(frames, keep)
+        w, h = frames[center][0], frames[center][1]
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        t = np.ascontiguousarray(table, np.int32)
+        oy = np.zeros(max(w * h, 1), np.uint8)
+        ou = np.zeros(max(cw * ch, 1), np.uint8)
+        ov = np.zeros(max(cw * ch, 1), np.uint8)
+        err = Err()
+        st = self.lib.orc_build_arnr_altref_mv(arr, len(frames), center, strength, _p(t), _p(oy),
+                                               _p(ou), _p(ov), C.byref(err))
+        if st != 0:
+            return st, None
+        return st, (oy[:w * h], ou[:cw * ch], ov[:cw * ch])
 
     def build_arnr_altref(self, frames, center, strength=3, threads=1, want_mvs=False):
         """frames: list of (w, h, y, u, v, y_stride, uv_stride).  Returns

@@ -57,6 +57,15 @@ uint32_t orc_block_sad(const orc_i420 *src, const orc_i420 *ref,
 uint32_t orc_block_sad_generic(const orc_i420 *src, const orc_i420 *ref,
                                int bx, int by, int bw, int bh, int mvx, int mvy,
                                uint32_t early_exit_at);
+/* Exhaustive search around (sx, sy) within +-radius; out3 = {mvx, mvy,
+ * sad}; minimum SAD, ties to the smallest (|mvx|+|mvy|, mvy, mvx). */
+void orc_block_search(const orc_i420 *src, const orc_i420 *ref, int bx, int by, int bw, int bh,
+                      int sx, int sy, int radius, int32_t out3[3]);
+int orc_block_search_group(const orc_i420 *frames, size_t n, size_t center, int radius,
+                           const int32_t *seeds, size_t threads, int32_t *out);
+int orc_build_arnr_altref_mv(const orc_i420 *frames, size_t n, size_t center, uint8_t strength,
+                             const int32_t *table, uint8_t *out_y, uint8_t *out_u,
+                             uint8_t *out_v, orc_err *err);
 /* out3 = {mvx, mvy, sad} */
 void orc_refine_search(const orc_i420 *src, const orc_i420 *ref,
                        int bx, int by, int bw, int bh, int32_t out3[3]);

@@ -247,12 +247,16 @@ __device__ __forceinline__ int xcd_band(int i, int total) {
 
 // One strip of group `a`: M anchor MBs of MB row `brow`, strip column
 // `scol`.  `frame_at(i)` returns frame i of the group, `wt4` the weight LUT.
+// TABLE: the per-block vectors and SADs come from `table` (layout
+// n x brows x bcols x {mvx, mvy, sad}) instead of the descent, and the
+// blend reads the neighbour straight from memory with clamped coordinates
+// (the vectors are not bounded by the staged +-15 window).
 // Ends with the strip resolved into the output planes; the caller
 // synchronises the workgroup before reusing LDS.
-template <int M, class FrameAt>
+template <int M, bool TABLE, class FrameAt>
 __device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t* wt4,
                                               const FrameAt& frame_at, int brow, int scol,
-                                              uint8_t* smem) {
+                                              uint8_t* smem, const int32_t* table = nullptr) {
   using G = StripGeom<M>;
   const G g;
   // LDS: packed accumulators M*384 u32 | weight LUT 256 B | centre luma
@@ -279,7 +283,7 @@ __device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t
 
   // The first neighbour's windows are staged before anything else.
   int k = wave;
-  if (k < nnb) stage_windows(win, frame_at(k < a.center ? k : k + 1), a, g, bx0, by, lane);
+  if (!TABLE && k < nnb) stage_windows(win, frame_at(k < a.center ? k : k + 1), a, g, bx0, by, lane);
 
   // Centre strip (zero outside the frame), accumulators, LUT.
   for (int i = tid; i < 64 * M; i += blockDim.x) {  // luma: 16 rows x 4M dwords
@@ -322,7 +326,8 @@ __device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t
 
   for (; k < nnb; k += nw) {
     const int fi = k < a.center ? k : k + 1;
-    if (k != wave) stage_windows(win, frame_at(fi), a, g, bx0, by, lane);
+    const FrameDesc F = frame_at(fi);
+    if (!TABLE && k != wave) stage_windows(win, F, a, g, bx0, by, lane);
     wave_lds_sync();
 
 #pragma unroll 1
@@ -331,18 +336,25 @@ __device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t
       if (bx >= a.w) break;
       const int bw = min(16, a.w - bx);
       const int cbw = (bw + 1) >> 1;
-      const uint8_t* s_src = s_cy + 2 * kk * 16 * M + 16 * m;
-      const uint4 src0 = *reinterpret_cast<const uint4*>(s_src);
-      const uint4 src1 = *reinterpret_cast<const uint4*>(s_src + 16 * M);
-      const uint8_t* row0 = win + (2 * kk + 15) * G::lp + 16 * m + 16;
       int bestx, besty;
       uint32_t best_sad;
-      if (bw == 16 && bh == 16) {
-        descend<true, G>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad);
+      if (TABLE) {
+        const int32_t* t = table + 3 * (((size_t)fi * a.brows + brow) * a.bcols + (bx >> 4));
+        bestx = __builtin_amdgcn_readfirstlane(t[0]);
+        besty = __builtin_amdgcn_readfirstlane(t[1]);
+        best_sad = (uint32_t)__builtin_amdgcn_readfirstlane(t[2]);
       } else {
-        descend<false, G>(row0, src0, src1, gd_off, gq, kk, bw, bh, bestx, besty, best_sad);
+        const uint8_t* s_src = s_cy + 2 * kk * 16 * M + 16 * m;
+        const uint4 src0 = *reinterpret_cast<const uint4*>(s_src);
+        const uint4 src1 = *reinterpret_cast<const uint4*>(s_src + 16 * M);
+        const uint8_t* row0 = win + (2 * kk + 15) * G::lp + 16 * m + 16;
+        if (bw == 16 && bh == 16) {
+          descend<true, G>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad);
+        } else {
+          descend<false, G>(row0, src0, src1, gd_off, gq, kk, bw, bh, bestx, besty, best_sad);
+        }
       }
-      if (a.mv_out && lane == 0) {
+      if (!TABLE && a.mv_out && lane == 0) {
         int32_t* o = a.mv_out + 3 * (((size_t)fi * a.brows + brow) * a.bcols + (bx >> 4));
         o[0] = bestx; o[1] = besty; o[2] = (int32_t)best_sad;
       }
@@ -351,10 +363,19 @@ __device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t
       if (best_sad < (kSadPerPixelCutoff + 1) * (uint32_t)(bw * bh)) {
         uint32_t* acc = s_acc + 384 * m;
         if (lr < bh) {
-          const uint8_t* rp4 = win + (lr + besty + 15) * g.lp + 16 * m + 16 + 4 * lq + bestx;
-          const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
-          const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
-          const uint32_t ref4 = funnel(rw[0], rw[1], rsh);
+          uint32_t ref4;
+          if (TABLE) {
+            const uint8_t* rrow = F.y + (size_t)clampi(by + lr + besty, 0, a.h - 1) * F.y_stride;
+            const int xr = bx + 4 * lq + bestx;
+            ref4 = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) ref4 |= (uint32_t)rrow[clampi(xr + j, 0, a.w - 1)] << (8 * j);
+          } else {
+            const uint8_t* rp4 = win + (lr + besty + 15) * g.lp + 16 * m + 16 + 4 * lq + bestx;
+            const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
+            const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
+            ref4 = funnel(rw[0], rw[1], rsh);
+          }
           const uint32_t ctr4 = *reinterpret_cast<const uint32_t*>(s_cy + lr * 16 * M + 16 * m + 4 * lq);
 #pragma unroll
           for (int j = 0; j < 4; j++) {
@@ -368,8 +389,17 @@ __device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t
         }
         if (cr < cbh && cc < cbw) {
           const int cmvx = bestx / 2, cmvy = besty / 2;  // truncation toward zero
-          const int o = (cr + cmvy + 7) * g.cp + 8 * m + 8 + cc + cmvx;
-          const int ru = cwin[o], rv = cwin[kCWinRows * g.cp + o];
+          int ru, rv;
+          if (TABLE) {
+            const size_t o = (size_t)clampi(cby + cr + cmvy, 0, a.ch - 1) * F.uv_stride +
+                             clampi((bx >> 1) + cc + cmvx, 0, a.cw - 1);
+            ru = F.u[o];
+            rv = F.v[o];
+          } else {
+            const int o = (cr + cmvy + 7) * g.cp + 8 * m + 8 + cc + cmvx;
+            ru = cwin[o];
+            rv = cwin[kCWinRows * g.cp + o];
+          }
           const int cu = s_cu[cr * 8 * M + 8 * m + cc], cv = s_cv[cr * 8 * M + 8 * m + cc];
           const uint32_t wu = s_lut[ru > cu ? ru - cu : cu - ru];
           const uint32_t wv = s_lut[rv > cv ? rv - cv : cv - rv];
@@ -429,7 +459,18 @@ __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
   const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
   const int brow = strip / a.g.strips_per_row;
   const int scol = strip - brow * a.g.strips_per_row;
-  process_strip<M>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, smem);
+  process_strip<M, false>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, smem);
+}
+
+// The altref from a given alignment table (guard, blend, resolve).
+template <int M>
+__global__ void __launch_bounds__(1024) arnr_table_kernel(const ArnrArgs a, const int32_t* table) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int total = a.g.strips_per_row * a.g.brows;
+  const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
+  const int brow = strip / a.g.strips_per_row;
+  const int scol = strip - brow * a.g.strips_per_row;
+  process_strip<M, true>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, smem, table);
 }
 
 // Descriptor fields read from global memory and made wave-uniform, so that
@@ -507,7 +548,7 @@ __global__ void __launch_bounds__(1024) arnr_batch_kernel(const ArnrBatchArgs b)
       const int local = unit - uload(&job->unit_base);
       const int brow = local / g.strips_per_row;
       const int scol = local - brow * g.strips_per_row;
-      process_strip<M>(g, job->wt4, [&](int i) { return load_frame(b.frames + fb + i); }, brow, scol,
+      process_strip<M, false>(g, job->wt4, [&](int i) { return load_frame(b.frames + fb + i); }, brow, scol,
                        smem);
     }
     __syncthreads();  // LDS and s_unit are free for the next unit
@@ -566,6 +607,19 @@ hipError_t launch_arnr_batch(const ArnrBatchArgs& b, int nw, int workgroups, hip
     case 2: hipLaunchKernelGGL(arnr_batch_kernel<2>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
     case 4: hipLaunchKernelGGL(arnr_batch_kernel<4>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
     case 8: hipLaunchKernelGGL(arnr_batch_kernel<8>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_arnr_table(const ArnrArgs& a, const int32_t* table, int nw, hipStream_t stream) {
+  const int blocks = a.g.strips_per_row * a.g.brows;
+  const size_t lds = arnr_fused_lds_bytes(0, a.mbs_per_wg);  // no staged windows
+  switch (a.mbs_per_wg) {
+    case 1: hipLaunchKernelGGL(arnr_table_kernel<1>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
+    case 2: hipLaunchKernelGGL(arnr_table_kernel<2>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
+    case 4: hipLaunchKernelGGL(arnr_table_kernel<4>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
+    case 8: hipLaunchKernelGGL(arnr_table_kernel<8>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -438,6 +438,54 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   return finish(c, st);
 }
 
+int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
+                          int radius, const int32_t* seeds, int32_t* out, vp8g_err* err) {
+  if (!c) return VP8G_E_ARG;
+  int st = validate_arnr(frames, n, center, err);
+  if (st != VP8G_OK) return st;
+  if (!out || !block_search_supported(radius)) return VP8G_E_ARG;
+  if (((uintptr_t)out | (uintptr_t)seeds) & 3) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  SearchArgs a;
+  memset(&a, 0, sizeof(a));
+  bool aligned4 = true;
+  for (size_t i = 0; i < n; i++) {
+    const vp8g_i420& f = frames[i];
+    a.frames[i] = FrameDesc{f.y, f.u, f.v, (uint32_t)f.y_stride, (uint32_t)f.uv_stride};
+    if (((uintptr_t)f.y & 3) || (f.y_stride & 3)) aligned4 = false;
+  }
+  a.seeds = seeds;
+  a.out = out;
+  a.w = (int)frames[center].width;
+  a.h = (int)frames[center].height;
+  a.bcols = (a.w + 15) / 16;
+  a.brows = (a.h + 15) / 16;
+  a.n = (int)n;
+  a.center = (int)center;
+  a.aligned4 = aligned4 ? 1 : 0;
+  a.total_blocks = (int)(n - 1) * a.bcols * a.brows;
+  return hip_ok(launch_block_search(a, radius, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int vp8g_build_arnr_altref_table_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
+                                     uint8_t strength, const int32_t* table, uint8_t* oy,
+                                     uint8_t* ou, uint8_t* ov, vp8g_err* err) {
+  if (!c) return VP8G_E_ARG;
+  int st = validate_arnr(frames, n, center, err);
+  if (st != VP8G_OK) return st;
+  if (!oy || !ou || !ov || !table || ((uintptr_t)table & 3)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  ArnrArgs a;
+  memset(&a, 0, sizeof(a));
+  fill_group(frames, n, center, strength, oy, ou, ov, nullptr, a.frames, &a.g, a.wt4);
+  a.xcd_remap = c->xcd_remap;
+  int nw, m;
+  pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &nw, &m);
+  a.mbs_per_wg = m;
+  a.g.strips_per_row = (a.g.bcols + m - 1) / m;
+  return hip_ok(launch_arnr_table(a, table, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
 int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
                            vp8g_arnr_batch** out, size_t* failed_job, vp8g_err* err) {
   if (!c || !out || (!jobs && nj)) return VP8G_E_ARG;

@@ -1,0 +1,259 @@
+// Exhaustive +-R whole-pel block search on gfx950 (a project-defined judge
+// and labeller beside the reference's 4-step ARNR descent).
+//
+// For every 16x16 block of the centre frame and every other frame of the
+// group, the SADs of all (2R+1)^2 candidates seed + (dx, dy) are computed
+// with the reference's block_sad semantics (clamped fetches at the visible
+// frame edges, src/arnr.rs:149-153, 200-225; partial edge blocks cover
+// bw x bh pixels) and the minimum is taken; ties go to the smallest
+// (|mvx| + |mvy|, mvy, mvx).
+//
+// Work decomposition: a lane group per block.  The block's reference
+// window (rows by+sy-R .. by+sy+R+15, dword-aligned columns) is staged into
+// LDS once; lane (j, db) owns the candidates dx in quad j (4 consecutive
+// positions) and K consecutive dy rows starting at db*K.  It slides down
+// the window rows: every row's five aligned dwords feed the K row
+// accumulators that need them through v_qsad_pk_u16_u8, which adds the
+// SADs of one 4-byte source dword against the 4 byte-shifted positions of
+// an 8-byte reference pair into four 16-bit accumulators (no byte-alignment
+// shifts; a full 16x16 SAD is at most 65280, so 16 bits never overflow).
+// The lane's 4K candidates are folded into one 64-bit key
+// (sad, |mv|_1, mvy, mvx) and the group reduces the key.
+#include "vp8g_device.h"
+#include "vp8g_internal.h"
+
+namespace vp8g {
+
+template <int R>
+struct SearchGeom {
+  static constexpr int D = 2 * R + 1;               // dy (and dx) candidates
+  static constexpr int J = R / 2 + 1;               // dx quads: covers shift0 + 2R for any shift0 <= 3
+  static constexpr int K = R <= 16 ? 5 : 9;         // dy rows per lane
+  static constexpr int DB = (D + K - 1) / K;        // dy blocks
+  static constexpr int LPB = J * DB;                // working lanes per block
+  static constexpr int G = LPB <= 2 ? 2 : (LPB <= 64 ? 64 : (LPB + 63) / 64 * 64);  // lane group
+  static constexpr int WROWS = DB * K + 15;         // window rows staged
+  static constexpr int PD = J + 4;                  // window pitch (dwords)
+  static constexpr int WIN_DW = WROWS * PD;
+  static constexpr int WG = G >= 256 ? G : (G <= 2 ? 64 : 256);  // threads per workgroup
+  static constexpr int BPW = WG / G;                // blocks per workgroup
+};
+
+__device__ __forceinline__ uint32_t gather4_clamped(const uint8_t* row, int x0, int w) {
+  uint32_t v = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) v |= (uint32_t)row[clampi(x0 + b, 0, w - 1)] << (8 * b);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t pack_key(uint32_t sad, int mvx, int mvy) {
+  const uint32_t l1 = (uint32_t)(abs(mvx) + abs(mvy));
+  return ((uint64_t)sad << 48) | ((uint64_t)l1 << 32) | ((uint64_t)(uint32_t)(mvy + 32768) << 16) |
+         (uint64_t)(uint32_t)(mvx + 32768);
+}
+
+__device__ __forceinline__ uint64_t min_u64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  const int lo = __shfl_xor((int)(uint32_t)v, m, 64);
+  const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Source access: registers (S) for groups of at most one wave, or an LDS
+// copy (s_src, read with wave-uniform addresses) for multi-wave groups,
+// where a 64-entry register array is not promoted out of scratch.
+template <int R>
+struct SrcRows {
+  const uint32_t (&S)[16][4];
+  const uint32_t* lds;
+  __device__ __forceinline__ uint4 row(int r) const {
+    if (SearchGeom<R>::G > 64) return *reinterpret_cast<const uint4*>(lds + 4 * r);
+    return make_uint4(S[r][0], S[r][1], S[r][2], S[r][3]);
+  }
+};
+
+template <int R, bool FULL>
+__device__ __forceinline__ void accumulate(const uint32_t* win, int j, int db, const SrcRows<R>& src,
+                                           int bh, int bw, uint64_t (&acc)[SearchGeom<R>::K]) {
+  using SG = SearchGeom<R>;
+  constexpr int K = SG::K;
+  const int bwq = bw >> 2, rem = bw & 3;
+  const uint32_t remmask = rem ? (0xFFFFFFFFu >> (32 - 8 * rem)) : 0u;
+#pragma unroll
+  for (int yy = 0; yy < K + 15; yy++) {
+    const uint32_t* wr = win + (db * K + yy) * SG::PD + j;
+    const uint32_t w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4];
+    const uint64_t p[4] = {(uint64_t)w0 | ((uint64_t)w1 << 32), (uint64_t)w1 | ((uint64_t)w2 << 32),
+                           (uint64_t)w2 | ((uint64_t)w3 << 32), (uint64_t)w3 | ((uint64_t)w4 << 32)};
+    const uint32_t wv[5] = {w0, w1, w2, w3, w4};
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int r = yy - k;
+      if (r < 0 || r > 15) continue;
+      if (!FULL && r >= bh) continue;
+      const uint4 s4 = src.row(r);
+      const uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
+      if (FULL) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[k] = __builtin_amdgcn_qsad_pk_u16_u8(p[q], sv[q], acc[k]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          if (q < bwq) {
+            acc[k] = __builtin_amdgcn_qsad_pk_u16_u8(p[q], sv[q], acc[k]);
+          } else if (q == bwq && rem) {
+            // Last, partial source dword: mask the reference bytes too.
+            uint64_t add = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              const uint32_t ref = funnel(wv[q], wv[q + 1], i) & remmask;
+              add |= (uint64_t)sad4(sv[q], ref, 0) << (16 * i);
+            }
+            acc[k] += add;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const SearchArgs a) {
+  using SG = SearchGeom<R>;
+  constexpr int G = SG::G, K = SG::K, J = SG::J;
+  __shared__ uint32_t s_win[SG::BPW][SG::WIN_DW];
+  __shared__ uint64_t s_red[SG::G > 64 ? SG::G / 64 : 1];
+  __shared__ __align__(16) uint32_t s_src[SG::G > 64 ? SG::BPW * 64 : 4];
+  const int tid = threadIdx.x;
+  const int slot = tid / G;         // block slot in the workgroup
+  const int l = tid - slot * G;     // lane within the block's group
+  const int per_frame = a.brows * a.bcols;
+  const long long bi = (long long)blockIdx.x * SG::BPW + slot;
+  const bool live = bi < a.total_blocks;
+  // Out-of-range slots still run the staging/barriers with a valid block.
+  const int b = live ? (int)bi : 0;
+  const int kf = b / per_frame;
+  const int fi = kf < a.center ? kf : kf + 1;
+  const int rest = b - kf * per_frame;
+  const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
+  const int bx = bcol * 16, by = brow * 16;
+  const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
+  int sx = 0, sy = 0;
+  const size_t tab = 3 * ((size_t)fi * per_frame + rest);
+  if (a.seeds) {
+    sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
+    sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
+  }
+  const FrameDesc F = a.frames[fi];
+  const FrameDesc C = a.frames[a.center];
+  const int x0 = bx + sx - R;                 // leftmost candidate column
+  const int x0a = x0 >= 0 ? (x0 & ~3) : -(((-x0) + 3) & ~3);
+  const int shift0 = x0 - x0a;                // 0..3
+  const int y0 = by + sy - R;
+
+  // Stage the clamped window.
+  uint32_t* win = s_win[slot];
+  const bool interior = a.aligned4 && x0a >= 0 && x0a + 4 * SG::PD <= a.w;
+  for (int s = l; s < SG::WIN_DW; s += G) {
+    const int wy = s / SG::PD, wd = s - wy * SG::PD;
+    const uint8_t* row = F.y + (size_t)clampi(y0 + wy, 0, a.h - 1) * F.y_stride;
+    win[s] = interior ? *reinterpret_cast<const uint32_t*>(row + x0a + 4 * wd)
+                      : gather4_clamped(row, x0a + 4 * wd, a.w);
+  }
+  // Source block (zero outside bw x bh).
+  auto src_dword = [&](int r, int q) -> uint32_t {
+    uint32_t v = 0;
+    if (r < bh) {
+      const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
+      if (bw == 16 && a.aligned4) {
+        v = *reinterpret_cast<const uint32_t*>(row + 4 * q);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (4 * q + e < bw) v |= (uint32_t)row[4 * q + e] << (8 * e);
+      }
+    }
+    return v;
+  };
+  uint32_t S[16][4];
+  if (G > 64) {
+    if (l < 64) s_src[slot * 64 + l] = src_dword(l >> 2, l & 3);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t v = src_dword(r, q);
+        // One block per wave: the source is wave-uniform and lives in SGPRs.
+        S[r][q] = G == 64 ? (uint32_t)__builtin_amdgcn_readfirstlane((int)v) : v;
+      }
+    }
+  }
+  __syncthreads();
+  const SrcRows<R> src{S, s_src + (G > 64 ? slot * 64 : 0)};
+
+  const int j = l % J, db = l / J;
+  uint64_t best = ~0ull;
+  if (l < SG::LPB) {
+    uint64_t acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0;
+    if (bw == 16 && bh == 16) {
+      accumulate<R, true>(win, j, db, src, bh, bw, acc);
+    } else {
+      accumulate<R, false>(win, j, db, src, bh, bw, acc);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int dyi = db * K + k;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int dxi = 4 * j + i - shift0;
+        if (dyi < SG::D && dxi >= 0 && dxi < SG::D) {
+          const uint32_t sad = (uint32_t)(acc[k] >> (16 * i)) & 0xFFFFu;
+          best = min_u64(best, pack_key(sad, sx - R + dxi, sy - R + dyi));
+        }
+      }
+    }
+  }
+  // Reduce over the block's lane group.
+#pragma unroll
+  for (int m = 1; m < (G < 64 ? G : 64); m <<= 1) best = min_u64(best, shfl_xor_u64(best, m));
+  if (G > 64) {
+    if ((tid & 63) == 0) s_red[tid >> 6] = best;
+    __syncthreads();
+    best = s_red[0];
+#pragma unroll
+    for (int wv = 1; wv < (G > 64 ? G / 64 : 1); wv++) best = min_u64(best, s_red[wv]);
+  }
+  if (live && l == 0) {
+    a.out[tab] = (int32_t)(best & 0xFFFF) - 32768;
+    a.out[tab + 1] = (int32_t)((best >> 16) & 0xFFFF) - 32768;
+    a.out[tab + 2] = (int32_t)(best >> 48);
+  }
+}
+
+bool block_search_supported(int radius) { return radius == 2 || radius == 16 || radius == 64; }
+
+hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream) {
+  if (a.total_blocks <= 0) return hipSuccess;
+  switch (radius) {
+#define VP8G_SEARCH_CASE(RR)                                                                       \
+  case RR: {                                                                                       \
+    using SG = SearchGeom<RR>;                                                                     \
+    const int grid = (a.total_blocks + SG::BPW - 1) / SG::BPW;                                     \
+    hipLaunchKernelGGL(block_search_kernel<RR>, dim3(grid), dim3(SG::WG), 0, stream, a);           \
+    break;                                                                                         \
+  }
+    VP8G_SEARCH_CASE(2)
+    VP8G_SEARCH_CASE(16)
+    VP8G_SEARCH_CASE(64)
+#undef VP8G_SEARCH_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace vp8g

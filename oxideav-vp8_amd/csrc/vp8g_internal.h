@@ -76,6 +76,20 @@ constexpr size_t kMaxLdsPerWorkgroup = 65536;
 size_t arnr_fused_lds_bytes(int nw, int mbs_per_wg);
 int arnr_supported_mbs(int m);  // largest compiled strip width <= m (1, 2, 4, 8)
 hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream);
+hipError_t launch_arnr_table(const ArnrArgs& a, const int32_t* table, int nw, hipStream_t stream);
+
+// Exhaustive block search.
+constexpr int kMaxSeed = 16384;  // seed components are clamped to +-kMaxSeed
+struct SearchArgs {
+  FrameDesc frames[kMaxFrames];
+  const int32_t* seeds;   // optional, table layout n x brows x bcols x 3 ({sx, sy, -})
+  int32_t* out;           // table layout; centre rows untouched
+  int w, h, bcols, brows, n, center;
+  int aligned4;           // every luma plane pointer and stride is a multiple of 4
+  int total_blocks;       // (n - 1) * brows * bcols
+};
+bool block_search_supported(int radius);
+hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream);
 
 // Motion compensation / SAD scoring.
 struct McPlanes {

@@ -95,6 +95,7 @@ EXPORTS = (
     "vp8g_half_plane_sads_batch",
     "vp8g_arnr_batch_create", "vp8g_arnr_batch_run", "vp8g_arnr_batch_shape",
     "vp8g_arnr_batch_destroy", "vp8g_ctx_load_tuning", "vp8g_tuning_lookup", "vp8g_arnr_shape",
+    "vp8g_block_search_dev", "vp8g_build_arnr_altref_table_dev",
 )
 
 
@@ -132,6 +133,9 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_tuning_lookup": (i32, [C.c_char_p, C.c_uint32, C.c_uint32, sz, i32, C.POINTER(i32),
                                      C.POINTER(i32), C.POINTER(i32)]),
         "vp8g_arnr_shape": (i32, [vp, C.c_uint32, C.c_uint32, sz, u8, C.POINTER(i32), C.POINTER(i32)]),
+        "vp8g_block_search_dev": (i32, [vp, C.POINTER(_I420), sz, sz, i32, vp, vp, C.POINTER(_Err)]),
+        "vp8g_build_arnr_altref_table_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, vp,
+                                                   C.POINTER(_Err)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -386,6 +390,22 @@ class Context:
         st = self.lib.vp8g_build_arnr_altref_dev(self.h, descs, n, center, strength, oy, ou, ov,
                                                  C.byref(err), mv_out)
         self._raise(st, "vp8g_build_arnr_altref_dev")
+
+    def block_search_dev(self, descs, n: int, center: int, radius: int, out: int,
+                         seeds: int | None = None) -> None:
+        """Exhaustive +-radius block search over device-resident frames into
+        the alignment-table layout at device pointer ``out``.  Enqueues only."""
+        err = _Err()
+        st = self.lib.vp8g_block_search_dev(self.h, descs, n, center, radius, seeds, out, C.byref(err))
+        self._raise(st, "vp8g_block_search_dev")
+
+    def build_arnr_altref_table_dev(self, descs, n: int, center: int, strength: int, table: int,
+                                    oy: int, ou: int, ov: int) -> None:
+        """The altref from a given device alignment table.  Enqueues only."""
+        err = _Err()
+        st = self.lib.vp8g_build_arnr_altref_table_dev(self.h, descs, n, center, strength, table, oy,
+                                                       ou, ov, C.byref(err))
+        self._raise(st, "vp8g_build_arnr_altref_table_dev")
 
     def create_arnr_batch(self, jobs) -> "ArnrBatch":
         """Batched ARNR over device-resident groups.  ``jobs`` is a sequence
