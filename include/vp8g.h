@@ -165,6 +165,79 @@ int vp8g_build_arnr_altref_table_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, siz
                                      uint8_t *out_y, uint8_t *out_u, uint8_t *out_v,
                                      vp8g_err *err);
 
+/* ---- learned motion proposer ----
+ * No counterpart in the reference: a ~60k-parameter BF16 CNN (hand-written
+ * MFMA kernels, fp32 accumulation and fp32 master parameters) that
+ * proposes one whole-pel vector per 16x16 block from the 4x-downsampled
+ * (anchor, neighbour) luma pair, for callers that seed a short exact search
+ * (vp8g_block_search_dev with radius 2) instead of the full +-16 one.
+ * Inputs ("planes"): u8 [image][2][4*ceil(h/16)][4*ceil(w/16)], the anchor
+ * then the neighbour, each pixel the rounded mean of a 4x4 full-resolution
+ * block (clamped at the frame edge).  Network: a fixed correlation layer
+ * (per block, the 81 mean absolute differences of its 4x4 anchor pixels
+ * against the neighbour displaced by -4..4 downsampled pixels, plus both
+ * frames' pixels; 128 channels), a 1x1 conv 128->32, 3x3 convs 32->32,
+ * 32->64, 64->32, 32->32 (leaky ReLU, slope 1/16), and a 1x1 head 32->2
+ * predicting mv / 8.
+ * Parameters: vp8g_proposer_param_count() fp32 values, per conv layer
+ * weights [taps][cout][cin] then bias[cout], then the head [2][32] and
+ * bias[2].  Labels: int32 [image][block][2] = {mvx, mvy}; training loss:
+ * mean Huber (delta 2 px) on mv / 8 over all label entries, where a block
+ * whose mvx is -32768 contributes zero.  All pointers device, 16-byte aligned;
+ * everything runs on the context's stream. */
+typedef struct vp8g_proposer vp8g_proposer;
+size_t vp8g_proposer_param_count(void);
+int vp8g_proposer_create(vp8g_ctx *ctx, vp8g_proposer **out);
+void vp8g_proposer_destroy(vp8g_proposer *p);
+/* Host copies of the parameters; setting them resets the optimizer. */
+int vp8g_proposer_set_params(vp8g_proposer *p, const float *params, size_t n);
+int vp8g_proposer_get_params(vp8g_proposer *p, float *params, size_t n);
+/* Gradients of the last backward pass and its mean Huber loss. */
+int vp8g_proposer_get_grads(vp8g_proposer *p, float *grads, size_t n, float *loss);
+/* Planes of the n_frames - 1 (centre, other frame) pairs, in frame order. */
+int vp8g_proposer_prepare_dev(vp8g_proposer *p, const vp8g_i420 *frames, size_t n_frames,
+                              size_t center, uint8_t *planes);
+/* Forward pass; a4_out (may be NULL) receives the last hidden layer,
+ * bf16 [image][block][32]. */
+int vp8g_proposer_forward_dev(vp8g_proposer *p, const uint8_t *planes, int images, int blocks_w,
+                              int blocks_h, uint16_t *a4_out);
+/* Proposals for every non-centre frame into an alignment table (layout of
+ * vp8g_block_search_dev's out; sad entries are 0), components rounded and
+ * clamped to +-clamp. */
+int vp8g_proposer_propose_dev(vp8g_proposer *p, const vp8g_i420 *frames, size_t n_frames,
+                              size_t center, int clamp, int32_t *table);
+/* Forward + backward (gradients kept, parameters unchanged). */
+int vp8g_proposer_backward_dev(vp8g_proposer *p, const uint8_t *planes, const int32_t *labels,
+                               int images, int blocks_w, int blocks_h);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr;
+ * loss_dev (device float, may be NULL) receives the step's loss. */
+int vp8g_proposer_train_step_dev(vp8g_proposer *p, const uint8_t *planes, const int32_t *labels,
+                                 int images, int blocks_w, int blocks_h, float lr, float *loss_dev);
+
+/* Synthetic region-motion luma frames on the device (training data for the
+ * proposer; no counterpart in the reference).  Frame t, pixel (x, y):
+ * a 3x3-box-blurred splitmix64 noise texture (texture_seed) sampled at
+ * (x - t*vx, y - t*vy) clamped, (vx, vy) = vel[8*ry + rx] for the 8x8
+ * region grid; then each rect of frame t (in order) replaced by its own
+ * blurred noise; then + Gaussian(sigma) noise (Box-Muller on splitmix64
+ * draws of noise_seeds[t]), rounded and clipped.  `out` and `clean` (may be
+ * NULL; the frames before the Gaussian stage) are device [n][h][w].
+ * Returns after the frames are written. */
+typedef struct {
+    uint64_t seed;
+    uint32_t x0, y0, w, h;
+} vp8g_synth_rect;
+typedef struct {
+    uint32_t width, height;
+    size_t n, rects_per_frame;
+    uint64_t texture_seed;
+    const int32_t *vel;             /* host [64][2] */
+    const vp8g_synth_rect *rects;   /* host [n][rects_per_frame] */
+    const uint64_t *noise_seeds;    /* host [n] */
+    double sigma;
+} vp8g_synth_spec;
+int vp8g_synth_luma_dev(vp8g_ctx *ctx, const vp8g_synth_spec *spec, uint8_t *out, uint8_t *clean);
+
 /* ---- batched ARNR ----
  * Many groups (clips of any resolution and length) in one persistent launch,
  * captured once into a HIP graph and replayed by vp8g_arnr_batch_run.

@@ -25,7 +25,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-SOURCES = ["capi.hip", "arnr_kernels.hip", "mc_kernels.hip", "search_kernels.hip", "tuning.cpp"]
+SOURCES = ["capi.hip", "arnr_kernels.hip", "mc_kernels.hip", "search_kernels.hip", "proposer_kernels.hip",
+           "proposer_host.hip", "synth_kernels.hip", "synth_host.hip", "tuning.cpp"]
 
 
 def build(verbose: bool = False) -> Path:

@@ -10,24 +10,10 @@
 
 #include "../../include/vp8g.h"
 #include "tuning.h"
+#include "vp8g_ctx.h"
 #include "vp8g_internal.h"
 
 using namespace vp8g;
-
-struct vp8g_ctx {
-  int device = 0;
-  hipStream_t own = nullptr;
-  hipStream_t stream = nullptr;
-  int arnr_waves = 0;
-  int xcd_remap = 1;
-  int arnr_mbs = 0;
-  int batch_wgs_per_cu = 0;
-  int cus = 0;
-  std::vector<TuneEntry> tuning;  // launch shapes by (resolution class, group length)
-  // Device arena for the host-pointer entry points (grown, never shrunk).
-  uint8_t* arena = nullptr;
-  size_t arena_bytes = 0;
-};
 
 struct vp8g_arnr_batch {
   vp8g_ctx* ctx = nullptr;
@@ -263,6 +249,10 @@ int finish(vp8g_ctx* c, int st) {
 }
 
 }  // namespace
+
+int vp8g_validate_group(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* err) {
+  return validate_arnr(frames, n, center, err);
+}
 
 extern "C" {
 

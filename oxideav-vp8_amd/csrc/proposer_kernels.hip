@@ -1,0 +1,508 @@
+// Learned motion proposer: a small BF16 CNN on MFMA (gfx950).
+//
+// Input: the 4x-downsampled (anchor, neighbour) luma pair, u8.
+// Network (per 16x16 macroblock "cell" = 4x4 downsampled pixels):
+//   x0  = features: the 81 mean absolute differences of the cell's 4x4
+//         anchor pixels against the neighbour displaced by (ddx, ddy) in
+//         [-4, 4]^2 downsampled pixels (a fixed correlation layer), then
+//         the cell's anchor and neighbour pixels / 255, zero-padded to 128
+//   a0  = lrelu(conv1x1(x0, 128 -> 32) + b0)
+//   a1  = lrelu(conv3x3(a0, 32 -> 32) + b1)
+//   a2  = lrelu(conv3x3(a1, 32 -> 64) + b2)
+//   a3  = lrelu(conv3x3(a2, 64 -> 32) + b3)
+//   a4  = lrelu(conv3x3(a3, 32 -> 32) + b4)
+//   (lrelu(z) = z for z > 0, z / 16 otherwise)
+//   mv  = 8 * (W5 a4 + b5)                      (1x1, 32 -> 2: mvx, mvy)
+// Activations are NHWC bf16 [image][cell row][cell col][channel]; products
+// run on v_mfma_f32_16x16x32_bf16 with fp32 accumulation.  Parameters are
+// fp32 masters; the kernels read bf16 copies of the conv weights.
+//
+// MFMA operand layout (lane l, element i): A[l%16][8*(l/16)+i],
+// B[8*(l/16)+i][l%16], D[4*(l/16)+i][l%16].
+#include "proposer.h"
+#include "vp8g_device.h"
+
+namespace vp8g {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(const uint4& v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ uint16_t bf16_bits(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ float bf16_val(uint16_t b) { return (float)__builtin_bit_cast(__bf16, b); }
+
+// ---------------------------------------------------------------- inputs
+// One thread per downsampled pixel of each (pair, frame) plane.
+__global__ void prep_downsample_kernel(const PrepArgs a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int dw = 4 * a.cw, dh = 4 * a.ch;
+  const long long plane = (long long)dw * dh;
+  if (t >= (long long)a.pairs * 2 * plane) return;
+  const int pf = (int)(t / plane);
+  const int rem = (int)(t - (long long)pf * plane);
+  const int y = rem / dw, x = rem - y * dw;
+  const int p = pf >> 1, f = pf & 1;
+  const uint8_t* src = f == 0 ? a.anchor : a.neighbours[p];
+  const uint32_t stride = f == 0 ? a.anchor_stride : a.neighbour_stride[p];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int yy = 0; yy < 4; yy++) {
+    const uint8_t* row = src + (size_t)min(4 * y + yy, a.h - 1) * stride;
+#pragma unroll
+    for (int xx = 0; xx < 4; xx++) sum += row[min(4 * x + xx, a.w - 1)];
+  }
+  a.out[t] = (uint8_t)((sum + 8) >> 4);
+}
+
+// One thread per (image, cell, feature channel).
+__global__ void features_kernel(const FeatArgs a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cells = a.cw * a.ch;
+  if (t >= (long long)a.images * cells * kFeat) return;
+  const int c = (int)(t % kFeat);
+  const long long ic = t / kFeat;
+  const int img = (int)(ic / cells);
+  const int cell = (int)(ic - (long long)img * cells);
+  const int cy = cell / a.cw, cx = cell - cy * a.cw;
+  const int dw = 4 * a.cw, dh = 4 * a.ch;
+  const uint8_t* A = a.planes + (size_t)img * 2 * dw * dh;
+  const uint8_t* B = A + (size_t)dw * dh;
+  const int x0 = 4 * cx, y0 = 4 * cy;
+  float v = 0.f;
+  if (c < kCostChannels) {
+    const int ddx = c % (2 * kSearch + 1) - kSearch, ddy = c / (2 * kSearch + 1) - kSearch;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint8_t* ar = A + (size_t)(y0 + j) * dw + x0;
+      const uint8_t* br = B + (size_t)clampi(y0 + j + ddy, 0, dh - 1) * dw;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int d = (int)ar[i] - (int)br[clampi(x0 + i + ddx, 0, dw - 1)];
+        s += (uint32_t)(d < 0 ? -d : d);
+      }
+    }
+    v = (float)s * (1.0f / 1024.0f);
+  } else if (c < kCostChannels + 32) {
+    const int k = c - kCostChannels;
+    const uint8_t* P = k < 16 ? A : B;
+    const int q = k & 15;
+    v = (float)P[(size_t)(y0 + (q >> 2)) * dw + x0 + (q & 3)] * (1.0f / 255.0f);
+  }
+  a.out[t] = bf16_bits(v);
+}
+
+// ---------------------------------------------------------------- conv3x3
+// LDS rows of 16-byte chunks are XOR-swizzled by the row index so that the
+// 16 lanes reading one chunk column of 16 consecutive rows hit distinct banks.
+template <int C>
+__device__ __forceinline__ int swz(int row, int chunk) {
+  return chunk ^ (row & (C / 8 - 1));
+}
+
+// out = EPI_BIAS_RELU: lrelu(conv(in, w) + bias)
+//       EPI_MASK:      conv(in, w) * (mask > 0 ? 1 : 1/16)  (backward through lrelu)
+// w: [TAPS][COUT][CIN] bf16; TAPS 9: 3x3, tap = 3 * dy + dx reads offset
+// (dy - 1, dx - 1), zero padded; TAPS 1: 1x1.
+// One persistent workgroup of 4 waves per tile of 64 cells of one cell row;
+// wave w computes cells 16w .. 16w + 15 for all COUT channels.
+template <int TAPS, int CIN, int COUT, int EPI>
+__global__ void __launch_bounds__(256) conv_kernel(const ConvArgs a) {
+  constexpr int CK = CIN / 8;   // 16-byte chunks per cell
+  constexpr int TW = 64;        // cells per tile
+  constexpr int HALO = TAPS == 9 ? 1 : 0;
+  constexpr int ROWS = 1 + 2 * HALO;
+  constexpr int IW = TW + 2 * HALO;  // staged cells per row
+  __shared__ __align__(16) uint4 s_w[TAPS * COUT * CK];
+  __shared__ __align__(16) uint4 s_in[ROWS * IW * CK];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+
+  const uint4* wg = reinterpret_cast<const uint4*>(a.w);
+  for (int i = tid; i < TAPS * COUT * CK; i += 256) {
+    const int row = i / CK, ck = i - row * CK;  // row = tap * COUT + co
+    s_w[row * CK + swz<CIN>(row, ck)] = wg[i];
+  }
+  const int segs = (a.cw + TW - 1) / TW;
+  const int total = a.images * a.ch * segs;
+  const uint4* in = reinterpret_cast<const uint4*>(a.in);
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const int img = tile / (a.ch * segs);
+    const int rem = tile - img * a.ch * segs;
+    const int cy = rem / segs, cx0 = (rem - cy * segs) * TW;
+    __syncthreads();  // previous tile's readers are done (and weights staged)
+    for (int i = tid; i < ROWS * IW * CK; i += 256) {
+      const int row = i / CK, ck = i - row * CK;  // row = r * IW + c
+      const int r = row / IW, c = row - r * IW;
+      const int y = cy + r - HALO, x = cx0 + c - HALO;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (y >= 0 && y < a.ch && x >= 0 && x < a.cw) v = in[(((size_t)img * a.ch + y) * a.cw + x) * CK + ck];
+      s_in[row * CK + swz<CIN>(row, ck)] = v;
+    }
+    __syncthreads();
+    f32x4 acc[COUT / 16];
+#pragma unroll
+    for (int m = 0; m < COUT / 16; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < TAPS; t++) {
+      const int dy = TAPS == 9 ? t / 3 : 0, dx = TAPS == 9 ? t - 3 * (t / 3) : 0;
+      const int brow = dy * IW + 16 * wave + (l & 15) + dx;
+#pragma unroll
+      for (int c = 0; c < CIN / 32; c++) {
+        const int ck = 4 * c + (l >> 4);
+        const bf16x8 b = as_bf16x8(s_in[brow * CK + swz<CIN>(brow, ck)]);
+#pragma unroll
+        for (int m = 0; m < COUT / 16; m++) {
+          const int arow = t * COUT + 16 * m + (l & 15);
+          const bf16x8 av = as_bf16x8(s_w[arow * CK + swz<CIN>(arow, ck)]);
+          acc[m] = mfma16(av, b, acc[m]);
+        }
+      }
+    }
+    const int cell_x = cx0 + 16 * wave + (l & 15);
+    if (cell_x < a.cw) {
+      const size_t cell = ((size_t)img * a.ch + cy) * a.cw + cell_x;
+#pragma unroll
+      for (int m = 0; m < COUT / 16; m++) {
+        const int co = 16 * m + 4 * (l >> 4);
+        float v[4] = {acc[m][0], acc[m][1], acc[m][2], acc[m][3]};
+        if (EPI == EPI_BIAS_RELU) {
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const float z = v[i] + a.bias[co + i];
+            v[i] = z > 0.f ? z : kLeak * z;
+          }
+        } else {
+          const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + cell * COUT + co);
+          const uint16_t mb[4] = {(uint16_t)mk.x, (uint16_t)(mk.x >> 16), (uint16_t)mk.y, (uint16_t)(mk.y >> 16)};
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            if (!(bf16_val(mb[i]) > 0.f)) v[i] *= kLeak;
+        }
+        uint2 o;
+        o.x = (uint32_t)bf16_bits(v[0]) | ((uint32_t)bf16_bits(v[1]) << 16);
+        o.y = (uint32_t)bf16_bits(v[2]) | ((uint32_t)bf16_bits(v[3]) << 16);
+        *reinterpret_cast<uint2*>(a.out + cell * COUT + co) = o;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- wgrad
+// Partial weight and bias gradients of one conv over a slice of 32-cell
+// chunks: dW[t][co][ci] = sum_p dy[p][co] * x[p + d_t][ci], db[co] =
+// sum_p dy[p][co].  dy and x are staged transposed (channel-major, 32 cells
+// per row) so both MFMA operands read 16 contiguous bytes; x is staged once
+// per tap (the dx shift is an element offset).  Each workgroup writes its
+// own partial slot; a reduce kernel sums the slots in order.
+template <int TAPS, int CIN, int COUT>
+__global__ void __launch_bounds__(256) wgrad_kernel(const WgradArgs a) {
+  constexpr int KC = 32;                         // cells per chunk (MFMA K)
+  constexpr int HALO = TAPS == 9 ? 1 : 0;
+  constexpr int TILES = TAPS * (COUT / 16) * (CIN / 16);
+  constexpr int TPW = TILES / 4;                 // tiles per wave
+  static_assert(TILES % 4 == 0, "tiles");
+  __shared__ __align__(16) uint16_t s_dy[COUT * KC];
+  __shared__ __align__(16) uint16_t s_x[TAPS * CIN * KC];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int segs = (a.cw + KC - 1) / KC;
+  const int total = a.images * a.ch * segs;
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  for (int chunk = blockIdx.x; chunk < total; chunk += gridDim.x) {
+    const int img = chunk / (a.ch * segs);
+    const int rem = chunk - img * a.ch * segs;
+    const int cy = rem / segs, cx0 = (rem - cy * segs) * KC;
+    __syncthreads();
+    // dy^T: s_dy[co][p]
+    for (int i = tid; i < KC * (COUT / 8); i += 256) {
+      const int p = i / (COUT / 8), ck = i - p * (COUT / 8);
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (cx0 + p < a.cw) v = *reinterpret_cast<const uint4*>(a.dy + (((size_t)img * a.ch + cy) * a.cw + cx0 + p) * COUT + 8 * ck);
+      const uint16_t e[8] = {(uint16_t)v.x, (uint16_t)(v.x >> 16), (uint16_t)v.y, (uint16_t)(v.y >> 16),
+                             (uint16_t)v.z, (uint16_t)(v.z >> 16), (uint16_t)v.w, (uint16_t)(v.w >> 16)};
+#pragma unroll
+      for (int j = 0; j < 8; j++) s_dy[(8 * ck + j) * KC + p] = e[j];
+    }
+    // x^T per tap: s_x[t][ci][p] = x[cy + dy - 1][cx0 + p + dx - 1][ci]
+    for (int i = tid; i < (1 + 2 * HALO) * (KC + 2 * HALO) * (CIN / 8); i += 256) {
+      const int rc = i / (CIN / 8), ck = i - rc * (CIN / 8);
+      const int r = rc / (KC + 2 * HALO), c = rc - r * (KC + 2 * HALO);
+      const int y = cy + r - HALO, x = cx0 + c - HALO;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (y >= 0 && y < a.ch && x >= 0 && x < a.cw)
+        v = *reinterpret_cast<const uint4*>(a.x + (((size_t)img * a.ch + y) * a.cw + x) * CIN + 8 * ck);
+      const uint16_t e[8] = {(uint16_t)v.x, (uint16_t)(v.x >> 16), (uint16_t)v.y, (uint16_t)(v.y >> 16),
+                             (uint16_t)v.z, (uint16_t)(v.z >> 16), (uint16_t)v.w, (uint16_t)(v.w >> 16)};
+#pragma unroll
+      for (int dx = 0; dx < 1 + 2 * HALO; dx++) {
+        const int p = c - dx;
+        if (p >= 0 && p < KC) {
+#pragma unroll
+          for (int j = 0; j < 8; j++) s_x[(((1 + 2 * HALO) * r + dx) * CIN + 8 * ck + j) * KC + p] = e[j];
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < COUT) {
+      float s = 0.f;
+      for (int p = 0; p < KC; p++) s += bf16_val(s_dy[tid * KC + p]);
+      bsum += s;
+    }
+#pragma unroll
+    for (int i = 0; i < TPW; i++) {
+      const int tile = wave * TPW + i;
+      const int t = tile / ((COUT / 16) * (CIN / 16));
+      const int mt = (tile / (CIN / 16)) % (COUT / 16);
+      const int nt = tile % (CIN / 16);
+      const bf16x8 av = as_bf16x8(*reinterpret_cast<const uint4*>(s_dy + (16 * mt + (l & 15)) * KC + 8 * (l >> 4)));
+      const bf16x8 bv = as_bf16x8(*reinterpret_cast<const uint4*>(s_x + (t * CIN + 16 * nt + (l & 15)) * KC + 8 * (l >> 4)));
+      acc[i] = mfma16(av, bv, acc[i]);
+    }
+  }
+  float* part = a.partials + (size_t)blockIdx.x * (TAPS * COUT * CIN + COUT);
+#pragma unroll
+  for (int i = 0; i < TPW; i++) {
+    const int tile = wave * TPW + i;
+    const int t = tile / ((COUT / 16) * (CIN / 16));
+    const int mt = (tile / (CIN / 16)) % (COUT / 16);
+    const int nt = tile % (CIN / 16);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int co = 16 * mt + 4 * (l >> 4) + e, ci = 16 * nt + (l & 15);
+      part[(t * COUT + co) * CIN + ci] = acc[i][e];
+    }
+  }
+  if (tid < COUT) part[TAPS * COUT * CIN + tid] = bsum;
+}
+
+// Sum `slots` partial vectors of length n (slot-major) into out (+= when
+// accumulate), in slot order.
+__global__ void reduce_partials_kernel(const float* partials, int slots, int n, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int k = 0; k < slots; k++) s += partials[(size_t)k * n + i];
+  out[i] = s;
+}
+
+// ---------------------------------------------------------------- head
+// Per cell: pred = W5 a4 + b5 (fp32 from bf16 activations).  Inference
+// writes the rounded, clamped vector into the alignment table; training
+// adds the Huber loss against label / 8, the gradient of a4's
+// pre-activation (bf16) and per-workgroup partial sums
+// {dW5[2][32], db5[2], loss}.
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) head_kernel(const HeadArgs a) {
+  __shared__ float s_red[4 * kHeadPartials];
+  const int tid = threadIdx.x;
+  const long long cell = (long long)blockIdx.x * 256 + tid;
+  const long long cells = (long long)a.images * a.cells;
+  const bool live = cell < cells;
+  float gw[2][32];
+  float g[2] = {0.f, 0.f}, loss = 0.f;
+#pragma unroll
+  for (int o = 0; o < 2; o++)
+#pragma unroll
+    for (int c = 0; c < 32; c++) gw[o][c] = 0.f;
+  if (live) {
+    float x[32];
+    const uint4* src = reinterpret_cast<const uint4*>(a.a4 + cell * 32);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint4 v = src[q];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        x[8 * q + 2 * j] = bf16_val((uint16_t)w[j]);
+        x[8 * q + 2 * j + 1] = bf16_val((uint16_t)(w[j] >> 16));
+      }
+    }
+    float pred[2];
+#pragma unroll
+    for (int o = 0; o < 2; o++) {
+      float s = a.params[64 + o];
+#pragma unroll
+      for (int c = 0; c < 32; c++) s += a.params[32 * o + c] * x[c];
+      pred[o] = s;
+    }
+    if (a.table) {
+      const int img = (int)(cell / a.cells), rest = (int)(cell - (long long)img * a.cells);
+      int32_t* t = a.table + 3 * ((size_t)a.frame_of_image[img] * a.cells + rest);
+#pragma unroll
+      for (int o = 0; o < 2; o++) {
+        float mv = rintf(pred[o] * kMvScale);
+        mv = fminf(fmaxf(mv, (float)-a.clamp), (float)a.clamp);
+        t[o] = (int32_t)mv;
+      }
+      t[2] = 0;
+    }
+    // A label whose mvx is kIgnoreLabel contributes no loss and no gradient.
+    if (a.labels && a.labels[2 * cell] != kIgnoreLabel) {
+#pragma unroll
+      for (int o = 0; o < 2; o++) {
+        const float d = pred[o] - (float)a.labels[2 * cell + o] * (1.0f / kMvScale);
+        const float ad = fabsf(d);
+        loss += (ad < kHuberDelta ? 0.5f * d * d / kHuberDelta : ad - 0.5f * kHuberDelta) * a.inv_count;
+        g[o] = (ad < kHuberDelta ? d / kHuberDelta : (d > 0.f ? 1.f : -1.f)) * a.inv_count;
+      }
+      uint4 out[4];
+      uint32_t* ow = reinterpret_cast<uint32_t*>(out);
+#pragma unroll
+      for (int c = 0; c < 32; c += 2) {
+        float v0 = a.params[c] * g[0] + a.params[32 + c] * g[1];
+        float v1 = a.params[c + 1] * g[0] + a.params[32 + c + 1] * g[1];
+        if (!(x[c] > 0.f)) v0 *= kLeak;
+        if (!(x[c + 1] > 0.f)) v1 *= kLeak;
+        ow[c / 2] = (uint32_t)bf16_bits(v0) | ((uint32_t)bf16_bits(v1) << 16);
+      }
+      uint4* dst = reinterpret_cast<uint4*>(a.g4 + cell * 32);
+#pragma unroll
+      for (int q = 0; q < 4; q++) dst[q] = out[q];
+#pragma unroll
+      for (int o = 0; o < 2; o++)
+#pragma unroll
+        for (int c = 0; c < 32; c++) gw[o][c] = g[o] * x[c];
+    } else if (a.labels) {
+      uint4* dst = reinterpret_cast<uint4*>(a.g4 + cell * 32);
+#pragma unroll
+      for (int q = 0; q < 4; q++) dst[q] = make_uint4(0, 0, 0, 0);
+    }
+  }
+  if (!a.labels) return;
+  // Workgroup sums of the 67 training statistics: wave sums, then the 4
+  // waves in order.
+  const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int k = 0; k < kHeadPartials; k++) {
+    const float v = wave_sum_f(k < 64 ? gw[k >> 5][k & 31] : (k < 66 ? g[k - 64] : loss));
+    if (lane == 0) s_red[wave * kHeadPartials + k] = v;
+  }
+  __syncthreads();
+  if (tid < kHeadPartials) {
+    float* part = a.partials + (size_t)blockIdx.x * kHeadPartials;
+    part[tid] = ((s_red[tid] + s_red[kHeadPartials + tid]) + s_red[2 * kHeadPartials + tid]) +
+                s_red[3 * kHeadPartials + tid];
+  }
+}
+
+// ---------------------------------------------------------------- Adam
+// params, grads, m, v: fp32 [n]; bf16 conv weights are rewritten from the
+// new masters: forward [9][COUT][CIN] and backward-data [9][CIN][COUT] with
+// the tap flipped (t -> 8 - t).
+__global__ void adam_kernel(const AdamArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const float g = a.grads[i];
+  const float m = a.b1 * a.m[i] + (1.f - a.b1) * g;
+  const float v = a.b2 * a.v[i] + (1.f - a.b2) * g * g;
+  a.m[i] = m;
+  a.v[i] = v;
+  const float p = a.params[i] - a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
+  a.params[i] = p;
+}
+
+__global__ void pack_weights_kernel(const PackArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.taps * a.cout * a.cin;
+  if (i >= n) return;
+  const int t = i / (a.cout * a.cin), r = i - t * a.cout * a.cin;
+  const int co = r / a.cin, ci = r - co * a.cin;
+  const uint16_t b = bf16_bits(a.master[i]);
+  a.fwd[i] = b;
+  if (a.bwd) a.bwd[((a.taps - 1 - t) * a.cin + ci) * a.cout + co] = b;
+}
+
+// ---------------------------------------------------------------- launchers
+static int persistent_grid(int total, int per_cu, int cus) {
+  const int g = per_cu * cus;
+  return total < g ? total : g;
+}
+
+hipError_t launch_prep_downsample(const PrepArgs& a, hipStream_t s) {
+  const long long n = (long long)a.pairs * 2 * (4 * a.cw) * (4 * a.ch);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_downsample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_features(const FeatArgs& a, hipStream_t s) {
+  const long long n = (long long)a.images * a.cw * a.ch * kFeat;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_conv(const ConvArgs& a, int taps, int cin, int cout, int epi, int cus, hipStream_t s) {
+  const int total = a.images * a.ch * ((a.cw + 63) / 64);
+  if (total == 0) return hipSuccess;
+  const dim3 grid(persistent_grid(total, 2, cus)), block(256);
+#define VP8G_CONV(T, CI, CO, E)                                                              \
+  if (taps == T && cin == CI && cout == CO && epi == E) {                                    \
+    hipLaunchKernelGGL((conv_kernel<T, CI, CO, E>), grid, block, 0, s, a);                   \
+    return hipGetLastError();                                                                \
+  }
+  VP8G_CONV(1, kFeat, 32, EPI_BIAS_RELU)
+  VP8G_CONV(9, 32, 32, EPI_BIAS_RELU)
+  VP8G_CONV(9, 32, 64, EPI_BIAS_RELU)
+  VP8G_CONV(9, 64, 32, EPI_BIAS_RELU)
+  VP8G_CONV(9, 32, 32, EPI_MASK)
+  VP8G_CONV(9, 64, 32, EPI_MASK)
+  VP8G_CONV(9, 32, 64, EPI_MASK)
+#undef VP8G_CONV
+  return hipErrorInvalidValue;
+}
+
+int wgrad_slots(const WgradArgs& a, int cus) {
+  const int total = a.images * a.ch * ((a.cw + 31) / 32);
+  return persistent_grid(total, 2, cus);
+}
+
+hipError_t launch_wgrad(const WgradArgs& a, int taps, int cin, int cout, int slots, hipStream_t s) {
+  if (slots <= 0) return hipSuccess;
+  const dim3 grid(slots), block(256);
+  if (taps == 1 && cin == kFeat && cout == 32) hipLaunchKernelGGL((wgrad_kernel<1, kFeat, 32>), grid, block, 0, s, a);
+  else if (taps == 9 && cin == 32 && cout == 32) hipLaunchKernelGGL((wgrad_kernel<9, 32, 32>), grid, block, 0, s, a);
+  else if (taps == 9 && cin == 32 && cout == 64) hipLaunchKernelGGL((wgrad_kernel<9, 32, 64>), grid, block, 0, s, a);
+  else if (taps == 9 && cin == 64 && cout == 32) hipLaunchKernelGGL((wgrad_kernel<9, 64, 32>), grid, block, 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const float* partials, int slots, int n, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partials, slots, n, out);
+  return hipGetLastError();
+}
+
+int head_blocks(long long cells) { return (int)((cells + 255) / 256); }
+
+hipError_t launch_head(const HeadArgs& a, hipStream_t s) {
+  const long long cells = (long long)a.images * a.cells;
+  if (cells == 0) return hipSuccess;
+  hipLaunchKernelGGL(head_kernel, dim3(head_blocks(cells)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_weights(const PackArgs& a, hipStream_t s) {
+  const int n = a.taps * a.cout * a.cin;
+  hipLaunchKernelGGL(pack_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace vp8g

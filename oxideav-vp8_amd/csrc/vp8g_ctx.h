@@ -1,0 +1,27 @@
+// The context object behind the C ABI's vp8g_ctx handle (host only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/vp8g.h"
+#include "tuning.h"
+
+struct vp8g_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  int arnr_waves = 0;
+  int xcd_remap = 1;
+  int arnr_mbs = 0;
+  int batch_wgs_per_cu = 0;
+  int cus = 0;
+  std::vector<vp8g::TuneEntry> tuning;  // launch shapes by (resolution class, group length)
+  // Device arena for the host-pointer entry points (grown, never shrunk).
+  uint8_t* arena = nullptr;
+  size_t arena_bytes = 0;
+};
+
+// Group validation in the reference's order (src/arnr.rs:494-515) plus the
+// library's own limits; shared by every group entry point.
+int vp8g_validate_group(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* err);

@@ -85,6 +85,18 @@ class _ArnrJob(C.Structure):
                 ("out_v", C.c_void_p), ("mv_out", C.c_void_p)]
 
 
+class _SynthRect(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("x0", C.c_uint32), ("y0", C.c_uint32), ("w", C.c_uint32),
+                ("h", C.c_uint32)]
+
+
+class _SynthSpec(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("n", C.c_size_t),
+                ("rects_per_frame", C.c_size_t), ("texture_seed", C.c_uint64),
+                ("vel", C.c_void_p), ("rects", C.c_void_p), ("noise_seeds", C.c_void_p),
+                ("sigma", C.c_double)]
+
+
 EXPORTS = (
     "vp8g_ctx_create", "vp8g_ctx_destroy", "vp8g_ctx_set_stream", "vp8g_ctx_stream",
     "vp8g_ctx_synchronize", "vp8g_ctx_set_option", "vp8g_status_str",
@@ -96,6 +108,10 @@ EXPORTS = (
     "vp8g_arnr_batch_create", "vp8g_arnr_batch_run", "vp8g_arnr_batch_shape",
     "vp8g_arnr_batch_destroy", "vp8g_ctx_load_tuning", "vp8g_tuning_lookup", "vp8g_arnr_shape",
     "vp8g_block_search_dev", "vp8g_build_arnr_altref_table_dev",
+    "vp8g_proposer_param_count", "vp8g_proposer_create", "vp8g_proposer_destroy",
+    "vp8g_proposer_set_params", "vp8g_proposer_get_params", "vp8g_proposer_get_grads",
+    "vp8g_proposer_prepare_dev", "vp8g_proposer_forward_dev", "vp8g_proposer_propose_dev",
+    "vp8g_proposer_backward_dev", "vp8g_proposer_train_step_dev", "vp8g_synth_luma_dev",
 )
 
 
@@ -136,6 +152,18 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_block_search_dev": (i32, [vp, C.POINTER(_I420), sz, sz, i32, vp, vp, C.POINTER(_Err)]),
         "vp8g_build_arnr_altref_table_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, vp,
                                                    C.POINTER(_Err)]),
+        "vp8g_proposer_param_count": (sz, []),
+        "vp8g_proposer_create": (i32, [vp, C.POINTER(vp)]),
+        "vp8g_proposer_destroy": (None, [vp]),
+        "vp8g_proposer_set_params": (i32, [vp, vp, sz]),
+        "vp8g_proposer_get_params": (i32, [vp, vp, sz]),
+        "vp8g_proposer_get_grads": (i32, [vp, vp, sz, C.POINTER(C.c_float)]),
+        "vp8g_proposer_prepare_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp]),
+        "vp8g_proposer_forward_dev": (i32, [vp, vp, i32, i32, i32, vp]),
+        "vp8g_proposer_propose_dev": (i32, [vp, C.POINTER(_I420), sz, sz, i32, vp]),
+        "vp8g_proposer_backward_dev": (i32, [vp, vp, vp, i32, i32, i32]),
+        "vp8g_proposer_train_step_dev": (i32, [vp, vp, vp, i32, i32, i32, C.c_float, vp]),
+        "vp8g_synth_luma_dev": (i32, [vp, C.POINTER(_SynthSpec), vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

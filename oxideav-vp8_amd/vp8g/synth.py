@@ -88,6 +88,7 @@ class Group:
     u: list   # per frame (ch, cw)
     v: list
     seed: int
+    clean: list | None = None  # per frame (y, u, v) before the noise stage
 
     @property
     def n(self) -> int:
@@ -160,7 +161,7 @@ def arnr_group_regions(width: int = 1920, height: int = 1080, n: int = 7, vmax: 
     u0 = blurred_noise(base + 2, ch, cw)
     v0 = blurred_noise(base + 3, ch, cw)
     vel = uniform_int(base + 4, 128, -vmax, vmax).reshape(64, 2)
-    ys, us, vs = [], [], []
+    ys, us, vs, clean = [], [], [], []
     for t in range(n):
         fy = _region_warp(y0, t, vel, 8, 1)
         fu = _region_warp(u0, t, vel, 8, 2)
@@ -169,10 +170,11 @@ def arnr_group_regions(width: int = 1920, height: int = 1080, n: int = 7, vmax: 
             fy = _occlude(fy, base + 500 + 7 * t, occluders, 1)
             fu = _occlude(fu, base + 501 + 7 * t, occluders, 2)
             fv = _occlude(fv, base + 502 + 7 * t, occluders, 2)
+        clean.append((fy, fu, fv))
         ys.append(add_noise(fy, base + 100 + 3 * t))
         us.append(add_noise(fu, base + 101 + 3 * t))
         vs.append(add_noise(fv, base + 102 + 3 * t))
-    return Group(width, height, n // 2, ys, us, vs, base)
+    return Group(width, height, n // 2, ys, us, vs, base, clean)
 
 
 BATCH_RESOLUTIONS = ((176, 144), (352, 288), (640, 360), (1280, 720), (1920, 1080))
