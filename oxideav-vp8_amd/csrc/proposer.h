@@ -77,7 +77,7 @@ struct HeadArgs {
   int images, cells;
   // inference
   int32_t* table;            // alignment table, or null
-  const int* frame_of_image; // table frame index of each image (device)
+  int frame_of_image[64];    // table frame index of each image
   int clamp;
   // training
   const int32_t* labels;     // [images][cells][2] {mvx, mvy}, or null

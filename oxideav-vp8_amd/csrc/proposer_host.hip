@@ -24,7 +24,6 @@ struct vp8g_proposer {
   int step = 0;
   uint8_t* arena = nullptr;  // activations, gradients, partials
   size_t arena_bytes = 0;
-  int* frame_map = nullptr;  // [64] table frame of each image
 };
 
 namespace {
@@ -172,8 +171,7 @@ int vp8g_proposer_create(vp8g_ctx* c, vp8g_proposer** out) {
   vp8g_proposer* p = new vp8g_proposer();
   p->ctx = c;
   bool good = ok(hipMalloc(&p->params, kParamCount * 4)) && ok(hipMalloc(&p->grads, (kParamCount + 1) * 4)) &&
-              ok(hipMalloc(&p->m, kParamCount * 4)) && ok(hipMalloc(&p->v, kParamCount * 4)) &&
-              ok(hipMalloc(&p->frame_map, 64 * sizeof(int)));
+              ok(hipMalloc(&p->m, kParamCount * 4)) && ok(hipMalloc(&p->v, kParamCount * 4));
   for (int l = 0; l < kLayers && good; l++)
     good = ok(hipMalloc(&p->wf[l], conv_weights(l) * 2)) && ok(hipMalloc(&p->wb[l], conv_weights(l) * 2));
   good = good && ok(hipMemsetAsync(p->params, 0, kParamCount * 4, c->stream)) &&
@@ -200,7 +198,6 @@ void vp8g_proposer_destroy(vp8g_proposer* p) {
     if (p->wf[l]) (void)hipFree(p->wf[l]);
     if (p->wb[l]) (void)hipFree(p->wb[l]);
   }
-  if (p->frame_map) (void)hipFree(p->frame_map);
   if (p->arena) (void)hipFree(p->arena);
   delete p;
 }
@@ -297,23 +294,17 @@ int vp8g_proposer_propose_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
   st = vp8g_proposer_prepare_dev(p, frames, n, center, planes);
   if (st == VP8G_OK) st = forward(p, planes, images, cw, ch, L);
   if (st != VP8G_OK) return st;
-  int fmap[64];
-  int k = 0;
-  for (size_t i = 0; i < n; i++)
-    if (i != center) fmap[k++] = (int)i;
   HeadArgs hd{};
   hd.a4 = reinterpret_cast<const uint16_t*>(p->arena + L.a[kLayers - 1]);
   hd.params = p->params + kHeadOffset;
   hd.images = images;
   hd.cells = cw * ch;
   hd.table = table;
-  hd.frame_of_image = p->frame_map;
   hd.clamp = clamp;
-  // The frame map upload is ordered before the head on the stream; the
-  // host array must stay valid until the copy has run.
-  if (!ok(hipMemcpyAsync(p->frame_map, fmap, sizeof(int) * images, hipMemcpyHostToDevice, p->ctx->stream)) ||
-      !ok(hipStreamSynchronize(p->ctx->stream)) || !ok(launch_head(hd, p->ctx->stream)))
-    return VP8G_E_DEVICE;
+  int k = 0;
+  for (size_t i = 0; i < n; i++)
+    if (i != center) hd.frame_of_image[k++] = (int)i;
+  if (!ok(launch_head(hd, p->ctx->stream))) return VP8G_E_DEVICE;
   return VP8G_OK;
 }
 

@@ -37,35 +37,57 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) { return __builtin_bit_ca
 __device__ __forceinline__ float bf16_val(uint16_t b) { return (float)__builtin_bit_cast(__bf16, b); }
 
 // ---------------------------------------------------------------- inputs
-// One thread per downsampled pixel of each (pair, frame) plane.
+// One thread per 4 horizontally adjacent downsampled pixels of a (pair,
+// frame) plane: 4 rows of 16 full-resolution bytes (dword loads when the
+// row segment is inside the frame and aligned).
 __global__ void prep_downsample_kernel(const PrepArgs a) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int dw = 4 * a.cw, dh = 4 * a.ch;
-  const long long plane = (long long)dw * dh;
-  if (t >= (long long)a.pairs * 2 * plane) return;
-  const int pf = (int)(t / plane);
-  const int rem = (int)(t - (long long)pf * plane);
-  const int y = rem / dw, x = rem - y * dw;
+  const int qw = a.cw;  // quads per downsampled row (dw / 4)
+  const long long quads = (long long)qw * dh;
+  if (t >= (long long)a.pairs * 2 * quads) return;
+  const int pf = (int)(t / quads);
+  const int rem = (int)(t - (long long)pf * quads);
+  const int y = rem / qw, xq = rem - y * qw;
   const int p = pf >> 1, f = pf & 1;
   const uint8_t* src = f == 0 ? a.anchor : a.neighbours[p];
   const uint32_t stride = f == 0 ? a.anchor_stride : a.neighbour_stride[p];
-  uint32_t sum = 0;
+  const int x0 = 16 * xq;
+  const bool fast = x0 + 16 <= a.w && ((((uintptr_t)src) | stride) & 3) == 0;
+  uint32_t sum[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int yy = 0; yy < 4; yy++) {
     const uint8_t* row = src + (size_t)min(4 * y + yy, a.h - 1) * stride;
+    if (fast) {
+      const uint32_t* r32 = reinterpret_cast<const uint32_t*>(row + x0);
 #pragma unroll
-    for (int xx = 0; xx < 4; xx++) sum += row[min(4 * x + xx, a.w - 1)];
+      for (int k = 0; k < 4; k++) {
+        const uint32_t v = r32[k];
+        sum[k] += (v & 255) + ((v >> 8) & 255) + ((v >> 16) & 255) + (v >> 24);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int xx = 0; xx < 4; xx++) sum[k] += row[min(x0 + 4 * k + xx, a.w - 1)];
+    }
   }
-  a.out[t] = (uint8_t)((sum + 8) >> 4);
+  const uint32_t packed = ((sum[0] + 8) >> 4) | (((sum[1] + 8) >> 4) << 8) | (((sum[2] + 8) >> 4) << 16) |
+                          (((sum[3] + 8) >> 4) << 24);
+  *reinterpret_cast<uint32_t*>(a.out + (size_t)pf * dw * dh + (size_t)y * dw + 4 * xq) = packed;
 }
 
-// One thread per (image, cell, feature channel).
+// Ten threads per (image, cell): thread k < 9 computes the 9 SADs of
+// displacement row ddy = k - 4 (the cell's 4x4 anchor pixels against 4 rows
+// of the clamped neighbour window, v_sad_u8); thread 9 writes both frames'
+// pixels and the zero channels.
 __global__ void features_kernel(const FeatArgs a) {
+  constexpr int TPC = 2 * kSearch + 2;  // threads per cell
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int cells = a.cw * a.ch;
-  if (t >= (long long)a.images * cells * kFeat) return;
-  const int c = (int)(t % kFeat);
-  const long long ic = t / kFeat;
+  if (t >= (long long)a.images * cells * TPC) return;
+  const long long ic = t / TPC;
+  const int k = (int)(t - ic * TPC);
   const int img = (int)(ic / cells);
   const int cell = (int)(ic - (long long)img * cells);
   const int cy = cell / a.cw, cx = cell - cy * a.cw;
@@ -73,28 +95,54 @@ __global__ void features_kernel(const FeatArgs a) {
   const uint8_t* A = a.planes + (size_t)img * 2 * dw * dh;
   const uint8_t* B = A + (size_t)dw * dh;
   const int x0 = 4 * cx, y0 = 4 * cy;
-  float v = 0.f;
-  if (c < kCostChannels) {
-    const int ddx = c % (2 * kSearch + 1) - kSearch, ddy = c / (2 * kSearch + 1) - kSearch;
-    uint32_t s = 0;
+  uint16_t* out = a.out + (size_t)ic * kFeat;
+  uint32_t arow[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) arow[j] = *reinterpret_cast<const uint32_t*>(A + (size_t)(y0 + j) * dw + x0);
+  if (k < 2 * kSearch + 1) {
+    // Window rows y0 + k - 4 + j, columns x0 - 4 .. x0 + 7 (3 dwords).
+    uint32_t win[4][3];
+    const bool inner = x0 >= 4 && x0 + 8 <= dw;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const uint8_t* ar = A + (size_t)(y0 + j) * dw + x0;
-      const uint8_t* br = B + (size_t)clampi(y0 + j + ddy, 0, dh - 1) * dw;
+      const uint8_t* row = B + (size_t)clampi(y0 + k - kSearch + j, 0, dh - 1) * dw;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int d = (int)ar[i] - (int)br[clampi(x0 + i + ddx, 0, dw - 1)];
-        s += (uint32_t)(d < 0 ? -d : d);
+      for (int q = 0; q < 3; q++) {
+        if (inner) {
+          win[j][q] = *reinterpret_cast<const uint32_t*>(row + x0 - 4 + 4 * q);
+        } else {
+          uint32_t v = 0;
+#pragma unroll
+          for (int b = 0; b < 4; b++) v |= (uint32_t)row[clampi(x0 - 4 + 4 * q + b, 0, dw - 1)] << (8 * b);
+          win[j][q] = v;
+        }
       }
     }
-    v = (float)s * (1.0f / 1024.0f);
-  } else if (c < kCostChannels + 32) {
-    const int k = c - kCostChannels;
-    const uint8_t* P = k < 16 ? A : B;
-    const int q = k & 15;
-    v = (float)P[(size_t)(y0 + (q >> 2)) * dw + x0 + (q & 3)] * (1.0f / 255.0f);
+#pragma unroll
+    for (int ddx = 0; ddx < 2 * kSearch + 1; ddx++) {
+      uint32_t s = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        // bytes ddx .. ddx + 3 of the 12-byte window row (alignbyte uses
+        // the shift mod 4, so the whole-dword offset 8 is taken directly)
+        const uint32_t ref = ddx < 4 ? funnel(win[j][0], win[j][1], ddx)
+                                     : (ddx < 8 ? funnel(win[j][1], win[j][2], ddx - 4) : win[j][2]);
+        s = sad4(arow[j], ref, s);
+      }
+      out[(2 * kSearch + 1) * k + ddx] = bf16_bits((float)s * (1.0f / 1024.0f));
+    }
+    return;
   }
-  a.out[t] = bf16_bits(v);
+  uint32_t brow[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) brow[j] = *reinterpret_cast<const uint32_t*>(B + (size_t)(y0 + j) * dw + x0);
+#pragma unroll
+  for (int p = 0; p < 32; p++) {
+    const uint32_t px = p < 16 ? (arow[p >> 2] >> (8 * (p & 3))) & 255 : (brow[(p - 16) >> 2] >> (8 * (p & 3))) & 255;
+    out[kCostChannels + p] = bf16_bits((float)px * (1.0f / 255.0f));
+  }
+#pragma unroll
+  for (int c = kCostChannels + 32; c < kFeat; c++) out[c] = 0;
 }
 
 // ---------------------------------------------------------------- conv3x3
@@ -431,14 +479,14 @@ static int persistent_grid(int total, int per_cu, int cus) {
 }
 
 hipError_t launch_prep_downsample(const PrepArgs& a, hipStream_t s) {
-  const long long n = (long long)a.pairs * 2 * (4 * a.cw) * (4 * a.ch);
+  const long long n = (long long)a.pairs * 2 * a.cw * (4 * a.ch);
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(prep_downsample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_features(const FeatArgs& a, hipStream_t s) {
-  const long long n = (long long)a.images * a.cw * a.ch * kFeat;
+  const long long n = (long long)a.images * a.cw * a.ch * (2 * kSearch + 2);
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();

@@ -234,6 +234,144 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
   }
 }
 
+// Radius 2 (the refinement around a proposal): one wave per block, 4 blocks
+// per workgroup.  Lane l < 40 takes dy = l / 8, the dx quad j = (l / 4) & 1
+// and the source rows 4h .. 4h + 3, h = l & 3; the four row quarters are
+// added with two lane swaps, then the wave reduces the keys.
+__global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
+  constexpr int R = 2, PD = 6, WROWS = 20, GROUP = 64, SLOTS = 256 / GROUP;
+  __shared__ uint32_t s_win[SLOTS][WROWS * PD];
+  __shared__ __align__(16) uint32_t s_src[SLOTS][64];
+  const int tid = threadIdx.x, slot = tid / GROUP, l = tid & (GROUP - 1);
+  const int per_frame = a.brows * a.bcols;
+  const long long bi = (long long)blockIdx.x * SLOTS + slot;
+  const bool live = bi < a.total_blocks;
+  const int b = live ? (int)bi : 0;
+  const int kf = b / per_frame;
+  const int fi = kf < a.center ? kf : kf + 1;
+  const int rest = b - kf * per_frame;
+  const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
+  const int bx = bcol * 16, by = brow * 16;
+  const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
+  const size_t tab = 3 * ((size_t)fi * per_frame + rest);
+  int sx = 0, sy = 0;
+  if (a.seeds) {
+    sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
+    sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
+  }
+  const FrameDesc F = a.frames[fi];
+  const FrameDesc C = a.frames[a.center];
+  const int x0 = bx + sx - R;
+  const int x0a = x0 >= 0 ? (x0 & ~3) : -(((-x0) + 3) & ~3);
+  const int shift0 = x0 - x0a;
+  const int y0 = by + sy - R;
+  uint32_t* win = s_win[slot];
+  const bool interior = a.aligned4 && x0a >= 0 && x0a + 4 * PD <= a.w;
+  // All staging loads of this lane are issued before any LDS store.
+  constexpr int WL = (WROWS * PD + GROUP - 1) / GROUP;  // window dwords per lane
+  constexpr int SL = 64 / GROUP;                        // source dwords per lane
+  uint32_t wv_[WL], sv_[SL];
+#pragma unroll
+  for (int k = 0; k < WL; k++) {
+    const int s = l + GROUP * k;
+    const int sc = s < WROWS * PD ? s : 0;
+    const int wy = sc / PD, wd = sc - wy * PD;
+    const uint8_t* row = F.y + (size_t)clampi(y0 + wy, 0, a.h - 1) * F.y_stride;
+    wv_[k] = interior ? *reinterpret_cast<const uint32_t*>(row + x0a + 4 * wd) : gather4_clamped(row, x0a + 4 * wd, a.w);
+  }
+#pragma unroll
+  for (int k = 0; k < SL; k++) {
+    const int s = l + GROUP * k;
+    const int r = s >> 2, q = s & 3;
+    uint32_t v = 0;
+    if (r < bh) {
+      const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
+      if (bw == 16 && a.aligned4) {
+        v = *reinterpret_cast<const uint32_t*>(row + 4 * q);
+      } else {
+        for (int e = 0; e < 4; e++)
+          if (4 * q + e < bw) v |= (uint32_t)row[4 * q + e] << (8 * e);
+      }
+    }
+    sv_[k] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < WL; k++)
+    if (l + GROUP * k < WROWS * PD) win[l + GROUP * k] = wv_[k];
+#pragma unroll
+  for (int k = 0; k < SL; k++) s_src[slot][l + GROUP * k] = sv_[k];
+  __syncthreads();
+  const int dy = l >> 3, j = (l >> 2) & 1, h = l & 3;
+  uint64_t acc = 0;
+  if (l < 40) {
+    if (bw == 16 && bh == 16) {
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int r = 4 * h + rr;
+        const uint32_t* wr = win + (dy + r) * PD + j;
+        const uint32_t w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4];
+        const uint4 s4 = *reinterpret_cast<const uint4*>(&s_src[slot][4 * r]);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w0 | ((uint64_t)w1 << 32), s4.x, acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w1 | ((uint64_t)w2 << 32), s4.y, acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w2 | ((uint64_t)w3 << 32), s4.z, acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w3 | ((uint64_t)w4 << 32), s4.w, acc);
+      }
+    } else {
+      const int bwq = bw >> 2, rem = bw & 3;
+      const uint32_t remmask = rem ? (0xFFFFFFFFu >> (32 - 8 * rem)) : 0u;
+      for (int rr = 0; rr < 4; rr++) {
+        const int r = 4 * h + rr;
+        if (r >= bh) break;
+        const uint32_t* wr = win + (dy + r) * PD + j;
+        const uint32_t wv[5] = {wr[0], wr[1], wr[2], wr[3], wr[4]};
+        const uint4 s4 = *reinterpret_cast<const uint4*>(&s_src[slot][4 * r]);
+        const uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint64_t pair = (uint64_t)wv[q] | ((uint64_t)wv[q + 1] << 32);
+          if (q < bwq) {
+            acc = __builtin_amdgcn_qsad_pk_u16_u8(pair, sv[q], acc);
+          } else if (q == bwq && rem) {
+            uint64_t add = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              add |= (uint64_t)sad4(sv[q], funnel(wv[q], wv[q + 1], i) & remmask, 0) << (16 * i);
+            acc += add;
+          }
+        }
+      }
+    }
+  }
+  acc += shfl_xor_u64(acc, 1);  // the other row quarters of the same (dy, j)
+  acc += shfl_xor_u64(acc, 2);
+  // Two 32-bit wave minima: the SAD, then among the candidates holding it
+  // the tie key (|mvx| + |mvy|, mvy, mvx); mvy and mvx are ordered through
+  // their offsets from the seed (constant over the block).
+  uint32_t lane_sad = 0xFFFFFFFFu, lane_tie = 0xFFFFFFFFu;
+  if (l < 40) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int dxi = 4 * j + i - shift0;
+      if (dxi >= 0 && dxi <= 2 * R) {
+        const uint32_t sad = (uint32_t)(acc >> (16 * i)) & 0xFFFFu;
+        const int mvx = sx - R + dxi, mvy = sy - R + dy;
+        const uint32_t tie = ((uint32_t)(abs(mvx) + abs(mvy)) << 6) | ((uint32_t)dy << 3) | (uint32_t)dxi;
+        if (sad < lane_sad || (sad == lane_sad && tie < lane_tie)) {
+          lane_sad = sad;
+          lane_tie = tie;
+        }
+      }
+    }
+  }
+  const uint32_t best_sad = wave_min_u32(lane_sad);
+  const uint32_t best_tie = wave_min_u32(lane_sad == best_sad ? lane_tie : 0xFFFFFFFFu);
+  if (live && l == 0) {
+    a.out[tab] = sx - R + (int)(best_tie & 7);
+    a.out[tab + 1] = sy - R + (int)((best_tie >> 3) & 7);
+    a.out[tab + 2] = (int32_t)best_sad;
+  }
+}
+
 bool block_search_supported(int radius) { return radius == 2 || radius == 16 || radius == 64; }
 
 hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream) {
@@ -246,7 +384,9 @@ hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stre
     hipLaunchKernelGGL(block_search_kernel<RR>, dim3(grid), dim3(SG::WG), 0, stream, a);           \
     break;                                                                                         \
   }
-    VP8G_SEARCH_CASE(2)
+    case 2:
+      hipLaunchKernelGGL(refine2_kernel, dim3((a.total_blocks + 3) / 4), dim3(256), 0, stream, a);
+      break;
     VP8G_SEARCH_CASE(16)
     VP8G_SEARCH_CASE(64)
 #undef VP8G_SEARCH_CASE
