@@ -112,6 +112,86 @@ def cpu_baseline(group, mc, seconds: float, cores: int):
     }
 
 
+def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
+    """The same group's altref three ways, each search + table-driven blend
+    timed with HIP events on the library's stream: the reference's 4-step
+    descent (the fused kernel), the exhaustive +-16 search, and the
+    learned-seeded search (proposer + exact +-2 refinement, trained weights
+    oxideav-vp8_amd/weights/proposer_1080p.safetensors).  Reports anchor-MB/s,
+    min-SAD agreement with the exhaustive search and luma PSNR against the
+    noise-free centre."""
+    import torch
+
+    import vp8g
+    from vp8g.proposer import WEIGHTS_DIR, Proposer, psnr
+    wpath = WEIGHTS_DIR / "proposer_1080p.safetensors"
+    if not wpath.exists():
+        return {"skipped": f"no weights at {wpath}"}
+    prop = Proposer(ctx, params=Proposer.load_params(wpath))
+    w, h, n, c = group.width, group.height, group.n, group.center
+    bw, bh = (w + 15) // 16, (h + 15) // 16
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+    dev = torch.device("cuda:0")
+    ex = torch.zeros((n, bh, bw, 3), dtype=torch.int32, device=dev)
+    seeds, ref = torch.zeros_like(ex), torch.zeros_like(ex)
+    oy = torch.empty(w * h, dtype=torch.uint8, device=dev)
+    ou = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
+    ov = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def altref_exhaustive():
+        ctx.block_search_dev(descs, n, c, 16, ex.data_ptr())
+        ctx.build_arnr_altref_table_dev(descs, n, c, 3, ex.data_ptr(), oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
+
+    def altref_learned():
+        prop.propose(descs, n, c, seeds.data_ptr(), 14)
+        ctx.block_search_dev(descs, n, c, 2, ref.data_ptr(), seeds.data_ptr())
+        ctx.build_arnr_altref_table_dev(descs, n, c, 3, ref.data_ptr(), oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
+
+    def altref_descent():
+        ctx.build_arnr_altref_dev(descs, n, c, 3, oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
+
+    anchors = bw * bh
+    clean = group.clean[c][0]
+    out = {}
+    for name, fn in (("descent", altref_descent), ("exhaustive16", altref_exhaustive), ("learned", altref_learned)):
+        ms = timed(fn)
+        ctx.synchronize()
+        out[name] = {"ms": ms, "anchor_mb_per_s": anchors / (ms * 1e-3),
+                     "psnr_y_vs_clean": psnr(oy.cpu().numpy().reshape(h, w), clean)}
+    t_search_ex = timed(lambda: ctx.block_search_dev(descs, n, c, 16, ex.data_ptr()))
+    t_search_le = timed(lambda: (prop.propose(descs, n, c, seeds.data_ptr(), 14),
+                                 ctx.block_search_dev(descs, n, c, 2, ref.data_ptr(), seeds.data_ptr())))
+    ctx.synchronize()
+    others = [i for i in range(n) if i != c]
+    e, r = ex.cpu().numpy()[others], ref.cpu().numpy()[others]
+    area = (np.minimum(16, h - 16 * np.arange(bh))[:, None] * np.minimum(16, w - 16 * np.arange(bw))[None, :])
+    matched = e[..., 2] < 13 * area[None]
+    same = r[..., 2] == e[..., 2]
+    out.update({
+        "agreement": float(same.mean()),
+        "agreement_matched": float(same[matched].mean()),
+        "matched_fraction": float(matched.mean()),
+        "psnr_loss_db": out["exhaustive16"]["psnr_y_vs_clean"] - out["learned"]["psnr_y_vs_clean"],
+        "search_ms": {"exhaustive16": t_search_ex, "learned": t_search_le},
+        "search_speedup": t_search_ex / t_search_le,
+        "weights": str(wpath.name),
+    })
+    prop.close()
+    return out
+
+
 def bench_batch(args) -> int:
     """64 clips (resolutions uniform over 176x144 .. 1920x1080, lengths over
     {3, 5, 7, 9, 11}, region motion + occluders) as one persistent,
@@ -201,6 +281,7 @@ def main() -> int:
     ap.add_argument("--arnr-waves", type=int, default=0)
     ap.add_argument("--no-xcd-remap", action="store_true")
     ap.add_argument("--arnr-mbs", type=int, default=0)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeded altref leg")
     ap.add_argument("--workload", choices=("step", "batch"), default="step",
                     help="step: the 1080p group step (default); batch: 64 uneven clips in one "
                          "persistent graph-captured launch against per-clip launches")
@@ -371,6 +452,8 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if not args.no_learned:
+        result["learned_seeded"] = bench_learned(ctx, group, descs, stream, reps=max(5, args.steps // 5))
     if not args.no_cpu:
         cores = max(1, min(os.cpu_count() or 1, 16))
         print(f"[bench] timing CPU oracle on {cores} threads", file=sys.stderr, flush=True)

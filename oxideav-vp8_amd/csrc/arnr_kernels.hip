@@ -169,7 +169,8 @@ constexpr uint32_t kProbeDy1 = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) | (
 template <bool FULL, class G>
 __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, const uint4 src1,
                                         int gd_off, int gq, int kk, int bw, int bh, int& bestx,
-                                        int& besty, uint32_t& best_sad) {
+                                        int& besty, uint32_t& best_sad, const uint8_t* zrow, uint32_t zsrc) {
+  static_assert((G::lp & 3) == 0, "window rows are dword aligned");
   uint32_t colmask[4] = {~0u, ~0u, ~0u, ~0u};
   uint32_t rowmask0 = ~0u, rowmask1 = ~0u;
   if (!FULL) {
@@ -181,17 +182,18 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     rowmask0 = 2 * kk < bh ? ~0u : 0u;
     rowmask1 = 2 * kk + 1 < bh ? ~0u : 0u;
   }
-  // 16 bytes at an arbitrary byte offset: five naturally aligned dword
-  // reads and byte funnel shifts (unaligned LDS reads stall the LDS pipe).
-  auto row16 = [&](const uint8_t* p) -> uint4 {
+  // The lane's two rows at byte offset `off`: one alignment for both (the
+  // row pitch is a dword multiple), ten naturally aligned dword reads and
+  // byte funnel shifts (unaligned LDS reads stall the LDS pipe).
+  auto sad_at = [&](int off) -> uint32_t {
+    const uint8_t* p = row0 + off;
     const uint32_t sh = (uint32_t)(uintptr_t)p & 3;
     const uint32_t* w = reinterpret_cast<const uint32_t*>(p - sh);
-    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
-    return make_uint4(funnel(w0, w1, sh), funnel(w1, w2, sh), funnel(w2, w3, sh), funnel(w3, w4, sh));
-  };
-  auto sad_at = [&](int off) -> uint32_t {
-    uint4 q0 = row16(row0 + off);
-    uint4 q1 = row16(row0 + off + G::lp);
+    constexpr int P = G::lp >> 2;
+    const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3], a4 = w[4];
+    const uint32_t b0 = w[P], b1 = w[P + 1], b2 = w[P + 2], b3 = w[P + 3], b4 = w[P + 4];
+    uint4 q0 = make_uint4(funnel(a0, a1, sh), funnel(a1, a2, sh), funnel(a2, a3, sh), funnel(a3, a4, sh));
+    uint4 q1 = make_uint4(funnel(b0, b1, sh), funnel(b1, b2, sh), funnel(b2, b3, sh), funnel(b3, b4, sh));
     if (!FULL) {
       q0.x &= colmask[0] & rowmask0; q0.y &= colmask[1] & rowmask0;
       q0.z &= colmask[2] & rowmask0; q0.w &= colmask[3] & rowmask0;
@@ -209,7 +211,17 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     return sum8(acc);
   };
   int bx = 0, by = 0, boff = 0;
-  uint32_t bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0));
+  uint32_t bsad;
+  if (FULL) {
+    // SAD at (0, 0) over the whole wave: lane = (row l / 4, dword l % 4),
+    // both operands dword aligned.
+    uint32_t z = sad4(zsrc, *reinterpret_cast<const uint32_t*>(zrow), 0);
+    z = sum16(z);
+    bsad = (uint32_t)__builtin_amdgcn_readlane((int)z, 0) + (uint32_t)__builtin_amdgcn_readlane((int)z, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)z, 32) + (uint32_t)__builtin_amdgcn_readlane((int)z, 48);
+  } else {
+    bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0));
+  }
 #pragma unroll
   for (int sh = 3; sh >= 0; sh--) {
     const uint32_t s = sad_at(boff + gd_off * (1 << sh));
@@ -218,9 +230,11 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     // reference's winner as one unsigned minimum over the wave.
     uint32_t key = (s << 4) | (uint32_t)(gq + 1);
     key = min(key, dpp<0x128>(key));  // row_ror:8: the row's other group
-    key = min(key, (uint32_t)__builtin_amdgcn_update_dpp((int)key, (int)key, 0x142, 0xA, 0xF, false));
-    key = min(key, (uint32_t)__builtin_amdgcn_update_dpp((int)key, (int)key, 0x143, 0xC, 0xF, false));
-    const uint32_t kmin = (uint32_t)__builtin_amdgcn_readlane((int)key, 63);
+    const uint32_t k0 = (uint32_t)__builtin_amdgcn_readlane((int)key, 0);
+    const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)key, 16);
+    const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)key, 32);
+    const uint32_t k3 = (uint32_t)__builtin_amdgcn_readlane((int)key, 48);
+    const uint32_t kmin = min(min(k0, k1), min(k2, k3));
     if (kmin < (bsad << 4)) {
       const int w = (int)(kmin & 15) - 1;
       const int dx = (int)((kProbeDx1 >> (2 * w)) & 3) - 1;
@@ -349,9 +363,12 @@ __device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t
         const uint4 src1 = *reinterpret_cast<const uint4*>(s_src + 16 * M);
         const uint8_t* row0 = win + (2 * kk + 15) * G::lp + 16 * m + 16;
         if (bw == 16 && bh == 16) {
-          descend<true, G>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad);
+          // (0, 0) operands of the whole-wave SAD: row lane / 4, dword lane % 4
+          const uint8_t* zrow = win + ((lane >> 2) + 15) * G::lp + 16 * m + 16 + 4 * (lane & 3);
+          const uint32_t zsrc = *reinterpret_cast<const uint32_t*>(s_cy + (lane >> 2) * 16 * M + 16 * m + 4 * (lane & 3));
+          descend<true, G>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad, zrow, zsrc);
         } else {
-          descend<false, G>(row0, src0, src1, gd_off, gq, kk, bw, bh, bestx, besty, best_sad);
+          descend<false, G>(row0, src0, src1, gd_off, gq, kk, bw, bh, bestx, besty, best_sad, nullptr, 0);
         }
       }
       if (!TABLE && a.mv_out && lane == 0) {
