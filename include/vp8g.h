@@ -312,6 +312,18 @@ int vp8g_quarter_pixel_refine_luma_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref
                                          const vp8g_mv *centers, size_t n,
                                          vp8g_search_result *out);
 
+/* The same over device arrays (enqueue only).  mb_xy components must be at
+ * most 0xFFFFF and centre components lie in [-1023, 1023] (components
+ * outside are clamped to it). */
+int vp8g_half_pixel_refine_luma_batch_dev(vp8g_ctx *ctx, const vp8g_luma_ref *ref,
+                                          const uint32_t *mb_xy, const uint8_t *src,
+                                          const vp8g_mv *centers, size_t n,
+                                          vp8g_search_result *out);
+int vp8g_quarter_pixel_refine_luma_batch_dev(vp8g_ctx *ctx, const vp8g_luma_ref *ref,
+                                             const uint32_t *mb_xy, const uint8_t *src,
+                                             const vp8g_mv *centers, size_t n,
+                                             vp8g_search_result *out);
+
 /* Replaces the encoder's half-pel probe set, fetch_window + HalfPlanes::new
  * + HalfPlanes::sad (src/encoder/inter_search.rs:266-346, used by
  * sub_pixel_refine :509-536), over a batch: block i's whole-pel position is

@@ -774,6 +774,42 @@ int vp8g_half_plane_sads_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const int3
   return finish(c, st);
 }
 
+static int refine_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                            const uint8_t* src, const vp8g_mv* centers, size_t n,
+                            vp8g_search_result* out, int step) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !src || !centers || !out || n > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (((uintptr_t)mb_xy | (uintptr_t)src | (uintptr_t)centers | (uintptr_t)out) & 3) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  RefineArgs a;
+  a.plane = ref->plane;
+  a.stride = (uint32_t)ref->stride;
+  a.w = (int)ref->width;
+  a.h = (int)ref->height;
+  a.mb_xy = mb_xy;
+  a.src = src;
+  a.center = reinterpret_cast<const int16_t*>(centers);
+  a.out = reinterpret_cast<uint32_t*>(out);
+  a.n = (int)n;
+  a.step = step;
+  return hip_ok(launch_refine_ladder(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int vp8g_half_pixel_refine_luma_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                                          const uint8_t* src, const vp8g_mv* centers, size_t n,
+                                          vp8g_search_result* out) {
+  return refine_batch_dev(c, ref, mb_xy, src, centers, n, out, 2);
+}
+
+int vp8g_quarter_pixel_refine_luma_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                                             const uint8_t* src, const vp8g_mv* centers, size_t n,
+                                             vp8g_search_result* out) {
+  return refine_batch_dev(c, ref, mb_xy, src, centers, n, out, 1);
+}
+
 int vp8g_half_pixel_refine_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref,
                                       const uint32_t* mb_xy, const uint8_t* src,
                                       const vp8g_mv* centers, size_t n, vp8g_search_result* out) {

@@ -263,7 +263,9 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
   uint8_t* win = smem + wave * (24 * 24 + 21 * 16);
   uint8_t* lt = win + 24 * 24;
   const int mb_col = (int)a.mb_xy[2 * i], mb_row = (int)a.mb_xy[2 * i + 1];
-  const int crow = a.center[2 * i], ccol = a.center[2 * i + 1];
+  // Centres outside +-1023 are outside the reference's contract (it
+  // debug-asserts); clamping keeps every candidate inside the staged window.
+  const int crow = clampi(a.center[2 * i], -1023, 1023), ccol = clampi(a.center[2 * i + 1], -1023, 1023);
   const int st = a.step;
   const int offr[8] = {-st, -st, -st, 0, 0, st, st, st};
   const int offc[8] = {-st, 0, st, -st, st, -st, 0, st};

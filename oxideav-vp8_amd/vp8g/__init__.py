@@ -112,6 +112,7 @@ EXPORTS = (
     "vp8g_proposer_set_params", "vp8g_proposer_get_params", "vp8g_proposer_get_grads",
     "vp8g_proposer_prepare_dev", "vp8g_proposer_forward_dev", "vp8g_proposer_propose_dev",
     "vp8g_proposer_backward_dev", "vp8g_proposer_train_step_dev", "vp8g_synth_luma_dev",
+    "vp8g_half_pixel_refine_luma_batch_dev", "vp8g_quarter_pixel_refine_luma_batch_dev",
 )
 
 
@@ -164,6 +165,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_proposer_backward_dev": (i32, [vp, vp, vp, i32, i32, i32]),
         "vp8g_proposer_train_step_dev": (i32, [vp, vp, vp, i32, i32, i32, C.c_float, vp]),
         "vp8g_synth_luma_dev": (i32, [vp, C.POINTER(_SynthSpec), vp, vp]),
+        "vp8g_half_pixel_refine_luma_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
+        "vp8g_quarter_pixel_refine_luma_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -531,6 +534,25 @@ class Context:
         res[:, 1] = out[:n, 1].view(np.int16)
         res[:, 2] = out[:n, 2].astype(np.int64) | (out[:n, 3].astype(np.int64) << 16)
         return res
+
+    def refine_batch_dev(self, plane: int, stride: int, width: int, height: int, mb_xy: int, src: int,
+                         centers: int, n: int, out: int, quarter: bool) -> None:
+        """Half- (quarter=False) or quarter-pel refinement over device arrays
+        (vp8g_search_result out[n]); enqueues only."""
+        ref = _LumaRef(plane, stride, width, height)
+        fn = (self.lib.vp8g_quarter_pixel_refine_luma_batch_dev if quarter
+              else self.lib.vp8g_half_pixel_refine_luma_batch_dev)
+        self._raise(fn(self.h, C.byref(ref), mb_xy, src, centers, n, out), "refine_batch_dev")
+
+    def predict_inter_mb_batch_dev(self, y: int, u: int, v: int, y_stride: int, uv_stride: int, mb_cols: int,
+                                   mb_rows: int, mb_xy: int, mvs: int, n: int, full_pixel: bool, filters,
+                                   out: int) -> None:
+        """predict_inter_mb over device arrays into out (n x 384); enqueues only."""
+        rp = _RefPlanes(y, u, v, y_stride, uv_stride, mb_cols, mb_rows)
+        f = np.ascontiguousarray(filters, np.int32)
+        self._raise(self.lib.vp8g_predict_inter_mb_batch_dev(self.h, C.byref(rp), mb_xy, mvs, n, int(full_pixel),
+                                                             f.ctypes.data, out),
+                    "vp8g_predict_inter_mb_batch_dev")
 
     def half_plane_sads_batch(self, reference: LumaRef, xy, src, taps, diagonals: bool = True) -> np.ndarray:
         """fetch_window + HalfPlanes (src/encoder/inter_search.rs:266-346):

@@ -208,3 +208,37 @@ def test_half_plane_sads_match_oracle(gpu, oracle, diagonals):
                         oracle.lib.orc_half_planes_sad(C.byref(hp), s.ctypes.data, dx, dy)
                     k += 1
         assert_same(got, want, f"half planes taps={list(taps)} diagonals={diagonals}")
+
+
+def test_device_refine_and_mc_entries_match_host_entries(gpu):
+    """The _dev refinement and MC entries give the host entries' results
+    (those are pinned against the oracle above) on the same inputs."""
+    import torch
+    import vp8g
+    import vp8g.synth as synth
+    g = synth.arnr_group_cif(96, 64, 3, clip=6)
+    w, h = g.width, g.height
+    bw, bh = w // 16, h // 16
+    rng = np.random.default_rng(12)
+    n = bw * bh
+    mb_xy = np.stack(np.meshgrid(np.arange(bw), np.arange(bh)), -1).reshape(-1, 2).astype(np.uint32)
+    centers = rng.integers(-40, 41, (n, 2)).astype(np.int16)
+    src = np.ascontiguousarray(g.y[1].reshape(bh, 16, bw, 16).transpose(0, 2, 1, 3).reshape(n, 256))
+    ref = vp8g.LumaRef(g.y[0].reshape(-1), w, w, h)
+    want_h = gpu.half_pixel_refine_luma_batch(ref, mb_xy, src, centers)
+    want_q = gpu.quarter_pixel_refine_luma_batch(ref, mb_xy, src, centers)
+    dy = torch.from_numpy(g.y[0].reshape(-1).copy()).cuda()
+    dxy = torch.from_numpy(mb_xy.astype(np.int32)).cuda()
+    dsrc = torch.from_numpy(src).cuda()
+    dc = torch.from_numpy(centers).cuda()
+    oh = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+    oq = torch.zeros_like(oh)
+    torch.cuda.synchronize()
+    gpu.refine_batch_dev(dy.data_ptr(), w, w, h, dxy.data_ptr(), dsrc.data_ptr(), dc.data_ptr(), n, oh.data_ptr(), False)
+    gpu.refine_batch_dev(dy.data_ptr(), w, w, h, dxy.data_ptr(), dsrc.data_ptr(), dc.data_ptr(), n, oq.data_ptr(), True)
+    gpu.synchronize()
+    for got, want in ((oh, want_h), (oq, want_q)):
+        raw = got.cpu().numpy()
+        rows = (raw[:, 0] & 0xFFFF).astype(np.uint16).view(np.int16)
+        cols = ((raw[:, 0] >> 16) & 0xFFFF).astype(np.uint16).view(np.int16)
+        assert np.array_equal(np.stack([rows, cols], 1), want[:, :2]) and np.array_equal(raw[:, 1].astype(np.uint32), want[:, 2])
