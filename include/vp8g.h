@@ -85,12 +85,14 @@ int vp8g_ctx_set_stream(vp8g_ctx *ctx, void *stream);
 void *vp8g_ctx_stream(vp8g_ctx *ctx);
 int vp8g_ctx_synchronize(vp8g_ctx *ctx);
 /* Launch-shape options; none of them changes any output byte.
- * VP8G_OPT_ARNR_WAVES: waves per ARNR workgroup (0 = one per neighbour
- *   frame, at most 16).
+ * VP8G_OPT_ARNR_WAVES: waves per anchor macroblock in an ARNR launch, each
+ *   taking a share of the neighbour frames (0 = enough for about 4096 waves
+ *   per launch; at most one per neighbour; waves x MBs per workgroup <= 16).
  * VP8G_OPT_XCD_REMAP: 1 (default) gives each XCD a contiguous band of
  *   macroblocks, 0 uses the launch order.
  * VP8G_OPT_ARNR_MBS: anchor macroblocks per ARNR workgroup, 1..8
- *   (0 = default 4; reduced automatically to fit 64 KiB of LDS).
+ *   (0 = default 4; reduced automatically to fit 16 waves and 64 KiB of
+ *   LDS).
  * VP8G_OPT_BATCH_WGS_PER_CU: persistent workgroups per CU of a batch,
  *   0..32 (0 = as many as fit). */
 #define VP8G_OPT_ARNR_WAVES 1
@@ -249,7 +251,7 @@ int vp8g_synth_luma_dev(vp8g_ctx *ctx, const vp8g_synth_spec *spec, uint8_t *out
  * job and `err` carries its payload.  Planes, outputs and mv_out are device
  * pointers fixed at creation: a run reads whatever the frame buffers hold
  * when it executes.  Launch shape comes from the context's options at
- * creation (VP8G_OPT_ARNR_MBS, VP8G_OPT_ARNR_WAVES — default 2 for a batch —
+ * creation (VP8G_OPT_ARNR_MBS, VP8G_OPT_ARNR_WAVES — default 1 for a batch —
  * and VP8G_OPT_BATCH_WGS_PER_CU).  At most 65535 jobs. */
 typedef struct {
     const vp8g_i420 *frames;   /* host array of n_frames device-plane frames */

@@ -1,13 +1,14 @@
 // ARNR altref synthesis on gfx950, search + occlusion guard + blend + resolve
 // fused into one launch.
 //
-// Work decomposition: one workgroup per strip of M horizontally adjacent
-// anchor macroblocks of one MB row; one wave per neighbour frame.  Each wave
-// stages its neighbour's clamped luma and chroma windows covering the whole
-// strip (+-15 px luma, +-7 px chroma) into LDS once, then runs the M block
-// searches and blends out of LDS.  The centre strip is staged once per
-// workgroup.  Per-pixel sums from all waves meet in LDS accumulators and the
-// workgroup resolves the strip.
+// Work decomposition: one wave per anchor macroblock (times K frame
+// splits), M horizontally adjacent MBs of one MB row per workgroup.  A wave
+// loops over its share of the neighbour frames; for each it stages the
+// frame's clamped luma and chroma window around the MB (+-15 px luma, +-7 px
+// chroma) into its own LDS slice, runs the block search there and adds the
+// blended pixels into per-lane register accumulators.  The next frame's
+// window is fetched into registers while the current one is searched.  With
+// K > 1 the split waves of one MB meet once, in LDS, before the resolve.
 //
 // Semantics follow src/arnr.rs:476-675 of the reference crate:
 //   * search: refine_search (src/arnr.rs:264-300) — SAD at (0,0), then for
@@ -23,9 +24,9 @@
 //   * blend: acc += wt[|ref - ctr|] * ref, wsum += wt[..] with the 256-entry
 //     table of src/arnr.rs:307-323; chroma uses (mv/2 truncated toward zero)
 //     and the chroma-plane clamp (src/arnr.rs:402-459, 620-639).  Both sums
-//     are packed into one u32 (acc < 2^20, wsum < 2^12 for <= 64 frames)
-//     and added with LDS atomics: integer addition is exact and commutative,
-//     so the result does not depend on wave scheduling.
+//     are packed into one u32 (acc < 2^20, wsum < 2^12 for <= 64 frames):
+//     integer addition is exact and commutative, so neither the frame order
+//     of a wave nor the split of frames over waves changes the result.
 //   * resolve: (acc + ws/2) / ws with the centre's weight-16 seed
 //     (src/arnr.rs:662-675).
 // Chroma output clamping (src/arnr.rs:442-444) never fires from the block
@@ -35,23 +36,23 @@
 
 namespace vp8g {
 
-
 constexpr int kWinRows = 46;   // by-15 .. by+30
-constexpr int kCWinRows = 22;  // cby-7 .. cby+14
 constexpr int kAccShift = 20;  // packed accumulator: wsum << 20 | acc
+constexpr uint32_t kAccOne = 1u << kAccShift;
 
-// Strip geometry for M MBs per workgroup; compile-time so that every
-// index division in the staging and resolve loops is a multiply-shift.
-template <int M_>
-struct StripGeom {
-  static constexpr int m = M_;                 // MBs per strip
-  static constexpr int lw = 16 * M_ + 32;      // luma window width (cols bx0-16 .. bx0+16M+15)
-  // Row pitch = width + 8 bytes: two rows apart (the row pair of one search
-  // lane) then land 4 banks apart per lane, so the 8 lanes of a probe group
-  // read disjoint LDS banks.
-  static constexpr int lp = lw + 8;
-  static constexpr int cp = 8 * M_ + 16;       // chroma window pitch (cols cbx0-8 .. cbx0+8M+7)
-  static constexpr int wave_bytes = (kWinRows * lp + 2 * kCWinRows * cp + 15) & ~15;
+// One wave's LDS windows: luma rows by-15..by+30, cols bx-16..bx+31 (three
+// aligned 16-byte pieces per row) at pitch 48, twice: the frame being
+// searched and the next one, which lands by DMA meanwhile.  A search lane
+// reads rows kk and kk + 8; at 12 dwords per row the 8 lanes of a probe
+// group then read 8 distinct banks.  Chroma is not staged: the blend reads
+// one U and one V byte per lane at the chosen vector straight from memory.
+constexpr int kLumaPitch = 48;
+constexpr int kLumaWin = kWinRows * kLumaPitch;  // 2208
+constexpr int kWaveWinBytes = 2 * kLumaWin;      // 4416
+constexpr int kLumaPieces = kWinRows * 3;        // 138 16-byte pieces
+
+struct MbGeom {
+  static constexpr int lp = kLumaPitch;
 };
 
 __device__ __forceinline__ uint32_t load_clamped4(const uint8_t* row, int x0, int w, bool aligned4) {
@@ -72,85 +73,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Stage frame f's clamped windows for the strip starting at (bx0, by):
-// luma rows by-15..by+30, cols bx0-16..bx0+16M+15; chroma rows
-// cby-7..cby+14, cols cbx0-8..cbx0+8M+7 (U then V).
-template <class G>
-__device__ void stage_windows(uint8_t* win, const FrameDesc f, const ArnrGroup& a, const G g,
-                              int bx0, int by, int lane) {
-  const int cbx0 = bx0 >> 1, cby = by >> 1;
-  const ptrdiff_t v_minus_u = f.v - f.u;  // plane choice as integer arithmetic
-  uint8_t* cwin = win + kWinRows * g.lp;
-  constexpr int lpieces = G::lw >> 4;  // 16-byte pieces per luma row
-  constexpr int cdw = G::cp >> 2;      // dwords per chroma row
-  const bool interior = a.aligned16 && bx0 >= 16 && bx0 + 16 * G::m + 16 <= a.w && by >= 15 &&
-                        by + 31 <= a.h;
-  if (interior) {
-    // Luma: whole aligned 16-byte pieces; chroma (then interior as well:
-    // cbx0 >= 8, cbx0 + 8M + 8 <= cw, cby >= 8, cby + 15 <= ch): dwords.
-    // Loads go out in batches of 4 per lane before their LDS stores.
-    const uint8_t* ybase = f.y + (size_t)(by - 15) * f.y_stride + bx0 - 16;
-    constexpr int total = kWinRows * lpieces;
-    auto yld = [&](int s) -> uint4 {
-      s = min(s, total - 1);
-      const int wy = s / lpieces, pc = s - wy * lpieces;
-      return *reinterpret_cast<const uint4*>(ybase + (size_t)wy * f.y_stride + 16 * pc);
-    };
-    auto yst = [&](int s, const uint4& v) {
-      if (s < total) {
-        const int wy = s / lpieces, pc = s - wy * lpieces;
-        uint2* d = reinterpret_cast<uint2*>(win + wy * G::lp + 16 * pc);  // 8-byte aligned
-        d[0] = make_uint2(v.x, v.y);
-        d[1] = make_uint2(v.z, v.w);
-      }
-    };
-    for (int s0 = lane; s0 < total; s0 += 256) {
-      const uint4 v0 = yld(s0), v1 = yld(s0 + 64), v2 = yld(s0 + 128), v3 = yld(s0 + 192);
-      yst(s0, v0);
-      yst(s0 + 64, v1);
-      yst(s0 + 128, v2);
-      yst(s0 + 192, v3);
-    }
-    const size_t coff = (size_t)(cby - 7) * f.uv_stride + cbx0 - 8;
-    constexpr int ctotal = 2 * kCWinRows * cdw;
-    const bool al4 = a.aligned4c != 0;
-    auto cld = [&](int s) -> uint32_t {
-      s = min(s, ctotal - 1);
-      const int pl = s >= kCWinRows * cdw;
-      const int t = s - pl * kCWinRows * cdw;
-      const int wy = t / cdw, dq = t - wy * cdw;
-      const uint8_t* p = f.u + (pl ? v_minus_u : 0) + coff + (size_t)wy * f.uv_stride + 4 * dq;
-      return al4 ? *reinterpret_cast<const uint32_t*>(p)
-                 : (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
-    };
-    uint32_t* cw32 = reinterpret_cast<uint32_t*>(cwin);
-    for (int s0 = lane; s0 < ctotal; s0 += 256) {
-      const uint32_t c0 = cld(s0), c1 = cld(s0 + 64), c2 = cld(s0 + 128), c3 = cld(s0 + 192);
-      cw32[s0] = c0;
-      if (s0 + 64 < ctotal) cw32[s0 + 64] = c1;
-      if (s0 + 128 < ctotal) cw32[s0 + 128] = c2;
-      if (s0 + 192 < ctotal) cw32[s0 + 192] = c3;
-    }
-    return;
-  }
-  // Frame-edge strips: clamped dword gathers.
-  constexpr int ldw = G::lw >> 2;
-  for (int s = lane; s < kWinRows * ldw; s += 64) {
-    const int wy = s / ldw, dq = s - wy * ldw;
-    const int y = clampi(by - 15 + wy, 0, a.h - 1);
-    reinterpret_cast<uint32_t*>(win)[wy * (G::lp >> 2) + dq] =
-        load_clamped4(f.y + (size_t)y * f.y_stride, bx0 - 16 + 4 * dq, a.w, a.aligned4 != 0);
-  }
-  for (int s = lane; s < 2 * kCWinRows * cdw; s += 64) {
-    const int pl = s >= kCWinRows * cdw;
-    const int t = s - pl * kCWinRows * cdw;
-    const int wy = t / cdw, dq = t - wy * cdw;
-    const int y = clampi(cby - 7 + wy, 0, a.ch - 1);
-    reinterpret_cast<uint32_t*>(cwin)[s] = load_clamped4(f.u + (pl ? v_minus_u : 0) + (size_t)y * f.uv_stride,
-                                                         cbx0 - 8 + 4 * dq, a.cw, a.aligned4c != 0);
-  }
-}
-
 // Probe g of a step sits at centre + s*(dx_g, dy_g) with the reference's
 // order (-1,-1),(0,-1),(1,-1),(-1,0),(1,0),(-1,1),(0,1),(1,1)
 // (src/arnr.rs:274-283); (d + 1) is packed 2 bits per probe.
@@ -160,8 +82,9 @@ constexpr uint32_t kProbeDy1 = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) | (
                                (2u << 12) | (2u << 14);
 
 // refine_search (src/arnr.rs:264-300) for one block against the staged
-// window.  Lanes are 8 probe groups x 8 row pairs; `row0` points at this
-// lane's first source row's (0,0)-displaced reference row.  The probes of
+// window.  Lanes are 8 probe groups x 8 row pairs (rows kk and kk + 8);
+// `row0` points at this lane's first source row's (0,0)-displaced reference
+// row.  The probes of
 // a step never leave +-15 (8+4+2+1), so the reference's range check never
 // fires and is not evaluated.  FULL: 16x16 block; otherwise columns >= bw
 // and rows >= bh are masked out of every SAD (the centre strip holds zeros
@@ -179,8 +102,8 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
       const int left = bw - 4 * q;
       colmask[q] = left >= 4 ? 0xFFFFFFFFu : (left <= 0 ? 0u : (0xFFFFFFFFu >> (32 - 8 * left)));
     }
-    rowmask0 = 2 * kk < bh ? ~0u : 0u;
-    rowmask1 = 2 * kk + 1 < bh ? ~0u : 0u;
+    rowmask0 = kk < bh ? ~0u : 0u;
+    rowmask1 = kk + 8 < bh ? ~0u : 0u;
   }
   // The lane's two rows at byte offset `off`: one alignment for both (the
   // row pitch is a dword multiple), ten naturally aligned dword reads and
@@ -189,7 +112,7 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     const uint8_t* p = row0 + off;
     const uint32_t sh = (uint32_t)(uintptr_t)p & 3;
     const uint32_t* w = reinterpret_cast<const uint32_t*>(p - sh);
-    constexpr int P = G::lp >> 2;
+    constexpr int P = 8 * G::lp >> 2;  // row kk + 8
     const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3], a4 = w[4];
     const uint32_t b0 = w[P], b1 = w[P + 1], b2 = w[P + 2], b3 = w[P + 3], b4 = w[P + 4];
     uint4 q0 = make_uint4(funnel(a0, a1, sh), funnel(a1, a2, sh), funnel(a2, a3, sh), funnel(a3, a4, sh));
@@ -259,235 +182,383 @@ __device__ __forceinline__ int xcd_band(int i, int total) {
   return (i & 7) * (total >> 3) + (i >> 3);
 }
 
-// One strip of group `a`: M anchor MBs of MB row `brow`, strip column
-// `scol`.  `frame_at(i)` returns frame i of the group, `wt4` the weight LUT.
-// TABLE: the per-block vectors and SADs come from `table` (layout
-// n x brows x bcols x {mvx, mvy, sad}) instead of the descent, and the
-// blend reads the neighbour straight from memory with clamped coordinates
-// (the vectors are not bounded by the staged +-15 window).
-// Ends with the strip resolved into the output planes; the caller
-// synchronises the workgroup before reusing LDS.
-template <int M, bool TABLE, class FrameAt>
-__device__ __forceinline__ void process_strip(const ArnrGroup& a, const uint32_t* wt4,
-                                              const FrameAt& frame_at, int brow, int scol,
-                                              uint8_t* smem, const int32_t* table = nullptr) {
-  using G = StripGeom<M>;
-  const G g;
-  // LDS: packed accumulators M*384 u32 | weight LUT 256 B | centre luma
-  // 16 x 16M | centre U, V 8 x 8M each | per-wave windows.
-  uint32_t* s_acc = reinterpret_cast<uint32_t*>(smem);
-  uint8_t* s_lut = smem + M * 384 * 4;
-  uint8_t* s_cy = s_lut + 256;
-  uint8_t* s_cu = s_cy + 256 * M;
-  uint8_t* s_cv = s_cu + 64 * M;
-  uint8_t* s_win = s_cv + 64 * M;
 
-  const int nw = blockDim.x >> 6;
-  const int tid = threadIdx.x;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lane = tid & 63;
+// DMA of one frame's luma window into `win` (global_load_lds_dwordx4: lane
+// l of an instruction lands at win + 16 l): piece p = lane + 64 i is window
+// row p / 3, 16-byte column p % 3, at byte 16 p of the window.  `origin` is
+// the frame address of window row 0, column 0 and off_i lane piece i's
+// offset from it (row * stride + 16 * column).  The data is in LDS once
+// vmcnt has drained (wait_vm).
+using LdsPtr = __attribute__((address_space(3))) void*;
+__device__ __forceinline__ void dma_window(const uint8_t* origin, uint32_t off0, uint32_t off1, uint32_t off2,
+                                           uint8_t* win, int lane) {
+  __builtin_amdgcn_global_load_lds(origin + off0, (LdsPtr)(win), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(origin + off1, (LdsPtr)(win + 1024), 16, 0, 0);
+  if (lane < kLumaPieces - 128) __builtin_amdgcn_global_load_lds(origin + off2, (LdsPtr)(win + 2048), 16, 0, 0);
+}
 
-  const int bx0 = scol * 16 * M, by = brow * 16;
-  const int cbx0 = bx0 >> 1, cby = by >> 1;
-  const int bh = min(16, a.h - by), cbh = (bh + 1) >> 1;
-  const FrameDesc C = frame_at(a.center);
-  const int nnb = a.filter_on ? a.n - 1 : 0;
-  uint8_t* win = s_win + wave * g.wave_bytes;
-  uint8_t* cwin = win + kWinRows * g.lp;
+// Every vector memory operation of the wave done (DMA writes included), and
+// no memory access moved across.
+__device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-  // The first neighbour's windows are staged before anything else.
-  int k = wave;
-  if (!TABLE && k < nnb) stage_windows(win, frame_at(k < a.center ? k : k + 1), a, g, bx0, by, lane);
-
-  // Centre strip (zero outside the frame), accumulators, LUT.
-  for (int i = tid; i < 64 * M; i += blockDim.x) {  // luma: 16 rows x 4M dwords
-    const int r = i / (4 * M), q = i - r * 4 * M;
-    const int x = bx0 + 4 * q;
-    uint32_t v = 0;
-    if (r < bh && x < a.w) {
-      const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride;
-      if (a.aligned4 && x + 4 <= a.w) {
-        v = *reinterpret_cast<const uint32_t*>(row + x);
+// MBs at the left or right frame edge (or planes without one stride or 16-
+// byte alignment): the window gathered with clamped rows and columns, in the
+// layout the DMA gives.  With 16-byte aligned planes and a width that is a
+// multiple of 16, every 16-byte piece lies wholly inside the frame (one
+// load) or wholly outside it (the edge pixel repeated): all three pieces of
+// a lane are loaded before the first store, so the window costs one memory
+// round trip.  Otherwise clamped dword gathers.
+__device__ __forceinline__ void stage_gather(uint8_t* win, const FrameDesc& f, const ArnrGroup& a, int bx, int by,
+                                             int lane) {
+  if (a.aligned16 && (a.w & 15) == 0) {
+    uint4 v[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const int p = min(lane + 64 * i, kLumaPieces - 1);
+      const int row = p / 3;
+      const uint8_t* rp = f.y + (size_t)clampi(by - 15 + row, 0, a.h - 1) * f.y_stride;
+      const int x0 = bx - 16 + 16 * (p - 3 * row);
+      if (x0 >= 0 && x0 + 16 <= a.w) {
+        v[i] = *reinterpret_cast<const uint4*>(rp + x0);
       } else {
-        for (int j = 0; j < 4; j++)
-          if (x + j < a.w) v |= (uint32_t)row[x + j] << (8 * j);
+        const uint32_t b = (uint32_t)rp[x0 < 0 ? 0 : a.w - 1] * 0x01010101u;
+        v[i] = make_uint4(b, b, b, b);
       }
     }
-    reinterpret_cast<uint32_t*>(s_cy)[i] = v;
-  }
-  for (int i = tid; i < 32 * M; i += blockDim.x) {  // chroma: 2 planes x 8 rows x 2M dwords
-    const int pl = i >= 16 * M;
-    const int t = i - pl * 16 * M;
-    const int r = t / (2 * M), q = t - r * 2 * M;
-    const int x = cbx0 + 4 * q;
-    uint32_t v = 0;
-    if (r < cbh && x < a.cw) {
-      const uint8_t* row = (pl ? C.v : C.u) + (size_t)(cby + r) * C.uv_stride;
-      for (int j = 0; j < 4; j++)
-        if (x + j < a.cw) v |= (uint32_t)row[x + j] << (8 * j);
-    }
-    reinterpret_cast<uint32_t*>(pl ? s_cv : s_cu)[t] = v;
-  }
-  for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
-  for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = wt4[i];
-  __syncthreads();
-
-  const int gq = lane >> 3, kk = lane & 7;   // search: probe group, row pair
-  // This lane's probe direction as a window offset per unit step.
-  const int gd_off = (((kProbeDy1 >> (2 * gq)) & 3) - 1) * G::lp + (((kProbeDx1 >> (2 * gq)) & 3) - 1);
-  const int lr = lane >> 2, lq = lane & 3;   // blend: luma row, dword
-  const int cr = lane >> 3, cc = lane & 7;   // blend: chroma pixel
-
-  for (; k < nnb; k += nw) {
-    const int fi = k < a.center ? k : k + 1;
-    const FrameDesc F = frame_at(fi);
-    if (!TABLE && k != wave) stage_windows(win, F, a, g, bx0, by, lane);
-    wave_lds_sync();
-
-#pragma unroll 1
-    for (int m = 0; m < M; m++) {
-      const int bx = bx0 + 16 * m;
-      if (bx >= a.w) break;
-      const int bw = min(16, a.w - bx);
-      const int cbw = (bw + 1) >> 1;
-      int bestx, besty;
-      uint32_t best_sad;
-      if (TABLE) {
-        const int32_t* t = table + 3 * (((size_t)fi * a.brows + brow) * a.bcols + (bx >> 4));
-        bestx = __builtin_amdgcn_readfirstlane(t[0]);
-        besty = __builtin_amdgcn_readfirstlane(t[1]);
-        best_sad = (uint32_t)__builtin_amdgcn_readfirstlane(t[2]);
-      } else {
-        const uint8_t* s_src = s_cy + 2 * kk * 16 * M + 16 * m;
-        const uint4 src0 = *reinterpret_cast<const uint4*>(s_src);
-        const uint4 src1 = *reinterpret_cast<const uint4*>(s_src + 16 * M);
-        const uint8_t* row0 = win + (2 * kk + 15) * G::lp + 16 * m + 16;
-        if (bw == 16 && bh == 16) {
-          // (0, 0) operands of the whole-wave SAD: row lane / 4, dword lane % 4
-          const uint8_t* zrow = win + ((lane >> 2) + 15) * G::lp + 16 * m + 16 + 4 * (lane & 3);
-          const uint32_t zsrc = *reinterpret_cast<const uint32_t*>(s_cy + (lane >> 2) * 16 * M + 16 * m + 4 * (lane & 3));
-          descend<true, G>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad, zrow, zsrc);
-        } else {
-          descend<false, G>(row0, src0, src1, gd_off, gq, kk, bw, bh, bestx, besty, best_sad, nullptr, 0);
-        }
-      }
-      if (!TABLE && a.mv_out && lane == 0) {
-        int32_t* o = a.mv_out + 3 * (((size_t)fi * a.brows + brow) * a.bcols + (bx >> 4));
-        o[0] = bestx; o[1] = besty; o[2] = (int32_t)best_sad;
-      }
-
-      // sad / (bw*bh) <= 12  <=>  sad < 13 * bw * bh (integer division).
-      if (best_sad < (kSadPerPixelCutoff + 1) * (uint32_t)(bw * bh)) {
-        uint32_t* acc = s_acc + 384 * m;
-        if (lr < bh) {
-          uint32_t ref4;
-          if (TABLE) {
-            const uint8_t* rrow = F.y + (size_t)clampi(by + lr + besty, 0, a.h - 1) * F.y_stride;
-            const int xr = bx + 4 * lq + bestx;
-            ref4 = 0;
 #pragma unroll
-            for (int j = 0; j < 4; j++) ref4 |= (uint32_t)rrow[clampi(xr + j, 0, a.w - 1)] << (8 * j);
-          } else {
-            const uint8_t* rp4 = win + (lr + besty + 15) * g.lp + 16 * m + 16 + 4 * lq + bestx;
-            const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
-            const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
-            ref4 = funnel(rw[0], rw[1], rsh);
-          }
-          const uint32_t ctr4 = *reinterpret_cast<const uint32_t*>(s_cy + lr * 16 * M + 16 * m + 4 * lq);
-#pragma unroll
-          for (int j = 0; j < 4; j++) {
-            if (4 * lq + j < bw) {
-              const int rp = (ref4 >> (8 * j)) & 255;
-              const int cp = (ctr4 >> (8 * j)) & 255;
-              const uint32_t wgt = s_lut[rp > cp ? rp - cp : cp - rp];
-              lds_add(&acc[lr * 16 + 4 * lq + j], (wgt << kAccShift) | (wgt * (uint32_t)rp));
-            }
-          }
-        }
-        if (cr < cbh && cc < cbw) {
-          const int cmvx = bestx / 2, cmvy = besty / 2;  // truncation toward zero
-          int ru, rv;
-          if (TABLE) {
-            const size_t o = (size_t)clampi(cby + cr + cmvy, 0, a.ch - 1) * F.uv_stride +
-                             clampi((bx >> 1) + cc + cmvx, 0, a.cw - 1);
-            ru = F.u[o];
-            rv = F.v[o];
-          } else {
-            const int o = (cr + cmvy + 7) * g.cp + 8 * m + 8 + cc + cmvx;
-            ru = cwin[o];
-            rv = cwin[kCWinRows * g.cp + o];
-          }
-          const int cu = s_cu[cr * 8 * M + 8 * m + cc], cv = s_cv[cr * 8 * M + 8 * m + cc];
-          const uint32_t wu = s_lut[ru > cu ? ru - cu : cu - ru];
-          const uint32_t wv = s_lut[rv > cv ? rv - cv : cv - rv];
-          lds_add(&acc[256 + cr * 8 + cc], (wu << kAccShift) | (wu * (uint32_t)ru));
-          lds_add(&acc[320 + cr * 8 + cc], (wv << kAccShift) | (wv * (uint32_t)rv));
-        }
-      }
-    }
-    wave_lds_sync();  // window reads done before the next neighbour restages it
+    for (int i = 0; i < 3; i++)
+      if (lane + 64 * i < kLumaPieces) reinterpret_cast<uint4*>(win)[lane + 64 * i] = v[i];
+    return;
   }
-  __syncthreads();
-
-  // Resolve the strip: each thread handles 4 horizontally adjacent pixels.
-  for (int i = tid; i < 96 * M; i += blockDim.x) {
-    const int m = i / 96, j = i - m * 96;
-    const int bx = bx0 + 16 * m;
-    if (bx >= a.w) continue;
-    const uint32_t* acc = s_acc + 384 * m;
-    int x, y, plane_w, base;
-    const uint8_t* ctrp;
-    uint8_t* out;
-    if (j < 64) {  // luma: 16 rows x 4 quads
-      const int r = j >> 2, q = j & 3;
-      x = bx + 4 * q; y = by + r; plane_w = a.w; base = r * 16 + 4 * q;
-      ctrp = s_cy + r * 16 * M + 16 * m + 4 * q;
-      out = a.out_y;
-      if (r >= bh) continue;
-    } else {  // chroma: 2 planes x 8 rows x 2 quads
-      const int t = j - 64, pl = t >= 16, u = t & 15, r = u >> 1, q = u & 1;
-      x = (bx >> 1) + 4 * q; y = cby + r; plane_w = a.cw; base = 256 + 64 * pl + r * 8 + 4 * q;
-      ctrp = (pl ? s_cv : s_cu) + r * 8 * M + 8 * m + 4 * q;
-      out = pl ? a.out_v : a.out_u;
-      if (r >= cbh) continue;
-    }
-    uint32_t px4 = 0;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const uint32_t packed = acc[base + e];
-      const uint32_t sum = kWMax * ctrp[e] + (packed & ((1u << kAccShift) - 1));
-      const uint32_t ws = kWMax + (packed >> kAccShift);
-      px4 |= ((sum + (ws >> 1)) / ws) << (8 * e);
-    }
-    uint8_t* dst = out + (size_t)y * plane_w + x;
-    if (x + 4 <= plane_w && ((((uintptr_t)dst) & 3) == 0)) {
-      *reinterpret_cast<uint32_t*>(dst) = px4;
-    } else {
-      for (int e = 0; e < 4; e++)
-        if (x + e < plane_w) dst[e] = (uint8_t)(px4 >> (8 * e));
-    }
+  constexpr int ldw = kLumaPitch / 4;  // dwords per window row
+  for (int s = lane; s < kWinRows * ldw; s += 64) {
+    const int wy = s / ldw, dq = s - wy * ldw;
+    const int y = clampi(by - 15 + wy, 0, a.h - 1);
+    reinterpret_cast<uint32_t*>(win)[s] =
+        load_clamped4(f.y + (size_t)y * f.y_stride, bx - 16 + 4 * dq, a.w, a.aligned4 != 0);
   }
 }
 
-template <int M>
+// Row r of the anchor MB as 16 bytes, zero beyond (bw, bh).
+__device__ __forceinline__ uint4 centre_row16(const FrameDesc& C, int bx, int by, int r, int bw, int bh, bool fast16) {
+  if (fast16) return *reinterpret_cast<const uint4*>(C.y + (size_t)(by + r) * C.y_stride + bx);
+  uint32_t d[4] = {0u, 0u, 0u, 0u};
+  if (r < bh) {
+    const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      if (j < bw) d[j >> 2] |= (uint32_t)row[j] << (8 * (j & 3));
+  }
+  return make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+// One anchor MB as a wave sees it, and the lane's packed accumulators
+// (wsum << 20 | acc) of luma pixels 4*lq .. 4*lq+3 of row lr and of chroma
+// pixel (cr, cc) of U and V.
+struct MbState {
+  int bx, by, bw, bh, cbx, cby, brow;
+  uint32_t ctr4, cu, cv;  // centre pixels of the lane's blend positions
+  uint32_t ay0, ay1, ay2, ay3, au, av;
+};
+
+// x = ref | 1 << 20: sad(x, ctr | 1 << 20) = |ref - ctr|, and wt * x adds wt
+// to the weight field and wt * ref to the sum.
+__device__ __forceinline__ void blend_px(uint32_t& acc, uint32_t ref, uint32_t ctr, const uint8_t* lut) {
+  const uint32_t x = ref | kAccOne;
+  acc += lut[sad4(x, ctr | kAccOne, 0)] * x;
+}
+
+// Frame loops of one wave, one per kind of MB, so that each loop keeps only
+// its own lane constants live:
+//   kPathFast:   interior 16x16 MB: the unmasked search, chroma unclamped.
+//   kPathDma:    MB in the top or bottom rows (or partial), columns
+//                interior: the masked search (identical to the unmasked one
+//                for a whole MB), chroma clamped.
+//   kPathGather: MB at the left or right edge, or planes not 16-byte
+//                aligned or without one stride: clamped gathers.
+//   kPathTable:  vectors from the table, neighbour read from memory.
+// Fast and Dma MBs (16-byte aligned luma, one stride for all frames,
+// columns bx-16..bx+31 inside the frame) take their window by DMA with
+// row-clamped piece offsets; the next frame's window arrives during this
+// frame's search.  A frame's chroma bytes are loaded after its search and
+// blended after the next frame's wait, so that their latency overlaps the
+// next search.
+enum { kPathTable = 0, kPathFast = 1, kPathDma = 2, kPathGather = 3 };
+
+template <int PATH, class FrameAt>
+__device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& frame_at, MbState& s, int split,
+                                           int K, int lane, uint8_t* win, const uint8_t* s_lut,
+                                           const int32_t* table) {
+  const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
+  const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
+  const int nnb = a.filter_on ? a.n - 1 : 0;
+  // search: lane = probe group gq x row pair kk (rows kk, kk + 8)
+  const int gq = lane >> 3, kk = lane & 7;
+  uint4 src0 = make_uint4(0, 0, 0, 0), src1 = src0;
+  uint32_t off0 = 0, off1 = 0, off2 = 0, coff = 0;
+  if (PATH != kPathTable) {
+    const FrameDesc C = frame_at(a.center);
+    src0 = centre_row16(C, s.bx, s.by, kk, s.bw, s.bh, PATH == kPathFast);
+    src1 = centre_row16(C, s.bx, s.by, kk + 8, s.bw, s.bh, PATH == kPathFast);
+    if (PATH == kPathFast || PATH == kPathDma) {
+      // plane offset of lane piece p: window row p / 3 (clamped to the
+      // frame), 16-byte column p % 3
+      auto piece = [&](int p) -> uint32_t {
+        const int row = p / 3;
+        const int y = clampi(s.by - 15 + row, 0, a.h - 1);
+        return (uint32_t)y * C.y_stride + (uint32_t)(s.bx - 16) + 16u * (uint32_t)(p - 3 * row);
+      };
+      off0 = piece(lane);
+      off1 = piece(lane + 64);
+      off2 = piece(min(lane + 128, kLumaPieces - 1));
+    }
+    if (PATH == kPathFast) coff = (uint32_t)cr * C.uv_stride + (uint32_t)cc;
+  }
+  constexpr bool kDma = PATH == kPathFast || PATH == kPathDma;
+
+  int k = split;
+  int cur = 0;  // window buffer of the frame being searched
+  FrameDesc F{};
+  if (k < nnb) {
+    F = frame_at(k < a.center ? k : k + 1);
+    if (kDma) dma_window(F.y, off0, off1, off2, win, lane);
+  }
+  bool pend = false;  // chroma bytes of the previous frame await their blend
+  uint32_t ru = 0, rv = 0;
+  for (; k < nnb; k += K) {
+    const int fi = k < a.center ? k : k + 1;
+    const int kn = k + K;
+    FrameDesc Fn{};
+    if (kn < nnb) Fn = frame_at(kn < a.center ? kn : kn + 1);
+    uint8_t* w = win + cur * kLumaWin;
+    if (kDma) wait_vm();  // this frame's window and the previous frame's chroma have landed
+    // The previous frame's chroma registers are read (unconditionally: the
+    // compiler's own waits for them then sit here) before the next DMA goes
+    // out, so none of those waits covers the DMA.
+    asm volatile("" ::"v"(ru), "v"(rv));
+    if (pend) {
+      blend_px(s.au, ru, s.cu, s_lut);
+      blend_px(s.av, rv, s.cv, s_lut);
+    }
+    if (kDma) {
+      if (kn < nnb) dma_window(Fn.y, off0, off1, off2, win + (cur ^ 1) * kLumaWin, lane);
+    } else if (PATH == kPathGather) {
+      wave_lds_sync();  // the previous frame's window reads are done
+      stage_gather(w, F, a, s.bx, s.by, lane);
+      wave_lds_sync();
+    }
+
+    int bestx, besty;
+    uint32_t best_sad;
+    if (PATH == kPathTable) {
+      const int32_t* t = table + 3 * (((size_t)fi * a.brows + s.brow) * a.bcols + (s.bx >> 4));
+      bestx = __builtin_amdgcn_readfirstlane(t[0]);
+      besty = __builtin_amdgcn_readfirstlane(t[1]);
+      best_sad = (uint32_t)__builtin_amdgcn_readfirstlane(t[2]);
+    } else {
+      const int gd_off = (((kProbeDy1 >> (2 * gq)) & 3) - 1) * kLumaPitch + (((kProbeDx1 >> (2 * gq)) & 3) - 1);
+      const uint8_t* row0 = w + (kk + 15) * kLumaPitch + 16;
+      if (PATH == kPathFast) {
+        // (0, 0) operands of the whole-wave SAD: row lane / 4, dword lane % 4
+        const uint8_t* zrow = w + (lr + 15) * kLumaPitch + 16 + 4 * lq;
+        descend<true, MbGeom>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad, zrow, s.ctr4);
+      } else {
+        descend<false, MbGeom>(row0, src0, src1, gd_off, gq, kk, s.bw, s.bh, bestx, besty, best_sad, nullptr, 0);
+      }
+      if (a.mv_out && lane == 0) {
+        int32_t* o = a.mv_out + 3 * (((size_t)fi * a.brows + s.brow) * a.bcols + (s.bx >> 4));
+        o[0] = bestx;
+        o[1] = besty;
+        o[2] = (int32_t)best_sad;
+      }
+    }
+
+    // sad / (bw*bh) <= 12  <=>  sad < 13 * bw * bh (integer division).
+    // Positions outside the block accumulate too (their window bytes are
+    // valid clamped samples); the resolve never writes them out.
+    pend = best_sad < (kSadPerPixelCutoff + 1) * (uint32_t)(s.bw * s.bh);
+    if (pend) {
+      const int cmvx = bestx / 2, cmvy = besty / 2;  // truncation toward zero
+      uint32_t ref4;
+      if (PATH == kPathTable) {
+        const uint8_t* rrow = F.y + (size_t)clampi(s.by + lr + besty, 0, a.h - 1) * F.y_stride;
+        const int xr = s.bx + 4 * lq + bestx;
+        ref4 = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) ref4 |= (uint32_t)rrow[clampi(xr + j, 0, a.w - 1)] << (8 * j);
+      } else {
+        const uint8_t* rp4 = w + (lr + besty + 15) * kLumaPitch + 16 + 4 * lq + bestx;
+        const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
+        ref4 = funnel(rw[0], rw[1], rsh);
+      }
+      if (PATH == kPathFast) {
+        const ptrdiff_t cb = (ptrdiff_t)(s.cby + cmvy) * F.uv_stride + s.cbx + cmvx;
+        ru = F.u[cb + coff];
+        rv = F.v[cb + coff];
+      } else {
+        const size_t o = (size_t)clampi(s.cby + cr + cmvy, 0, a.ch - 1) * F.uv_stride +
+                         clampi(s.cbx + cc + cmvx, 0, a.cw - 1);
+        ru = F.u[o];
+        rv = F.v[o];
+      }
+      blend_px(s.ay0, ref4 & 255, s.ctr4 & 255, s_lut);
+      blend_px(s.ay1, (ref4 >> 8) & 255, (s.ctr4 >> 8) & 255, s_lut);
+      blend_px(s.ay2, (ref4 >> 16) & 255, (s.ctr4 >> 16) & 255, s_lut);
+      blend_px(s.ay3, ref4 >> 24, s.ctr4 >> 24, s_lut);
+    }
+    F = Fn;
+    cur ^= 1;
+  }
+  if (pend) {
+    blend_px(s.au, ru, s.cu, s_lut);
+    blend_px(s.av, rv, s.cv, s_lut);
+  }
+}
+
+// Anchor MB `mi` of strip (brow, scol) of group `a`, frames split `split` of
+// K: the wave's frames are k = split, split + K, ... of the nnb neighbours.
+// `frame_at(i)` returns frame i of the group, `wt4` the weight LUT.
+// TABLE: the per-block vectors and SADs come from `table` (layout
+// n x brows x bcols x {mvx, mvy, sad}) instead of the descent, and the
+// blend reads the neighbour straight from memory with clamped coordinates
+// (the vectors are not bounded by the +-15 window).
+// LDS: weight LUT 256 B | (K > 1) M x 384 packed accumulators | per-wave
+// window pairs.  Every wave of the workgroup calls this (inactive MBs past the
+// right frame edge included: they take part in the barriers); the caller
+// synchronises the workgroup before reusing LDS.
+template <bool TABLE, class FrameAt>
+__device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* wt4, const FrameAt& frame_at,
+                                            int brow, int scol, int M, uint8_t* smem,
+                                            const int32_t* table = nullptr) {
+  const int nwaves = blockDim.x >> 6;
+  const int K = nwaves / M;
+  uint8_t* s_lut = smem;
+  uint32_t* s_acc = reinterpret_cast<uint32_t*>(smem + 256);
+  uint8_t* s_win = smem + 256 + (K > 1 ? M * 384 * 4 : 0);
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int mi = wave % M, split = wave / M;
+
+  for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = wt4[i];
+  if (K > 1)
+    for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
+  __syncthreads();
+
+  const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
+  const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
+  MbState s;
+  s.bx = (scol * M + mi) * 16;
+  s.by = brow * 16;
+  s.brow = brow;
+  const bool active = s.bx < a.w;
+  s.bw = min(16, a.w - s.bx);
+  s.bh = min(16, a.h - s.by);
+  s.cbx = s.bx >> 1;
+  s.cby = s.by >> 1;
+  const int cbw = (s.bw + 1) >> 1, cbh = (s.bh + 1) >> 1;
+  s.ay0 = s.ay1 = s.ay2 = s.ay3 = s.au = s.av = 0;
+  s.ctr4 = s.cu = s.cv = 0;
+
+  if (active) {
+    const FrameDesc C = frame_at(a.center);
+    // centre pixels of this lane's blend positions (zero outside the frame)
+    if (s.bw == 16 && s.bh == 16 && a.aligned4) {
+      s.ctr4 = *reinterpret_cast<const uint32_t*>(C.y + (size_t)(s.by + lr) * C.y_stride + s.bx + 4 * lq);
+    } else if (lr < s.bh) {
+      const uint8_t* row = C.y + (size_t)(s.by + lr) * C.y_stride + s.bx + 4 * lq;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (4 * lq + j < s.bw) s.ctr4 |= (uint32_t)row[j] << (8 * j);
+    }
+    if (cr < cbh && cc < cbw) {
+      const size_t o = (size_t)(s.cby + cr) * C.uv_stride + s.cbx + cc;
+      s.cu = C.u[o];
+      s.cv = C.v[o];
+    }
+    uint8_t* win = s_win + wave * kWaveWinBytes;
+    // DMA: window columns inside the frame; fast: also the window rows and
+    // every chroma read at +-7
+    const bool dma = a.aligned16 && a.uniform_strides && s.bx >= 16 && s.bx + 32 <= a.w;
+    const bool fast = dma && s.by >= 15 && s.by + 31 <= a.h && s.cbx >= 7 && s.cbx + 15 <= a.cw && s.cby >= 7 &&
+                      s.cby + 15 <= a.ch;
+    if (TABLE)
+      frame_loop<kPathTable>(a, frame_at, s, split, K, lane, win, s_lut, table);
+    else if (fast)
+      frame_loop<kPathFast>(a, frame_at, s, split, K, lane, win, s_lut, table);
+    else if (dma)
+      frame_loop<kPathDma>(a, frame_at, s, split, K, lane, win, s_lut, table);
+    else
+      frame_loop<kPathGather>(a, frame_at, s, split, K, lane, win, s_lut, table);
+  }
+
+  if (K > 1) {
+    uint32_t* acc = s_acc + 384 * mi;
+    if (active) {
+      lds_add(&acc[lr * 16 + 4 * lq + 0], s.ay0);
+      lds_add(&acc[lr * 16 + 4 * lq + 1], s.ay1);
+      lds_add(&acc[lr * 16 + 4 * lq + 2], s.ay2);
+      lds_add(&acc[lr * 16 + 4 * lq + 3], s.ay3);
+      lds_add(&acc[256 + cr * 8 + cc], s.au);
+      lds_add(&acc[320 + cr * 8 + cc], s.av);
+    }
+    __syncthreads();
+    if (active && split == 0) {
+      const uint4 v = *reinterpret_cast<const uint4*>(&acc[lr * 16 + 4 * lq]);
+      s.ay0 = v.x;
+      s.ay1 = v.y;
+      s.ay2 = v.z;
+      s.ay3 = v.w;
+      s.au = acc[256 + cr * 8 + cc];
+      s.av = acc[320 + cr * 8 + cc];
+    }
+  }
+  if (!active || split != 0) return;
+
+  // Resolve: (16 * ctr + acc + ws / 2) / ws with ws = 16 + wsum.
+  auto resolve = [](uint32_t packed, uint32_t ctr) -> uint32_t {
+    const uint32_t sum = kWMax * ctr + (packed & (kAccOne - 1));
+    const uint32_t ws = kWMax + (packed >> kAccShift);
+    return (sum + (ws >> 1)) / ws;
+  };
+  if (lr < s.bh) {
+    const uint32_t c = s.ctr4;
+    const uint32_t px4 = resolve(s.ay0, c & 255) | (resolve(s.ay1, (c >> 8) & 255) << 8) |
+                         (resolve(s.ay2, (c >> 16) & 255) << 16) | (resolve(s.ay3, c >> 24) << 24);
+    uint8_t* dst = a.out_y + (size_t)(s.by + lr) * a.w + s.bx + 4 * lq;
+    if (4 * lq + 4 <= s.bw && ((((uintptr_t)dst) & 3) == 0)) {
+      *reinterpret_cast<uint32_t*>(dst) = px4;
+    } else {
+      for (int e = 0; e < 4; e++)
+        if (4 * lq + e < s.bw) dst[e] = (uint8_t)(px4 >> (8 * e));
+    }
+  }
+  if (cr < cbh && cc < cbw) {
+    const size_t o = (size_t)(s.cby + cr) * a.cw + s.cbx + cc;
+    a.out_u[o] = (uint8_t)resolve(s.au, s.cu);
+    a.out_v[o] = (uint8_t)resolve(s.av, s.cv);
+  }
+}
+
 __global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int total = a.g.strips_per_row * a.g.brows;
   const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
   const int brow = strip / a.g.strips_per_row;
   const int scol = strip - brow * a.g.strips_per_row;
-  process_strip<M, false>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, smem);
+  process_mbs<false>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, a.mbs_per_wg, smem);
 }
 
 // The altref from a given alignment table (guard, blend, resolve).
-template <int M>
 __global__ void __launch_bounds__(1024) arnr_table_kernel(const ArnrArgs a, const int32_t* table) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int total = a.g.strips_per_row * a.g.brows;
   const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
   const int brow = strip / a.g.strips_per_row;
   const int scol = strip - brow * a.g.strips_per_row;
-  process_strip<M, true>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, smem, table);
+  process_mbs<true>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, a.mbs_per_wg, smem, table);
 }
 
 // Descriptor fields read from global memory and made wave-uniform, so that
@@ -532,14 +603,15 @@ __device__ __forceinline__ ArnrGroup load_group(const ArnrGroup* p) {
   g.aligned4 = uload(&p->aligned4);
   g.aligned16 = uload(&p->aligned16);
   g.aligned4c = uload(&p->aligned4c);
+  g.uniform_strides = uload(&p->uniform_strides);
   g.strips_per_row = uload(&p->strips_per_row);
   return g;
 }
 
+
 // Persistent batch kernel: every workgroup loops taking units (one strip of
 // one group) until all chunks are drained; a unit is processed exactly as
 // the single-group kernel processes that strip.
-template <int M>
 __global__ void __launch_bounds__(1024) arnr_batch_kernel(const ArnrBatchArgs b) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_unit;
@@ -565,8 +637,8 @@ __global__ void __launch_bounds__(1024) arnr_batch_kernel(const ArnrBatchArgs b)
       const int local = unit - uload(&job->unit_base);
       const int brow = local / g.strips_per_row;
       const int scol = local - brow * g.strips_per_row;
-      process_strip<M, false>(g, job->wt4, [&](int i) { return load_frame(b.frames + fb + i); }, brow, scol,
-                       smem);
+      process_mbs<false>(g, job->wt4, [&](int i) { return load_frame(b.frames + fb + i); }, brow, scol,
+                         b.mbs_per_wg, smem);
     }
     __syncthreads();  // LDS and s_unit are free for the next unit
   }
@@ -577,68 +649,45 @@ __global__ void zero_u32_kernel(uint32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-size_t arnr_fused_lds_bytes(int nw, int m) {
-  const int lp = 16 * m + 40, cp = 8 * m + 16;  // StripGeom<m>::lp, ::cp
-  const int wave_bytes = (kWinRows * lp + 2 * kCWinRows * cp + 15) & ~15;
-  return (size_t)m * 384 * 4 + 256 + 256 * m + 128 * m + (size_t)nw * wave_bytes;
+
+// LDS of one workgroup: LUT, the split accumulators when K > 1, and one
+// window per wave unless the vectors come from a table.
+size_t arnr_lds_bytes(int k, int m, bool table) {
+  return 256 + (k > 1 ? (size_t)m * 384 * 4 : 0) + (table ? 0 : (size_t)m * k * kWaveWinBytes);
 }
 
 int arnr_supported_mbs(int m) { return m >= 8 ? 8 : (m >= 4 ? 4 : (m >= 2 ? 2 : 1)); }
 
-hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream) {
+hipError_t launch_arnr_fused(const ArnrArgs& a, int k, hipStream_t stream) {
   const int blocks = a.g.strips_per_row * a.g.brows;
-  const size_t lds = arnr_fused_lds_bytes(nw, a.mbs_per_wg);
-  switch (a.mbs_per_wg) {
-    case 1: hipLaunchKernelGGL(arnr_fused_kernel<1>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
-    case 2: hipLaunchKernelGGL(arnr_fused_kernel<2>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
-    case 4: hipLaunchKernelGGL(arnr_fused_kernel<4>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
-    case 8: hipLaunchKernelGGL(arnr_fused_kernel<8>, dim3(blocks), dim3(64 * nw), lds, stream, a); break;
-    default: return hipErrorInvalidValue;
-  }
+  const int m = a.mbs_per_wg;
+  if (m < 1 || k < 1 || m * k > 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(arnr_fused_kernel, dim3(blocks), dim3(64 * m * k), arnr_lds_bytes(k, m, false), stream, a);
   return hipGetLastError();
 }
 
-template <int M>
-static int batch_occupancy(int nw) {
+int arnr_batch_workgroups_per_cu(int k, int m) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, arnr_batch_kernel<M>, 64 * nw,
-                                                   arnr_fused_lds_bytes(nw, M)) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, arnr_batch_kernel, 64 * m * k,
+                                                   arnr_lds_bytes(k, m, false)) != hipSuccess)
     return 0;
   return per_cu;
 }
 
-int arnr_batch_workgroups_per_cu(int nw, int m) {
-  switch (m) {
-    case 1: return batch_occupancy<1>(nw);
-    case 2: return batch_occupancy<2>(nw);
-    case 4: return batch_occupancy<4>(nw);
-    case 8: return batch_occupancy<8>(nw);
-    default: return 0;
-  }
-}
-
-hipError_t launch_arnr_batch(const ArnrBatchArgs& b, int nw, int workgroups, hipStream_t stream) {
-  const size_t lds = arnr_fused_lds_bytes(nw, b.mbs_per_wg);
-  switch (b.mbs_per_wg) {
-    case 1: hipLaunchKernelGGL(arnr_batch_kernel<1>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
-    case 2: hipLaunchKernelGGL(arnr_batch_kernel<2>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
-    case 4: hipLaunchKernelGGL(arnr_batch_kernel<4>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
-    case 8: hipLaunchKernelGGL(arnr_batch_kernel<8>, dim3(workgroups), dim3(64 * nw), lds, stream, b); break;
-    default: return hipErrorInvalidValue;
-  }
+hipError_t launch_arnr_batch(const ArnrBatchArgs& b, int k, int workgroups, hipStream_t stream) {
+  const int m = b.mbs_per_wg;
+  if (m < 1 || k < 1 || m * k > 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(arnr_batch_kernel, dim3(workgroups), dim3(64 * m * k), arnr_lds_bytes(k, m, false), stream,
+                     b);
   return hipGetLastError();
 }
 
-hipError_t launch_arnr_table(const ArnrArgs& a, const int32_t* table, int nw, hipStream_t stream) {
+hipError_t launch_arnr_table(const ArnrArgs& a, const int32_t* table, int k, hipStream_t stream) {
   const int blocks = a.g.strips_per_row * a.g.brows;
-  const size_t lds = arnr_fused_lds_bytes(0, a.mbs_per_wg);  // no staged windows
-  switch (a.mbs_per_wg) {
-    case 1: hipLaunchKernelGGL(arnr_table_kernel<1>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
-    case 2: hipLaunchKernelGGL(arnr_table_kernel<2>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
-    case 4: hipLaunchKernelGGL(arnr_table_kernel<4>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
-    case 8: hipLaunchKernelGGL(arnr_table_kernel<8>, dim3(blocks), dim3(64 * nw), lds, stream, a, table); break;
-    default: return hipErrorInvalidValue;
-  }
+  const int m = a.mbs_per_wg;
+  if (m < 1 || k < 1 || m * k > 16) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(arnr_table_kernel, dim3(blocks), dim3(64 * m * k), arnr_lds_bytes(k, m, true), stream, a,
+                     table);
   return hipGetLastError();
 }
 

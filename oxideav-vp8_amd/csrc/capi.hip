@@ -69,32 +69,39 @@ int validate_arnr(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* er
   return VP8G_OK;
 }
 
+// Shrink (M, K) until M * K <= 16 waves and the workgroup's LDS fits:
+// M first, then K.
+void fit_shape(int* k, int* m, bool table) {
+  while (*m > 1 && (*m * *k > 16 || arnr_lds_bytes(*k, *m, table) > kMaxLdsPerWorkgroup))
+    *m = arnr_supported_mbs(*m - 1);
+  while (*k > 1 && (*m * *k > 16 || arnr_lds_bytes(*k, *m, table) > kMaxLdsPerWorkgroup)) (*k)--;
+}
+
 // Launch shape of one group: explicit options first, then the tuning
-// table's entry for (resolution class, group length), then the defaults
-// (one wave per neighbour up to 8, 4 MBs per strip); shrunk until the LDS
-// fits.
-void pick_arnr_shape(const vp8g_ctx* c, int w, int h, int n, int nnb, int* nw_out, int* m_out) {
+// table's entry for (resolution class, group length), then the defaults:
+// 4 MBs per workgroup, and K waves per MB (frame splits) such that the
+// launch holds about 4096 waves, at most one per neighbour.
+void pick_arnr_shape(const vp8g_ctx* c, int w, int h, int n, int nnb, int* k_out, int* m_out) {
   const TuneEntry* e = lookup_tuning(c->tuning, resolution_class(w, h), n);
-  int nw = c->arnr_waves > 0 ? c->arnr_waves : (e && e->waves > 0 ? e->waves : (nnb < 8 ? nnb : 8));
-  if (nw < 1) nw = 1;
-  if (nw > 16) nw = 16;
+  const int mbs = ((w + 15) / 16) * ((h + 15) / 16);
+  int k = c->arnr_waves > 0 ? c->arnr_waves : (e && e->waves > 0 ? e->waves : (4096 + mbs - 1) / mbs);
+  if (k > nnb) k = nnb;
+  if (k < 1) k = 1;
+  if (k > 16) k = 16;
   int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : (e && e->mbs > 0 ? e->mbs : 4));
-  while (m > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) m = arnr_supported_mbs(m - 1);
-  while (nw > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) nw--;
-  *nw_out = nw;
+  fit_shape(&k, &m, false);
+  *k_out = k;
   *m_out = m;
 }
 
-// Batch launch shape: options, then the table's "batch" entry, then 2
-// waves (every group length 3..11 leaves an even number of neighbours) and
-// 4 MBs; same LDS limit.
-void pick_batch_shape(const vp8g_ctx* c, int* nw_out, int* m_out, int* wgs_per_cu) {
+// Batch launch shape: options, then the table's "batch" entry, then one
+// wave per MB and 4 MBs per workgroup; same limits.
+void pick_batch_shape(const vp8g_ctx* c, int* k_out, int* m_out, int* wgs_per_cu) {
   const TuneEntry* e = lookup_tuning(c->tuning, "batch", 0);
-  int nw = c->arnr_waves > 0 ? c->arnr_waves : (e && e->waves > 0 ? e->waves : 2);
+  int k = c->arnr_waves > 0 ? c->arnr_waves : (e && e->waves > 0 ? e->waves : 1);
   int m = arnr_supported_mbs(c->arnr_mbs > 0 ? c->arnr_mbs : (e && e->mbs > 0 ? e->mbs : 4));
-  while (m > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) m = arnr_supported_mbs(m - 1);
-  while (nw > 1 && arnr_fused_lds_bytes(nw, m) > kMaxLdsPerWorkgroup) nw--;
-  *nw_out = nw;
+  fit_shape(&k, &m, false);
+  *k_out = k;
   *m_out = m;
   *wgs_per_cu = c->batch_wgs_per_cu > 0 ? c->batch_wgs_per_cu : (e ? e->wgs_per_cu : 0);
 }
@@ -127,12 +134,15 @@ void fill_group(const vp8g_i420* frames, size_t n, size_t center, uint8_t streng
   memset(g, 0, sizeof(*g));
   const vp8g_i420& ctr = frames[center];
   bool aligned4 = true, aligned16 = true, aligned4c = true;
+  const size_t rows = ctr.height, crows = (rows + 1) / 2;
+  bool uniform_strides = rows * frames[0].y_stride < (1u << 31) && crows * frames[0].uv_stride < (1u << 31);
   for (size_t i = 0; i < n; i++) {
     const vp8g_i420& f = frames[i];
     fd[i] = FrameDesc{f.y, f.u, f.v, (uint32_t)f.y_stride, (uint32_t)f.uv_stride};
     if (((uintptr_t)f.y & 3) || (f.y_stride & 3)) aligned4 = false;
     if (((uintptr_t)f.y & 15) || (f.y_stride & 15)) aligned16 = false;
     if (((uintptr_t)f.u & 3) || ((uintptr_t)f.v & 3) || (f.uv_stride & 3)) aligned4c = false;
+    if (f.y_stride != frames[0].y_stride || f.uv_stride != frames[0].uv_stride) uniform_strides = false;
   }
   g->out_y = oy;
   g->out_u = ou;
@@ -150,6 +160,7 @@ void fill_group(const vp8g_i420* frames, size_t n, size_t center, uint8_t streng
   g->aligned4 = aligned4 ? 1 : 0;
   g->aligned16 = aligned16 ? 1 : 0;
   g->aligned4c = aligned4c ? 1 : 0;
+  g->uniform_strides = uniform_strides ? 1 : 0;
   // weight_table(8 << min(strength, 6)), src/arnr.rs:307-323, 533-536.
   const uint32_t s = 8u << (strength < kMaxStrength ? strength : kMaxStrength);
   uint8_t wt[256];
@@ -164,11 +175,11 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   memset(&a, 0, sizeof(a));
   fill_group(frames, n, center, strength, oy, ou, ov, mv_out, a.frames, &a.g, a.wt4);
   a.xcd_remap = c->xcd_remap;
-  int nw, m;
-  pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &nw, &m);
+  int k, m;
+  pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &k, &m);
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
-  return hip_ok(launch_arnr_fused(a, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return hip_ok(launch_arnr_fused(a, k, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
 int check_ref_planes(const vp8g_ref_planes* r) {
@@ -324,10 +335,10 @@ int vp8g_arnr_shape(vp8g_ctx* c, uint32_t width, uint32_t height, size_t n_frame
   if (!c || width == 0 || height == 0 || n_frames == 0 || n_frames > VP8G_MAX_FRAMES) return VP8G_E_ARG;
   const int n = (int)n_frames;
   const bool on = strength != 0 && n >= 2;
-  int nw, m;
-  pick_arnr_shape(c, (int)width, (int)height, n, on ? n - 1 : 1, &nw, &m);
+  int k, m;
+  pick_arnr_shape(c, (int)width, (int)height, n, on ? n - 1 : 1, &k, &m);
   if (mbs) *mbs = m;
-  if (waves) *waves = nw;
+  if (waves) *waves = k;
   return VP8G_OK;
 }
 
@@ -469,11 +480,11 @@ int vp8g_build_arnr_altref_table_dev(vp8g_ctx* c, const vp8g_i420* frames, size_
   memset(&a, 0, sizeof(a));
   fill_group(frames, n, center, strength, oy, ou, ov, nullptr, a.frames, &a.g, a.wt4);
   a.xcd_remap = c->xcd_remap;
-  int nw, m;
-  pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &nw, &m);
+  int k, m;
+  pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &k, &m);
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
-  return hip_ok(launch_arnr_table(a, table, nw, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return hip_ok(launch_arnr_table(a, table, k, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
 int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
@@ -493,8 +504,8 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
     total_frames += jb.n_frames;
   }
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
-  int nw, m, want_per_cu;
-  pick_batch_shape(c, &nw, &m, &want_per_cu);
+  int k, m, want_per_cu;
+  pick_batch_shape(c, &k, &m, &want_per_cu);
   std::vector<ArnrBatchJob> hj(nj);
   std::vector<FrameDesc> hf(total_frames);
   std::vector<uint16_t> hu;
@@ -514,7 +525,7 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
     fb += jb.n_frames;
   }
   if (units > 0x7FFFFFFF) return VP8G_E_ARG;
-  int per_cu = arnr_batch_workgroups_per_cu(nw, m);
+  int per_cu = arnr_batch_workgroups_per_cu(k, m);
   if (per_cu < 1) return VP8G_E_DEVICE;
   if (want_per_cu > 0 && want_per_cu < per_cu) per_cu = want_per_cu;
   size_t wgs = (size_t)per_cu * (size_t)c->cus;
@@ -523,7 +534,7 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
   vp8g_arnr_batch* b = new vp8g_arnr_batch();
   b->ctx = c;
   b->mbs = m;
-  b->waves = nw;
+  b->waves = k;
   b->workgroups = (int)wgs;
   b->units = units;
   const size_t o_jobs = 0, o_frames = align_up(sizeof(ArnrBatchJob) * nj, 256);
@@ -553,7 +564,7 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
          hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     if (ok) {
       const bool launched = hip_ok(launch_zero_u32(a.ticket, kBatchChunks, cs)) &&
-                            hip_ok(launch_arnr_batch(a, nw, (int)wgs, cs));
+                            hip_ok(launch_arnr_batch(a, k, (int)wgs, cs));
       ok = hip_ok(hipStreamEndCapture(cs, &b->graph)) && launched;
     }
     ok = ok && hip_ok(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));

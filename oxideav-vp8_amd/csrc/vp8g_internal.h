@@ -33,6 +33,7 @@ struct ArnrGroup {
   int aligned4;     // every luma plane pointer and stride is a multiple of 4
   int aligned16;    // ... of 16
   int aligned4c;    // every chroma plane pointer and stride is a multiple of 4
+  int uniform_strides;  // all frames share y_stride and uv_stride; plane offsets < 2^31
   int strips_per_row;
 };
 
@@ -44,6 +45,8 @@ struct ArnrArgs {
   int xcd_remap;    // map workgroups to contiguous MB bands per XCD
   int mbs_per_wg;   // M: anchor MBs per workgroup (one strip of an MB row)
 };
+// Launch shape: a workgroup holds M anchor MBs x K waves per MB (K splits the
+// neighbour frames), 64 * M * K threads, M * K <= 16.
 
 // Batched launch: many groups of any size and length in one persistent
 // kernel; workgroups take (group, strip) units from an atomic ticket.
@@ -68,15 +71,15 @@ struct ArnrBatchArgs {
   int mbs_per_wg;
 };
 
-hipError_t launch_arnr_batch(const ArnrBatchArgs& a, int nw, int workgroups, hipStream_t stream);
+hipError_t launch_arnr_batch(const ArnrBatchArgs& a, int k, int workgroups, hipStream_t stream);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t stream);
-int arnr_batch_workgroups_per_cu(int nw, int mbs_per_wg);
+int arnr_batch_workgroups_per_cu(int k, int mbs_per_wg);
 
 constexpr size_t kMaxLdsPerWorkgroup = 65536;
-size_t arnr_fused_lds_bytes(int nw, int mbs_per_wg);
-int arnr_supported_mbs(int m);  // largest compiled strip width <= m (1, 2, 4, 8)
-hipError_t launch_arnr_fused(const ArnrArgs& a, int nw, hipStream_t stream);
-hipError_t launch_arnr_table(const ArnrArgs& a, const int32_t* table, int nw, hipStream_t stream);
+size_t arnr_lds_bytes(int k, int mbs_per_wg, bool table);
+int arnr_supported_mbs(int m);  // strip width rounded down to 1, 2, 4 or 8
+hipError_t launch_arnr_fused(const ArnrArgs& a, int k, hipStream_t stream);
+hipError_t launch_arnr_table(const ArnrArgs& a, const int32_t* table, int k, hipStream_t stream);
 
 // Exhaustive block search.
 constexpr int kMaxSeed = 16384;  // seed components are clamped to +-kMaxSeed

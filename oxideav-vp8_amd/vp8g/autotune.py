@@ -1,14 +1,15 @@
 """Launch-shape autotuner for the ARNR kernels.
 
 Sweeps the shapes that never change output bytes — anchor MBs per
-workgroup (``mbs``), waves per workgroup (``waves``) and, for batches,
+workgroup (``mbs``), waves per anchor MB (``waves``, each taking a share
+of the neighbour frames; mbs x waves <= 16) and, for batches,
 persistent workgroups per CU (``wgs_per_cu``) — over device-resident
 synthetic workloads, timing each with HIP events on the library's stream,
 and writes the fastest shape per (resolution class, group length) as a
 JSON table that libvp8g.so reads when a context is created:
 
     {"arch": "gfx950",
-     "entries": [{"class": "1080p", "n": 7, "mbs": 4, "waves": 6,
+     "entries": [{"class": "1080p", "n": 7, "mbs": 4, "waves": 1,
                   "wgs_per_cu": 0, "ms": 0.061}, ...]}
 
 Class "batch" (n = 0) holds the shape of batched launches.  Resolution
@@ -121,7 +122,7 @@ def tune_single(torch, vp8g, ctx, clip: DeviceClip, log=print):
     best = None
     for mbs in MBS:
         for waves in WAVES:
-            if waves > max(1, len(clip.frames) - 1):
+            if waves > max(1, len(clip.frames) - 1) or mbs * waves > 16:
                 continue
             ctx.apply_tuning({"mbs": mbs, "waves": waves})
             ms = time_single(torch, vp8g, ctx, clip)
@@ -137,6 +138,8 @@ def tune_batch(torch, vp8g, ctx, clips, log=print):
     best = None
     for mbs in MBS:
         for waves in (1, 2, 4):
+            if mbs * waves > 16:
+                continue
             for wgs in WGS_PER_CU:
                 ctx.apply_tuning({"mbs": mbs, "waves": waves, "wgs_per_cu": wgs})
                 ms, shape = time_batch(torch, vp8g, ctx, clips, reps=5)
