@@ -133,43 +133,35 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     acc = sad4(src1.w, q1.w, acc);
     return sum8(acc);
   };
-  int bx = 0, by = 0, boff = 0;
+  // boff: window offset of the current best vector (dy * lp + dx)
+  int boff = 0;
   uint32_t bsad;
   if (FULL) {
     // SAD at (0, 0) over the whole wave: lane = (row l / 4, dword l % 4),
     // both operands dword aligned.
     uint32_t z = sad4(zsrc, *reinterpret_cast<const uint32_t*>(zrow), 0);
-    z = sum16(z);
-    bsad = (uint32_t)__builtin_amdgcn_readlane((int)z, 0) + (uint32_t)__builtin_amdgcn_readlane((int)z, 16) +
-           (uint32_t)__builtin_amdgcn_readlane((int)z, 32) + (uint32_t)__builtin_amdgcn_readlane((int)z, 48);
+    bsad = wave_sum_to_sgpr(sum16(z));
   } else {
     bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0));
   }
+  // Probe order is the group index and the incumbent keeps ties, so the key
+  // sad << 11 | group << 7 | (probe offset + 64) against best << 11 yields
+  // exactly the reference's winner as one unsigned minimum over the wave,
+  // and carries the winner's window offset (|offset| <= lp + 1 < 64).
+  static_assert(G::lp + 1 < 64, "probe offsets fit the key's 7-bit field");
+  const uint32_t key_lo = ((uint32_t)gq << 7) | (uint32_t)(gd_off + 64);
 #pragma unroll
   for (int sh = 3; sh >= 0; sh--) {
     const uint32_t s = sad_at(boff + gd_off * (1 << sh));
-    // Probe order is the group index and the incumbent keeps ties, so the
-    // key (sad << 4 | group + 1) against (best << 4) yields exactly the
-    // reference's winner as one unsigned minimum over the wave.
-    uint32_t key = (s << 4) | (uint32_t)(gq + 1);
-    key = min(key, dpp<0x128>(key));  // row_ror:8: the row's other group
-    const uint32_t k0 = (uint32_t)__builtin_amdgcn_readlane((int)key, 0);
-    const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)key, 16);
-    const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)key, 32);
-    const uint32_t k3 = (uint32_t)__builtin_amdgcn_readlane((int)key, 48);
-    const uint32_t kmin = min(min(k0, k1), min(k2, k3));
-    if (kmin < (bsad << 4)) {
-      const int w = (int)(kmin & 15) - 1;
-      const int dx = (int)((kProbeDx1 >> (2 * w)) & 3) - 1;
-      const int dy = (int)((kProbeDy1 >> (2 * w)) & 3) - 1;
-      bsad = kmin >> 4;
-      bx += dx * (1 << sh);
-      by += dy * (1 << sh);
-      boff += (dy * G::lp + dx) * (1 << sh);
+    const uint32_t kmin = wave_min_to_sgpr((s << 11) | key_lo);
+    if (kmin < (bsad << 11)) {
+      bsad = kmin >> 11;
+      boff += ((int)(kmin & 127) - 64) * (1 << sh);
     }
   }
-  bestx = bx;
-  besty = by;
+  // |dx| <= 15 < lp / 2: dy is boff / lp rounded to nearest, dx the rest
+  besty = (boff + 16 * G::lp + G::lp / 2) / G::lp - 16;
+  bestx = boff - besty * G::lp;
   best_sad = bsad;
 }
 
@@ -190,46 +182,26 @@ __device__ __forceinline__ int xcd_band(int i, int total) {
 // offset from it (row * stride + 16 * column).  The data is in LDS once
 // vmcnt has drained (wait_vm).
 using LdsPtr = __attribute__((address_space(3))) void*;
+// MASKED: lane piece i is skipped when bit i of `outside` is set (a piece
+// wholly outside the frame's columns; the caller fills it).
+template <bool MASKED>
 __device__ __forceinline__ void dma_window(const uint8_t* origin, uint32_t off0, uint32_t off1, uint32_t off2,
-                                           uint8_t* win, int lane) {
-  __builtin_amdgcn_global_load_lds(origin + off0, (LdsPtr)(win), 16, 0, 0);
-  __builtin_amdgcn_global_load_lds(origin + off1, (LdsPtr)(win + 1024), 16, 0, 0);
-  if (lane < kLumaPieces - 128) __builtin_amdgcn_global_load_lds(origin + off2, (LdsPtr)(win + 2048), 16, 0, 0);
+                                           uint8_t* win, int lane, uint32_t outside) {
+  if (!MASKED || !(outside & 1)) __builtin_amdgcn_global_load_lds(origin + off0, (LdsPtr)(win), 16, 0, 0);
+  if (!MASKED || !(outside & 2)) __builtin_amdgcn_global_load_lds(origin + off1, (LdsPtr)(win + 1024), 16, 0, 0);
+  if (lane < kLumaPieces - 128 && (!MASKED || !(outside & 4)))
+    __builtin_amdgcn_global_load_lds(origin + off2, (LdsPtr)(win + 2048), 16, 0, 0);
 }
 
 // Every vector memory operation of the wave done (DMA writes included), and
 // no memory access moved across.
 __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// MBs at the left or right frame edge (or planes without one stride or 16-
-// byte alignment): the window gathered with clamped rows and columns, in the
-// layout the DMA gives.  With 16-byte aligned planes and a width that is a
-// multiple of 16, every 16-byte piece lies wholly inside the frame (one
-// load) or wholly outside it (the edge pixel repeated): all three pieces of
-// a lane are loaded before the first store, so the window costs one memory
-// round trip.  Otherwise clamped dword gathers.
+// Windows the DMA cannot take (planes not 16-byte aligned or without one
+// stride, or a window piece straddling the frame's right edge): clamped
+// dword gathers into LDS.
 __device__ __forceinline__ void stage_gather(uint8_t* win, const FrameDesc& f, const ArnrGroup& a, int bx, int by,
                                              int lane) {
-  if (a.aligned16 && (a.w & 15) == 0) {
-    uint4 v[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const int p = min(lane + 64 * i, kLumaPieces - 1);
-      const int row = p / 3;
-      const uint8_t* rp = f.y + (size_t)clampi(by - 15 + row, 0, a.h - 1) * f.y_stride;
-      const int x0 = bx - 16 + 16 * (p - 3 * row);
-      if (x0 >= 0 && x0 + 16 <= a.w) {
-        v[i] = *reinterpret_cast<const uint4*>(rp + x0);
-      } else {
-        const uint32_t b = (uint32_t)rp[x0 < 0 ? 0 : a.w - 1] * 0x01010101u;
-        v[i] = make_uint4(b, b, b, b);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-      if (lane + 64 * i < kLumaPieces) reinterpret_cast<uint4*>(win)[lane + 64 * i] = v[i];
-    return;
-  }
   constexpr int ldw = kLumaPitch / 4;  // dwords per window row
   for (int s = lane; s < kWinRows * ldw; s += 64) {
     const int wy = s / ldw, dq = s - wy * ldw;
@@ -295,33 +267,58 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   // search: lane = probe group gq x row pair kk (rows kk, kk + 8)
   const int gq = lane >> 3, kk = lane & 7;
   uint4 src0 = make_uint4(0, 0, 0, 0), src1 = src0;
-  uint32_t off0 = 0, off1 = 0, off2 = 0, coff = 0;
+  // DMA paths: plane offset of lane piece i (window row p / 3 clamped to the
+  // frame, 16-byte column p % 3, p = lane + 64 i); kPathDma: bit i of
+  // `outside` set when the piece lies wholly outside the frame's columns,
+  // its offset then that of the row's edge pixel, which is repeated.
+  uint32_t off0 = 0, off1 = 0, off2 = 0, coff = 0, outside = 0;
   if (PATH != kPathTable) {
     const FrameDesc C = frame_at(a.center);
     src0 = centre_row16(C, s.bx, s.by, kk, s.bw, s.bh, PATH == kPathFast);
     src1 = centre_row16(C, s.bx, s.by, kk + 8, s.bw, s.bh, PATH == kPathFast);
     if (PATH == kPathFast || PATH == kPathDma) {
-      // plane offset of lane piece p: window row p / 3 (clamped to the
-      // frame), 16-byte column p % 3
-      auto piece = [&](int p) -> uint32_t {
+      auto piece = [&](int p, int i) -> uint32_t {
         const int row = p / 3;
-        const int y = clampi(s.by - 15 + row, 0, a.h - 1);
-        return (uint32_t)y * C.y_stride + (uint32_t)(s.bx - 16) + 16u * (uint32_t)(p - 3 * row);
+        const uint32_t y = (uint32_t)clampi(s.by - 15 + row, 0, a.h - 1) * C.y_stride;
+        const int x0 = s.bx - 16 + 16 * (p - 3 * row);
+        if (PATH == kPathDma && (x0 < 0 || x0 + 16 > a.w)) {
+          outside |= 1u << i;
+          return y + (uint32_t)(x0 < 0 ? 0 : a.w - 1);
+        }
+        return y + (uint32_t)x0;
       };
-      off0 = piece(lane);
-      off1 = piece(lane + 64);
-      off2 = piece(min(lane + 128, kLumaPieces - 1));
+      off0 = piece(lane, 0);
+      off1 = piece(lane + 64, 1);
+      off2 = piece(min(lane + 128, kLumaPieces - 1), 2);
     }
     if (PATH == kPathFast) coff = (uint32_t)cr * C.uv_stride + (uint32_t)cc;
   }
   constexpr bool kDma = PATH == kPathFast || PATH == kPathDma;
+  constexpr bool kMasked = PATH == kPathDma;
+  // edge pixels of the outside pieces of the frame in flight (kPathDma)
+  uint32_t e0 = 0, e1 = 0, e2 = 0;
+  auto fetch_edges = [&](const uint8_t* plane) {
+    if (outside & 1) e0 = plane[off0];
+    if (outside & 2) e1 = plane[off1];
+    if (outside & 4) e2 = plane[off2];
+  };
+  auto rep16 = [](uint32_t b) {
+    const uint32_t r = b * 0x01010101u;
+    return make_uint4(r, r, r, r);
+  };
+  auto fill_edges = [&](uint8_t* wb) {
+    if (outside & 1) reinterpret_cast<uint4*>(wb)[lane] = rep16(e0);
+    if (outside & 2) reinterpret_cast<uint4*>(wb)[lane + 64] = rep16(e1);
+    if ((outside & 4) && lane < kLumaPieces - 128) reinterpret_cast<uint4*>(wb)[lane + 128] = rep16(e2);
+  };
 
   int k = split;
   int cur = 0;  // window buffer of the frame being searched
   FrameDesc F{};
   if (k < nnb) {
     F = frame_at(k < a.center ? k : k + 1);
-    if (kDma) dma_window(F.y, off0, off1, off2, win, lane);
+    if (kDma) dma_window<kMasked>(F.y, off0, off1, off2, win, lane, outside);
+    if (kMasked) fetch_edges(F.y);
   }
   bool pend = false;  // chroma bytes of the previous frame await their blend
   uint32_t ru = 0, rv = 0;
@@ -332,16 +329,20 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     if (kn < nnb) Fn = frame_at(kn < a.center ? kn : kn + 1);
     uint8_t* w = win + cur * kLumaWin;
     if (kDma) wait_vm();  // this frame's window and the previous frame's chroma have landed
-    // The previous frame's chroma registers are read (unconditionally: the
-    // compiler's own waits for them then sit here) before the next DMA goes
+    // Registers loaded for this frame are read here (unconditionally: the
+    // compiler's own waits for them then sit here), before the next DMA goes
     // out, so none of those waits covers the DMA.
-    asm volatile("" ::"v"(ru), "v"(rv));
+    asm volatile("" ::"v"(ru), "v"(rv), "v"(e0), "v"(e1), "v"(e2));
+    if (kMasked) fill_edges(w);
     if (pend) {
       blend_px(s.au, ru, s.cu, s_lut);
       blend_px(s.av, rv, s.cv, s_lut);
     }
     if (kDma) {
-      if (kn < nnb) dma_window(Fn.y, off0, off1, off2, win + (cur ^ 1) * kLumaWin, lane);
+      if (kn < nnb) {
+        dma_window<kMasked>(Fn.y, off0, off1, off2, win + (cur ^ 1) * kLumaWin, lane, outside);
+        if (kMasked) fetch_edges(Fn.y);
+      }
     } else if (PATH == kPathGather) {
       wave_lds_sync();  // the previous frame's window reads are done
       stage_gather(w, F, a, s.bx, s.by, lane);
@@ -479,11 +480,12 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
       s.cv = C.v[o];
     }
     uint8_t* win = s_win + wave * kWaveWinBytes;
-    // DMA: window columns inside the frame; fast: also the window rows and
-    // every chroma read at +-7
-    const bool dma = a.aligned16 && a.uniform_strides && s.bx >= 16 && s.bx + 32 <= a.w;
-    const bool fast = dma && s.by >= 15 && s.by + 31 <= a.h && s.cbx >= 7 && s.cbx + 15 <= a.cw && s.cby >= 7 &&
-                      s.cby + 15 <= a.ch;
+    // DMA: every window piece wholly inside or wholly outside the frame's
+    // columns; fast: the whole window and every chroma read at +-7 inside
+    const bool cols_inside = s.bx >= 16 && s.bx + 32 <= a.w;
+    const bool dma = a.aligned16 && a.uniform_strides && (cols_inside || (a.w & 15) == 0);
+    const bool fast = dma && cols_inside && s.by >= 15 && s.by + 31 <= a.h && s.cbx >= 7 && s.cbx + 15 <= a.cw &&
+                      s.cby >= 7 && s.cby + 15 <= a.ch;
     if (TABLE)
       frame_loop<kPathTable>(a, frame_at, s, split, K, lane, win, s_lut, table);
     else if (fast)

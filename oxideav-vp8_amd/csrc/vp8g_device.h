@@ -65,6 +65,34 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   return min(min(r0, r1), min(r2, r3));
 }
 
+// Row broadcasts (gfx9 DPP): lane 15 of each row to the next row
+// (row_bcast:15), lane 31 to rows 2 and 3 (row_bcast:31).  Rows outside
+// `RowMask` receive `old`.
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint32_t dpp_rows(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, Ctrl, RowMask, 0xF, false);
+}
+
+// Given a value that is uniform within each row of 16 lanes: the sum over
+// the four rows, wave-uniform (lane 63 after two row broadcasts).
+__device__ __forceinline__ uint32_t wave_sum_to_sgpr(uint32_t v) {
+  v += dpp_rows<kDppRowBcast15, 0xA>(0u, v);  // rows 1, 3: + previous row
+  v += dpp_rows<kDppRowBcast31, 0xC>(0u, v);  // rows 2, 3: + rows 0..1
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Given a value that is uniform within each aligned group of 8 lanes: the
+// minimum over the wave, wave-uniform.
+__device__ __forceinline__ uint32_t wave_min_to_sgpr(uint32_t v) {
+  v = min(v, dpp<0x128>(v));                       // row_ror:8: the row's other group
+  v = min(v, dpp_rows<kDppRowBcast15, 0xA>(~0u, v));  // rows 1, 3: min with previous row
+  v = min(v, dpp_rows<kDppRowBcast31, 0xC>(~0u, v));  // rows 2, 3: min with rows 0..1
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 }  // namespace vp8g

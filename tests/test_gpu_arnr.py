@@ -134,6 +134,51 @@ def test_device_entry_and_mv_table(gpu, oracle):
     assert (got == mvs).all()
 
 
+@pytest.mark.parametrize("layout", ["aligned", "odd_offsets", "mixed_strides", "odd_width"])
+def test_device_entry_plane_layouts(gpu, oracle, layout):
+    """Device-resident frames in every layout the kernel stages differently:
+    16-byte aligned planes of one stride (windows by DMA, pieces outside the
+    left and right edges filled from the edge pixel), planes at odd byte
+    offsets and frames of different strides (clamped gathers), and a width
+    that is not a multiple of 16 (gathers at the right edge only)."""
+    import torch
+    import vp8g
+    import vp8g.synth as synth
+    w, h = (136, 72) if layout == "odd_width" else (144, 80)
+    g = synth.arnr_group_cif(w, h, 5, clip=6)
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+    keep, descs, tuples = [], (vp8g._I420 * g.n)(), []
+    for i, (y, u, v) in enumerate(zip(g.y, g.u, g.v)):
+        pad = {"mixed_strides": 16 * (i % 3), "odd_width": 8}.get(layout, 0)
+        off = (1 + 2 * i) if layout == "odd_offsets" else 0
+        ys, cs = w + pad, cw + pad
+
+        def plane(p, pw, ph, stride):
+            buf = np.full(off + stride * ph, 77, np.uint8)
+            buf[off:].reshape(ph, stride)[:, :pw] = np.asarray(p).reshape(ph, pw)
+            t = torch.from_numpy(buf).cuda()
+            keep.append(t)
+            return t.data_ptr() + off, buf[off:]
+        py, hy = plane(y, w, h, ys)
+        pu, hu = plane(u, cw, ch, cs)
+        pv, hv = plane(v, cw, ch, cs)
+        descs[i] = vp8g._I420(w, h, py, pu, pv, ys, cs)
+        tuples.append((w, h, hy, hu, hv, ys, cs))
+    st, want, _, mvs = oracle.build_arnr_altref(tuples, g.center, 3, want_mvs=True)
+    assert st == 0
+    oy = torch.empty(w * h, dtype=torch.uint8, device="cuda")
+    ou = torch.empty(cw * ch, dtype=torch.uint8, device="cuda")
+    ov = torch.empty(cw * ch, dtype=torch.uint8, device="cuda")
+    mv_dev = torch.zeros(mvs.size, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    gpu.build_arnr_altref_dev(descs, g.n, g.center, 3, oy.data_ptr(), ou.data_ptr(), ov.data_ptr(),
+                              mv_dev.data_ptr())
+    gpu.synchronize()
+    assert (mv_dev.cpu().numpy().reshape(mvs.shape) == mvs).all()
+    assert (oy.cpu().numpy() == want[0]).all()
+    assert (ou.cpu().numpy() == want[1]).all() and (ov.cpu().numpy() == want[2]).all()
+
+
 def test_1080p_region_motion_group(gpu, oracle):
     """The 1920x1080 seven-frame region-motion group with occluders."""
     import vp8g.synth as synth
