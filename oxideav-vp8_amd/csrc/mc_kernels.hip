@@ -96,6 +96,142 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---- Dot-product form of the six-tap passes --------------------------------
+// A tap row as two signed-byte dwords for v_dot4_i32_i8 (taps 0..3, taps
+// 4..5).  With every sample s given as s ^ 0x80 (= s - 128 as a signed
+// byte), sum_k (s_k - 128) t_k + 128 * sum_k t_k is the unsigned sum; the
+// form applies to rows whose taps lie in [-128, 127] and sum to 128, which
+// is every non-identity row of the VP8 six-tap and bilinear tables
+// (dot_table_ok checks a caller's table).  Identity rows are copies.
+struct DotTaps {
+  int32_t lo, hi;
+  bool ident;
+};
+
+__device__ __forceinline__ DotTaps dot_taps(const int32_t* row, bool ident) {
+  DotTaps k;
+  k.ident = ident || (row[0] == 0 && row[1] == 0 && row[2] == 128 && row[3] == 0 && row[4] == 0 && row[5] == 0);
+  k.lo = (int32_t)((uint32_t)(row[0] & 255) | ((uint32_t)(row[1] & 255) << 8) | ((uint32_t)(row[2] & 255) << 16) |
+                   ((uint32_t)(row[3] & 255) << 24));
+  k.hi = (int32_t)((uint32_t)(row[4] & 255) | ((uint32_t)(row[5] & 255) << 8));
+  return k;
+}
+
+constexpr uint32_t kBias = 0x80808080u;
+
+// clamp255((sum_k s_k t_k + 64) >> 7): `a` holds samples 0..3 and `b` samples
+// 4, 5 at the byte positions of the set taps of `hi`, all as s ^ 0x80.
+__device__ __forceinline__ uint32_t tap6(uint32_t a, int32_t lo, uint32_t b, int32_t hi) {
+  const int x = __builtin_amdgcn_sdot4((int)a, lo, __builtin_amdgcn_sdot4((int)b, hi, 16384 + 64, false), false);
+  return (uint32_t)min(max(x, 0), 32767) >> 7;
+}
+
+// Horizontal pass over the (OW+5) x (OW+5) halo whose row r starts `sh`
+// bytes into the dword at halo + r * hp; writes the (OW+5) x OW result to
+// `tmp` (pitch OW) as samples ^ 0x80.  A unit is 4 outputs of one row: its
+// 9 support samples lie in 3 aligned dwords.
+template <int OW>
+__device__ __forceinline__ void hpass_dot(const uint8_t* halo, int hp, uint32_t sh, uint8_t* tmp, const DotTaps& f,
+                                          int lane) {
+  constexpr int UPR = OW / 4;
+  constexpr int UNITS = (OW + 5) * UPR;
+  for (int u = lane; u < UNITS; u += 64) {
+    const int r = u / UPR, c0 = 4 * (u % UPR);
+    const uint32_t* s32 = reinterpret_cast<const uint32_t*>(halo + r * hp + c0);
+    const uint32_t w0 = s32[0], w1 = s32[1], w2 = s32[2];
+    const uint32_t u0 = funnel(w0, w1, sh) ^ kBias, u1 = funnel(w1, w2, sh) ^ kBias;  // samples 0..3, 4..7
+    uint32_t packed;
+    if (f.ident) {
+      packed = funnel(u0, u1, 2);
+    } else {
+      const uint32_t u2 = funnel(w2, w2, sh) ^ kBias;  // byte 0: sample 8
+      const uint32_t o0 = tap6(u0, f.lo, u1, f.hi);
+      const uint32_t o1 = tap6(funnel(u0, u1, 1), f.lo, u1, f.hi << 8);
+      const uint32_t o2 = tap6(funnel(u0, u1, 2), f.lo, u1, f.hi << 16);
+      const uint32_t o3 = tap6(funnel(u0, u1, 3), f.lo, funnel(u1, u2, 3), f.hi);
+      packed = (o0 | (o1 << 8) | (o2 << 16) | (o3 << 24)) ^ kBias;
+    }
+    *reinterpret_cast<uint32_t*>(tmp + r * OW + c0) = packed;
+  }
+}
+
+// Vertical pass for unit u (row u / (OW/4), columns 4 (u % (OW/4)) .. +3)
+// over `tmp` as hpass_dot leaves it; returns the 4 output samples.  The
+// 6 x 4 support block is transposed with byte permutes so that each
+// column's samples sit in one dword.
+template <int OW>
+__device__ __forceinline__ uint32_t vpass_dot(const uint8_t* tmp, const DotTaps& f, int u) {
+  constexpr int UPR = OW / 4;
+  constexpr int RD = OW / 4;  // dword pitch
+  const int r = u / UPR, c0 = 4 * (u % UPR);
+  const uint32_t* t = reinterpret_cast<const uint32_t*>(tmp + r * OW + c0);
+  if (f.ident) return t[2 * RD] ^ kBias;
+  const uint32_t r0 = t[0], r1 = t[RD], r2 = t[2 * RD], r3 = t[3 * RD], r4 = t[4 * RD], r5 = t[5 * RD];
+  // v_perm_b32(s0, s1, sel): selector byte 0..3 picks s1's bytes, 4..7 s0's
+  const uint32_t t0 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);  // r0b0 r1b0 r0b1 r1b1
+  const uint32_t t1 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);  // r0b2 r1b2 r0b3 r1b3
+  const uint32_t t2 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
+  const uint32_t t3 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
+  const uint32_t b01 = __builtin_amdgcn_perm(r5, r4, 0x05010400u);  // r4b0 r5b0 r4b1 r5b1
+  const uint32_t b23 = __builtin_amdgcn_perm(r5, r4, 0x07030602u);
+  const uint32_t o0 = tap6(__builtin_amdgcn_perm(t2, t0, 0x05040100u), f.lo, b01, f.hi);
+  const uint32_t o1 = tap6(__builtin_amdgcn_perm(t2, t0, 0x07060302u), f.lo, b01, f.hi << 16);
+  const uint32_t o2 = tap6(__builtin_amdgcn_perm(t3, t1, 0x05040100u), f.lo, b23, f.hi);
+  const uint32_t o3 = tap6(__builtin_amdgcn_perm(t3, t1, 0x07060302u), f.lo, b23, f.hi << 16);
+  return o0 | (o1 << 8) | (o2 << 16) | (o3 << 24);
+}
+
+// Unclamped gather of ROWS x DW dwords from the dword-aligned column xa of
+// rows y0 .. y0+ROWS-1 (the caller checked the rectangle lies inside the
+// clamp bounds and the plane is 4-byte aligned), stored to LDS at `pitch`.
+template <int ROWS, int DW>
+struct GatherDw {
+  static constexpr int kTotal = ROWS * DW;
+  static constexpr int kIters = (kTotal + 63) / 64;
+  uint32_t v[kIters];
+
+  __device__ __forceinline__ void load(const uint8_t* plane, uint32_t stride, int xa, int y0, int lane) {
+#pragma unroll
+    for (int it = 0; it < kIters; it++) {
+      const int s = min(lane + 64 * it, kTotal - 1);
+      const int r = s / DW, q = s - r * DW;
+      v[it] = *reinterpret_cast<const uint32_t*>(plane + (size_t)(y0 + r) * stride + xa + 4 * q);
+    }
+  }
+
+  __device__ __forceinline__ void store(uint8_t* dst, int pitch, int lane) const {
+#pragma unroll
+    for (int it = 0; it < kIters; it++) {
+      const int s = lane + 64 * it;
+      if (s < kTotal) {
+        const int r = s / DW, q = s - r * DW;
+        *reinterpret_cast<uint32_t*>(dst + r * pitch + 4 * q) = v[it];
+      }
+    }
+  }
+};
+
+// Stage the ROWS x COLS halo at (x0, y0) of a plane clamped to (w, h) into
+// LDS (pitch `pitch`, a multiple of 4 with room for COLS + 3 bytes); returns
+// the byte shift of the halo within each LDS row.  Interior halos of
+// 4-byte aligned planes are read as whole dwords.
+template <int ROWS, int COLS>
+__device__ __forceinline__ uint32_t stage_halo(const uint8_t* plane, uint32_t stride, int w, int h, int x0, int y0,
+                                               bool aligned4, uint8_t* dst, int pitch, int lane) {
+  constexpr int DW = (COLS + 3 + 3) / 4;  // dwords covering COLS bytes at any shift
+  const int xa = x0 & ~3;
+  if (aligned4 && xa >= 0 && xa + 4 * DW <= w && y0 >= 0 && y0 + ROWS <= h) {
+    GatherDw<ROWS, DW> g;
+    g.load(plane, stride, xa, y0, lane);
+    g.store(dst, pitch, lane);
+    return (uint32_t)(x0 - xa);
+  }
+  Gather<ROWS, COLS> g;
+  g.load(plane, stride, w, h, x0, y0, lane);
+  g.store(dst, pitch, lane);
+  return 0;
+}
+
 // Two-pass filter of an OW x OW block from its (OW+5)^2 halo held in LDS
 // (pitch hp); the horizontal result goes to `tmp` (pitch OW).  Returns, for
 // the calling lane, the 4 output samples of unit `u` packed little-endian:
@@ -164,6 +300,7 @@ __device__ __forceinline__ int avg4(int c) {
 // 13 x 16, two chroma temps 13 x 8.
 constexpr int kMcLdsPerWave = 21 * 24 + 21 * 16 + 2 * 13 * 16 + 2 * 13 * 8;
 
+template <bool DOT>
 __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * kMcLdsPerWave];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -183,18 +320,45 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
   }
   // A whole-pixel vector is a plain copy whatever row 0 of the caller's
   // table holds (src/motion_comp.rs:2022-2028, 2044-2067).
+  const bool ycopy = (yc & 7) == 0 && (yr & 7) == 0, ccopy = (uc & 7) == 0 && (ur & 7) == 0;
+  const int lx0 = mb_col * 16 + (yc >> 3) - 2, ly0 = mb_row * 16 + (yr >> 3) - 2;
+  const int cx0 = mb_col * 8 + (uc >> 3) - 2, cy0 = mb_row * 8 + (ur >> 3) - 2;
+  uint8_t* out = a.out + (size_t)i * 384;
+  if (DOT) {
+    const DotTaps fh = dot_taps(a.filters[yc & 7], ycopy), fv = dot_taps(a.filters[yr & 7], ycopy);
+    const DotTaps cfh = dot_taps(a.filters[uc & 7], ccopy), cfv = dot_taps(a.filters[ur & 7], ccopy);
+    const bool al = a.aligned4 != 0;
+    const uint32_t lsh = stage_halo<21, 21>(a.ref.y, a.ref.y_stride, a.ref.lw, a.ref.lh, lx0, ly0, al, lh, 24, lane);
+    const uint32_t ush =
+        stage_halo<13, 13>(a.ref.u, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, al, chalo, 16, lane);
+    const uint32_t vsh =
+        stage_halo<13, 13>(a.ref.v, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, al, chalo + 13 * 16, 16, lane);
+    wave_lds_sync();
+    hpass_dot<16>(lh, 24, lsh, lt, fh, lane);
+    if (lane < 26) {
+      hpass_dot<8>(chalo, 16, ush, ctmp, cfh, lane);  // lanes 0..25 -> U
+    } else if (lane < 52) {
+      hpass_dot<8>(chalo + 13 * 16, 16, vsh, ctmp + 13 * 8, cfh, lane - 26);  // V
+    }
+    wave_lds_sync();
+    *reinterpret_cast<uint32_t*>(out + 4 * lane) = vpass_dot<16>(lt, fv, lane);
+    if (lane < 16) {
+      *reinterpret_cast<uint32_t*>(out + 256 + 4 * lane) = vpass_dot<8>(ctmp, cfv, lane);
+    } else if (lane < 32) {
+      *reinterpret_cast<uint32_t*>(out + 320 + 4 * (lane - 16)) = vpass_dot<8>(ctmp + 13 * 8, cfv, lane - 16);
+    }
+    return;
+  }
   Taps fh = load_taps(a.filters[yc & 7]);
   Taps fv = load_taps(a.filters[yr & 7]);
   Taps cfh = load_taps(a.filters[uc & 7]);
   Taps cfv = load_taps(a.filters[ur & 7]);
-  if ((yc & 7) == 0 && (yr & 7) == 0) fh.ident = fv.ident = true;
-  if ((uc & 7) == 0 && (ur & 7) == 0) cfh.ident = cfv.ident = true;
+  if (ycopy) fh.ident = fv.ident = true;
+  if (ccopy) cfh.ident = cfv.ident = true;
 
   Gather<21, 21> gy;
   Gather<13, 13> gu, gv;
-  gy.load(a.ref.y, a.ref.y_stride, a.ref.lw, a.ref.lh, mb_col * 16 + (yc >> 3) - 2,
-          mb_row * 16 + (yr >> 3) - 2, lane);
-  const int cx0 = mb_col * 8 + (uc >> 3) - 2, cy0 = mb_row * 8 + (ur >> 3) - 2;
+  gy.load(a.ref.y, a.ref.y_stride, a.ref.lw, a.ref.lh, lx0, ly0, lane);
   gu.load(a.ref.u, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, lane);
   gv.load(a.ref.v, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, lane);
   gy.store(lh, 24, lane);
@@ -208,7 +372,6 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
     hpass<8>(chalo + 13 * 16, 16, ctmp + 13 * 8, cfh, lane - 26);  // V
   }
   wave_lds_sync();
-  uint8_t* out = a.out + (size_t)i * 384;
   *reinterpret_cast<uint32_t*>(out + 4 * lane) = vpass_unit<16>(lt, fv, lane);
   if (lane < 16) {
     *reinterpret_cast<uint32_t*>(out + 256 + 4 * lane) = vpass_unit<8>(ctmp, cfv, lane);
@@ -218,15 +381,17 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
   }
 }
 
-// 16x16 luma prediction for eighth-pel MV (r8, c8) whose halo lies inside
-// `win` (pitch 24) at offset (ox, oy); returns this lane's 4 samples.
-__device__ __forceinline__ uint32_t luma_pred_from_window(const uint8_t* win, int ox, int oy,
-                                                          int r8, int c8, uint8_t* lt, int lane) {
-  const Taps fh = load_taps(kSixtap[c8 & 7]);
-  const Taps fv = load_taps(kSixtap[r8 & 7]);
-  hpass<16>(win + oy * 24 + ox, 24, lt, fh, lane);
+// 16x16 luma prediction (six-tap) for eighth-pel MV (r8, c8) whose halo
+// lies inside `win` (pitch 24, at most 3 columns of slack) at offset
+// (ox, oy) plus a per-row byte shift `sh`; returns this lane's 4 samples.
+__device__ __forceinline__ uint32_t luma_pred_from_window(const uint8_t* win, int ox, int oy, uint32_t sh, int r8,
+                                                          int c8, uint8_t* lt, int lane) {
+  const bool copy = (r8 & 7) == 0 && (c8 & 7) == 0;
+  const DotTaps fh = dot_taps(kSixtap[c8 & 7], copy), fv = dot_taps(kSixtap[r8 & 7], copy);
+  const uint32_t x = (uint32_t)ox + sh;
+  hpass_dot<16>(win + oy * 24 + (x & ~3u), 24, x & 3u, lt, fh, lane);
   wave_lds_sync();
-  const uint32_t p = vpass_unit<16>(lt, fv, lane);
+  const uint32_t p = vpass_dot<16>(lt, fv, lane);
   wave_lds_sync();
   return p;
 }
@@ -240,13 +405,11 @@ __global__ void __launch_bounds__(256) mb_luma_sad_at_mv_kernel(const SadArgs a)
   uint8_t* lt = lh + 21 * 24;
   const int mb_col = (int)a.mb_xy[2 * i], mb_row = (int)a.mb_xy[2 * i + 1];
   const int16_t r8 = dbl16(a.mv[2 * i]), c8 = dbl16(a.mv[2 * i + 1]);
-  Gather<21, 21> gy;
-  gy.load(a.plane, a.stride, a.w, a.h, mb_col * 16 + (c8 >> 3) - 2, mb_row * 16 + (r8 >> 3) - 2,
-          lane);
   const uint32_t s = *reinterpret_cast<const uint32_t*>(a.src + (size_t)i * 256 + 4 * lane);
-  gy.store(lh, 24, lane);
+  const uint32_t sh = stage_halo<21, 21>(a.plane, a.stride, a.w, a.h, mb_col * 16 + (c8 >> 3) - 2,
+                                         mb_row * 16 + (r8 >> 3) - 2, a.aligned4 != 0, lh, 24, lane);
   wave_lds_sync();
-  const uint32_t p = luma_pred_from_window(lh, 0, 0, r8, c8, lt, lane);
+  const uint32_t p = luma_pred_from_window(lh, 0, 0, sh, r8, c8, lt, lane);
   const uint32_t tot = wave_sum(sad4(s, p, 0));
   if (lane == 0) a.sad[i] = tot;
 }
@@ -292,7 +455,7 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
   for (int k = 0; k < 9; k++) {
     const int16_t r8 = dbl16((int16_t)cr[k]), c8 = dbl16((int16_t)ccl[k]);
     if (k > 0 && cr[k] == best_r && ccl[k] == best_c) continue;  // collapsed onto the best
-    const uint32_t p = luma_pred_from_window(win, (c8 >> 3) - minc, (r8 >> 3) - minr, r8, c8, lt, lane);
+    const uint32_t p = luma_pred_from_window(win, (c8 >> 3) - minc, (r8 >> 3) - minr, 0, r8, c8, lt, lane);
     const uint32_t tot = wave_sum(sad4(s, p, 0));
     if (k == 0 || tot < best_sad) {
       best_sad = tot;
@@ -306,14 +469,41 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
   }
 }
 
-hipError_t launch_predict_inter_mb(const McArgs& a, hipStream_t stream) {
-  if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(predict_inter_mb_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+// Every row either the identity {0, 0, 128, 0, 0, 0} or taps in [-128, 127]
+// summing to 128 (the dot-product form's precondition).
+static bool dot_table_ok(const int32_t f[8][6]) {
+  for (int r = 0; r < 8; r++) {
+    const int32_t* t = f[r];
+    if (t[0] == 0 && t[1] == 0 && t[2] == 128 && t[3] == 0 && t[4] == 0 && t[5] == 0) continue;
+    int64_t sum = 0;
+    for (int k = 0; k < 6; k++) {
+      if (t[k] < -128 || t[k] > 127) return false;
+      sum += t[k];
+    }
+    if (sum != 128) return false;
+  }
+  return true;
+}
+
+static bool aligned4(const void* p, uint32_t stride) { return (((uintptr_t)p | stride) & 3) == 0; }
+
+hipError_t launch_predict_inter_mb(const McArgs& args, hipStream_t stream) {
+  if (args.n <= 0) return hipSuccess;
+  McArgs a = args;
+  a.aligned4 = aligned4(a.ref.y, a.ref.y_stride) && aligned4(a.ref.u, a.ref.uv_stride) &&
+               aligned4(a.ref.v, a.ref.uv_stride);
+  a.dot = dot_table_ok(a.filters);
+  if (a.dot)
+    hipLaunchKernelGGL(predict_inter_mb_kernel<true>, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL(predict_inter_mb_kernel<false>, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_mb_luma_sad_at_mv(const SadArgs& a, hipStream_t stream) {
-  if (a.n <= 0) return hipSuccess;
+hipError_t launch_mb_luma_sad_at_mv(const SadArgs& args, hipStream_t stream) {
+  if (args.n <= 0) return hipSuccess;
+  SadArgs a = args;
+  a.aligned4 = aligned4(a.plane, a.stride);
   hipLaunchKernelGGL(mb_luma_sad_at_mv_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   return hipGetLastError();
 }

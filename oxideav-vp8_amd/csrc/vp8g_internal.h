@@ -112,6 +112,8 @@ struct McArgs {
   int n;
   int full_pixel;
   int32_t filters[8][6];
+  int aligned4;            // set by the launcher: planes and strides 4-byte aligned
+  int dot;                 // set by the launcher: the table suits the dot-product form
 };
 
 struct SadArgs {
@@ -123,6 +125,7 @@ struct SadArgs {
   const int16_t* mv;       // n x {row, col}
   uint32_t* sad;           // n
   int n;
+  int aligned4;            // set by the launcher: plane and stride 4-byte aligned
 };
 
 struct HalfPlaneArgs {
