@@ -413,50 +413,92 @@ def main() -> int:
     lib = ctx.lib
     torch.cuda.synchronize()
 
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-
-    def step(marks=None):
-        if marks:
-            marks[0].record(stream)
+    def step():
         st = lib.vp8g_build_arnr_altref_dev(ctx.h, descs, n, group.center, 3, oy.data_ptr(),
                                             ou.data_ptr(), ov.data_ptr(), None, None)
-        if marks:
-            marks[1].record(stream)
         st |= lib.vp8g_predict_inter_mb_batch_dev(ctx.h, rp, xy.data_ptr(), mvs.data_ptr(), n_mb, 0,
                                                   filt.ctypes.data, pred.data_ptr())
-        if marks:
-            marks[2].record(stream)
         st |= lib.vp8g_mb_luma_sad_at_mv_batch_dev(ctx.h, lr, xy.data_ptr(), src.data_ptr(),
                                                    mvs.data_ptr(), n_mb, sad.data_ptr())
-        if marks:
-            marks[3].record(stream)
         if st:
             raise RuntimeError(f"vp8g launch failed: {st}")
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    ref_out = oy.clone()
+    ref_out, ref_pred, ref_sad = oy.clone(), pred.clone(), sad.clone()
     print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
 
+    # The step: three C-ABI calls from Python on the library's stream, host
+    # launch cost included (no events between the kernels).
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
     start.record(stream)
-    for k in range(args.steps):
-        step(ev[k])
+    for _ in range(args.steps):
+        step()
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
-    ms_total = start.elapsed_time(end)
-    ms_step = ms_total / args.steps
-    arnr_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
-    mc_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
-    sad_ms = float(np.mean([e[2].elapsed_time(e[3]) for e in ev]))
+    ms_step = start.elapsed_time(end) / args.steps
+    if not (torch.equal(ref_out, oy) and torch.equal(ref_pred, pred) and torch.equal(ref_sad, sad)):
+        raise RuntimeError("step output changed between steps")
+
+    def capture(fn, times=1):
+        """fn's launches, `times` over, as one HIP graph captured on a stream
+        the library is pointed at."""
+        g = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream(device=dev)
+        torch.cuda.synchronize()
+        ctx.set_stream(cap.cuda_stream)
+        try:
+            with torch.cuda.graph(g, stream=cap):
+                for _ in range(times):
+                    fn()
+        finally:
+            ctx.set_stream(None)
+        torch.cuda.synchronize()
+        return g, cap
+
+    def replay_ms(g, cap, reps):
+        with torch.cuda.stream(cap):
+            g.replay()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(cap)
+            for _ in range(reps):
+                g.replay()
+            b.record(cap)
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    # The same step as one HIP graph replay (reported; not the headline).
+    g_step, cap = capture(step)
+    graph_ms_step = replay_ms(g_step, cap, args.steps)
+    if not (torch.equal(ref_out, oy) and torch.equal(ref_pred, pred) and torch.equal(ref_sad, sad)):
+        raise RuntimeError("graph replay output differs from the eager step")
+
+    # Per-kernel durations for the roofline: each kernel alone, 10 launches
+    # per graph replay, so the host never paces the GPU.
+    def only(i):
+        def fn():
+            if i == 0:
+                st = lib.vp8g_build_arnr_altref_dev(ctx.h, descs, n, group.center, 3, oy.data_ptr(),
+                                                    ou.data_ptr(), ov.data_ptr(), None, None)
+            elif i == 1:
+                st = lib.vp8g_predict_inter_mb_batch_dev(ctx.h, rp, xy.data_ptr(), mvs.data_ptr(), n_mb, 0,
+                                                         filt.ctypes.data, pred.data_ptr())
+            else:
+                st = lib.vp8g_mb_luma_sad_at_mv_batch_dev(ctx.h, lr, xy.data_ptr(), src.data_ptr(),
+                                                          mvs.data_ptr(), n_mb, sad.data_ptr())
+            if st:
+                raise RuntimeError(f"vp8g launch failed: {st}")
+        return fn
+    kreps = max(2, args.steps // 10)
+    arnr_ms, mc_ms, sad_ms = (replay_ms(*capture(only(i), 10), kreps) / 10 for i in range(3))
     if not torch.equal(ref_out, oy):
         raise RuntimeError("altref output changed between steps")
-    print(f"[bench] timed {args.steps} steps: {ms_step:.4f} ms/step (wall {wall:.3f}s)",
-          file=sys.stderr, flush=True)
+    print(f"[bench] timed {args.steps} steps: {ms_step:.4f} ms/step (one-graph replay {graph_ms_step:.4f}; "
+          f"wall {wall:.3f}s)", file=sys.stderr, flush=True)
 
     anchors = ((w + 15) // 16) * ((h + 15) // 16)
     bytes_alg, ops_alg = arnr_algorithmic(w, h, n)
@@ -486,7 +528,10 @@ def main() -> int:
             "anchor_mbs_per_step": anchors,
             "group_frames": n,
             "parallelism": "single GPU",
+            "launch": "three C-ABI calls per step from Python on the library stream",
         },
+        "graph_ms_per_step": graph_ms_step,
+        "kernel_timing": "each kernel alone, 10 launches per HIP graph replay",
         "kernels_ms": {"arnr_fused": arnr_ms, "predict_inter_mb": mc_ms, "mb_luma_sad_at_mv": sad_ms},
         "arnr_only_anchor_mb_per_s": anchors / (arnr_ms * 1e-3),
         "roofline": {
