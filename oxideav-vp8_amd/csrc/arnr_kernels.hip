@@ -67,12 +67,6 @@ __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Probe g of a step sits at centre + s*(dx_g, dy_g) with the reference's
 // order (-1,-1),(0,-1),(1,-1),(-1,0),(1,0),(-1,1),(0,1),(1,1)
 // (src/arnr.rs:274-283); (d + 1) is packed 2 bits per probe.

@@ -90,12 +90,6 @@ struct Gather {
   }
 };
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // ---- Dot-product form of the six-tap passes --------------------------------
 // A tap row as two signed-byte dwords for v_dot4_i32_i8 (taps 0..3, taps
 // 4..5).  With every sample s given as s ^ 0x80 (= s - 128 as a signed
@@ -303,7 +297,7 @@ constexpr int kMcLdsPerWave = 21 * 24 + 21 * 16 + 2 * 13 * 16 + 2 * 13 * 8;
 template <bool DOT>
 __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * kMcLdsPerWave];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + wave;
   if (i >= a.n) return;  // whole wave exits together
   uint8_t* lh = smem + wave * kMcLdsPerWave;
@@ -398,7 +392,7 @@ __device__ __forceinline__ uint32_t luma_pred_from_window(const uint8_t* win, in
 
 __global__ void __launch_bounds__(256) mb_luma_sad_at_mv_kernel(const SadArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * (21 * 24 + 21 * 16)];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + wave;
   if (i >= a.n) return;
   uint8_t* lh = smem + wave * (21 * 24 + 21 * 16);
@@ -420,7 +414,7 @@ __global__ void __launch_bounds__(256) mb_luma_sad_at_mv_kernel(const SadArgs a)
 // on the absolute coordinate, so a sub-window equals a direct fetch).
 __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * (24 * 24 + 21 * 16)];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + wave;
   if (i >= a.n) return;
   uint8_t* win = smem + wave * (24 * 24 + 21 * 16);
@@ -520,7 +514,7 @@ constexpr int kHpLds = 22 * 24 + 22 * 17 + 17 * 16 + 17 * 17;
 
 __global__ void __launch_bounds__(256) half_plane_sads_kernel(const HalfPlaneArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * ((kHpLds + 15) & ~15)];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + wave;
   if (i >= a.n) return;
   uint8_t* win = smem + wave * ((kHpLds + 15) & ~15);

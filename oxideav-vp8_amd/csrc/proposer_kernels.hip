@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs a) {
   constexpr int IW = TW + 2 * HALO;  // staged cells per row
   __shared__ __align__(16) uint4 s_w[TAPS * COUT * CK];
   __shared__ __align__(16) uint4 s_in[ROWS * IW * CK];
-  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
 
   const uint4* wg = reinterpret_cast<const uint4*>(a.w);
   for (int i = tid; i < TAPS * COUT * CK; i += 256) {
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(256) wgrad_kernel(const WgradArgs a) {
   static_assert(TILES % 4 == 0, "tiles");
   __shared__ __align__(16) uint16_t s_dy[COUT * KC];
   __shared__ __align__(16) uint16_t s_x[TAPS * CIN * KC];
-  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
   const int segs = (a.cw + KC - 1) / KC;
   const int total = a.images * a.ch * segs;
   f32x4 acc[TPW];

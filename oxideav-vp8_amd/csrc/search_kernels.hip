@@ -234,141 +234,163 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
   }
 }
 
-// Radius 2 (the refinement around a proposal): one wave per block, 4 blocks
-// per workgroup.  Lane l < 40 takes dy = l / 8, the dx quad j = (l / 4) & 1
-// and the source rows 4h .. 4h + 3, h = l & 3; the four row quarters are
-// added with two lane swaps, then the wave reduces the keys.
+// Radius 2 (the refinement around a proposal): one wave per block position,
+// 4 per workgroup, walking every neighbour frame.  The centre block sits in
+// registers; the next frame's 20 x 24 window (at its own seed) is loaded
+// into registers while the current one is scored out of LDS.  Lane l < 40
+// takes dy = l / 8, the dx quad j = (l / 4) & 1 and the source rows
+// 4h .. 4h + 3, h = l & 3: 16 v_qsad_pk_u16_u8 give its 4 candidates' partial
+// SADs, two DPP quad swaps add the row quarters.  Candidates outside the
+// +-2 square are forced to 0xFFFF (a real SAD is at most 65280), then two
+// wave minima pick the SAD and, among the candidates holding it, the tie
+// key (|mvx| + |mvy|, mvy, mvx).
 __global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
-  constexpr int R = 2, PD = 6, WROWS = 20, GROUP = 64, SLOTS = 256 / GROUP;
-  __shared__ uint32_t s_win[SLOTS][WROWS * PD];
-  __shared__ __align__(16) uint32_t s_src[SLOTS][64];
-  const int tid = threadIdx.x, slot = tid / GROUP, l = tid & (GROUP - 1);
+  constexpr int R = 2, PD = 6, WROWS = 20, WDW = WROWS * PD;
+  __shared__ __align__(16) uint32_t s_win[4][WDW];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const int per_frame = a.brows * a.bcols;
-  const long long bi = (long long)blockIdx.x * SLOTS + slot;
-  const bool live = bi < a.total_blocks;
-  const int b = live ? (int)bi : 0;
-  const int kf = b / per_frame;
-  const int fi = kf < a.center ? kf : kf + 1;
-  const int rest = b - kf * per_frame;
+  const int rest = blockIdx.x * 4 + wave;
+  if (rest >= per_frame) return;  // whole wave; no workgroup barrier follows
   const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
   const int bx = bcol * 16, by = brow * 16;
   const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
-  const size_t tab = 3 * ((size_t)fi * per_frame + rest);
-  int sx = 0, sy = 0;
-  if (a.seeds) {
-    sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
-    sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
-  }
-  const FrameDesc F = a.frames[fi];
-  const FrameDesc C = a.frames[a.center];
-  const int x0 = bx + sx - R;
-  const int x0a = x0 >= 0 ? (x0 & ~3) : -(((-x0) + 3) & ~3);
-  const int shift0 = x0 - x0a;
-  const int y0 = by + sy - R;
-  uint32_t* win = s_win[slot];
-  const bool interior = a.aligned4 && x0a >= 0 && x0a + 4 * PD <= a.w;
-  // All staging loads of this lane are issued before any LDS store.
-  constexpr int WL = (WROWS * PD + GROUP - 1) / GROUP;  // window dwords per lane
-  constexpr int SL = 64 / GROUP;                        // source dwords per lane
-  uint32_t wv_[WL], sv_[SL];
-#pragma unroll
-  for (int k = 0; k < WL; k++) {
-    const int s = l + GROUP * k;
-    const int sc = s < WROWS * PD ? s : 0;
-    const int wy = sc / PD, wd = sc - wy * PD;
-    const uint8_t* row = F.y + (size_t)clampi(y0 + wy, 0, a.h - 1) * F.y_stride;
-    wv_[k] = interior ? *reinterpret_cast<const uint32_t*>(row + x0a + 4 * wd) : gather4_clamped(row, x0a + 4 * wd, a.w);
-  }
-#pragma unroll
-  for (int k = 0; k < SL; k++) {
-    const int s = l + GROUP * k;
-    const int r = s >> 2, q = s & 3;
+  const bool full = bw == 16 && bh == 16;
+  const int nnb = a.n - 1;
+  uint32_t* win = s_win[wave];
+  const int dy = l >> 3, j = (l >> 2) & 1, h = l & 3;
+  // the centre block (zero outside it) staged through LDS once, one dword
+  // per lane, then the lane's source rows 4h .. 4h + 3 kept in registers
+  {
+    const FrameDesc C = a.frames[a.center];
+    const int r = l >> 2, q = l & 3;
     uint32_t v = 0;
     if (r < bh) {
       const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
-      if (bw == 16 && a.aligned4) {
+      if (full && a.aligned4) {
         v = *reinterpret_cast<const uint32_t*>(row + 4 * q);
       } else {
         for (int e = 0; e < 4; e++)
           if (4 * q + e < bw) v |= (uint32_t)row[4 * q + e] << (8 * e);
       }
     }
-    sv_[k] = v;
+    win[l] = v;
   }
+  wave_lds_sync();
+  uint4 src[4];
 #pragma unroll
-  for (int k = 0; k < WL; k++)
-    if (l + GROUP * k < WROWS * PD) win[l + GROUP * k] = wv_[k];
-#pragma unroll
-  for (int k = 0; k < SL; k++) s_src[slot][l + GROUP * k] = sv_[k];
-  __syncthreads();
-  const int dy = l >> 3, j = (l >> 2) & 1, h = l & 3;
-  uint64_t acc = 0;
-  if (l < 40) {
-    if (bw == 16 && bh == 16) {
+  for (int rr = 0; rr < 4; rr++) src[rr] = reinterpret_cast<const uint4*>(win)[4 * h + rr];
+  wave_lds_sync();  // the source is read before the first window overwrites it
+  // lane k < nnb holds neighbour k's seed
+  int my_sx = 0, my_sy = 0;
+  if (a.seeds && l < nnb) {
+    const int fi = l < a.center ? l : l + 1;
+    const size_t tab = 3 * ((size_t)fi * per_frame + rest);
+    my_sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
+    my_sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
+  }
+  // window dwords l and l + 64 (< 120): row s / 6, dword s % 6
+  const int wy0 = l / PD, wd0 = l - PD * wy0;
+  const int s1 = min(l + 64, WDW - 1), wy1 = s1 / PD, wd1 = s1 - PD * wy1;
+  auto floor4 = [](int x) { return x >= 0 ? (x & ~3) : -(((-x) + 3) & ~3); };
+  auto load_window = [&](int k, uint32_t& v0, uint32_t& v1) {
+    const FrameDesc F = a.frames[k < a.center ? k : k + 1];
+    const int sx = __builtin_amdgcn_readlane(my_sx, k), sy = __builtin_amdgcn_readlane(my_sy, k);
+    const int x0a = floor4(bx + sx - R), y0 = by + sy - R;
+    if (a.aligned4 && x0a >= 0 && x0a + 4 * PD <= a.w && y0 >= 0 && y0 + WROWS <= a.h) {
+      const uint8_t* base = F.y + (size_t)y0 * F.y_stride + x0a;
+      v0 = *reinterpret_cast<const uint32_t*>(base + (uint32_t)wy0 * F.y_stride + 4 * wd0);
+      v1 = *reinterpret_cast<const uint32_t*>(base + (uint32_t)wy1 * F.y_stride + 4 * wd1);
+    } else {
+      v0 = gather4_clamped(F.y + (size_t)clampi(y0 + wy0, 0, a.h - 1) * F.y_stride, x0a + 4 * wd0, a.w);
+      v1 = gather4_clamped(F.y + (size_t)clampi(y0 + wy1, 0, a.h - 1) * F.y_stride, x0a + 4 * wd1, a.w);
+    }
+  };
+  uint32_t v0 = 0, v1 = 0;
+  if (nnb > 0) load_window(0, v0, v1);
+  for (int k = 0; k < nnb; k++) {
+    wave_lds_sync();  // the previous frame's window reads are done
+    win[l] = v0;
+    if (l + 64 < WDW) win[l + 64] = v1;
+    wave_lds_sync();
+    const int sx = __builtin_amdgcn_readlane(my_sx, k), sy = __builtin_amdgcn_readlane(my_sy, k);
+    const int x0 = bx + sx - R;
+    const int shift0 = x0 - floor4(x0);
+    if (k + 1 < nnb) load_window(k + 1, v0, v1);  // in flight during the scoring
+    uint64_t acc = 0;
+    if (full) {
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int r = 4 * h + rr;
-        const uint32_t* wr = win + (dy + r) * PD + j;
+        const uint32_t* wr = win + (dy + 4 * h + rr) * PD + j;
         const uint32_t w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4];
-        const uint4 s4 = *reinterpret_cast<const uint4*>(&s_src[slot][4 * r]);
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w0 | ((uint64_t)w1 << 32), s4.x, acc);
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w1 | ((uint64_t)w2 << 32), s4.y, acc);
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w2 | ((uint64_t)w3 << 32), s4.z, acc);
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w3 | ((uint64_t)w4 << 32), s4.w, acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w0 | ((uint64_t)w1 << 32), src[rr].x, acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w1 | ((uint64_t)w2 << 32), src[rr].y, acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w2 | ((uint64_t)w3 << 32), src[rr].z, acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w3 | ((uint64_t)w4 << 32), src[rr].w, acc);
       }
     } else {
       const int bwq = bw >> 2, rem = bw & 3;
       const uint32_t remmask = rem ? (0xFFFFFFFFu >> (32 - 8 * rem)) : 0u;
+#pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int r = 4 * h + rr;
-        if (r >= bh) break;
-        const uint32_t* wr = win + (dy + r) * PD + j;
-        const uint32_t wv[5] = {wr[0], wr[1], wr[2], wr[3], wr[4]};
-        const uint4 s4 = *reinterpret_cast<const uint4*>(&s_src[slot][4 * r]);
-        const uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
+        if (4 * h + rr < bh) {
+          const uint32_t* wr = win + (dy + 4 * h + rr) * PD + j;
+          const uint32_t wv[5] = {wr[0], wr[1], wr[2], wr[3], wr[4]};
+          const uint32_t sv[4] = {src[rr].x, src[rr].y, src[rr].z, src[rr].w};
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const uint64_t pair = (uint64_t)wv[q] | ((uint64_t)wv[q + 1] << 32);
-          if (q < bwq) {
-            acc = __builtin_amdgcn_qsad_pk_u16_u8(pair, sv[q], acc);
-          } else if (q == bwq && rem) {
-            uint64_t add = 0;
+          for (int q = 0; q < 4; q++) {
+            const uint64_t pair = (uint64_t)wv[q] | ((uint64_t)wv[q + 1] << 32);
+            if (q < bwq) {
+              acc = __builtin_amdgcn_qsad_pk_u16_u8(pair, sv[q], acc);
+            } else if (q == bwq && rem) {
+              uint64_t add = 0;
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-              add |= (uint64_t)sad4(sv[q], funnel(wv[q], wv[q + 1], i) & remmask, 0) << (16 * i);
-            acc += add;
+              for (int i = 0; i < 4; i++)
+                add |= (uint64_t)sad4(sv[q], funnel(wv[q], wv[q + 1], i) & remmask, 0) << (16 * i);
+              acc += add;
+            }
           }
         }
       }
     }
-  }
-  acc += shfl_xor_u64(acc, 1);  // the other row quarters of the same (dy, j)
-  acc += shfl_xor_u64(acc, 2);
-  // Two 32-bit wave minima: the SAD, then among the candidates holding it
-  // the tie key (|mvx| + |mvy|, mvy, mvx); mvy and mvx are ordered through
-  // their offsets from the seed (constant over the block).
-  uint32_t lane_sad = 0xFFFFFFFFu, lane_tie = 0xFFFFFFFFu;
-  if (l < 40) {
+    // row quarters: the four lanes of a quad (no 16-bit field overflows, so
+    // the halves add independently)
+    uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
+    lo += dpp<kDppQuadXor1>(lo);
+    hi += dpp<kDppQuadXor1>(hi);
+    lo += dpp<kDppQuadXor2>(lo);
+    hi += dpp<kDppQuadXor2>(hi);
+    // candidate i of the lane: dxi = 4j + i - shift0, inside the square iff
+    // 0 <= dxi <= 4; outside ones and idle lanes read 0xFFFF
+    const uint32_t lo_bad = j == 0 ? (shift0 >= 2 ? 0xFFFFFFFFu : (shift0 == 1 ? 0x0000FFFFu : 0u))
+                                   : (shift0 >= 1 ? 0u : 0xFFFF0000u);
+    const uint32_t hi_bad = j == 0 ? (shift0 == 3 ? 0x0000FFFFu : 0u)
+                                   : (shift0 == 3 ? 0u : (shift0 == 2 ? 0xFFFF0000u : 0xFFFFFFFFu));
+    if (l < 40) {
+      lo |= lo_bad;
+      hi |= hi_bad;
+    } else {
+      lo = hi = 0xFFFFFFFFu;
+    }
+    const uint32_t lane_sad = min(min(lo & 0xFFFFu, lo >> 16), min(hi & 0xFFFFu, hi >> 16));
+    const uint32_t best_sad = wave_min_any(lane_sad);
+    const int mvy = sy - R + dy;
+    const int mvx0 = sx - R + 4 * j - shift0;  // mvx of candidate i is mvx0 + i
+    // tie key of candidate i: (|mvx| + |mvy|) << 6 | dy << 3 | dxi
+    const uint32_t tie_hi = (uint32_t)abs(mvy), tie_lo = (uint32_t)((dy << 3) + 4 * j - shift0);
+    uint32_t lane_tie = 0xFFFFFFFFu;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int dxi = 4 * j + i - shift0;
-      if (dxi >= 0 && dxi <= 2 * R) {
-        const uint32_t sad = (uint32_t)(acc >> (16 * i)) & 0xFFFFu;
-        const int mvx = sx - R + dxi, mvy = sy - R + dy;
-        const uint32_t tie = ((uint32_t)(abs(mvx) + abs(mvy)) << 6) | ((uint32_t)dy << 3) | (uint32_t)dxi;
-        if (sad < lane_sad || (sad == lane_sad && tie < lane_tie)) {
-          lane_sad = sad;
-          lane_tie = tie;
-        }
-      }
+      const uint32_t sad = ((i < 2 ? lo : hi) >> (16 * (i & 1))) & 0xFFFFu;
+      const uint32_t tie = (((uint32_t)abs(mvx0 + i) + tie_hi) << 6) | (tie_lo + (uint32_t)i);
+      if (sad == best_sad) lane_tie = min(lane_tie, tie);
     }
-  }
-  const uint32_t best_sad = wave_min_u32(lane_sad);
-  const uint32_t best_tie = wave_min_u32(lane_sad == best_sad ? lane_tie : 0xFFFFFFFFu);
-  if (live && l == 0) {
-    a.out[tab] = sx - R + (int)(best_tie & 7);
-    a.out[tab + 1] = sy - R + (int)((best_tie >> 3) & 7);
-    a.out[tab + 2] = (int32_t)best_sad;
+    const uint32_t best_tie = wave_min_any(lane_tie);
+    if (l == 0) {
+      const size_t tab = 3 * ((size_t)(k < a.center ? k : k + 1) * per_frame + rest);
+      a.out[tab] = sx - R + (int)(best_tie & 7);
+      a.out[tab + 1] = sy - R + (int)((best_tie >> 3) & 7);
+      a.out[tab + 2] = (int32_t)best_sad;
+    }
   }
 }
 
@@ -385,7 +407,7 @@ hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stre
     break;                                                                                         \
   }
     case 2:
-      hipLaunchKernelGGL(refine2_kernel, dim3((a.total_blocks + 3) / 4), dim3(256), 0, stream, a);
+      hipLaunchKernelGGL(refine2_kernel, dim3((a.brows * a.bcols + 3) / 4), dim3(256), 0, stream, a);
       break;
     VP8G_SEARCH_CASE(16)
     VP8G_SEARCH_CASE(64)

@@ -84,6 +84,17 @@ __device__ __forceinline__ uint32_t wave_sum_to_sgpr(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// Minimum over the whole wave, wave-uniform (any lane values).
+__device__ __forceinline__ uint32_t wave_min_any(uint32_t v) {
+  v = min(v, dpp<kDppQuadXor1>(v));
+  v = min(v, dpp<kDppQuadXor2>(v));
+  v = min(v, dpp<kDppRowHalfMirror>(v));
+  v = min(v, dpp<kDppRowMirror>(v));
+  v = min(v, dpp_rows<kDppRowBcast15, 0xA>(~0u, v));
+  v = min(v, dpp_rows<kDppRowBcast31, 0xC>(~0u, v));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Given a value that is uniform within each aligned group of 8 lanes: the
 // minimum over the wave, wave-uniform.
 __device__ __forceinline__ uint32_t wave_min_to_sgpr(uint32_t v) {
@@ -91,6 +102,14 @@ __device__ __forceinline__ uint32_t wave_min_to_sgpr(uint32_t v) {
   v = min(v, dpp_rows<kDppRowBcast15, 0xA>(~0u, v));  // rows 1, 3: min with previous row
   v = min(v, dpp_rows<kDppRowBcast31, 0xC>(~0u, v));  // rows 2, 3: min with rows 0..1
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// LDS writes of the wave visible to the wave's later LDS reads (and no
+// access moved across): lanes exchange data through LDS within one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
