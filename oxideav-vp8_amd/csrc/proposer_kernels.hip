@@ -40,15 +40,15 @@ __device__ __forceinline__ float bf16_val(uint16_t b) { return (float)__builtin_
 // One thread per 4 horizontally adjacent downsampled pixels of a (pair,
 // frame) plane: 4 rows of 16 full-resolution bytes (dword loads when the
 // row segment is inside the frame and aligned).
+// Grid: x over the plane's quads, y over the 2 * pairs planes (so the
+// plane, its source frame and stride are uniform per workgroup).
 __global__ void prep_downsample_kernel(const PrepArgs a) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pf = blockIdx.y;
   const int dw = 4 * a.cw, dh = 4 * a.ch;
-  const int qw = a.cw;  // quads per downsampled row (dw / 4)
-  const long long quads = (long long)qw * dh;
-  if (t >= (long long)a.pairs * 2 * quads) return;
-  const int pf = (int)(t / quads);
-  const int rem = (int)(t - (long long)pf * quads);
-  const int y = rem / qw, xq = rem - y * qw;
+  const uint32_t qw = (uint32_t)a.cw;  // quads per downsampled row (dw / 4)
+  if (t >= qw * (uint32_t)dh) return;
+  const int y = (int)(t / qw), xq = (int)(t - (uint32_t)y * qw);
   const int p = pf >> 1, f = pf & 1;
   const uint8_t* src = f == 0 ? a.anchor : a.neighbours[p];
   const uint32_t stride = f == 0 ? a.anchor_stride : a.neighbour_stride[p];
@@ -61,10 +61,7 @@ __global__ void prep_downsample_kernel(const PrepArgs a) {
     if (fast) {
       const uint32_t* r32 = reinterpret_cast<const uint32_t*>(row + x0);
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const uint32_t v = r32[k];
-        sum[k] += (v & 255) + ((v >> 8) & 255) + ((v >> 16) & 255) + (v >> 24);
-      }
+      for (int k = 0; k < 4; k++) sum[k] = sad4(r32[k], 0u, sum[k]);  // + the dword's 4 bytes
     } else {
 #pragma unroll
       for (int k = 0; k < 4; k++)
@@ -81,68 +78,93 @@ __global__ void prep_downsample_kernel(const PrepArgs a) {
 // displacement row ddy = k - 4 (the cell's 4x4 anchor pixels against 4 rows
 // of the clamped neighbour window, v_sad_u8); thread 9 writes both frames'
 // pixels and the zero channels.
-__global__ void features_kernel(const FeatArgs a) {
+// 25 cells per workgroup, 10 threads per cell (thread k < 9: the 9 cost
+// channels of row offset k - 4; thread 9: the pixel and zero channels).  The
+// cells' 128-channel rows are assembled in LDS and leave as coalesced
+// 16-byte stores (the cells of a workgroup are consecutive in the output).
+constexpr int kFeatCellsPerWg = 25;
+
+__global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
   constexpr int TPC = 2 * kSearch + 2;  // threads per cell
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int cells = a.cw * a.ch;
-  if (t >= (long long)a.images * cells * TPC) return;
-  const long long ic = t / TPC;
-  const int k = (int)(t - ic * TPC);
-  const int img = (int)(ic / cells);
-  const int cell = (int)(ic - (long long)img * cells);
-  const int cy = cell / a.cw, cx = cell - cy * a.cw;
-  const int dw = 4 * a.cw, dh = 4 * a.ch;
-  const uint8_t* A = a.planes + (size_t)img * 2 * dw * dh;
-  const uint8_t* B = A + (size_t)dw * dh;
-  const int x0 = 4 * cx, y0 = 4 * cy;
-  uint16_t* out = a.out + (size_t)ic * kFeat;
-  uint32_t arow[4];
+  __shared__ __align__(16) uint16_t s_out[kFeatCellsPerWg * kFeat];
+  // image and cell counts are bounded (<= 4096 images of <= 1024 x 1024
+  // cells), so every cell index fits 32 bits
+  const uint32_t cells = (uint32_t)(a.cw * a.ch);
+  const uint32_t total = (uint32_t)a.images * cells;
+  const uint32_t ic0 = blockIdx.x * kFeatCellsPerWg;
+  const int tid = threadIdx.x;
+  const int jc = tid / TPC, k = tid - jc * TPC;
+  const uint32_t ic = ic0 + (uint32_t)jc;
+  if (jc < kFeatCellsPerWg && ic < total) {
+    const int img = (int)(ic / cells);
+    const int cell = (int)(ic - (uint32_t)img * cells);
+    const int cy = cell / a.cw, cx = cell - cy * a.cw;
+    const int dw = 4 * a.cw, dh = 4 * a.ch;
+    const uint8_t* A = a.planes + (size_t)img * 2 * dw * dh;
+    const uint8_t* B = A + (size_t)dw * dh;
+    const int x0 = 4 * cx, y0 = 4 * cy;
+    uint16_t* out = s_out + jc * kFeat;
+    uint32_t arow[4];
 #pragma unroll
-  for (int j = 0; j < 4; j++) arow[j] = *reinterpret_cast<const uint32_t*>(A + (size_t)(y0 + j) * dw + x0);
-  if (k < 2 * kSearch + 1) {
-    // Window rows y0 + k - 4 + j, columns x0 - 4 .. x0 + 7 (3 dwords).
-    uint32_t win[4][3];
-    const bool inner = x0 >= 4 && x0 + 8 <= dw;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint8_t* row = B + (size_t)clampi(y0 + k - kSearch + j, 0, dh - 1) * dw;
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        if (inner) {
-          win[j][q] = *reinterpret_cast<const uint32_t*>(row + x0 - 4 + 4 * q);
-        } else {
-          uint32_t v = 0;
-#pragma unroll
-          for (int b = 0; b < 4; b++) v |= (uint32_t)row[clampi(x0 - 4 + 4 * q + b, 0, dw - 1)] << (8 * b);
-          win[j][q] = v;
-        }
-      }
-    }
-#pragma unroll
-    for (int ddx = 0; ddx < 2 * kSearch + 1; ddx++) {
-      uint32_t s = 0;
+    for (int j = 0; j < 4; j++) arow[j] = *reinterpret_cast<const uint32_t*>(A + (size_t)(y0 + j) * dw + x0);
+    if (k < 2 * kSearch + 1) {
+      // Window rows y0 + k - 4 + j, columns x0 - 4 .. x0 + 7 (3 dwords).
+      uint32_t win[4][3];
+      const bool inner = x0 >= 4 && x0 + 8 <= dw;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        // bytes ddx .. ddx + 3 of the 12-byte window row (alignbyte uses
-        // the shift mod 4, so the whole-dword offset 8 is taken directly)
-        const uint32_t ref = ddx < 4 ? funnel(win[j][0], win[j][1], ddx)
-                                     : (ddx < 8 ? funnel(win[j][1], win[j][2], ddx - 4) : win[j][2]);
-        s = sad4(arow[j], ref, s);
+        const uint8_t* row = B + (size_t)clampi(y0 + k - kSearch + j, 0, dh - 1) * dw;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          if (inner) {
+            win[j][q] = *reinterpret_cast<const uint32_t*>(row + x0 - 4 + 4 * q);
+          } else {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) v |= (uint32_t)row[clampi(x0 - 4 + 4 * q + b, 0, dw - 1)] << (8 * b);
+            win[j][q] = v;
+          }
+        }
       }
-      out[(2 * kSearch + 1) * k + ddx] = bf16_bits((float)s * (1.0f / 1024.0f));
+#pragma unroll
+      for (int ddx = 0; ddx < 2 * kSearch + 1; ddx++) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          // bytes ddx .. ddx + 3 of the 12-byte window row (alignbyte uses
+          // the shift mod 4, so the whole-dword offset 8 is taken directly)
+          const uint32_t ref = ddx < 4 ? funnel(win[j][0], win[j][1], ddx)
+                                       : (ddx < 8 ? funnel(win[j][1], win[j][2], ddx - 4) : win[j][2]);
+          sum = sad4(arow[j], ref, sum);
+        }
+        out[(2 * kSearch + 1) * k + ddx] = bf16_bits((float)sum * (1.0f / 1024.0f));
+      }
+    } else {
+      uint32_t brow[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) brow[j] = *reinterpret_cast<const uint32_t*>(B + (size_t)(y0 + j) * dw + x0);
+      auto px = [&](int p) -> uint32_t {  // pixel channel p: A then B, / 255
+        const uint32_t v = p < 16 ? (arow[p >> 2] >> (8 * (p & 3))) & 255 : (brow[(p - 16) >> 2] >> (8 * (p & 3))) & 255;
+        return bf16_bits((float)v * (1.0f / 255.0f));
+      };
+      // channel 81 alone (odd), then channel pairs 82/83 .. 126/127 as dwords
+      out[kCostChannels] = (uint16_t)px(0);
+      uint32_t* o32 = reinterpret_cast<uint32_t*>(out + kCostChannels + 1);
+#pragma unroll
+      for (int c = 0; c < (kFeat - kCostChannels - 1) / 2; c++) {
+        const int p0 = 1 + 2 * c, p1 = 2 + 2 * c;  // pixel indices of the pair
+        const uint32_t lo = p0 < 32 ? px(p0) : 0u, hi = p1 < 32 ? px(p1) : 0u;
+        o32[c] = lo | (hi << 16);
+      }
     }
-    return;
   }
-  uint32_t brow[4];
-#pragma unroll
-  for (int j = 0; j < 4; j++) brow[j] = *reinterpret_cast<const uint32_t*>(B + (size_t)(y0 + j) * dw + x0);
-#pragma unroll
-  for (int p = 0; p < 32; p++) {
-    const uint32_t px = p < 16 ? (arow[p >> 2] >> (8 * (p & 3))) & 255 : (brow[(p - 16) >> 2] >> (8 * (p & 3))) & 255;
-    out[kCostChannels + p] = bf16_bits((float)px * (1.0f / 255.0f));
-  }
-#pragma unroll
-  for (int c = kCostChannels + 32; c < kFeat; c++) out[c] = 0;
+  __syncthreads();
+  // the workgroup's cells: consecutive rows of kFeat bf16 = 16 uint4 each
+  const uint32_t left = total - ic0;
+  const int ncells = left < (uint32_t)kFeatCellsPerWg ? (int)left : kFeatCellsPerWg;
+  uint4* dst = reinterpret_cast<uint4*>(a.out + (size_t)ic0 * kFeat);
+  const uint4* srcv = reinterpret_cast<const uint4*>(s_out);
+  for (int i = tid; i < ncells * (kFeat / 8); i += 256) dst[i] = srcv[i];
 }
 
 // ---------------------------------------------------------------- conv3x3
@@ -170,26 +192,52 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs a) {
   __shared__ __align__(16) uint4 s_in[ROWS * IW * CK];
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
 
+  constexpr int WITERS = (TAPS * COUT * CK + 255) / 256;  // weight chunks per thread
+  constexpr int IITERS = (ROWS * IW * CK + 255) / 256;    // input chunks per thread
   const uint4* wg = reinterpret_cast<const uint4*>(a.w);
-  for (int i = tid; i < TAPS * COUT * CK; i += 256) {
-    const int row = i / CK, ck = i - row * CK;  // row = tap * COUT + co
-    s_w[row * CK + swz<CIN>(row, ck)] = wg[i];
-  }
   const int segs = (a.cw + TW - 1) / TW;
   const int total = a.images * a.ch * segs;
   const uint4* in = reinterpret_cast<const uint4*>(a.in);
+  bool first = true;
   for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
     const int img = tile / (a.ch * segs);
     const int rem = tile - img * a.ch * segs;
     const int cy = rem / segs, cx0 = (rem - cy * segs) * TW;
-    __syncthreads();  // previous tile's readers are done (and weights staged)
-    for (int i = tid; i < ROWS * IW * CK; i += 256) {
+    __syncthreads();  // previous tile's readers are done
+    // Every staging load of the thread (the weights on the first tile, then
+    // the input rows) is issued before the first LDS store.
+    uint4 wv[WITERS], iv[IITERS];
+    if (first) {
+#pragma unroll
+      for (int k = 0; k < WITERS; k++) wv[k] = wg[min(tid + 256 * k, TAPS * COUT * CK - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < IITERS; k++) {
+      const int i = min(tid + 256 * k, ROWS * IW * CK - 1);
       const int row = i / CK, ck = i - row * CK;  // row = r * IW + c
       const int r = row / IW, c = row - r * IW;
       const int y = cy + r - HALO, x = cx0 + c - HALO;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (y >= 0 && y < a.ch && x >= 0 && x < a.cw) v = in[(((size_t)img * a.ch + y) * a.cw + x) * CK + ck];
-      s_in[row * CK + swz<CIN>(row, ck)] = v;
+      iv[k] = make_uint4(0, 0, 0, 0);
+      if (y >= 0 && y < a.ch && x >= 0 && x < a.cw) iv[k] = in[(((size_t)img * a.ch + y) * a.cw + x) * CK + ck];
+    }
+    if (first) {
+#pragma unroll
+      for (int k = 0; k < WITERS; k++) {
+        const int i = tid + 256 * k;
+        if (i < TAPS * COUT * CK) {
+          const int row = i / CK, ck = i - row * CK;  // row = tap * COUT + co
+          s_w[row * CK + swz<CIN>(row, ck)] = wv[k];
+        }
+      }
+      first = false;
+    }
+#pragma unroll
+    for (int k = 0; k < IITERS; k++) {
+      const int i = tid + 256 * k;
+      if (i < ROWS * IW * CK) {
+        const int row = i / CK, ck = i - row * CK;
+        s_in[row * CK + swz<CIN>(row, ck)] = iv[k];
+      }
     }
     __syncthreads();
     f32x4 acc[COUT / 16];
@@ -479,23 +527,25 @@ static int persistent_grid(int total, int per_cu, int cus) {
 }
 
 hipError_t launch_prep_downsample(const PrepArgs& a, hipStream_t s) {
-  const long long n = (long long)a.pairs * 2 * a.cw * (4 * a.ch);
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_downsample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  const long long quads = (long long)a.cw * (4 * a.ch);  // per plane
+  if (quads == 0 || a.pairs == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_downsample_kernel, dim3((unsigned)((quads + 255) / 256), (unsigned)(2 * a.pairs)), dim3(256),
+                     0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_features(const FeatArgs& a, hipStream_t s) {
-  const long long n = (long long)a.images * a.cw * a.ch * (2 * kSearch + 2);
+  const long long n = (long long)a.images * a.cw * a.ch;  // cells
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(features_kernel, dim3((unsigned)((n + kFeatCellsPerWg - 1) / kFeatCellsPerWg)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_conv(const ConvArgs& a, int taps, int cin, int cout, int epi, int cus, hipStream_t s) {
   const int total = a.images * a.ch * ((a.cw + 63) / 64);
   if (total == 0) return hipSuccess;
-  const dim3 grid(persistent_grid(total, 2, cus)), block(256);
+  // one tile per workgroup up to 8 workgroups per CU, then persistent
+  const dim3 grid(persistent_grid(total, 8, cus)), block(256);
 #define VP8G_CONV(T, CI, CO, E)                                                              \
   if (taps == T && cin == CI && cout == CO && epi == E) {                                    \
     hipLaunchKernelGGL((conv_kernel<T, CI, CO, E>), grid, block, 0, s, a);                   \
