@@ -74,75 +74,86 @@ __global__ void prep_downsample_kernel(const PrepArgs a) {
   *reinterpret_cast<uint32_t*>(a.out + (size_t)pf * dw * dh + (size_t)y * dw + 4 * xq) = packed;
 }
 
-// Ten threads per (image, cell): thread k < 9 computes the 9 SADs of
+// Features of one segment of 32 cells of one cell row of one image per
+// workgroup, 10 threads per cell: thread k < 9 computes the 9 SADs of
 // displacement row ddy = k - 4 (the cell's 4x4 anchor pixels against 4 rows
 // of the clamped neighbour window, v_sad_u8); thread 9 writes both frames'
-// pixels and the zero channels.
-// 25 cells per workgroup, 10 threads per cell (thread k < 9: the 9 cost
-// channels of row offset k - 4; thread 9: the pixel and zero channels).  The
-// cells' 128-channel rows are assembled in LDS and leave as coalesced
-// 16-byte stores (the cells of a workgroup are consecutive in the output).
-constexpr int kFeatCellsPerWg = 25;
+// pixels and the zero channels.  The segment's 4 anchor rows and its 12
+// clamped neighbour rows (columns x0 - 4 .. x0 + 131) are staged in LDS with
+// coalesced loads; the cells' 128-channel rows are assembled in LDS and
+// leave as coalesced 16-byte stores.
+constexpr int kFeatSeg = 32;
 
-__global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
+__global__ void __launch_bounds__(320) features_kernel(const FeatArgs a) {
+  constexpr int AD = kFeatSeg;          // anchor dwords per row
+  constexpr int BD = kFeatSeg + 2;      // neighbour dwords per row
+  constexpr int BR = 4 + 2 * kSearch;   // neighbour rows
+  constexpr int NA = 4 * AD, NB = BR * BD;
   constexpr int TPC = 2 * kSearch + 2;  // threads per cell
-  __shared__ __align__(16) uint16_t s_out[kFeatCellsPerWg * kFeat];
-  // image and cell counts are bounded (<= 4096 images of <= 1024 x 1024
-  // cells), so every cell index fits 32 bits
-  const uint32_t cells = (uint32_t)(a.cw * a.ch);
-  const uint32_t total = (uint32_t)a.images * cells;
-  const uint32_t ic0 = blockIdx.x * kFeatCellsPerWg;
+  __shared__ uint32_t sA[NA];
+  __shared__ uint32_t sB[NB];
+  __shared__ __align__(16) uint16_t s_out[kFeatSeg * kFeat];
+  const int cx0 = blockIdx.x * kFeatSeg, cy = blockIdx.y, img = blockIdx.z;
+  const int dw = 4 * a.cw, dh = 4 * a.ch;
+  const int ncell = min(kFeatSeg, a.cw - cx0);
+  const uint8_t* A = a.planes + (size_t)img * 2 * dw * dh;
+  const uint8_t* B = A + (size_t)dw * dh;
+  const int x0 = 4 * cx0, y0 = 4 * cy;
   const int tid = threadIdx.x;
+  // staging dword i: anchor (row i / AD) or neighbour (row (i - NA) / BD,
+  // clamped to the plane; columns clamped per byte at the edges)
+  auto fetch = [&](int i) -> uint32_t {
+    if (i < NA) {
+      const int r = i / AD, q = i - r * AD;
+      const int x = x0 + 4 * q;
+      return x + 4 <= dw ? *reinterpret_cast<const uint32_t*>(A + (size_t)(y0 + r) * dw + x) : 0u;
+    }
+    const int t = i - NA, r = t / BD, q = t - r * BD;
+    const uint8_t* row = B + (size_t)clampi(y0 - kSearch + r, 0, dh - 1) * dw;
+    const int x = x0 - 4 + 4 * q;
+    if (x >= 0 && x + 4 <= dw) return *reinterpret_cast<const uint32_t*>(row + x);
+    uint32_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) v |= (uint32_t)row[clampi(x + b, 0, dw - 1)] << (8 * b);
+    return v;
+  };
+  {
+    const int i0 = tid, i1 = tid + 320;
+    const uint32_t v0 = fetch(i0), v1 = i1 < NA + NB ? fetch(i1) : 0u;
+    (i0 < NA ? sA[i0] : sB[i0 - NA]) = v0;
+    if (i1 < NA + NB) sB[i1 - NA] = v1;
+  }
+  __syncthreads();
   const int jc = tid / TPC, k = tid - jc * TPC;
-  const uint32_t ic = ic0 + (uint32_t)jc;
-  if (jc < kFeatCellsPerWg && ic < total) {
-    const int img = (int)(ic / cells);
-    const int cell = (int)(ic - (uint32_t)img * cells);
-    const int cy = cell / a.cw, cx = cell - cy * a.cw;
-    const int dw = 4 * a.cw, dh = 4 * a.ch;
-    const uint8_t* A = a.planes + (size_t)img * 2 * dw * dh;
-    const uint8_t* B = A + (size_t)dw * dh;
-    const int x0 = 4 * cx, y0 = 4 * cy;
+  if (jc < ncell) {
     uint16_t* out = s_out + jc * kFeat;
     uint32_t arow[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) arow[j] = *reinterpret_cast<const uint32_t*>(A + (size_t)(y0 + j) * dw + x0);
+    for (int r = 0; r < 4; r++) arow[r] = sA[r * AD + jc];
     if (k < 2 * kSearch + 1) {
-      // Window rows y0 + k - 4 + j, columns x0 - 4 .. x0 + 7 (3 dwords).
+      // window rows y0 + k - 4 + r, columns of the cell - 4 .. + 7
       uint32_t win[4][3];
-      const bool inner = x0 >= 4 && x0 + 8 <= dw;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const uint8_t* row = B + (size_t)clampi(y0 + k - kSearch + j, 0, dh - 1) * dw;
+      for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-          if (inner) {
-            win[j][q] = *reinterpret_cast<const uint32_t*>(row + x0 - 4 + 4 * q);
-          } else {
-            uint32_t v = 0;
-#pragma unroll
-            for (int b = 0; b < 4; b++) v |= (uint32_t)row[clampi(x0 - 4 + 4 * q + b, 0, dw - 1)] << (8 * b);
-            win[j][q] = v;
-          }
-        }
-      }
+        for (int q = 0; q < 3; q++) win[r][q] = sB[(k + r) * BD + jc + q];
 #pragma unroll
       for (int ddx = 0; ddx < 2 * kSearch + 1; ddx++) {
         uint32_t sum = 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int r = 0; r < 4; r++) {
           // bytes ddx .. ddx + 3 of the 12-byte window row (alignbyte uses
           // the shift mod 4, so the whole-dword offset 8 is taken directly)
-          const uint32_t ref = ddx < 4 ? funnel(win[j][0], win[j][1], ddx)
-                                       : (ddx < 8 ? funnel(win[j][1], win[j][2], ddx - 4) : win[j][2]);
-          sum = sad4(arow[j], ref, sum);
+          const uint32_t ref = ddx < 4 ? funnel(win[r][0], win[r][1], ddx)
+                                       : (ddx < 8 ? funnel(win[r][1], win[r][2], ddx - 4) : win[r][2]);
+          sum = sad4(arow[r], ref, sum);
         }
         out[(2 * kSearch + 1) * k + ddx] = bf16_bits((float)sum * (1.0f / 1024.0f));
       }
     } else {
       uint32_t brow[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) brow[j] = *reinterpret_cast<const uint32_t*>(B + (size_t)(y0 + j) * dw + x0);
+      for (int r = 0; r < 4; r++) brow[r] = sB[(kSearch + r) * BD + jc + 1];
       auto px = [&](int p) -> uint32_t {  // pixel channel p: A then B, / 255
         const uint32_t v = p < 16 ? (arow[p >> 2] >> (8 * (p & 3))) & 255 : (brow[(p - 16) >> 2] >> (8 * (p & 3))) & 255;
         return bf16_bits((float)v * (1.0f / 255.0f));
@@ -159,12 +170,10 @@ __global__ void __launch_bounds__(256) features_kernel(const FeatArgs a) {
     }
   }
   __syncthreads();
-  // the workgroup's cells: consecutive rows of kFeat bf16 = 16 uint4 each
-  const uint32_t left = total - ic0;
-  const int ncells = left < (uint32_t)kFeatCellsPerWg ? (int)left : kFeatCellsPerWg;
-  uint4* dst = reinterpret_cast<uint4*>(a.out + (size_t)ic0 * kFeat);
+  // the segment's cells: consecutive rows of kFeat bf16 = 16 uint4 each
+  uint4* dst = reinterpret_cast<uint4*>(a.out + (((size_t)img * a.ch + cy) * a.cw + cx0) * kFeat);
   const uint4* srcv = reinterpret_cast<const uint4*>(s_out);
-  for (int i = tid; i < ncells * (kFeat / 8); i += 256) dst[i] = srcv[i];
+  for (int i = tid; i < ncell * (kFeat / 8); i += 320) dst[i] = srcv[i];
 }
 
 // ---------------------------------------------------------------- conv3x3
@@ -535,9 +544,10 @@ hipError_t launch_prep_downsample(const PrepArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_features(const FeatArgs& a, hipStream_t s) {
-  const long long n = (long long)a.images * a.cw * a.ch;  // cells
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(features_kernel, dim3((unsigned)((n + kFeatCellsPerWg - 1) / kFeatCellsPerWg)), dim3(256), 0, s, a);
+  if (a.images == 0 || a.cw == 0 || a.ch == 0) return hipSuccess;
+  hipLaunchKernelGGL(features_kernel, dim3((unsigned)((a.cw + kFeatSeg - 1) / kFeatSeg), (unsigned)a.ch,
+                                           (unsigned)a.images),
+                     dim3(320), 0, s, a);
   return hipGetLastError();
 }
 
