@@ -120,15 +120,15 @@ __device__ __forceinline__ uint32_t tap6(uint32_t a, int32_t lo, uint32_t b, int
   return (uint32_t)min(max(x, 0), 32767) >> 7;
 }
 
-// Horizontal pass over the (OW+5) x (OW+5) halo whose row r starts `sh`
-// bytes into the dword at halo + r * hp; writes the (OW+5) x OW result to
-// `tmp` (pitch OW) as samples ^ 0x80.  A unit is 4 outputs of one row: its
+// Horizontal pass over ROWS rows (default OW + 5) of the halo whose row r
+// starts `sh` bytes into the dword at halo + r * hp; writes the ROWS x OW
+// result to `tmp` (pitch OW) as samples ^ 0x80.  A unit is 4 outputs of one row: its
 // 9 support samples lie in 3 aligned dwords.
-template <int OW>
+template <int OW, int ROWS = OW + 5>
 __device__ __forceinline__ void hpass_dot(const uint8_t* halo, int hp, uint32_t sh, uint8_t* tmp, const DotTaps& f,
                                           int lane) {
   constexpr int UPR = OW / 4;
-  constexpr int UNITS = (OW + 5) * UPR;
+  constexpr int UNITS = ROWS * UPR;
   for (int u = lane; u < UNITS; u += 64) {
     const int r = u / UPR, c0 = 4 * (u % UPR);
     const uint32_t* s32 = reinterpret_cast<const uint32_t*>(halo + r * hp + c0);
@@ -411,50 +411,75 @@ __global__ void __launch_bounds__(256) mb_luma_sad_at_mv_kernel(const SadArgs a)
 // Centre + 8 offsets: every candidate's integer part lies within one
 // pixel of the centre's, so one clamped 24x24 window at the smallest
 // integer origin minus 2 holds all nine 21x21 halos (clamping depends only
-// on the absolute coordinate, so a sub-window equals a direct fetch).
+// on the absolute coordinate, so a sub-window equals a direct fetch).  The
+// nine candidates take three column positions and three row positions: one
+// horizontal pass per column position over all 24 window rows, then one
+// vertical pass per candidate.  All nine SADs are computed, then the ladder
+// decides in the reference's order (strict '<', a candidate equal to the
+// current best skipped).
 __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) {
-  __shared__ __align__(16) uint8_t smem[4 * (24 * 24 + 21 * 16)];
+  constexpr int WP = 28;  // window pitch: 24 columns at any dword shift
+  constexpr int kWave = 24 * WP + 3 * 24 * 16;
+  __shared__ __align__(16) uint8_t smem[4 * kWave];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + wave;
   if (i >= a.n) return;
-  uint8_t* win = smem + wave * (24 * 24 + 21 * 16);
-  uint8_t* lt = win + 24 * 24;
+  uint8_t* win = smem + wave * kWave;
+  uint8_t* lt = win + 24 * WP;  // three 24 x 16 horizontal-pass results
   const int mb_col = (int)a.mb_xy[2 * i], mb_row = (int)a.mb_xy[2 * i + 1];
   // Centres outside +-1023 are outside the reference's contract (it
   // debug-asserts); clamping keeps every candidate inside the staged window.
   const int crow = clampi(a.center[2 * i], -1023, 1023), ccol = clampi(a.center[2 * i + 1], -1023, 1023);
   const int st = a.step;
-  const int offr[8] = {-st, -st, -st, 0, 0, st, st, st};
-  const int offc[8] = {-st, 0, st, -st, st, -st, 0, st};
-  int cr[9], ccl[9];
-  cr[0] = crow; ccl[0] = ccol;
+  // candidate k: row index rk(k) (0..2 -> -st, 0, +st), column index ck(k)
+  const int rowv[3] = {clampi(crow - st, -1023, 1023), crow, clampi(crow + st, -1023, 1023)};
+  const int colv[3] = {clampi(ccol - st, -1023, 1023), ccol, clampi(ccol + st, -1023, 1023)};
   int minr = 1 << 30, minc = 1 << 30;
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    if (k > 0) {
-      cr[k] = clampi(crow + offr[k - 1], -1023, 1023);
-      ccl[k] = clampi(ccol + offc[k - 1], -1023, 1023);
-    }
-    minr = min(minr, (int)(dbl16((int16_t)cr[k]) >> 3));
-    minc = min(minc, (int)(dbl16((int16_t)ccl[k]) >> 3));
+  for (int t = 0; t < 3; t++) {
+    minr = min(minr, (int)(dbl16((int16_t)rowv[t]) >> 3));
+    minc = min(minc, (int)(dbl16((int16_t)colv[t]) >> 3));
   }
-  Gather<24, 24> gw;
-  gw.load(a.plane, a.stride, a.w, a.h, mb_col * 16 + minc - 2, mb_row * 16 + minr - 2, lane);
   const uint32_t s = *reinterpret_cast<const uint32_t*>(a.src + (size_t)i * 256 + 4 * lane);
-  gw.store(win, 24, lane);
+  const uint32_t wsh = stage_halo<24, 24>(a.plane, a.stride, a.w, a.h, mb_col * 16 + minc - 2, mb_row * 16 + minr - 2,
+                                          a.aligned4 != 0, win, WP, lane);
   wave_lds_sync();
-  int best_r = cr[0], best_c = ccl[0];
-  uint32_t best_sad = 0;
-#pragma unroll 1
+  // horizontal passes: column position t at window column (c8 >> 3) - minc
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const int16_t c8 = dbl16((int16_t)colv[t]);
+    const uint32_t x = (uint32_t)((c8 >> 3) - minc) + wsh;
+    // a whole-pel column position is a copy in both passes when the row is
+    // whole-pel too; the vertical pass below makes that choice per
+    // candidate, so the horizontal identity is exact here either way
+    const DotTaps fh = dot_taps(kSixtap[c8 & 7], false);
+    hpass_dot<16, 24>(win + (x & ~3u), WP, x & 3u, lt + t * 24 * 16, fh, lane);
+  }
+  wave_lds_sync();
+  uint32_t sads[9];
+#pragma unroll
   for (int k = 0; k < 9; k++) {
-    const int16_t r8 = dbl16((int16_t)cr[k]), c8 = dbl16((int16_t)ccl[k]);
-    if (k > 0 && cr[k] == best_r && ccl[k] == best_c) continue;  // collapsed onto the best
-    const uint32_t p = luma_pred_from_window(win, (c8 >> 3) - minc, (r8 >> 3) - minr, 0, r8, c8, lt, lane);
-    const uint32_t tot = wave_sum(sad4(s, p, 0));
-    if (k == 0 || tot < best_sad) {
-      best_sad = tot;
-      best_r = cr[k];
-      best_c = ccl[k];
+    // k = 0: centre; k = 1..8: (-st,-st), (-st,0), (-st,+st), (0,-st),
+    // (0,+st), (+st,-st), (+st,0), (+st,+st)
+    const int rk = k == 0 ? 1 : (k <= 3 ? 0 : (k <= 5 ? 1 : 2));
+    const int ck = k == 0 ? 1 : (k <= 3 ? k - 1 : (k == 4 ? 0 : (k == 5 ? 2 : k - 6)));
+    const int16_t r8 = dbl16((int16_t)rowv[rk]);
+    const DotTaps fv = dot_taps(kSixtap[r8 & 7], false);
+    const uint8_t* tmp = lt + ck * 24 * 16 + ((r8 >> 3) - minr) * 16;
+    sads[k] = wave_sum_to_sgpr(sum16(sad4(s, vpass_dot<16>(tmp, fv, lane), 0)));
+  }
+  int best_r = crow, best_c = ccol;
+  uint32_t best_sad = sads[0];
+#pragma unroll
+  for (int k = 1; k < 9; k++) {
+    const int rk = k <= 3 ? 0 : (k <= 5 ? 1 : 2);
+    const int ck = k <= 3 ? k - 1 : (k == 4 ? 0 : (k == 5 ? 2 : k - 6));
+    const int r = rowv[rk], c = colv[ck];
+    if (r == best_r && c == best_c) continue;  // collapsed onto the best
+    if (sads[k] < best_sad) {
+      best_sad = sads[k];
+      best_r = r;
+      best_c = c;
     }
   }
   if (lane == 0) {
@@ -576,8 +601,10 @@ hipError_t launch_half_plane_sads(const HalfPlaneArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_refine_ladder(const RefineArgs& a, hipStream_t stream) {
-  if (a.n <= 0) return hipSuccess;
+hipError_t launch_refine_ladder(const RefineArgs& args, hipStream_t stream) {
+  if (args.n <= 0) return hipSuccess;
+  RefineArgs a = args;
+  a.aligned4 = aligned4(a.plane, a.stride);
   hipLaunchKernelGGL(refine_ladder_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   return hipGetLastError();
 }

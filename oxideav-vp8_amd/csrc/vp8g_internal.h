@@ -152,6 +152,7 @@ struct RefineArgs {
   uint32_t* out;           // n x {row | col << 16, sad} (vp8g_search_result layout)
   int n;
   int step;                // 2: half-pel ladder, 1: quarter-pel ladder
+  int aligned4;            // set by the launcher: plane and stride 4-byte aligned
 };
 
 hipError_t launch_predict_inter_mb(const McArgs& a, hipStream_t stream);
