@@ -18,6 +18,7 @@ oracle on the same workload on this host's cores.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -109,7 +110,21 @@ def cpu_baseline(group, mc, seconds: float, cores: int):
                    f"in {el:.1f} s on {cores} threads; 1 thread: {anchors * r1 / e1:.0f} anchor-MB/s "
                    f"({r1} steps in {e1:.1f} s)"),
         "value_1thread": anchors * r1 / e1,
+        "host": host_info(),
     }
+
+
+def host_info() -> dict:
+    """CPU model and thread counts of the machine the CPU leg ran on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
 
 
 def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
@@ -552,6 +567,28 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    # End to end through the host-pointer entry point: frames copied in,
+    # the altref copied out (PCIe included); not the headline.
+    frames_host = [vp8g.I420Frame(*t) for t in group.frames()]
+    ctx.build_arnr_altref(frames_host, group.center, vp8g.ArnrConfig(3))
+    e2e_reps, t0 = 0, time.perf_counter()
+    while e2e_reps < 5 or time.perf_counter() - t0 < 0.5:
+        pic = ctx.build_arnr_altref(frames_host, group.center, vp8g.ArnrConfig(3))
+        e2e_reps += 1
+    e2e_s = (time.perf_counter() - t0) / e2e_reps
+    if not np.array_equal(pic.y, ref_out.cpu().numpy()):
+        raise RuntimeError("host entry point output differs from the device entry point")
+    in_bytes = sum(p.size for t in group.frames() for p in t[2:5])
+    result["end_to_end"] = {
+        "ms_per_group": e2e_s * 1e3, "anchor_mb_per_s": anchors / e2e_s,
+        "bytes_in": int(in_bytes), "bytes_out": int(pic.y.size + pic.u.size + pic.v.size),
+        "note": "vp8g_build_arnr_altref on host buffers: H2D copy, kernel, D2H copy",
+    }
+    h = hashlib.sha256()
+    for t in group.frames():
+        for p in t[2:5]:
+            h.update(np.ascontiguousarray(p).tobytes())
+    result["inputs"] = {"prng": "splitmix64", "seed_base": synth._seed(1, 0, 0), "group_sha256": h.hexdigest()}
     if not args.no_learned:
         result["learned_seeded"] = bench_learned(ctx, group, descs, stream, reps=max(5, args.steps // 5))
     if not args.no_cpu:

@@ -5,6 +5,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include <string>
 #include <vector>
 
@@ -43,6 +45,29 @@ int ensure_arena(vp8g_ctx* c, size_t bytes) {
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+int ensure_pinned(vp8g_ctx* c, size_t bytes) {
+  if (bytes <= c->pinned_bytes) return VP8G_OK;
+  if (c->pinned) {
+    if (!hip_ok(hipStreamSynchronize(c->stream))) return VP8G_E_DEVICE;
+    (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_bytes = 0;
+  }
+  size_t want = bytes + (bytes >> 2);
+  if (!hip_ok(hipHostMalloc(reinterpret_cast<void**>(&c->pinned), want, hipHostMallocDefault))) return VP8G_E_DEVICE;
+  c->pinned_bytes = want;
+  return VP8G_OK;
+}
+
+// Rows of `w` bytes from a strided host plane into a packed run of pitch `p`.
+void pack_plane(uint8_t* dst, size_t p, const uint8_t* src, size_t stride, size_t w, size_t h) {
+  if (stride == p && w == p) {
+    memcpy(dst, src, p * h);
+    return;
+  }
+  for (size_t r = 0; r < h; r++) memcpy(dst + r * p, src + r * stride, w);
+}
 
 // Reference check order, src/arnr.rs:494-515.
 int validate_arnr(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* err) {
@@ -347,6 +372,7 @@ void vp8g_ctx_destroy(vp8g_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->arena) (void)hipFree(c->arena);
+  if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -416,16 +442,19 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   const size_t per = ysz + 2 * csz;
   const size_t out_bytes = align_up(w * h, 256) + 2 * align_up(cw * ch, 256);
   st = ensure_arena(c, per * n + out_bytes);
+  if (st == VP8G_OK) st = ensure_pinned(c, std::max(per * n, out_bytes));
   if (st != VP8G_OK) return st;
+  // Each frame is packed into pinned memory in the device layout and its
+  // copy issued at once, so packing the next frame overlaps the transfer.
   std::vector<vp8g_i420> dev(n);
   for (size_t i = 0; i < n; i++) {
     uint8_t* base = c->arena + per * i;
+    uint8_t* pin = c->pinned + per * i;
     const vp8g_i420& f = frames[i];
-    if (!hip_ok(hipMemcpy2DAsync(base, yp, f.y, f.y_stride, w, h, hipMemcpyHostToDevice, c->stream)) ||
-        !hip_ok(hipMemcpy2DAsync(base + ysz, cp, f.u, f.uv_stride, cw, ch, hipMemcpyHostToDevice,
-                                 c->stream)) ||
-        !hip_ok(hipMemcpy2DAsync(base + ysz + csz, cp, f.v, f.uv_stride, cw, ch,
-                                 hipMemcpyHostToDevice, c->stream)))
+    pack_plane(pin, yp, f.y, f.y_stride, w, h);
+    pack_plane(pin + ysz, cp, f.u, f.uv_stride, cw, ch);
+    pack_plane(pin + ysz + csz, cp, f.v, f.uv_stride, cw, ch);
+    if (!hip_ok(hipMemcpyAsync(base, pin, per, hipMemcpyHostToDevice, c->stream)))
       return finish(c, VP8G_E_DEVICE);
     dev[i] = vp8g_i420{f.width, f.height, base, base + ysz, base + ysz + csz, yp, cp};
   }
@@ -433,10 +462,15 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   uint8_t* du = dy + align_up(w * h, 256);
   uint8_t* dv = du + align_up(cw * ch, 256);
   st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr);
-  if (st == VP8G_OK) st = copy_out(c, oy, dy, w * h);
-  if (st == VP8G_OK) st = copy_out(c, ou, du, cw * ch);
-  if (st == VP8G_OK) st = copy_out(c, ov, dv, cw * ch);
-  return finish(c, st);
+  // the staging buffer is reused for the output once the inputs have left
+  if (st == VP8G_OK && !hip_ok(hipMemcpyAsync(c->pinned, dy, out_bytes, hipMemcpyDeviceToHost, c->stream)))
+    st = VP8G_E_DEVICE;
+  st = finish(c, st);
+  if (st != VP8G_OK) return st;
+  memcpy(oy, c->pinned, w * h);
+  memcpy(ou, c->pinned + (du - dy), cw * ch);
+  memcpy(ov, c->pinned + (dv - dy), cw * ch);
+  return VP8G_OK;
 }
 
 int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,

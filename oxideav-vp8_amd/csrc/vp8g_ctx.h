@@ -20,6 +20,11 @@ struct vp8g_ctx {
   // Device arena for the host-pointer entry points (grown, never shrunk).
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
+  // Pinned host staging for the same entry points (hipHostMalloc; grown,
+  // never shrunk): host planes are packed here and cross PCIe in large
+  // copies.
+  uint8_t* pinned = nullptr;
+  size_t pinned_bytes = 0;
 };
 
 // Group validation in the reference's order (src/arnr.rs:494-515) plus the
