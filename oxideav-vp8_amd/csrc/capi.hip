@@ -35,30 +35,23 @@ int ensure_arena(vp8g_ctx* c, size_t bytes) {
   if (c->arena) {
     if (!hip_ok(hipStreamSynchronize(c->stream))) return VP8G_E_DEVICE;
     (void)hipFree(c->arena);
+    (void)hipHostFree(c->pinned);
     c->arena = nullptr;
+    c->pinned = nullptr;
     c->arena_bytes = 0;
   }
   size_t want = bytes + (bytes >> 2);
   if (!hip_ok(hipMalloc(&c->arena, want))) return VP8G_E_DEVICE;
+  if (!hip_ok(hipHostMalloc(reinterpret_cast<void**>(&c->pinned), want, hipHostMallocDefault))) {
+    (void)hipFree(c->arena);
+    c->arena = nullptr;
+    return VP8G_E_DEVICE;
+  }
   c->arena_bytes = want;
   return VP8G_OK;
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-int ensure_pinned(vp8g_ctx* c, size_t bytes) {
-  if (bytes <= c->pinned_bytes) return VP8G_OK;
-  if (c->pinned) {
-    if (!hip_ok(hipStreamSynchronize(c->stream))) return VP8G_E_DEVICE;
-    (void)hipHostFree(c->pinned);
-    c->pinned = nullptr;
-    c->pinned_bytes = 0;
-  }
-  size_t want = bytes + (bytes >> 2);
-  if (!hip_ok(hipHostMalloc(reinterpret_cast<void**>(&c->pinned), want, hipHostMallocDefault))) return VP8G_E_DEVICE;
-  c->pinned_bytes = want;
-  return VP8G_OK;
-}
 
 // Rows of `w` bytes from a strided host plane into a packed run of pitch `p`.
 void pack_plane(uint8_t* dst, size_t p, const uint8_t* src, size_t stride, size_t w, size_t h) {
@@ -264,24 +257,31 @@ bool mb_xy_in_range(const uint32_t* mb_xy, size_t n) {
 // Bytes of a strided plane that the kernels may read.
 size_t plane_span(size_t stride, size_t width, size_t rows) { return rows ? (rows - 1) * stride + width : 0; }
 
+// Host <-> arena copies through the pinned mirror (same offset as the
+// device pointer in the arena).  Outputs reach the caller in finish().
 int copy_out(vp8g_ctx* c, void* host, const void* dev, size_t bytes) {
   if (!bytes) return VP8G_OK;
-  return hip_ok(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream)) ? VP8G_OK
-                                                                                     : VP8G_E_DEVICE;
+  const size_t off = static_cast<const uint8_t*>(dev) - c->arena;
+  if (!hip_ok(hipMemcpyAsync(c->pinned + off, dev, bytes, hipMemcpyDeviceToHost, c->stream))) return VP8G_E_DEVICE;
+  c->pending.push_back({host, off, bytes});
+  return VP8G_OK;
 }
 
 int copy_in(vp8g_ctx* c, void* dev, const void* host, size_t bytes) {
   if (!bytes) return VP8G_OK;
-  return hip_ok(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream)) ? VP8G_OK
-                                                                                    : VP8G_E_DEVICE;
+  const size_t off = static_cast<uint8_t*>(dev) - c->arena;
+  memcpy(c->pinned + off, host, bytes);
+  return hip_ok(hipMemcpyAsync(dev, c->pinned + off, bytes, hipMemcpyHostToDevice, c->stream)) ? VP8G_OK
+                                                                                              : VP8G_E_DEVICE;
 }
 
 int finish(vp8g_ctx* c, int st) {
-  if (st != VP8G_OK) {
-    (void)hipStreamSynchronize(c->stream);
-    return st;
-  }
-  return hip_ok(hipStreamSynchronize(c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  const bool synced = hip_ok(hipStreamSynchronize(c->stream));
+  if (st == VP8G_OK && !synced) st = VP8G_E_DEVICE;
+  if (st == VP8G_OK)
+    for (const auto& p : c->pending) memcpy(p.host, c->pinned + p.off, p.bytes);
+  c->pending.clear();
+  return st;
 }
 
 }  // namespace
@@ -372,7 +372,7 @@ void vp8g_ctx_destroy(vp8g_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->arena) (void)hipFree(c->arena);
-  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->pinned) (void)hipHostFree(c->pinned);  // allocated with the arena
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -442,7 +442,6 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   const size_t per = ysz + 2 * csz;
   const size_t out_bytes = align_up(w * h, 256) + 2 * align_up(cw * ch, 256);
   st = ensure_arena(c, per * n + out_bytes);
-  if (st == VP8G_OK) st = ensure_pinned(c, std::max(per * n, out_bytes));
   if (st != VP8G_OK) return st;
   // Each frame is packed into pinned memory in the device layout and its
   // copy issued at once, so packing the next frame overlaps the transfer.
@@ -462,15 +461,10 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   uint8_t* du = dy + align_up(w * h, 256);
   uint8_t* dv = du + align_up(cw * ch, 256);
   st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr);
-  // the staging buffer is reused for the output once the inputs have left
-  if (st == VP8G_OK && !hip_ok(hipMemcpyAsync(c->pinned, dy, out_bytes, hipMemcpyDeviceToHost, c->stream)))
-    st = VP8G_E_DEVICE;
-  st = finish(c, st);
-  if (st != VP8G_OK) return st;
-  memcpy(oy, c->pinned, w * h);
-  memcpy(ou, c->pinned + (du - dy), cw * ch);
-  memcpy(ov, c->pinned + (dv - dy), cw * ch);
-  return VP8G_OK;
+  if (st == VP8G_OK) st = copy_out(c, oy, dy, w * h);
+  if (st == VP8G_OK) st = copy_out(c, ou, du, cw * ch);
+  if (st == VP8G_OK) st = copy_out(c, ov, dv, cw * ch);
+  return finish(c, st);
 }
 
 int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,

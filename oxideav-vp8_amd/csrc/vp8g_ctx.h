@@ -20,11 +20,15 @@ struct vp8g_ctx {
   // Device arena for the host-pointer entry points (grown, never shrunk).
   uint8_t* arena = nullptr;
   size_t arena_bytes = 0;
-  // Pinned host staging for the same entry points (hipHostMalloc; grown,
-  // never shrunk): host planes are packed here and cross PCIe in large
-  // copies.
+  // Pinned host mirror of the arena (hipHostMalloc, same size and offsets):
+  // host inputs are packed here and cross PCIe from pinned memory; outputs
+  // land here and are copied to the caller once the stream has finished.
   uint8_t* pinned = nullptr;
-  size_t pinned_bytes = 0;
+  struct PendingOut {
+    void* host;
+    size_t off, bytes;
+  };
+  std::vector<PendingOut> pending;
 };
 
 // Group validation in the reference's order (src/arnr.rs:494-515) plus the
