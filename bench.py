@@ -114,6 +114,23 @@ def cpu_baseline(group, mc, seconds: float, cores: int):
     }
 
 
+def end_to_end(ctx, vp8g, frames_host, group, anchors: int, ref_out) -> dict:
+    """vp8g_build_arnr_altref on host buffers (H2D copy, kernel, D2H copy),
+    timed on the wall clock over at least 5 calls and 0.5 s."""
+    ctx.build_arnr_altref(frames_host, group.center, vp8g.ArnrConfig(3))
+    reps, t0 = 0, time.perf_counter()
+    while reps < 5 or time.perf_counter() - t0 < 0.5:
+        pic = ctx.build_arnr_altref(frames_host, group.center, vp8g.ArnrConfig(3))
+        reps += 1
+    secs = (time.perf_counter() - t0) / reps
+    if not np.array_equal(pic.y, ref_out.cpu().numpy()):
+        raise RuntimeError("host entry point output differs from the device entry point")
+    in_bytes = sum(p.size for t in group.frames() for p in t[2:5])
+    return {"ms_per_group": secs * 1e3, "anchor_mb_per_s": anchors / secs, "bytes_in": int(in_bytes),
+            "bytes_out": int(pic.y.size + pic.u.size + pic.v.size),
+            "note": "vp8g_build_arnr_altref on host buffers: H2D copy, kernel, D2H copy"}
+
+
 def host_info() -> dict:
     """CPU model and thread counts of the machine the CPU leg ran on."""
     model = "unknown"
@@ -348,6 +365,7 @@ def main() -> int:
     ap.add_argument("--no-xcd-remap", action="store_true")
     ap.add_argument("--arnr-mbs", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeded altref leg")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end (host buffers) leg")
     ap.add_argument("--workload", choices=("step", "batch", "4k"), default="step",
                     help="step: the 1080p group step (default); batch: 64 uneven clips in one "
                          "persistent graph-captured launch against per-clip launches; 4k: 3840x2160 "
@@ -569,21 +587,9 @@ def main() -> int:
     }
     # End to end through the host-pointer entry point: frames copied in,
     # the altref copied out (PCIe included); not the headline.
-    frames_host = [vp8g.I420Frame(*t) for t in group.frames()]
-    ctx.build_arnr_altref(frames_host, group.center, vp8g.ArnrConfig(3))
-    e2e_reps, t0 = 0, time.perf_counter()
-    while e2e_reps < 5 or time.perf_counter() - t0 < 0.5:
-        pic = ctx.build_arnr_altref(frames_host, group.center, vp8g.ArnrConfig(3))
-        e2e_reps += 1
-    e2e_s = (time.perf_counter() - t0) / e2e_reps
-    if not np.array_equal(pic.y, ref_out.cpu().numpy()):
-        raise RuntimeError("host entry point output differs from the device entry point")
-    in_bytes = sum(p.size for t in group.frames() for p in t[2:5])
-    result["end_to_end"] = {
-        "ms_per_group": e2e_s * 1e3, "anchor_mb_per_s": anchors / e2e_s,
-        "bytes_in": int(in_bytes), "bytes_out": int(pic.y.size + pic.u.size + pic.v.size),
-        "note": "vp8g_build_arnr_altref on host buffers: H2D copy, kernel, D2H copy",
-    }
+    frames_host = [vp8g.I420Frame(*t) for t in group.frames()] if not args.no_e2e else None
+    if frames_host is not None:
+        result["end_to_end"] = end_to_end(ctx, vp8g, frames_host, group, anchors, ref_out)
     h = hashlib.sha256()
     for t in group.frames():
         for p in t[2:5]:
