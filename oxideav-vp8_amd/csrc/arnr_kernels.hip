@@ -159,14 +159,6 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
   best_sad = bsad;
 }
 
-// Maps the launch index onto strips so that each of the 8 XCDs (which
-// receive workgroups round-robin) works on one contiguous band and keeps
-// that band's frame rows in its own L2.
-__device__ __forceinline__ int xcd_band(int i, int total) {
-  const int full = (total >> 3) << 3;
-  if (i >= full) return i;
-  return (i & 7) * (total >> 3) + (i >> 3);
-}
 
 
 // DMA of one frame's luma window into `win` (global_load_lds_dwordx4: lane

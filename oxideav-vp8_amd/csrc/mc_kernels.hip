@@ -23,11 +23,6 @@
 
 namespace vp8g {
 
-__constant__ int32_t kSixtap[8][6] = {
-    {0, 0, 128, 0, 0, 0},     {0, -6, 123, 12, -1, 0}, {2, -11, 108, 36, -8, 1},
-    {0, -9, 93, 50, -6, 0},   {3, -16, 77, 77, -16, 3}, {0, -6, 50, 93, -9, 0},
-    {1, -8, 36, 108, -11, 2}, {0, -1, 12, 123, -6, 0}};
-
 struct Taps {
   int32_t t[6];
   bool ident;
@@ -102,12 +97,45 @@ struct DotTaps {
   bool ident;
 };
 
-__device__ __forceinline__ DotTaps dot_taps(const int32_t* row, bool ident) {
+// A table row packed for the dot-product form: taps 0..3 and 4..5 as
+// signed bytes (host and device).
+constexpr uint32_t pack_lo(const int32_t* t) {
+  return (uint32_t)(t[0] & 255) | ((uint32_t)(t[1] & 255) << 8) | ((uint32_t)(t[2] & 255) << 16) |
+         ((uint32_t)(t[3] & 255) << 24);
+}
+constexpr uint32_t pack_hi(const int32_t* t) { return (uint32_t)(t[4] & 255) | ((uint32_t)(t[5] & 255) << 8); }
+constexpr bool is_identity(const int32_t* t) {
+  return t[0] == 0 && t[1] == 0 && t[2] == 128 && t[3] == 0 && t[4] == 0 && t[5] == 0;
+}
+
+// The six-tap set (RFC 6386 section 18.3), packed at compile time; row 0 is
+// the identity.
+constexpr int32_t kSixtapRows[8][6] = {
+    {0, 0, 128, 0, 0, 0},     {0, -6, 123, 12, -1, 0}, {2, -11, 108, 36, -8, 1},
+    {0, -9, 93, 50, -6, 0},   {3, -16, 77, 77, -16, 3}, {0, -6, 50, 93, -9, 0},
+    {1, -8, 36, 108, -11, 2}, {0, -1, 12, 123, -6, 0}};
+__constant__ uint32_t kSixtapLo[8] = {pack_lo(kSixtapRows[0]), pack_lo(kSixtapRows[1]), pack_lo(kSixtapRows[2]),
+                                      pack_lo(kSixtapRows[3]), pack_lo(kSixtapRows[4]), pack_lo(kSixtapRows[5]),
+                                      pack_lo(kSixtapRows[6]), pack_lo(kSixtapRows[7])};
+__constant__ uint32_t kSixtapHi[8] = {pack_hi(kSixtapRows[0]), pack_hi(kSixtapRows[1]), pack_hi(kSixtapRows[2]),
+                                      pack_hi(kSixtapRows[3]), pack_hi(kSixtapRows[4]), pack_hi(kSixtapRows[5]),
+                                      pack_hi(kSixtapRows[6]), pack_hi(kSixtapRows[7])};
+
+// Taps of fraction f of the six-tap set (only row 0 is an identity).
+__device__ __forceinline__ DotTaps sixtap_dot(int f, bool ident) {
   DotTaps k;
-  k.ident = ident || (row[0] == 0 && row[1] == 0 && row[2] == 128 && row[3] == 0 && row[4] == 0 && row[5] == 0);
-  k.lo = (int32_t)((uint32_t)(row[0] & 255) | ((uint32_t)(row[1] & 255) << 8) | ((uint32_t)(row[2] & 255) << 16) |
-                   ((uint32_t)(row[3] & 255) << 24));
-  k.hi = (int32_t)((uint32_t)(row[4] & 255) | ((uint32_t)(row[5] & 255) << 8));
+  k.lo = (int32_t)kSixtapLo[f];
+  k.hi = (int32_t)kSixtapHi[f];
+  k.ident = ident || f == 0;
+  return k;
+}
+
+// Taps of fraction f of a caller's table packed on the host (McArgs).
+__device__ __forceinline__ DotTaps table_dot(const McArgs& a, int f, bool ident) {
+  DotTaps k;
+  k.lo = (int32_t)a.dot_lo[f];
+  k.hi = (int32_t)a.dot_hi[f];
+  k.ident = ident || ((a.ident_rows >> f) & 1);
   return k;
 }
 
@@ -205,25 +233,49 @@ struct GatherDw {
   }
 };
 
-// Stage the ROWS x COLS halo at (x0, y0) of a plane clamped to (w, h) into
-// LDS (pitch `pitch`, a multiple of 4 with room for COLS + 3 bytes); returns
-// the byte shift of the halo within each LDS row.  Interior halos of
-// 4-byte aligned planes are read as whole dwords.
+// The ROWS x COLS halo at (x0, y0) of a plane clamped to (w, h), staged in
+// two phases so that several halos' loads are in flight together: load()
+// issues the loads into registers (whole dwords from the dword-aligned
+// column below x0 for interior halos of 4-byte aligned planes, clamped
+// bytes otherwise), store() writes them to LDS (pitch `pitch`, a multiple of
+// 4 with room for COLS + 3 bytes) and returns the halo's byte shift within
+// each LDS row.
+template <int ROWS, int COLS>
+struct HaloStage {
+  static constexpr int DW = (COLS + 3 + 3) / 4;  // dwords covering COLS bytes at any shift
+  GatherDw<ROWS, DW> dw;
+  Gather<ROWS, COLS> bytes;
+  bool fast;
+  uint32_t sh;
+
+  __device__ __forceinline__ void load(const uint8_t* plane, uint32_t stride, int w, int h, int x0, int y0,
+                                       bool aligned4, int lane) {
+    const int xa = x0 & ~3;
+    fast = aligned4 && xa >= 0 && xa + 4 * DW <= w && y0 >= 0 && y0 + ROWS <= h;
+    if (fast) {
+      dw.load(plane, stride, xa, y0, lane);
+      sh = (uint32_t)(x0 - xa);
+    } else {
+      bytes.load(plane, stride, w, h, x0, y0, lane);
+      sh = 0;
+    }
+  }
+
+  __device__ __forceinline__ uint32_t store(uint8_t* dst, int pitch, int lane) const {
+    if (fast)
+      dw.store(dst, pitch, lane);
+    else
+      bytes.store(dst, pitch, lane);
+    return sh;
+  }
+};
+
 template <int ROWS, int COLS>
 __device__ __forceinline__ uint32_t stage_halo(const uint8_t* plane, uint32_t stride, int w, int h, int x0, int y0,
                                                bool aligned4, uint8_t* dst, int pitch, int lane) {
-  constexpr int DW = (COLS + 3 + 3) / 4;  // dwords covering COLS bytes at any shift
-  const int xa = x0 & ~3;
-  if (aligned4 && xa >= 0 && xa + 4 * DW <= w && y0 >= 0 && y0 + ROWS <= h) {
-    GatherDw<ROWS, DW> g;
-    g.load(plane, stride, xa, y0, lane);
-    g.store(dst, pitch, lane);
-    return (uint32_t)(x0 - xa);
-  }
-  Gather<ROWS, COLS> g;
-  g.load(plane, stride, w, h, x0, y0, lane);
-  g.store(dst, pitch, lane);
-  return 0;
+  HaloStage<ROWS, COLS> st;
+  st.load(plane, stride, w, h, x0, y0, aligned4, lane);
+  return st.store(dst, pitch, lane);
 }
 
 // Two-pass filter of an OW x OW block from its (OW+5)^2 halo held in LDS
@@ -298,7 +350,8 @@ template <bool DOT>
 __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * kMcLdsPerWave];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + wave;
+  // MBs are usually in raster order: an XCD takes a band of them
+  const int i = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
   if (i >= a.n) return;  // whole wave exits together
   uint8_t* lh = smem + wave * kMcLdsPerWave;
   uint8_t* lt = lh + 21 * 24;
@@ -319,14 +372,17 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
   const int cx0 = mb_col * 8 + (uc >> 3) - 2, cy0 = mb_row * 8 + (ur >> 3) - 2;
   uint8_t* out = a.out + (size_t)i * 384;
   if (DOT) {
-    const DotTaps fh = dot_taps(a.filters[yc & 7], ycopy), fv = dot_taps(a.filters[yr & 7], ycopy);
-    const DotTaps cfh = dot_taps(a.filters[uc & 7], ccopy), cfv = dot_taps(a.filters[ur & 7], ccopy);
+    const DotTaps fh = table_dot(a, yc & 7, ycopy), fv = table_dot(a, yr & 7, ycopy);
+    const DotTaps cfh = table_dot(a, uc & 7, ccopy), cfv = table_dot(a, ur & 7, ccopy);
     const bool al = a.aligned4 != 0;
-    const uint32_t lsh = stage_halo<21, 21>(a.ref.y, a.ref.y_stride, a.ref.lw, a.ref.lh, lx0, ly0, al, lh, 24, lane);
-    const uint32_t ush =
-        stage_halo<13, 13>(a.ref.u, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, al, chalo, 16, lane);
-    const uint32_t vsh =
-        stage_halo<13, 13>(a.ref.v, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, al, chalo + 13 * 16, 16, lane);
+    HaloStage<21, 21> hy;
+    HaloStage<13, 13> hu, hv;
+    hy.load(a.ref.y, a.ref.y_stride, a.ref.lw, a.ref.lh, lx0, ly0, al, lane);
+    hu.load(a.ref.u, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, al, lane);
+    hv.load(a.ref.v, a.ref.uv_stride, a.ref.cw, a.ref.ch, cx0, cy0, al, lane);
+    const uint32_t lsh = hy.store(lh, 24, lane);
+    const uint32_t ush = hu.store(chalo, 16, lane);
+    const uint32_t vsh = hv.store(chalo + 13 * 16, 16, lane);
     wave_lds_sync();
     hpass_dot<16>(lh, 24, lsh, lt, fh, lane);
     if (lane < 26) {
@@ -381,7 +437,7 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
 __device__ __forceinline__ uint32_t luma_pred_from_window(const uint8_t* win, int ox, int oy, uint32_t sh, int r8,
                                                           int c8, uint8_t* lt, int lane) {
   const bool copy = (r8 & 7) == 0 && (c8 & 7) == 0;
-  const DotTaps fh = dot_taps(kSixtap[c8 & 7], copy), fv = dot_taps(kSixtap[r8 & 7], copy);
+  const DotTaps fh = sixtap_dot(c8 & 7, copy), fv = sixtap_dot(r8 & 7, copy);
   const uint32_t x = (uint32_t)ox + sh;
   hpass_dot<16>(win + oy * 24 + (x & ~3u), 24, x & 3u, lt, fh, lane);
   wave_lds_sync();
@@ -393,7 +449,8 @@ __device__ __forceinline__ uint32_t luma_pred_from_window(const uint8_t* win, in
 __global__ void __launch_bounds__(256) mb_luma_sad_at_mv_kernel(const SadArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * (21 * 24 + 21 * 16)];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + wave;
+  // MBs are usually in raster order: an XCD takes a band of them
+  const int i = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
   if (i >= a.n) return;
   uint8_t* lh = smem + wave * (21 * 24 + 21 * 16);
   uint8_t* lt = lh + 21 * 24;
@@ -422,7 +479,8 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
   constexpr int kWave = 24 * WP + 3 * 24 * 16;
   __shared__ __align__(16) uint8_t smem[4 * kWave];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + wave;
+  // MBs are usually in raster order: an XCD takes a band of them
+  const int i = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
   if (i >= a.n) return;
   uint8_t* win = smem + wave * kWave;
   uint8_t* lt = win + 24 * WP;  // three 24 x 16 horizontal-pass results
@@ -452,7 +510,7 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
     // a whole-pel column position is a copy in both passes when the row is
     // whole-pel too; the vertical pass below makes that choice per
     // candidate, so the horizontal identity is exact here either way
-    const DotTaps fh = dot_taps(kSixtap[c8 & 7], false);
+    const DotTaps fh = sixtap_dot(c8 & 7, false);
     hpass_dot<16, 24>(win + (x & ~3u), WP, x & 3u, lt + t * 24 * 16, fh, lane);
   }
   wave_lds_sync();
@@ -464,7 +522,7 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
     const int rk = k == 0 ? 1 : (k <= 3 ? 0 : (k <= 5 ? 1 : 2));
     const int ck = k == 0 ? 1 : (k <= 3 ? k - 1 : (k == 4 ? 0 : (k == 5 ? 2 : k - 6)));
     const int16_t r8 = dbl16((int16_t)rowv[rk]);
-    const DotTaps fv = dot_taps(kSixtap[r8 & 7], false);
+    const DotTaps fv = sixtap_dot(r8 & 7, false);
     const uint8_t* tmp = lt + ck * 24 * 16 + ((r8 >> 3) - minr) * 16;
     sads[k] = wave_sum_to_sgpr(sum16(sad4(s, vpass_dot<16>(tmp, fv, lane), 0)));
   }
@@ -512,6 +570,12 @@ hipError_t launch_predict_inter_mb(const McArgs& args, hipStream_t stream) {
   a.aligned4 = aligned4(a.ref.y, a.ref.y_stride) && aligned4(a.ref.u, a.ref.uv_stride) &&
                aligned4(a.ref.v, a.ref.uv_stride);
   a.dot = dot_table_ok(a.filters);
+  a.ident_rows = 0;
+  for (int r = 0; r < 8; r++) {
+    a.dot_lo[r] = pack_lo(a.filters[r]);
+    a.dot_hi[r] = pack_hi(a.filters[r]);
+    if (is_identity(a.filters[r])) a.ident_rows |= 1u << r;
+  }
   if (a.dot)
     hipLaunchKernelGGL(predict_inter_mb_kernel<true>, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   else
@@ -540,7 +604,8 @@ constexpr int kHpLds = 22 * 24 + 22 * 17 + 17 * 16 + 17 * 17;
 __global__ void __launch_bounds__(256) half_plane_sads_kernel(const HalfPlaneArgs a) {
   __shared__ __align__(16) uint8_t smem[4 * ((kHpLds + 15) & ~15)];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 4 + wave;
+  // MBs are usually in raster order: an XCD takes a band of them
+  const int i = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
   if (i >= a.n) return;
   uint8_t* win = smem + wave * ((kHpLds + 15) & ~15);
   uint8_t* hp = win + 22 * 24;

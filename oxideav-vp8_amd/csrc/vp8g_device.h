@@ -112,6 +112,15 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Maps a launch index onto work items so that each of the 8 XCDs (which
+// receive workgroups round-robin) works on one contiguous band of items and
+// keeps that band's frame rows in its own L2.
+__device__ __forceinline__ int xcd_band(int i, int total) {
+  const int full = (total >> 3) << 3;
+  if (i >= full) return i;
+  return (i & 7) * (total >> 3) + (i >> 3);
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 }  // namespace vp8g

@@ -114,6 +114,8 @@ struct McArgs {
   int32_t filters[8][6];
   int aligned4;            // set by the launcher: planes and strides 4-byte aligned
   int dot;                 // set by the launcher: the table suits the dot-product form
+  uint32_t dot_lo[8], dot_hi[8];  // set by the launcher: rows packed as signed bytes
+  uint32_t ident_rows;     // set by the launcher: bit r = row r is {0, 0, 128, 0, 0, 0}
 };
 
 struct SadArgs {
