@@ -234,31 +234,29 @@ __device__ __forceinline__ void blend_px(uint32_t& acc, uint32_t ref, uint32_t c
 //                for a whole MB), chroma clamped.
 //   kPathGather: MB at the left or right edge, or planes not 16-byte
 //                aligned or without one stride: clamped gathers.
-//   kPathTable:  vectors from the table, neighbour read from memory.
 // Fast and Dma MBs (16-byte aligned luma, one stride for all frames,
 // columns bx-16..bx+31 inside the frame) take their window by DMA with
 // row-clamped piece offsets; the next frame's window arrives during this
 // frame's search.  A frame's chroma bytes are loaded after its search and
 // blended after the next frame's wait, so that their latency overlaps the
 // next search.
-enum { kPathTable = 0, kPathFast = 1, kPathDma = 2, kPathGather = 3 };
+enum { kPathFast = 1, kPathDma = 2, kPathGather = 3 };
 
 template <int PATH, class FrameAt>
 __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& frame_at, MbState& s, int split,
-                                           int K, int lane, uint8_t* win, const uint8_t* s_lut,
-                                           const int32_t* table) {
+                                           int K, int lane, uint8_t* win, const uint8_t* s_lut) {
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
   const int nnb = a.filter_on ? a.n - 1 : 0;
   // search: lane = probe group gq x row pair kk (rows kk, kk + 8)
   const int gq = lane >> 3, kk = lane & 7;
-  uint4 src0 = make_uint4(0, 0, 0, 0), src1 = src0;
+  uint4 src0, src1;
   // DMA paths: plane offset of lane piece i (window row p / 3 clamped to the
   // frame, 16-byte column p % 3, p = lane + 64 i); kPathDma: bit i of
   // `outside` set when the piece lies wholly outside the frame's columns,
   // its offset then that of the row's edge pixel, which is repeated.
   uint32_t off0 = 0, off1 = 0, off2 = 0, coff = 0, outside = 0;
-  if (PATH != kPathTable) {
+  {
     const FrameDesc C = frame_at(a.center);
     src0 = centre_row16(C, s.bx, s.by, kk, s.bw, s.bh, PATH == kPathFast);
     src1 = centre_row16(C, s.bx, s.by, kk + 8, s.bw, s.bh, PATH == kPathFast);
@@ -337,12 +335,7 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
 
     int bestx, besty;
     uint32_t best_sad;
-    if (PATH == kPathTable) {
-      const int32_t* t = table + 3 * (((size_t)fi * a.brows + s.brow) * a.bcols + (s.bx >> 4));
-      bestx = __builtin_amdgcn_readfirstlane(t[0]);
-      besty = __builtin_amdgcn_readfirstlane(t[1]);
-      best_sad = (uint32_t)__builtin_amdgcn_readfirstlane(t[2]);
-    } else {
+    {
       const int gd_off = (((kProbeDy1 >> (2 * gq)) & 3) - 1) * kLumaPitch + (((kProbeDx1 >> (2 * gq)) & 3) - 1);
       const uint8_t* row0 = w + (kk + 15) * kLumaPitch + 16;
       if (PATH == kPathFast) {
@@ -366,19 +359,10 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     pend = best_sad < (kSadPerPixelCutoff + 1) * (uint32_t)(s.bw * s.bh);
     if (pend) {
       const int cmvx = bestx / 2, cmvy = besty / 2;  // truncation toward zero
-      uint32_t ref4;
-      if (PATH == kPathTable) {
-        const uint8_t* rrow = F.y + (size_t)clampi(s.by + lr + besty, 0, a.h - 1) * F.y_stride;
-        const int xr = s.bx + 4 * lq + bestx;
-        ref4 = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) ref4 |= (uint32_t)rrow[clampi(xr + j, 0, a.w - 1)] << (8 * j);
-      } else {
-        const uint8_t* rp4 = w + (lr + besty + 15) * kLumaPitch + 16 + 4 * lq + bestx;
-        const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
-        const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
-        ref4 = funnel(rw[0], rw[1], rsh);
-      }
+      const uint8_t* rp4 = w + (lr + besty + 15) * kLumaPitch + 16 + 4 * lq + bestx;
+      const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
+      const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
+      const uint32_t ref4 = funnel(rw[0], rw[1], rsh);
       if (PATH == kPathFast) {
         const ptrdiff_t cb = (ptrdiff_t)(s.cby + cmvy) * F.uv_stride + s.cbx + cmvx;
         ru = F.u[cb + coff];
@@ -398,6 +382,83 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     cur ^= 1;
   }
   if (pend) {
+    blend_px(s.au, ru, s.cu, s_lut);
+    blend_px(s.av, rv, s.cv, s_lut);
+  }
+}
+
+struct TableEntry {
+  int x = 0, y = 0;
+  uint32_t sad = 0xFFFFFFFFu;
+};
+
+// Lane j: the table entry of the wave's j-th frame (sad ~0 past the last).
+__device__ __forceinline__ TableEntry load_table_entry(const ArnrGroup& a, const MbState& s, int split, int K, int lane,
+                                                      const int32_t* table) {
+  const int nnb = a.filter_on ? a.n - 1 : 0;
+  const int count = split < nnb ? (nnb - split + K - 1) / K : 0;
+  TableEntry e;
+  if (lane < count) {
+    const int k = split + lane * K;
+    const int fi = k < a.center ? k : k + 1;
+    const int32_t* t = table + 3 * (((size_t)fi * a.brows + s.brow) * a.bcols + (s.bx >> 4));
+    e.x = t[0];
+    e.y = t[1];
+    e.sad = (uint32_t)t[2];
+  }
+  return e;
+}
+
+// The altref blend from a vector table (n x brows x bcols x {mvx, mvy,
+// sad}).  The wave's frames are k = split, split + K, ... of the nnb
+// neighbours; lane j loads the table entry of its j-th frame, and a ballot
+// of the entries under the SAD cutoff gives the frames that blend.  For each
+// of those, every lane gathers its 4 luma and 2 chroma neighbour bytes with
+// clamped coordinates (the vectors are not bounded by a staged window).
+// The kernel is bound by the count of memory instructions, so with 4-byte
+// aligned luma planes and strides and rows of 8 pixels or more (A4) the 4
+// luma bytes come from one dword-aligned 8-byte read whose start is the
+// clamped first column rounded down to a dword and moved left, when needed,
+// so that both dwords start inside the row; one byte permute then applies
+// the column clamp.
+template <bool A4, class FrameAt>
+__device__ __forceinline__ void table_loop(const ArnrGroup& a, const FrameAt& frame_at, MbState& s, int split, int K,
+                                           int lane, const uint8_t* s_lut, const TableEntry& e) {
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2), aligned(4)));
+  const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
+  const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
+  const int last4 = ((a.w - 1) & ~3) - 4;   // last start of an 8-byte read
+  // sad / (bw*bh) <= 12  <=>  sad < 13 * bw * bh (integer division)
+  uint64_t todo = __builtin_amdgcn_ballot_w64(e.sad < (kSadPerPixelCutoff + 1) * (uint32_t)(s.bw * s.bh));
+  while (todo) {
+    const int j = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    const int k = split + j * K;
+    const FrameDesc F = frame_at(k < a.center ? k : k + 1);
+    const int mvx = __builtin_amdgcn_readlane(e.x, j), mvy = __builtin_amdgcn_readlane(e.y, j);
+    const uint8_t* rrow = F.y + (size_t)clampi(s.by + lr + mvy, 0, a.h - 1) * F.y_stride;
+    const int xr = s.bx + 4 * lq + mvx;
+    uint32_t r4;
+    if (A4) {
+      const int p0 = clampi(xr, 0, a.w - 1);
+      const int a0 = min(p0 & ~3, last4);
+      const uint32_t sel = (uint32_t)(p0 - a0) | (uint32_t)(clampi(xr + 1, 0, a.w - 1) - a0) << 8 |
+                           (uint32_t)(clampi(xr + 2, 0, a.w - 1) - a0) << 16 |
+                           (uint32_t)(clampi(xr + 3, 0, a.w - 1) - a0) << 24;
+      const u32x2 d = *reinterpret_cast<const u32x2*>(rrow + a0);
+      r4 = __builtin_amdgcn_perm(d.y, d.x, sel);
+    } else {
+      r4 = (uint32_t)rrow[clampi(xr, 0, a.w - 1)] | (uint32_t)rrow[clampi(xr + 1, 0, a.w - 1)] << 8 |
+           (uint32_t)rrow[clampi(xr + 2, 0, a.w - 1)] << 16 | (uint32_t)rrow[clampi(xr + 3, 0, a.w - 1)] << 24;
+    }
+    const int cmvx = mvx / 2, cmvy = mvy / 2;  // truncation toward zero
+    const size_t o =
+        (size_t)clampi(s.cby + cr + cmvy, 0, a.ch - 1) * F.uv_stride + clampi(s.cbx + cc + cmvx, 0, a.cw - 1);
+    const uint32_t ru = F.u[o], rv = F.v[o];
+    blend_px(s.ay0, r4 & 255, s.ctr4 & 255, s_lut);
+    blend_px(s.ay1, (r4 >> 8) & 255, (s.ctr4 >> 8) & 255, s_lut);
+    blend_px(s.ay2, (r4 >> 16) & 255, (s.ctr4 >> 16) & 255, s_lut);
+    blend_px(s.ay3, r4 >> 24, s.ctr4 >> 24, s_lut);
     blend_px(s.au, ru, s.cu, s_lut);
     blend_px(s.av, rv, s.cv, s_lut);
   }
@@ -429,11 +490,6 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int mi = wave % M, split = wave / M;
 
-  for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = wt4[i];
-  if (K > 1)
-    for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
-  __syncthreads();
-
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
   MbState s;
@@ -449,6 +505,10 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   s.ay0 = s.ay1 = s.ay2 = s.ay3 = s.au = s.av = 0;
   s.ctr4 = s.cu = s.cv = 0;
 
+  // The table entries and centre pixels are loaded before the LUT and the
+  // barrier, so that their latency overlaps the LUT's.
+  TableEntry e;
+  if (TABLE && active) e = load_table_entry(a, s, split, K, lane, table);
   if (active) {
     const FrameDesc C = frame_at(a.center);
     // centre pixels of this lane's blend positions (zero outside the frame)
@@ -465,6 +525,13 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
       s.cu = C.u[o];
       s.cv = C.v[o];
     }
+  }
+  for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = wt4[i];
+  if (K > 1)
+    for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
+  __syncthreads();
+
+  if (active) {
     uint8_t* win = s_win + wave * kWaveWinBytes;
     // DMA: every window piece wholly inside or wholly outside the frame's
     // columns; fast: the whole window and every chroma read at +-7 inside
@@ -472,14 +539,16 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
     const bool dma = a.aligned16 && a.uniform_strides && (cols_inside || (a.w & 15) == 0);
     const bool fast = dma && cols_inside && s.by >= 15 && s.by + 31 <= a.h && s.cbx >= 7 && s.cbx + 15 <= a.cw &&
                       s.cby >= 7 && s.cby + 15 <= a.ch;
-    if (TABLE)
-      frame_loop<kPathTable>(a, frame_at, s, split, K, lane, win, s_lut, table);
+    if (TABLE && a.aligned4 && a.w >= 8)
+      table_loop<true>(a, frame_at, s, split, K, lane, s_lut, e);
+    else if (TABLE)
+      table_loop<false>(a, frame_at, s, split, K, lane, s_lut, e);
     else if (fast)
-      frame_loop<kPathFast>(a, frame_at, s, split, K, lane, win, s_lut, table);
+      frame_loop<kPathFast>(a, frame_at, s, split, K, lane, win, s_lut);
     else if (dma)
-      frame_loop<kPathDma>(a, frame_at, s, split, K, lane, win, s_lut, table);
+      frame_loop<kPathDma>(a, frame_at, s, split, K, lane, win, s_lut);
     else
-      frame_loop<kPathGather>(a, frame_at, s, split, K, lane, win, s_lut, table);
+      frame_loop<kPathGather>(a, frame_at, s, split, K, lane, win, s_lut);
   }
 
   if (K > 1) {

@@ -168,13 +168,23 @@ def test_table_altref_from_exhaustive_vectors(gpu, oracle, radius):
     got = table_altref(gpu, dev, g.center, 3, tab)
     for name, a, b in zip("yuv", got, want):
         assert np.array_equal(a, b), name
-    # large vectors and ragged geometry
-    tuples = tuples_of(45, 37, 3, 17, 3)
-    rng = np.random.default_rng(3)
-    t2 = np.zeros((3, 3, 3, 3), np.int32)
-    t2[..., :2] = rng.integers(-80, 81, (3, 3, 3, 2))
-    t2[..., 2] = rng.integers(0, 4000, (3, 3, 3))
-    st, want = oracle.build_arnr_altref_mv(tuples, 1, 5, t2)
-    got = table_altref(gpu, Dev(tuples), 1, 5, t2)
+
+
+# (w, h, pad): 4-byte strides with a ragged right edge (dword reads), odd
+# strides and rows under 8 pixels (byte reads)
+@pytest.mark.parametrize("geom", [(45, 37, 3), (13, 7, 0), (6, 20, 2), (37, 29, 0), (8, 9, 0), (64, 33, 0)],
+                         ids=lambda g: "x".join(map(str, g)))
+def test_table_altref_large_vectors_ragged(gpu, oracle, geom):
+    w, h, pad = geom
+    tuples = tuples_of(w, h, 5, 17, pad)
+    rng = np.random.default_rng(w * h)
+    bh, bw = (h + 15) // 16, (w + 15) // 16
+    t2 = np.zeros((5, bh, bw, 3), np.int32)
+    t2[..., :2] = rng.integers(-80, 81, (5, bh, bw, 2))
+    t2[..., :2][rng.random((5, bh, bw)) < 0.5] = rng.integers(-3, 4, 2)
+    t2[..., 2] = rng.integers(0, 20 * min(w, 16) * min(h, 16), (5, bh, bw))
+    st, want = oracle.build_arnr_altref_mv(tuples, 2, 5, t2)
+    assert st == 0
+    got = table_altref(gpu, Dev(tuples), 2, 5, t2)
     for name, a, b in zip("yuv", got, want):
         assert np.array_equal(a, b), name
