@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
   __shared__ __align__(16) uint32_t s_win[4][WDW];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const int per_frame = a.brows * a.bcols;
-  const int rest = blockIdx.x * 4 + wave;
+  const int rest = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
   if (rest >= per_frame) return;  // whole wave; no workgroup barrier follows
   const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
   const int bx = bcol * 16, by = brow * 16;
@@ -258,6 +258,14 @@ __global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
   const int nnb = a.n - 1;
   uint32_t* win = s_win[wave];
   const int dy = l >> 3, j = (l >> 2) & 1, h = l & 3;
+  // lane k < nnb holds neighbour k's seed
+  int my_sx = 0, my_sy = 0;
+  if (a.seeds && l < nnb) {
+    const int fi = l < a.center ? l : l + 1;
+    const size_t tab = 3 * ((size_t)fi * per_frame + rest);
+    my_sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
+    my_sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
+  }
   // the centre block (zero outside it) staged through LDS once, one dword
   // per lane, then the lane's source rows 4h .. 4h + 3 kept in registers
   {
@@ -280,14 +288,6 @@ __global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
 #pragma unroll
   for (int rr = 0; rr < 4; rr++) src[rr] = reinterpret_cast<const uint4*>(win)[4 * h + rr];
   wave_lds_sync();  // the source is read before the first window overwrites it
-  // lane k < nnb holds neighbour k's seed
-  int my_sx = 0, my_sy = 0;
-  if (a.seeds && l < nnb) {
-    const int fi = l < a.center ? l : l + 1;
-    const size_t tab = 3 * ((size_t)fi * per_frame + rest);
-    my_sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
-    my_sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
-  }
   // window dwords l and l + 64 (< 120): row s / 6, dword s % 6
   const int wy0 = l / PD, wd0 = l - PD * wy0;
   const int s1 = min(l + 64, WDW - 1), wy1 = s1 / PD, wd1 = s1 - PD * wy1;
@@ -371,25 +371,23 @@ __global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
     } else {
       lo = hi = 0xFFFFFFFFu;
     }
-    const uint32_t lane_sad = min(min(lo & 0xFFFFu, lo >> 16), min(hi & 0xFFFFu, hi >> 16));
-    const uint32_t best_sad = wave_min_any(lane_sad);
     const int mvy = sy - R + dy;
     const int mvx0 = sx - R + 4 * j - shift0;  // mvx of candidate i is mvx0 + i
-    // tie key of candidate i: (|mvx| + |mvy|) << 6 | dy << 3 | dxi
-    const uint32_t tie_hi = (uint32_t)abs(mvy), tie_lo = (uint32_t)((dy << 3) + 4 * j - shift0);
-    uint32_t lane_tie = 0xFFFFFFFFu;
+    const int l1_base = abs(sx) + abs(sy) - 2 * R;
+    const uint32_t key_lo = (uint32_t)((dy << 3) + 4 * j - shift0);
+    uint32_t lane_key = 0xFFFFFFFFu;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const uint32_t sad = ((i < 2 ? lo : hi) >> (16 * (i & 1))) & 0xFFFFu;
-      const uint32_t tie = (((uint32_t)abs(mvx0 + i) + tie_hi) << 6) | (tie_lo + (uint32_t)i);
-      if (sad == best_sad) lane_tie = min(lane_tie, tie);
+      const uint32_t rel = (uint32_t)(abs(mvx0 + i) + abs(mvy) - l1_base);
+      lane_key = min(lane_key, (sad << 10) | (rel << 6) | (key_lo + (uint32_t)i));
     }
-    const uint32_t best_tie = wave_min_any(lane_tie);
+    const uint32_t best = wave_min_any(lane_key);
     if (l == 0) {
       const size_t tab = 3 * ((size_t)(k < a.center ? k : k + 1) * per_frame + rest);
-      a.out[tab] = sx - R + (int)(best_tie & 7);
-      a.out[tab + 1] = sy - R + (int)((best_tie >> 3) & 7);
-      a.out[tab + 2] = (int32_t)best_sad;
+      a.out[tab] = sx - R + (int)(best & 7);
+      a.out[tab + 1] = sy - R + (int)((best >> 3) & 7);
+      a.out[tab + 2] = (int32_t)(best >> 10);
     }
   }
 }
