@@ -63,6 +63,9 @@ __device__ __forceinline__ uint32_t load_clamped4(const uint8_t* row, int x0, in
   return v;
 }
 
+using LdsBytes = __attribute__((address_space(3))) const uint8_t*;
+using LdsWords = __attribute__((address_space(3))) const uint32_t*;
+
 __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -105,7 +108,10 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
   auto sad_at = [&](int off) -> uint32_t {
     const uint8_t* p = row0 + off;
     const uint32_t sh = (uint32_t)(uintptr_t)p & 3;
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(p - sh);
+    // One opaque base for all ten reads: their offsets (at most 400 bytes)
+    // then fit the 8-bit dword offsets of ds_read2_b32.
+    LdsWords w = (LdsWords)((LdsBytes)p - sh);
+    asm volatile("" : "+v"(w));
     constexpr int P = 8 * G::lp >> 2;  // row kk + 8
     const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3], a4 = w[4];
     const uint32_t b0 = w[P], b1 = w[P + 1], b2 = w[P + 2], b3 = w[P + 3], b4 = w[P + 4];
@@ -221,7 +227,7 @@ struct MbState {
 
 // x = ref | 1 << 20: sad(x, ctr | 1 << 20) = |ref - ctr|, and wt * x adds wt
 // to the weight field and wt * ref to the sum.
-__device__ __forceinline__ void blend_px(uint32_t& acc, uint32_t ref, uint32_t ctr, const uint8_t* lut) {
+__device__ __forceinline__ void blend_px(uint32_t& acc, uint32_t ref, uint32_t ctr, LdsBytes lut) {
   const uint32_t x = ref | kAccOne;
   acc += lut[sad4(x, ctr | kAccOne, 0)] * x;
 }
@@ -244,7 +250,7 @@ enum { kPathFast = 1, kPathDma = 2, kPathGather = 3 };
 
 template <int PATH, class FrameAt>
 __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& frame_at, MbState& s, int split,
-                                           int K, int lane, uint8_t* win, const uint8_t* s_lut) {
+                                           int K, int lane, uint8_t* win, LdsBytes s_lut) {
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
   const int nnb = a.filter_on ? a.n - 1 : 0;
@@ -423,7 +429,7 @@ __device__ __forceinline__ TableEntry load_table_entry(const ArnrGroup& a, const
 // the column clamp.
 template <bool A4, class FrameAt>
 __device__ __forceinline__ void table_loop(const ArnrGroup& a, const FrameAt& frame_at, MbState& s, int split, int K,
-                                           int lane, const uint8_t* s_lut, const TableEntry& e) {
+                                           int lane, LdsBytes s_lut, const TableEntry& e) {
   typedef uint32_t u32x2 __attribute__((ext_vector_type(2), aligned(4)));
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
@@ -475,15 +481,20 @@ __device__ __forceinline__ void table_loop(const ArnrGroup& a, const FrameAt& fr
 // window pairs.  Every wave of the workgroup calls this (inactive MBs past the
 // right frame edge included: they take part in the barriers); the caller
 // synchronises the workgroup before reusing LDS.
+// The weight LUT sits in static LDS: its address is a compile-time constant
+// that folds into the ds_read offsets of the blend.  The dynamic LDS (split
+// accumulators, windows) follows it.
+__shared__ __align__(16) uint32_t s_arnr_lut[64];
+
 template <bool TABLE, class FrameAt>
 __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* wt4, const FrameAt& frame_at,
                                             int brow, int scol, int M, uint8_t* smem,
                                             const int32_t* table = nullptr) {
   const int nwaves = blockDim.x >> 6;
   const int K = nwaves / M;
-  uint8_t* s_lut = smem;
-  uint32_t* s_acc = reinterpret_cast<uint32_t*>(smem + 256);
-  uint8_t* s_win = smem + 256 + (K > 1 ? M * 384 * 4 : 0);
+  LdsBytes s_lut = (LdsBytes)s_arnr_lut;
+  uint32_t* s_acc = reinterpret_cast<uint32_t*>(smem);
+  uint8_t* s_win = smem + (K > 1 ? M * 384 * 4 : 0);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -526,7 +537,7 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
       s.cv = C.v[o];
     }
   }
-  for (int i = tid; i < 64; i += blockDim.x) reinterpret_cast<uint32_t*>(s_lut)[i] = wt4[i];
+  for (int i = tid; i < 64; i += blockDim.x) s_arnr_lut[i] = wt4[i];
   if (K > 1)
     for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
   __syncthreads();
@@ -599,7 +610,9 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   }
 }
 
-__global__ void __launch_bounds__(1024) arnr_fused_kernel(const ArnrArgs a) {
+// At most 64 VGPRs: 8 waves per SIMD.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+arnr_fused_kernel(const ArnrArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int total = a.g.strips_per_row * a.g.brows;
   const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
@@ -707,10 +720,15 @@ __global__ void zero_u32_kernel(uint32_t* p, int n) {
 }
 
 
-// LDS of one workgroup: LUT, the split accumulators when K > 1, and one
+// Dynamic LDS of one workgroup: the split accumulators when K > 1, and one
 // window per wave unless the vectors come from a table.
+static size_t arnr_dynamic_lds_bytes(int k, int m, bool table) {
+  return (k > 1 ? (size_t)m * 384 * 4 : 0) + (table ? 0 : (size_t)m * k * kWaveWinBytes);
+}
+
+// All LDS of one workgroup: the static LUT and the dynamic part.
 size_t arnr_lds_bytes(int k, int m, bool table) {
-  return 256 + (k > 1 ? (size_t)m * 384 * 4 : 0) + (table ? 0 : (size_t)m * k * kWaveWinBytes);
+  return sizeof(s_arnr_lut) + arnr_dynamic_lds_bytes(k, m, table);
 }
 
 int arnr_supported_mbs(int m) { return m >= 8 ? 8 : (m >= 4 ? 4 : (m >= 2 ? 2 : 1)); }
@@ -719,14 +737,14 @@ hipError_t launch_arnr_fused(const ArnrArgs& a, int k, hipStream_t stream) {
   const int blocks = a.g.strips_per_row * a.g.brows;
   const int m = a.mbs_per_wg;
   if (m < 1 || k < 1 || m * k > 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(arnr_fused_kernel, dim3(blocks), dim3(64 * m * k), arnr_lds_bytes(k, m, false), stream, a);
+  hipLaunchKernelGGL(arnr_fused_kernel, dim3(blocks), dim3(64 * m * k), arnr_dynamic_lds_bytes(k, m, false), stream, a);
   return hipGetLastError();
 }
 
 int arnr_batch_workgroups_per_cu(int k, int m) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, arnr_batch_kernel, 64 * m * k,
-                                                   arnr_lds_bytes(k, m, false)) != hipSuccess)
+                                                   arnr_dynamic_lds_bytes(k, m, false)) != hipSuccess)
     return 0;
   return per_cu;
 }
@@ -734,7 +752,7 @@ int arnr_batch_workgroups_per_cu(int k, int m) {
 hipError_t launch_arnr_batch(const ArnrBatchArgs& b, int k, int workgroups, hipStream_t stream) {
   const int m = b.mbs_per_wg;
   if (m < 1 || k < 1 || m * k > 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(arnr_batch_kernel, dim3(workgroups), dim3(64 * m * k), arnr_lds_bytes(k, m, false), stream,
+  hipLaunchKernelGGL(arnr_batch_kernel, dim3(workgroups), dim3(64 * m * k), arnr_dynamic_lds_bytes(k, m, false), stream,
                      b);
   return hipGetLastError();
 }
@@ -743,7 +761,7 @@ hipError_t launch_arnr_table(const ArnrArgs& a, const int32_t* table, int k, hip
   const int blocks = a.g.strips_per_row * a.g.brows;
   const int m = a.mbs_per_wg;
   if (m < 1 || k < 1 || m * k > 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(arnr_table_kernel, dim3(blocks), dim3(64 * m * k), arnr_lds_bytes(k, m, true), stream, a,
+  hipLaunchKernelGGL(arnr_table_kernel, dim3(blocks), dim3(64 * m * k), arnr_dynamic_lds_bytes(k, m, true), stream, a,
                      table);
   return hipGetLastError();
 }
