@@ -496,7 +496,11 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   uint32_t* s_acc = reinterpret_cast<uint32_t*>(smem);
   uint8_t* s_win = smem + (K > 1 ? M * 384 * 4 : 0);
 
-  const int tid = threadIdx.x;
+  // Opaque to the compiler, so that in the persistent batch loop the lane
+  // constants derived from it are recomputed per unit instead of being
+  // hoisted and held across units (96 -> 72 VGPRs there).
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int mi = wave % M, split = wave / M;
