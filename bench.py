@@ -247,9 +247,12 @@ def bench_4k(args) -> int:
     if not wpath.exists():
         wpath = WEIGHTS_DIR / "proposer_1080p.safetensors"
     prop = Proposer(ctx, params=Proposer.load_params(wpath))
-    r = evaluate_group(ctx, prop, g, radius=64, refine=2, reps=max(1, min(args.steps, 3)))
+    ppath = WEIGHTS_DIR / "proposer_4k_l2.safetensors"
+    pprop = Proposer(ctx, params=Proposer.load_params(ppath)) if ppath.exists() else None
+    r = evaluate_group(ctx, prop, g, radius=64, refine=2, reps=max(1, min(args.steps, 3)), pyramid_prop=pprop,
+                       pyramid_levels=2)
     anchors = 240 * 135
-    le = r["learned"]
+    le = r["learned_pyramid"] if pprop is not None else r["learned"]
     result = {
         "metric": "4K learned-seeded search + sub-pel refine + six-tap MC, anchor-MB/s (vs exhaustive +-64)",
         "value": le["anchor_mb_per_s"],
@@ -264,12 +267,17 @@ def bench_4k(args) -> int:
         "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": "3840x2160 I420, 11-frame group (centre 5), 8x8 region motion [-48,48] px/frame, "
-                               "40 occluders/frame, sigma 4 noise", "anchor_mbs": anchors, "weights": wpath.name,
-                   "parallelism": "single GPU"},
+                               "40 occluders/frame, sigma 4 noise", "anchor_mbs": anchors,
+                   "search": ("coarse to fine: proposals at quarter resolution, exact +-2 at quarter, half and full"
+                              if pprop is not None else "proposals at full resolution, exact +-2"),
+                   "weights": ppath.name if pprop is not None else wpath.name,
+                   "single_level_weights": wpath.name, "parallelism": "single GPU"},
         "report": r,
         "stream_bytes_qi44": "not measurable here (reference encoder not buildable); residual SAD/SSE reported",
     }
     prop.close()
+    if pprop is not None:
+        pprop.close()
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -167,6 +167,24 @@ int vp8g_build_arnr_altref_table_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, siz
                                      uint8_t *out_y, uint8_t *out_u, uint8_t *out_v,
                                      vp8g_err *err);
 
+/* ---- coarse-to-fine pyramid ----
+ * No counterpart in the reference: the two steps that let the short exact
+ * search (radius 2) follow a proposal made at a coarser level.
+ * vp8g_downsample2_i420_dev writes one pyramid level of each of the
+ * n_frames (<= 64) frames src[i] into dst[i] (device planes; all src frames
+ * share one size w x h): dst[i] is ceil(w/2) x ceil(h/2), and each of its
+ * Y, U and V samples is the rounded mean (a + b + c + d + 2) >> 2 of the
+ * 2x2 source block, coordinates clamped to the plane.  dst[i].width and
+ * .height must be ceil(w/2) and ceil(h/2).  vp8g_upsample_table_dev maps an alignment
+ * table of n_frames x coarse_h x coarse_w blocks onto n_frames x fine_h x
+ * fine_w blocks: fine block (r, c) takes coarse block (min(r/2, coarse_h-1),
+ * min(c/2, coarse_w-1)) with both vector components doubled and sad 0 (the
+ * seeds of vp8g_block_search_dev one level finer).  All on the context's
+ * stream. */
+int vp8g_downsample2_i420_dev(vp8g_ctx *ctx, const vp8g_i420 *src, const vp8g_i420 *dst, size_t n_frames);
+int vp8g_upsample_table_dev(vp8g_ctx *ctx, const int32_t *coarse, size_t n_frames, int coarse_w,
+                            int coarse_h, int32_t *fine, int fine_w, int fine_h);
+
 /* ---- learned motion proposer ----
  * No counterpart in the reference: a ~60k-parameter BF16 CNN (hand-written
  * MFMA kernels, fp32 accumulation and fp32 master parameters) that

@@ -496,6 +496,47 @@ int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t
   return hip_ok(launch_block_search(a, radius, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
+int vp8g_downsample2_i420_dev(vp8g_ctx* c, const vp8g_i420* src, const vp8g_i420* dst, size_t n) {
+  if (!c || !src || !dst || n == 0 || n > (size_t)VP8G_MAX_FRAMES) return VP8G_E_ARG;
+  const size_t w = src[0].width, h = src[0].height;
+  if (w == 0 || h == 0 || w > 0x3FFF || h > 0x3FFF) return VP8G_E_INVALID_DIMENSIONS;
+  for (size_t i = 0; i < n; i++) {
+    if (src[i].width != w || src[i].height != h) return VP8G_E_REF_DIMS_MISMATCH;
+    if (dst[i].width != (w + 1) / 2 || dst[i].height != (h + 1) / 2) return VP8G_E_REF_DIMS_MISMATCH;
+    const vp8g_i420* fs[2] = {src + i, dst + i};
+    for (const vp8g_i420* f : fs) {
+      if (!f->y || !f->u || !f->v || f->y_stride < f->width || f->uv_stride < (f->width + 1) / 2) return VP8G_E_ARG;
+      if (f->y_stride > 0xFFFFFFFFu || f->uv_stride > 0xFFFFFFFFu) return VP8G_E_ARG;
+    }
+  }
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  for (size_t i0 = 0; i0 < n; i0 += kDownsampleFrames) {
+    const int cnt = (int)std::min<size_t>(kDownsampleFrames, n - i0);
+    Downsample2Args a;
+    memset(&a, 0, sizeof(a));
+    a.w = (int)w;
+    a.h = (int)h;
+    for (int k = 0; k < cnt; k++) {
+      const vp8g_i420 &s = src[i0 + k], &d = dst[i0 + k];
+      // dst's planes are the outputs (the descriptor type carries const pointers)
+      a.f[k] = Downsample2Frame{s.y, s.u, s.v, const_cast<uint8_t*>(d.y), const_cast<uint8_t*>(d.u),
+                                const_cast<uint8_t*>(d.v), (uint32_t)s.y_stride, (uint32_t)s.uv_stride,
+                                (uint32_t)d.y_stride, (uint32_t)d.uv_stride};
+    }
+    if (!hip_ok(launch_downsample2(a, cnt, c->stream))) return VP8G_E_DEVICE;
+  }
+  return VP8G_OK;
+}
+
+int vp8g_upsample_table_dev(vp8g_ctx* c, const int32_t* coarse, size_t n, int cw, int ch, int32_t* fine, int fw,
+                            int fh) {
+  if (!c || !coarse || !fine || n == 0 || n > (size_t)VP8G_MAX_FRAMES) return VP8G_E_ARG;
+  if (cw <= 0 || ch <= 0 || fw <= 0 || fh <= 0 || fw > 0x3FFF || fh > 0x3FFF) return VP8G_E_ARG;
+  if (((uintptr_t)coarse | (uintptr_t)fine) & 3) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  return hip_ok(launch_upsample_table(coarse, cw, ch, fine, fw, fh, (int)n, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
 int vp8g_build_arnr_altref_table_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
                                      uint8_t strength, const int32_t* table, uint8_t* oy,
                                      uint8_t* ou, uint8_t* ov, vp8g_err* err) {

@@ -392,6 +392,85 @@ __global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- pyramid
+// One level of a resolution pyramid for coarse-to-fine search: each output
+// pixel is the rounded mean of a 2x2 input block, input coordinates clamped
+// to the plane (odd sizes repeat the last row or column).  blockIdx.z =
+// 3 * frame + plane (Y, U, V); one thread per 4 consecutive output pixels of
+// a row.
+__global__ void __launch_bounds__(256) downsample2_kernel(const Downsample2Args a) {
+  const int fi = blockIdx.z / 3, p = blockIdx.z - 3 * fi;
+  const Downsample2Frame& F = a.f[fi];
+  const uint8_t* src = p == 0 ? F.src_y : (p == 1 ? F.src_u : F.src_v);
+  uint8_t* dst = p == 0 ? F.dst_y : (p == 1 ? F.dst_u : F.dst_v);
+  const int w = p == 0 ? a.w : (a.w + 1) >> 1, h = p == 0 ? a.h : (a.h + 1) >> 1;
+  const uint32_t ss = p == 0 ? F.src_y_stride : F.src_uv_stride;
+  const uint32_t ds = p == 0 ? F.dst_y_stride : F.dst_uv_stride;
+  const int ow = (w + 1) >> 1, oh = (h + 1) >> 1;
+  const int x4 = 4 * (blockIdx.x * 64 + (threadIdx.x & 63)), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x4 >= ow || y >= oh) return;
+  const uint8_t* r0 = src + (size_t)(2 * y) * ss;
+  const uint8_t* r1 = src + (size_t)min(2 * y + 1, h - 1) * ss;
+  uint8_t* o = dst + (size_t)y * ds;
+  // Interior: the 8 source bytes of each row are one aligned uint2 (8-byte
+  // aligned planes and strides), the 4 outputs one aligned dword.
+  if (((uintptr_t)r0 & 7) == 0 && ((uintptr_t)r1 & 7) == 0 && ((uintptr_t)(o + x4) & 3) == 0 && 2 * x4 + 8 <= w &&
+      x4 + 4 <= ow) {
+    const uint2 a = *reinterpret_cast<const uint2*>(r0 + 2 * x4), b = *reinterpret_cast<const uint2*>(r1 + 2 * x4);
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t wa = k < 2 ? a.x : a.y, wb = k < 2 ? b.x : b.y;
+      const int sh = 16 * (k & 1);
+      const uint32_t sum = ((wa >> sh) & 255) + ((wa >> (sh + 8)) & 255) + ((wb >> sh) & 255) + ((wb >> (sh + 8)) & 255);
+      out |= ((sum + 2) >> 2) << (8 * k);
+    }
+    *reinterpret_cast<uint32_t*>(o + x4) = out;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int x = x4 + k;
+    if (x < ow) {
+      const int x0 = 2 * x, x1 = min(2 * x + 1, w - 1);
+      o[x] = (uint8_t)((r0[x0] + r0[x1] + r1[x0] + r1[x1] + 2) >> 2);
+    }
+  }
+}
+
+// Seeds for the next finer level: fine block (r, c) of frame f takes the
+// vector of coarse block (min(r / 2, cbh - 1), min(c / 2, cbw - 1)), both
+// components doubled; the third entry is 0.
+__global__ void __launch_bounds__(256) upsample_table_kernel(const int32_t* coarse, int cbw, int cbh, int32_t* fine,
+                                                             int fbw, int fbh, int total) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const int per = fbw * fbh;
+  const int f = i / per, rem = i - f * per;
+  const int r = rem / fbw, c = rem - r * fbw;
+  const int32_t* src = coarse + 3 * (((size_t)f * cbh + min(r >> 1, cbh - 1)) * cbw + min(c >> 1, cbw - 1));
+  int32_t* dst = fine + 3 * (size_t)i;
+  dst[0] = 2 * src[0];
+  dst[1] = 2 * src[1];
+  dst[2] = 0;
+}
+
+hipError_t launch_downsample2(const Downsample2Args& a, int frames, hipStream_t stream) {
+  if (frames <= 0 || frames > kDownsampleFrames) return hipErrorInvalidValue;
+  const int ow = (a.w + 1) >> 1, oh = (a.h + 1) >> 1;
+  hipLaunchKernelGGL(downsample2_kernel, dim3((ow + 255) / 256, (oh + 3) / 4, 3 * frames), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_upsample_table(const int32_t* coarse, int cbw, int cbh, int32_t* fine, int fbw, int fbh, int n,
+                                 hipStream_t stream) {
+  const int total = n * fbw * fbh;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(upsample_table_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, coarse, cbw, cbh, fine,
+                     fbw, fbh, total);
+  return hipGetLastError();
+}
+
 bool block_search_supported(int radius) { return radius == 2 || radius == 16 || radius == 64; }
 
 hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream) {

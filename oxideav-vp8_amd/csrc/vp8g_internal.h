@@ -91,6 +91,20 @@ struct SearchArgs {
   int aligned4;           // every luma plane pointer and stride is a multiple of 4
   int total_blocks;       // (n - 1) * brows * bcols
 };
+// Pyramid level (2x2 rounded mean of Y, U and V) and seed upsampling.
+constexpr int kDownsampleFrames = 16;  // frames per downsample launch
+struct Downsample2Frame {
+  const uint8_t *src_y, *src_u, *src_v;
+  uint8_t *dst_y, *dst_u, *dst_v;
+  uint32_t src_y_stride, src_uv_stride, dst_y_stride, dst_uv_stride;
+};
+struct Downsample2Args {
+  Downsample2Frame f[kDownsampleFrames];
+  int w, h;  // source luma size (shared by the frames)
+};
+hipError_t launch_downsample2(const Downsample2Args& a, int frames, hipStream_t stream);
+hipError_t launch_upsample_table(const int32_t* coarse, int cbw, int cbh, int32_t* fine, int fbw, int fbh, int n,
+                                 hipStream_t stream);
 bool block_search_supported(int radius);
 hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream);
 

@@ -108,6 +108,7 @@ EXPORTS = (
     "vp8g_arnr_batch_create", "vp8g_arnr_batch_run", "vp8g_arnr_batch_shape",
     "vp8g_arnr_batch_destroy", "vp8g_ctx_load_tuning", "vp8g_tuning_lookup", "vp8g_arnr_shape",
     "vp8g_block_search_dev", "vp8g_build_arnr_altref_table_dev",
+    "vp8g_downsample2_i420_dev", "vp8g_upsample_table_dev",
     "vp8g_proposer_param_count", "vp8g_proposer_create", "vp8g_proposer_destroy",
     "vp8g_proposer_set_params", "vp8g_proposer_get_params", "vp8g_proposer_get_grads",
     "vp8g_proposer_prepare_dev", "vp8g_proposer_forward_dev", "vp8g_proposer_propose_dev",
@@ -151,6 +152,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
                                      C.POINTER(i32), C.POINTER(i32)]),
         "vp8g_arnr_shape": (i32, [vp, C.c_uint32, C.c_uint32, sz, u8, C.POINTER(i32), C.POINTER(i32)]),
         "vp8g_block_search_dev": (i32, [vp, C.POINTER(_I420), sz, sz, i32, vp, vp, C.POINTER(_Err)]),
+        "vp8g_downsample2_i420_dev": (i32, [vp, C.POINTER(_I420), C.POINTER(_I420), sz]),
+        "vp8g_upsample_table_dev": (i32, [vp, vp, sz, i32, i32, vp, i32, i32]),
         "vp8g_build_arnr_altref_table_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, vp,
                                                    C.POINTER(_Err)]),
         "vp8g_proposer_param_count": (sz, []),
@@ -429,6 +432,22 @@ class Context:
         err = _Err()
         st = self.lib.vp8g_block_search_dev(self.h, descs, n, center, radius, seeds, out, C.byref(err))
         self._raise(st, "vp8g_block_search_dev")
+
+    def downsample2_dev(self, src, dst, n: int = 1) -> None:
+        """One pyramid level of ``n`` frames: ``dst`` (``_I420`` descriptors of
+        ceil(w/2) x ceil(h/2), device planes; one descriptor or an array)
+        receives the 2x2 rounded means of ``src``'s Y, U and V.  Enqueues
+        only."""
+        sp = src if isinstance(src, C.Array) else C.pointer(src)
+        dp = dst if isinstance(dst, C.Array) else C.pointer(dst)
+        self._raise(self.lib.vp8g_downsample2_i420_dev(self.h, sp, dp, n), "vp8g_downsample2_i420_dev")
+
+    def upsample_table_dev(self, coarse: int, n: int, coarse_w: int, coarse_h: int, fine: int, fine_w: int,
+                           fine_h: int) -> None:
+        """Seeds one pyramid level finer: fine block (r, c) takes coarse block
+        (r // 2, c // 2) (clamped) with its vector doubled.  Enqueues only."""
+        self._raise(self.lib.vp8g_upsample_table_dev(self.h, coarse, n, coarse_w, coarse_h, fine, fine_w, fine_h),
+                    "vp8g_upsample_table_dev")
 
     def build_arnr_altref_table_dev(self, descs, n: int, center: int, strength: int, table: int,
                                     oy: int, ou: int, ov: int) -> None:

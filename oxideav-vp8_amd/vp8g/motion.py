@@ -86,10 +86,67 @@ def subpel_and_mc(ctx: Context, planes, width: int, height: int, center: int, ta
             "mc_bytes": pairs * n_mb * (384 + 384 + 4)}
 
 
-def evaluate_group(ctx: Context, prop, g, radius: int = 64, refine: int = 2, reps: int = 3) -> dict:
+class Pyramid:
+    """Coarser levels of a device group for coarse-to-fine search: level
+    k + 1 is vp8g_downsample2_i420_dev of level k (level 0 = the group).
+    Each level holds its frames' planes, descriptors, block grid and two
+    alignment tables (seeds and search result)."""
+
+    def __init__(self, torch, ctx: Context, d, levels: int = 2):
+        from . import _I420
+        self.ctx, self.n = ctx, d.g.n
+        dev = torch.device("cuda:0")
+        w, h = d.g.width, d.g.height
+        self.levels = [{"w": w, "h": h, "descs": d.descs, "bw": d.bw, "bh": d.bh}]
+        for _ in range(levels):
+            w, h = (w + 1) // 2, (h + 1) // 2
+            cw, ch = (w + 1) // 2, (h + 1) // 2
+            planes = [(torch.empty(w * h, dtype=torch.uint8, device=dev),
+                       torch.empty(cw * ch, dtype=torch.uint8, device=dev),
+                       torch.empty(cw * ch, dtype=torch.uint8, device=dev)) for _ in range(self.n)]
+            descs = (_I420 * self.n)()
+            for i, (y, u, v) in enumerate(planes):
+                descs[i] = _I420(w, h, y.data_ptr(), u.data_ptr(), v.data_ptr(), w, cw)
+            self.levels.append({"w": w, "h": h, "descs": descs, "planes": planes,
+                                "bw": (w + 15) // 16, "bh": (h + 15) // 16})
+        for lv in self.levels:
+            shape = (self.n, lv["bh"], lv["bw"], 3)
+            lv["seeds"] = torch.zeros(shape, dtype=torch.int32, device=dev)
+            lv["out"] = torch.zeros(shape, dtype=torch.int32, device=dev)
+
+    def build(self) -> None:
+        """Enqueue the downsampling of every frame to every level."""
+        for k in range(1, len(self.levels)):
+            self.ctx.downsample2_dev(self.levels[k - 1]["descs"], self.levels[k]["descs"], self.n)
+
+    def search(self, prop, center: int, refine: int = 2) -> None:
+        """Proposals at the coarsest level, then at every level from the
+        coarsest to level 0 an exact +-refine search around seeds (the
+        proposals, then the coarser level's result with the vector doubled).
+        The result is level 0's "out" table."""
+        top = self.levels[-1]
+        prop.propose(top["descs"], self.n, center, top["seeds"].data_ptr(), 16 - refine)
+        for k in range(len(self.levels) - 1, -1, -1):
+            lv = self.levels[k]
+            if k < len(self.levels) - 1:
+                up = self.levels[k + 1]
+                self.ctx.upsample_table_dev(up["out"].data_ptr(), self.n, up["bw"], up["bh"], lv["seeds"].data_ptr(),
+                                            lv["bw"], lv["bh"])
+            self.ctx.block_search_dev(lv["descs"], self.n, center, refine, lv["out"].data_ptr(),
+                                      lv["seeds"].data_ptr())
+
+    @property
+    def result(self):
+        return self.levels[0]["out"]
+
+
+def evaluate_group(ctx: Context, prop, g, radius: int = 64, refine: int = 2, reps: int = 3, pyramid_prop=None,
+                   pyramid_levels: int = 2) -> dict:
     """Exhaustive +-radius vs learned-seeded (+-refine) whole-pel search on a
     host group, each followed by half/quarter-pel refinement and six-tap MC
-    of every macroblock of every non-centre frame."""
+    of every macroblock of every non-centre frame.  With `pyramid_prop` (a
+    proposer trained at that pyramid level) the learned search also runs
+    coarse to fine (Pyramid: building the levels is included in its time)."""
     import torch
     from .proposer import DeviceGroup, psnr
     stream = torch.cuda.ExternalStream(ctx.stream())
@@ -121,8 +178,20 @@ def evaluate_group(ctx: Context, prop, g, radius: int = 64, refine: int = 2, rep
            "matched_fraction": float(matched.mean()),
            "ms_search_exhaustive": t_ex, "ms_search_learned": t_pr + t_rf, "ms_propose": t_pr, "ms_refine": t_rf,
            "search_speedup": t_ex / (t_pr + t_rf)}
+    legs = [("exhaustive", ex, t_ex), ("learned", ref, t_pr + t_rf)]
+    if pyramid_prop is not None:
+        pyr = Pyramid(torch, ctx, d, levels=pyramid_levels)
+        t_build = timed(pyr.build)
+        t_psearch = timed(lambda: pyr.search(pyramid_prop, c, refine))
+        ctx.synchronize()
+        pr = pyr.result.cpu().numpy()[others]
+        psame = pr[..., 2] == e[..., 2]
+        out["pyramid"] = {"levels": pyramid_levels, "ms_build": t_build, "ms_search": t_psearch,
+                          "agreement": float(psame.mean()), "agreement_matched": float(psame[matched].mean()),
+                          "search_speedup": t_ex / (t_build + t_psearch)}
+        legs.append(("learned_pyramid", pyr.result, t_build + t_psearch))
     anchors = d.bw * d.bh
-    for name, tab, t_search in (("exhaustive", ex, t_ex), ("learned", ref, t_pr + t_rf)):
+    for name, tab, t_search in legs:
         r = subpel_and_mc(ctx, d.planes, g.width, g.height, c, tab, stream)
         total = t_search + r["ms_subpel"] + r["ms_mc"]
         r.update({"ms_total": total, "anchor_mb_per_s": anchors / (total * 1e-3),

@@ -179,26 +179,40 @@ class Dataset:
 
 def build_dataset(ctx: Context, prop: Proposer, groups: int, width: int = 1920, height: int = 1080,
                   n: int = 7, vmax: int = 12, occluders: int = 20, config: int = 30, clip0: int = 0,
-                  radius: int = 16, log=None) -> Dataset:
-    """`groups` synthetic groups -> (n - 1) * groups labelled pairs."""
+                  radius: int = 16, log=None, level: int = 0) -> Dataset:
+    """`groups` synthetic groups -> (n - 1) * groups labelled pairs.  With
+    level > 0 the groups are generated at width x height and the pairs and
+    labels are taken `level` pyramid levels down (vp8g_downsample2_i420_dev
+    applied `level` times), for a proposer that runs at that level."""
     import torch
     dev = torch.device("cuda:0")
+    full_w, full_h = width, height
+    for _ in range(level):
+        width, height = (width + 1) // 2, (height + 1) // 2
     bw, bh = (width + 15) // 16, (height + 15) // 16
     cells = bw * bh
     pairs = groups * (n - 1)
     inputs = torch.empty((pairs, 2, 4 * bh, 4 * bw), dtype=torch.uint8, device=dev)
     labels = torch.empty((pairs, cells, 2), dtype=torch.int32, device=dev)
     sads = torch.empty((pairs, cells), dtype=torch.int32, device=dev)
-    frames = torch.empty((n, height, width), dtype=torch.uint8, device=dev)
+    chain = []  # (frames, descs, dummy) from full resolution down to the level
+    w, h = full_w, full_h
+    for k in range(level + 1):
+        f = torch.empty((n, h, w), dtype=torch.uint8, device=dev)
+        chain.append((f,) + luma_descs(torch, f, w, h))
+        w, h = (w + 1) // 2, (h + 1) // 2
+    full = chain[0][0]
+    frames, descs, dummy = chain[-1]
     table = torch.zeros((n, bh, bw, 3), dtype=torch.int32, device=dev)
-    descs, dummy = luma_descs(torch, frames, width, height)
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(ctx.stream())
     center = n // 2
     others = torch.tensor([i for i in range(n) if i != center], device=dev)
     t0 = time.perf_counter()
     for g in range(groups):
-        synth_group_luma(ctx, frames.data_ptr(), None, width, height, n, vmax, occluders, clip0 + g, config)
+        synth_group_luma(ctx, full.data_ptr(), None, full_w, full_h, n, vmax, occluders, clip0 + g, config)
+        for k in range(1, level + 1):
+            ctx.downsample2_dev(chain[k - 1][1], chain[k][1], n)
         ctx.block_search_dev(descs, n, center, radius, table.data_ptr())
         k0 = g * (n - 1)
         prop.prepare(descs, n, center, inputs[k0].data_ptr())
@@ -210,7 +224,7 @@ def build_dataset(ctx: Context, prop: Proposer, groups: int, width: int = 1920, 
             ctx.synchronize()
             log(f"[dataset] {g + 1}/{groups} groups ({(g + 1) * (n - 1)} pairs) in {time.perf_counter() - t0:.1f}s")
     ctx.synchronize()
-    del dummy
+    del chain, dummy
     return Dataset(inputs, labels, sads, bw, bh)
 
 
@@ -387,6 +401,8 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--config", type=int, default=30, help="seed family of the training content")
     ap.add_argument("--eval-groups", type=int, default=2)
+    ap.add_argument("--level", type=int, default=0,
+                    help="pyramid level of the training pairs (groups are generated at width x height)")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -399,7 +415,7 @@ def main(argv=None) -> int:
     groups = (args.pairs + args.frames - 2) // (args.frames - 1)
     t0 = time.perf_counter()
     data = build_dataset(ctx, prop, groups, args.width, args.height, args.frames, args.vmax, args.occluders,
-                         args.config, radius=args.radius, log=log)
+                         args.config, radius=args.radius, log=log, level=args.level)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
     losses = train(prop, data, args.steps, args.batch, args.lr, args.seed, log=log,
@@ -408,13 +424,16 @@ def main(argv=None) -> int:
     del data
     meta = {"pairs": groups * (args.frames - 1), "steps": args.steps, "batch": args.batch, "lr": args.lr,
             "width": args.width, "height": args.height, "frames": args.frames, "vmax": args.vmax,
-            "occluders": args.occluders, "radius": args.radius, "config": args.config,
+            "occluders": args.occluders, "radius": args.radius, "config": args.config, "level": args.level,
             "init": args.init or "he", "final_loss": float(losses[-100:].mean())}
     prop.save(args.out, meta)
     log(f"[proposer] saved {args.out}")
-    evals = [_synth.arnr_group_regions(args.width, args.height, args.frames, args.vmax, args.occluders,
-                                       clip=i, config=3) for i in range(args.eval_groups)]
-    report = evaluate(ctx, prop, evals, radius=args.radius)
+    if args.level == 0:
+        evals = [_synth.arnr_group_regions(args.width, args.height, args.frames, args.vmax, args.occluders,
+                                           clip=i, config=3) for i in range(args.eval_groups)]
+        report = evaluate(ctx, prop, evals, radius=args.radius)
+    else:  # a pyramid-level proposer is evaluated through the pyramid search (vp8g.motion)
+        report = {}
     report.update({"train": meta, "dataset_s": t_data, "train_s": t_train})
     print(json.dumps(report), flush=True)
     prop.close()
