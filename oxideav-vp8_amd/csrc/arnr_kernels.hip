@@ -248,9 +248,11 @@ __device__ __forceinline__ void blend_px(uint32_t& acc, uint32_t ref, uint32_t c
 // next search.
 enum { kPathFast = 1, kPathDma = 2, kPathGather = 3 };
 
-template <int PATH, class FrameAt>
+// `lut_sync` fills the weight LUT and synchronises the workgroup; every wave
+// calls it exactly once, here after its first window DMA is in flight.
+template <int PATH, class FrameAt, class LutSync>
 __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& frame_at, MbState& s, int split,
-                                           int K, int lane, uint8_t* win, LdsBytes s_lut) {
+                                           int K, int lane, uint8_t* win, LdsBytes s_lut, const LutSync& lut_sync) {
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
   const int nnb = a.filter_on ? a.n - 1 : 0;
@@ -310,6 +312,7 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     if (kDma) dma_window<kMasked>(F.y, off0, off1, off2, win, lane, outside);
     if (kMasked) fetch_edges(F.y);
   }
+  lut_sync();
   bool pend = false;  // chroma bytes of the previous frame await their blend
   uint32_t ru = 0, rv = 0;
   for (; k < nnb; k += K) {
@@ -427,9 +430,10 @@ __device__ __forceinline__ TableEntry load_table_entry(const ArnrGroup& a, const
 // clamped first column rounded down to a dword and moved left, when needed,
 // so that both dwords start inside the row; one byte permute then applies
 // the column clamp.
-template <bool A4, class FrameAt>
+template <bool A4, class FrameAt, class LutSync>
 __device__ __forceinline__ void table_loop(const ArnrGroup& a, const FrameAt& frame_at, MbState& s, int split, int K,
-                                           int lane, LdsBytes s_lut, const TableEntry& e) {
+                                           int lane, LdsBytes s_lut, const TableEntry& e, const LutSync& lut_sync) {
+  lut_sync();
   typedef uint32_t u32x2 __attribute__((ext_vector_type(2), aligned(4)));
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
@@ -541,10 +545,15 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
       s.cv = C.v[o];
     }
   }
-  for (int i = tid; i < 64; i += blockDim.x) s_arnr_lut[i] = wt4[i];
-  if (K > 1)
-    for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
-  __syncthreads();
+  // The LUT and the split accumulators are written, and the workgroup
+  // synchronised, by each wave's frame loop once its first window DMA is
+  // issued, so that the DMA's latency overlaps the centre and LUT loads.
+  auto lut_sync = [&]() {
+    for (int i = tid; i < 64; i += blockDim.x) s_arnr_lut[i] = wt4[i];
+    if (K > 1)
+      for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
+    __syncthreads();
+  };
 
   if (active) {
     uint8_t* win = s_win + wave * kWaveWinBytes;
@@ -555,15 +564,17 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
     const bool fast = dma && cols_inside && s.by >= 15 && s.by + 31 <= a.h && s.cbx >= 7 && s.cbx + 15 <= a.cw &&
                       s.cby >= 7 && s.cby + 15 <= a.ch;
     if (TABLE && a.aligned4 && a.w >= 8)
-      table_loop<true>(a, frame_at, s, split, K, lane, s_lut, e);
+      table_loop<true>(a, frame_at, s, split, K, lane, s_lut, e, lut_sync);
     else if (TABLE)
-      table_loop<false>(a, frame_at, s, split, K, lane, s_lut, e);
+      table_loop<false>(a, frame_at, s, split, K, lane, s_lut, e, lut_sync);
     else if (fast)
-      frame_loop<kPathFast>(a, frame_at, s, split, K, lane, win, s_lut);
+      frame_loop<kPathFast>(a, frame_at, s, split, K, lane, win, s_lut, lut_sync);
     else if (dma)
-      frame_loop<kPathDma>(a, frame_at, s, split, K, lane, win, s_lut);
+      frame_loop<kPathDma>(a, frame_at, s, split, K, lane, win, s_lut, lut_sync);
     else
-      frame_loop<kPathGather>(a, frame_at, s, split, K, lane, win, s_lut);
+      frame_loop<kPathGather>(a, frame_at, s, split, K, lane, win, s_lut, lut_sync);
+  } else {
+    lut_sync();
   }
 
   if (K > 1) {
@@ -686,7 +697,9 @@ __device__ __forceinline__ ArnrGroup load_group(const ArnrGroup* p) {
 // Persistent batch kernel: every workgroup loops taking units (one strip of
 // one group) until all chunks are drained; a unit is processed exactly as
 // the single-group kernel processes that strip.
-__global__ void __launch_bounds__(1024) arnr_batch_kernel(const ArnrBatchArgs b) {
+// At most 72 VGPRs: 7 waves per SIMD.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 7)))
+arnr_batch_kernel(const ArnrBatchArgs b) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_unit;
   const int first = blockIdx.x % kBatchChunks;
