@@ -600,11 +600,19 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   }
   if (!active || split != 0) return;
 
-  // Resolve: (16 * ctr + acc + ws / 2) / ws with ws = 16 + wsum.
+  // Resolve: (16 * ctr + acc + ws / 2) / ws with ws = 16 + wsum.  The
+  // numerator n is below 256 * ws and ws lies in [16, 1024] (at most 64
+  // frames), so the quotient is a pixel and n / ws has a fractional part of
+  // 0 or at least 1/1024.  n * rcp(ws) (v_rcp_f32, within 1 ulp) is then
+  // within 2^-14 of n / ws, and adding 2^-12 before truncating gives the
+  // exact floor: an integer quotient cannot fall below itself, a fractional
+  // one cannot reach the next integer (checked for every ws and both ends
+  // of every quotient's interval, with the reciprocal off by up to 2 ulp).
   auto resolve = [](uint32_t packed, uint32_t ctr) -> uint32_t {
     const uint32_t sum = kWMax * ctr + (packed & (kAccOne - 1));
     const uint32_t ws = kWMax + (packed >> kAccShift);
-    return (sum + (ws >> 1)) / ws;
+    const float r = __builtin_amdgcn_rcpf((float)ws);
+    return (uint32_t)__builtin_fmaf((float)(sum + (ws >> 1)), r, 0x1p-12f);
   };
   if (lr < s.bh) {
     const uint32_t c = s.ctr4;

@@ -358,8 +358,10 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
   uint8_t* chalo = lt + 21 * 16;  // U at 0, V at 13*16
   uint8_t* ctmp = chalo + 2 * 13 * 16;
 
-  const int mb_col = (int)a.mb_xy[2 * i], mb_row = (int)a.mb_xy[2 * i + 1];
-  int16_t yr = dbl16(a.mv[2 * i]), yc = dbl16(a.mv[2 * i + 1]);
+  const int mb_col = (int)uniform_u32(a.mb_xy + 2 * i), mb_row = (int)uniform_u32(a.mb_xy + 2 * i + 1);
+  int16_t mvr, mvc;
+  uniform_i16x2(a.mv + 2 * i, mvr, mvc);
+  int16_t yr = dbl16(mvr), yc = dbl16(mvc);
   int16_t ur = (int16_t)avg4(yr), uc = (int16_t)avg4(yc);
   if (a.full_pixel) {
     yr = (int16_t)(yr & ~7); yc = (int16_t)(yc & ~7);
@@ -454,8 +456,10 @@ __global__ void __launch_bounds__(256) mb_luma_sad_at_mv_kernel(const SadArgs a)
   if (i >= a.n) return;
   uint8_t* lh = smem + wave * (21 * 24 + 21 * 16);
   uint8_t* lt = lh + 21 * 24;
-  const int mb_col = (int)a.mb_xy[2 * i], mb_row = (int)a.mb_xy[2 * i + 1];
-  const int16_t r8 = dbl16(a.mv[2 * i]), c8 = dbl16(a.mv[2 * i + 1]);
+  const int mb_col = (int)uniform_u32(a.mb_xy + 2 * i), mb_row = (int)uniform_u32(a.mb_xy + 2 * i + 1);
+  int16_t mvr, mvc;
+  uniform_i16x2(a.mv + 2 * i, mvr, mvc);
+  const int16_t r8 = dbl16(mvr), c8 = dbl16(mvc);
   const uint32_t s = *reinterpret_cast<const uint32_t*>(a.src + (size_t)i * 256 + 4 * lane);
   const uint32_t sh = stage_halo<21, 21>(a.plane, a.stride, a.w, a.h, mb_col * 16 + (c8 >> 3) - 2,
                                          mb_row * 16 + (r8 >> 3) - 2, a.aligned4 != 0, lh, 24, lane);
@@ -484,10 +488,12 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
   if (i >= a.n) return;
   uint8_t* win = smem + wave * kWave;
   uint8_t* lt = win + 24 * WP;  // three 24 x 16 horizontal-pass results
-  const int mb_col = (int)a.mb_xy[2 * i], mb_row = (int)a.mb_xy[2 * i + 1];
+  const int mb_col = (int)uniform_u32(a.mb_xy + 2 * i), mb_row = (int)uniform_u32(a.mb_xy + 2 * i + 1);
   // Centres outside +-1023 are outside the reference's contract (it
   // debug-asserts); clamping keeps every candidate inside the staged window.
-  const int crow = clampi(a.center[2 * i], -1023, 1023), ccol = clampi(a.center[2 * i + 1], -1023, 1023);
+  int16_t cr16, cc16;
+  uniform_i16x2(a.center + 2 * i, cr16, cc16);
+  const int crow = clampi(cr16, -1023, 1023), ccol = clampi(cc16, -1023, 1023);
   const int st = a.step;
   // candidate k: row index rk(k) (0..2 -> -st, 0, +st), column index ck(k)
   const int rowv[3] = {clampi(crow - st, -1023, 1023), crow, clampi(crow + st, -1023, 1023)};

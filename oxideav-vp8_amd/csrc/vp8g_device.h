@@ -123,4 +123,17 @@ __device__ __forceinline__ int xcd_band(int i, int total) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Wave-uniform reads of per-item inputs the kernel never writes, through the
+// constant address space: scalar loads (s_load) into SGPRs, whose latency is
+// below a vector load's.  `p` must be 4-byte aligned.
+__device__ __forceinline__ uint32_t uniform_u32(const void* p) {
+  return *(const __attribute__((address_space(4))) uint32_t*)p;
+}
+// A pair of int16 (little-endian: element 0 in the low half).
+__device__ __forceinline__ void uniform_i16x2(const int16_t* p, int16_t& e0, int16_t& e1) {
+  const uint32_t w = uniform_u32(p);
+  e0 = (int16_t)(w & 0xFFFFu);
+  e1 = (int16_t)(w >> 16);
+}
+
 }  // namespace vp8g
