@@ -51,8 +51,14 @@ struct PrepArgs {
 
 struct FeatArgs {
   const uint8_t* planes;  // [images][2][4 ch][4 cw]
-  uint16_t* out;          // [images][ch][cw][kFeat] bf16
+  uint16_t* out;          // [images][ch][cw][kFeat] bf16 (when w0 is null)
   int images, cw, ch;
+  // Inference: the first layer (1x1 conv kFeat -> 32, bias, leaky ReLU)
+  // applied in the same kernel, out0 = [images][ch][cw][32] bf16 instead of
+  // the features.
+  const uint16_t* w0;     // [32][kFeat] bf16, or null
+  const float* b0;        // [32]
+  uint16_t* out0;
 };
 
 struct ConvArgs {

@@ -91,13 +91,22 @@ int pack_all(vp8g_proposer* p) {
   return VP8G_OK;
 }
 
-int forward(vp8g_proposer* p, const uint8_t* planes, int images, int cw, int ch, const Layout& L) {
+// fuse_first (inference): the features kernel applies the first layer
+// itself and the 128-channel features are never stored; training keeps them
+// for the first layer's weight gradient.
+int forward(vp8g_proposer* p, const uint8_t* planes, int images, int cw, int ch, const Layout& L,
+            bool fuse_first = false) {
   uint8_t* b = p->arena;
   hipStream_t s = p->ctx->stream;
-  FeatArgs f{planes, reinterpret_cast<uint16_t*>(b + L.x0), images, cw, ch};
+  FeatArgs f{planes, reinterpret_cast<uint16_t*>(b + L.x0), images, cw, ch, nullptr, nullptr, nullptr};
+  if (fuse_first) {
+    f.w0 = p->wf[0];
+    f.b0 = p->params + conv_offset(0) + conv_weights(0);
+    f.out0 = reinterpret_cast<uint16_t*>(b + L.a[0]);
+  }
   if (!ok(launch_features(f, s))) return VP8G_E_DEVICE;
-  const uint16_t* in = f.out;
-  for (int l = 0; l < kLayers; l++) {
+  const uint16_t* in = fuse_first ? f.out0 : f.out;
+  for (int l = fuse_first ? 1 : 0; l < kLayers; l++) {
     ConvArgs a{};
     a.in = in;
     a.out = reinterpret_cast<uint16_t*>(b + L.a[l]);
@@ -292,7 +301,7 @@ int vp8g_proposer_propose_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
   if (st != VP8G_OK) return st;
   uint8_t* planes = p->arena + L.planes;
   st = vp8g_proposer_prepare_dev(p, frames, n, center, planes);
-  if (st == VP8G_OK) st = forward(p, planes, images, cw, ch, L);
+  if (st == VP8G_OK) st = forward(p, planes, images, cw, ch, L, true);
   if (st != VP8G_OK) return st;
   HeadArgs hd{};
   hd.a4 = reinterpret_cast<const uint16_t*>(p->arena + L.a[kLayers - 1]);

@@ -170,6 +170,40 @@ __global__ void __launch_bounds__(320) features_kernel(const FeatArgs a) {
     }
   }
   __syncthreads();
+  if (a.w0) {
+    // The first layer straight from LDS: wave w < 4 computes output channels
+    // 16 (w >> 1) .. +15 of cells 16 (w & 1) .. +15, with the operand layout,
+    // summation order and epilogue of conv_kernel<1, kFeat, 32>, so the
+    // activations are the ones that kernel writes.  Columns of cells past
+    // the segment hold stale LDS; a column only reaches its own outputs,
+    // which are not stored.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+    if (wave < 4) {
+      const int m = wave >> 1, cell = 16 * (wave & 1) + (l & 15);
+      const uint4* wrow = reinterpret_cast<const uint4*>(a.w0 + (size_t)(16 * m + (l & 15)) * kFeat);
+      const uint4* xrow = reinterpret_cast<const uint4*>(s_out + cell * kFeat);
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < kFeat / 32; c++) {
+        const int ck = 4 * c + (l >> 4);
+        acc = mfma16(as_bf16x8(wrow[ck]), as_bf16x8(xrow[ck]), acc);
+      }
+      if (cell < ncell) {
+        const int co = 16 * m + 4 * (l >> 4);
+        float v[4] = {acc[0], acc[1], acc[2], acc[3]};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const float z = v[i] + a.b0[co + i];
+          v[i] = z > 0.f ? z : kLeak * z;
+        }
+        uint2 o;
+        o.x = (uint32_t)bf16_bits(v[0]) | ((uint32_t)bf16_bits(v[1]) << 16);
+        o.y = (uint32_t)bf16_bits(v[2]) | ((uint32_t)bf16_bits(v[3]) << 16);
+        *reinterpret_cast<uint2*>(a.out0 + (((size_t)img * a.ch + cy) * a.cw + cx0 + cell) * 32 + co) = o;
+      }
+    }
+    return;
+  }
   // the segment's cells: consecutive rows of kFeat bf16 = 16 uint4 each
   uint4* dst = reinterpret_cast<uint4*>(a.out + (((size_t)img * a.ch + cy) * a.cw + cx0) * kFeat);
   const uint4* srcv = reinterpret_cast<const uint4*>(s_out);

@@ -109,6 +109,46 @@ def test_forward_matches_bf16_reference(gpu, prop, shape):
     assert (err > tol).mean() < 2e-3, (err > tol).mean()
 
 
+def test_propose_fused_first_layer_matches_forward(gpu, prop):
+    """propose applies the first layer inside the features kernel; its
+    proposals must equal the unfused forward pass (features, then the 1x1
+    conv kernel) followed by the head, computed here on the host in fp32
+    (a value at a rounding tie may land one unit off)."""
+    import torch
+    import vp8g.synth as synth
+    from test_gpu_search import Dev
+    rng = np.random.default_rng(3)
+    params = ref().he_init(17).astype(np.float32)
+    params[4096:4128] = rng.normal(0, 0.05, 32)
+    prop.set_params(params)
+    g = synth.arnr_group_regions(200, 120, 5, vmax=8, occluders=4, clip=2, config=3)
+    dev = Dev(g.frames())
+    n, c = len(dev.tuples), g.center
+    bh, bw = dev.shape[1], dev.shape[2]
+    table = dev.table(-5)
+    planes = torch.zeros((n - 1, 2, 4 * bh, 4 * bw), dtype=torch.uint8, device="cuda")
+    a4 = torch.zeros((n - 1, bh * bw, 32), dtype=torch.int16, device="cuda")
+    torch.cuda.synchronize()
+    prop.propose(dev.descs, n, c, table.data_ptr(), 14)
+    prop.prepare(dev.descs, n, c, planes.data_ptr())
+    prop.forward(planes.data_ptr(), n - 1, bw, bh, a4.data_ptr())
+    gpu.synchronize()
+    x = bits_to_f64(a4).astype(np.float32).reshape(n - 1, bh, bw, 32)
+    head = params[-66:].astype(np.float32)  # [2][32] weights then [2] bias
+    pred = np.empty((n - 1, bh, bw, 2), np.float32)
+    for o in range(2):
+        s = np.full((n - 1, bh, bw), head[64 + o], np.float32)
+        for k in range(32):
+            s = (s.astype(np.float64) + head[32 * o + k].astype(np.float64) * x[..., k]).astype(np.float32)
+        pred[..., o] = s
+    want = np.clip(np.rint(pred * 8), -14, 14).astype(np.int32)
+    got = table.cpu().numpy()[[i for i in range(n) if i != c]]
+    assert (got[..., 2] == 0).all()
+    diff = np.abs(got[..., :2] - want)
+    assert diff.max() <= 1
+    assert (diff > 0).mean() < 1e-3, (diff > 0).mean()
+
+
 def test_gradients_match_reference(gpu, prop):
     import torch
     rng = np.random.default_rng(4)
