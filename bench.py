@@ -148,8 +148,10 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     """The same group's altref three ways, each search + table-driven blend
     timed with HIP events on the library's stream: the reference's 4-step
     descent (the fused kernel), the exhaustive +-16 search, and the
-    learned-seeded search (proposer + exact +-2 refinement, trained weights
-    oxideav-vp8_amd/weights/proposer_1080p.safetensors).  Reports anchor-MB/s,
+    learned-seeded search (proposer and exact +-2 refinement; trained
+    weights oxideav-vp8_amd/weights/proposer_1080p.safetensors), and the
+    same with the optional second stage (exact +-2 around each block's
+    border vector and its neighbours' vectors).  Reports anchor-MB/s,
     min-SAD agreement with the exhaustive search and luma PSNR against the
     noise-free centre."""
     import torch
@@ -165,7 +167,7 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     cw, ch = (w + 1) // 2, (h + 1) // 2
     dev = torch.device("cuda:0")
     ex = torch.zeros((n, bh, bw, 3), dtype=torch.int32, device=dev)
-    seeds, ref = torch.zeros_like(ex), torch.zeros_like(ex)
+    seeds, first, ref = torch.zeros_like(ex), torch.zeros_like(ex), torch.zeros_like(ex)
     oy = torch.empty(w * h, dtype=torch.uint8, device=dev)
     ou = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
     ov = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
@@ -186,9 +188,20 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
         ctx.block_search_dev(descs, n, c, 16, ex.data_ptr())
         ctx.build_arnr_altref_table_dev(descs, n, c, 3, ex.data_ptr(), oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
 
-    def altref_learned():
+    def search_learned():
         prop.propose(descs, n, c, seeds.data_ptr(), 14)
-        ctx.block_search_dev(descs, n, c, 2, ref.data_ptr(), seeds.data_ptr())
+        ctx.block_search_dev(descs, n, c, 2, first.data_ptr(), seeds.data_ptr())
+
+    def search_learned_two_stage():
+        search_learned()
+        ctx.refine_candidates_dev(descs, n, c, seeds.data_ptr(), first.data_ptr(), ref.data_ptr())
+
+    def altref_learned():
+        search_learned()
+        ctx.build_arnr_altref_table_dev(descs, n, c, 3, first.data_ptr(), oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
+
+    def altref_learned_two_stage():
+        search_learned_two_stage()
         ctx.build_arnr_altref_table_dev(descs, n, c, 3, ref.data_ptr(), oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
 
     def altref_descent():
@@ -197,27 +210,33 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     anchors = bw * bh
     clean = group.clean[c][0]
     out = {}
-    for name, fn in (("descent", altref_descent), ("exhaustive16", altref_exhaustive), ("learned", altref_learned)):
+    for name, fn in (("descent", altref_descent), ("exhaustive16", altref_exhaustive), ("learned", altref_learned),
+                     ("learned_two_stage", altref_learned_two_stage)):
         ms = timed(fn)
         ctx.synchronize()
         out[name] = {"ms": ms, "anchor_mb_per_s": anchors / (ms * 1e-3),
                      "psnr_y_vs_clean": psnr(oy.cpu().numpy().reshape(h, w), clean)}
     t_search_ex = timed(lambda: ctx.block_search_dev(descs, n, c, 16, ex.data_ptr()))
-    t_search_le = timed(lambda: (prop.propose(descs, n, c, seeds.data_ptr(), 14),
-                                 ctx.block_search_dev(descs, n, c, 2, ref.data_ptr(), seeds.data_ptr())))
+    t_search_le = timed(search_learned)
+    t_search_le2 = timed(search_learned_two_stage)
     ctx.synchronize()
     others = [i for i in range(n) if i != c]
-    e, r = ex.cpu().numpy()[others], ref.cpu().numpy()[others]
+    e, r1, r2 = ex.cpu().numpy()[others], first.cpu().numpy()[others], ref.cpu().numpy()[others]
     area = (np.minimum(16, h - 16 * np.arange(bh))[:, None] * np.minimum(16, w - 16 * np.arange(bw))[None, :])
     matched = e[..., 2] < 13 * area[None]
-    same = r[..., 2] == e[..., 2]
+    same = r1[..., 2] == e[..., 2]
+    same2 = r2[..., 2] == e[..., 2]
     out.update({
         "agreement": float(same.mean()),
         "agreement_matched": float(same[matched].mean()),
         "matched_fraction": float(matched.mean()),
         "psnr_loss_db": out["exhaustive16"]["psnr_y_vs_clean"] - out["learned"]["psnr_y_vs_clean"],
-        "search_ms": {"exhaustive16": t_search_ex, "learned": t_search_le},
+        "search_ms": {"exhaustive16": t_search_ex, "learned": t_search_le, "learned_two_stage": t_search_le2},
         "search_speedup": t_search_ex / t_search_le,
+        "two_stage": {"agreement_matched": float(same2[matched].mean()),
+                      "psnr_loss_db": out["exhaustive16"]["psnr_y_vs_clean"] -
+                      out["learned_two_stage"]["psnr_y_vs_clean"],
+                      "search_speedup": t_search_ex / t_search_le2},
         "weights": str(wpath.name),
     })
     prop.close()

@@ -157,6 +157,19 @@ int vp8g_build_arnr_altref_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_
  * +-16384.  radius: 2, 16 or 64.  Validation as vp8g_build_arnr_altref. */
 int vp8g_block_search_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames, size_t center,
                           int radius, const int32_t *seeds, int32_t *out, vp8g_err *err);
+/* Second stage of a seeded search, no counterpart in the reference: for
+ * every block and frame, exact +-2 searches (vp8g_block_search_dev's
+ * semantics) around extra centres, and the best of those and the first
+ * stage's result `first` (the table vp8g_block_search_dev wrote with radius
+ * 2 around `seeds`; seeds may be NULL = zero, as there).  The centres are
+ * the block's own first-stage vector when it lies on the border of the
+ * first square (2 away from the seed in x or y), then the first-stage
+ * vectors of the left, right, upper and lower blocks, each once and only
+ * when it differs from the seed and the own vector.  Ties go to the
+ * smaller SAD, then (|mvx| + |mvy|, mvy, mvx).  `out` (device) must not be
+ * `first`; centre rows are left untouched. */
+int vp8g_refine_candidates_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames, size_t center,
+                               const int32_t *seeds, const int32_t *first, int32_t *out, vp8g_err *err);
 /* The altref of build_arnr_altref (src/arnr.rs:488-650) with the search
  * replaced by a given alignment table (device, the layout above; the
  * centre's rows are not read): occlusion guard on the table's SAD, blend

@@ -496,6 +496,35 @@ int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t
   return hip_ok(launch_block_search(a, radius, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
+int vp8g_refine_candidates_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, const int32_t* seeds,
+                               const int32_t* first, int32_t* out, vp8g_err* err) {
+  if (!c) return VP8G_E_ARG;
+  int st = validate_arnr(frames, n, center, err);
+  if (st != VP8G_OK) return st;
+  if (!out || !first || out == first) return VP8G_E_ARG;
+  if (((uintptr_t)out | (uintptr_t)seeds | (uintptr_t)first) & 3) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  SearchArgs a;
+  memset(&a, 0, sizeof(a));
+  bool aligned4 = true;
+  for (size_t i = 0; i < n; i++) {
+    const vp8g_i420& f = frames[i];
+    a.frames[i] = FrameDesc{f.y, f.u, f.v, (uint32_t)f.y_stride, (uint32_t)f.uv_stride};
+    if (((uintptr_t)f.y & 3) || (f.y_stride & 3)) aligned4 = false;
+  }
+  a.seeds = seeds;
+  a.out = out;
+  a.w = (int)frames[center].width;
+  a.h = (int)frames[center].height;
+  a.bcols = (a.w + 15) / 16;
+  a.brows = (a.h + 15) / 16;
+  a.n = (int)n;
+  a.center = (int)center;
+  a.aligned4 = aligned4 ? 1 : 0;
+  a.total_blocks = (int)(n - 1) * a.bcols * a.brows;
+  return hip_ok(launch_refine_candidates(a, first, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
 int vp8g_downsample2_i420_dev(vp8g_ctx* c, const vp8g_i420* src, const vp8g_i420* dst, size_t n) {
   if (!c || !src || !dst || n == 0 || n > (size_t)VP8G_MAX_FRAMES) return VP8G_E_ARG;
   const size_t w = src[0].width, h = src[0].height;

@@ -471,6 +471,226 @@ hipError_t launch_upsample_table(const int32_t* coarse, int cbw, int cbh, int32_
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- candidates
+// Exact +-2 search of one block against one neighbour frame around centre
+// (sx, sy), for refine_candidates_kernel: the window is loaded and scored
+// at once (no prefetch).  Lane layout, SADs and tie keys as in
+// refine2_kernel.  Returns {mvx, mvy, sad} of the best candidate.
+struct R2Best {
+  int mvx, mvy;
+  uint32_t sad;
+};
+
+__device__ __forceinline__ R2Best r2_search_at(const SearchArgs& a, const FrameDesc& F, uint32_t* win,
+                                               const uint4 (&src)[4], int bx, int by, int bw, int bh, bool full,
+                                               int sx, int sy, int l) {
+  constexpr int R = 2, PD = 6, WROWS = 20, WDW = WROWS * PD;
+  const int dy = l >> 3, j = (l >> 2) & 1, h = l & 3;
+  auto floor4 = [](int x) { return x >= 0 ? (x & ~3) : -(((-x) + 3) & ~3); };
+  const int wy0 = l / PD, wd0 = l - PD * wy0;
+  const int s1 = min(l + 64, WDW - 1), wy1 = s1 / PD, wd1 = s1 - PD * wy1;
+  const int x0a = floor4(bx + sx - R), y0 = by + sy - R;
+  uint32_t v0, v1;
+  if (a.aligned4 && x0a >= 0 && x0a + 4 * PD <= a.w && y0 >= 0 && y0 + WROWS <= a.h) {
+    const uint8_t* base = F.y + (size_t)y0 * F.y_stride + x0a;
+    v0 = *reinterpret_cast<const uint32_t*>(base + (uint32_t)wy0 * F.y_stride + 4 * wd0);
+    v1 = *reinterpret_cast<const uint32_t*>(base + (uint32_t)wy1 * F.y_stride + 4 * wd1);
+  } else {
+    v0 = gather4_clamped(F.y + (size_t)clampi(y0 + wy0, 0, a.h - 1) * F.y_stride, x0a + 4 * wd0, a.w);
+    v1 = gather4_clamped(F.y + (size_t)clampi(y0 + wy1, 0, a.h - 1) * F.y_stride, x0a + 4 * wd1, a.w);
+  }
+  wave_lds_sync();  // the previous search's window reads are done
+  win[l] = v0;
+  if (l + 64 < WDW) win[l + 64] = v1;
+  wave_lds_sync();
+  const int shift0 = (bx + sx - R) - floor4(bx + sx - R);
+  uint64_t acc = 0;
+  if (full) {
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const uint32_t* wr = win + (dy + 4 * h + rr) * PD + j;
+      const uint32_t w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4];
+      acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w0 | ((uint64_t)w1 << 32), src[rr].x, acc);
+      acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w1 | ((uint64_t)w2 << 32), src[rr].y, acc);
+      acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w2 | ((uint64_t)w3 << 32), src[rr].z, acc);
+      acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w3 | ((uint64_t)w4 << 32), src[rr].w, acc);
+    }
+  } else {
+    const int bwq = bw >> 2, rem = bw & 3;
+    const uint32_t remmask = rem ? (0xFFFFFFFFu >> (32 - 8 * rem)) : 0u;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      if (4 * h + rr < bh) {
+        const uint32_t* wr = win + (dy + 4 * h + rr) * PD + j;
+        const uint32_t wv[5] = {wr[0], wr[1], wr[2], wr[3], wr[4]};
+        const uint32_t sv[4] = {src[rr].x, src[rr].y, src[rr].z, src[rr].w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint64_t pair = (uint64_t)wv[q] | ((uint64_t)wv[q + 1] << 32);
+          if (q < bwq) {
+            acc = __builtin_amdgcn_qsad_pk_u16_u8(pair, sv[q], acc);
+          } else if (q == bwq && rem) {
+            uint64_t add = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              add |= (uint64_t)sad4(sv[q], funnel(wv[q], wv[q + 1], i) & remmask, 0) << (16 * i);
+            acc += add;
+          }
+        }
+      }
+    }
+  }
+  uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
+  lo += dpp<kDppQuadXor1>(lo);
+  hi += dpp<kDppQuadXor1>(hi);
+  lo += dpp<kDppQuadXor2>(lo);
+  hi += dpp<kDppQuadXor2>(hi);
+  const int mvy = sy - R + dy, mvx0 = sx - R + 4 * j - shift0;
+  const int l1_base = abs(sx) + abs(sy) - 2 * R;
+  const uint32_t key_lo = (uint32_t)((dy << 3) + 4 * j - shift0);
+  uint32_t lane_key = 0xFFFFFFFFu;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int dxi = 4 * j + i - shift0;
+    const uint32_t sad = ((i < 2 ? lo : hi) >> (16 * (i & 1))) & 0xFFFFu;
+    const uint32_t rel = (uint32_t)(abs(mvx0 + i) + abs(mvy) - l1_base);
+    if (l < 40 && dxi >= 0 && dxi <= 2 * R) lane_key = min(lane_key, (sad << 10) | (rel << 6) | (key_lo + (uint32_t)i));
+  }
+  const uint32_t best = wave_min_any(lane_key);
+  return R2Best{sx - R + (int)(best & 7), sy - R + (int)((best >> 3) & 7), best >> 10};
+}
+
+// (sad, |mvx| + |mvy|, mvy, mvx) as one ordered 64-bit key (components of at
+// most 16386 in magnitude).
+__device__ __forceinline__ uint64_t abs_key(int mvx, int mvy, uint32_t sad) {
+  return ((uint64_t)sad << 48) | ((uint64_t)(abs(mvx) + abs(mvy)) << 32) | ((uint64_t)(mvy + 32768) << 16) |
+         (uint64_t)(mvx + 32768);
+}
+
+// Second stage of the learned-seeded search: for every block and neighbour
+// frame, exact +-2 searches around extra centres, and the best of those and
+// the first stage's result.  The centres are the block's own first-stage
+// vector when it lies on the border of the first square around the seed
+// (the minimum may continue past it), then the first-stage vectors of the
+// left, right, upper and lower blocks, each taken once and only when it
+// differs from the seed and the own vector.  One wave per block position, 4
+// per workgroup.
+__global__ void __launch_bounds__(256) refine_candidates_kernel(const SearchArgs a, const int32_t* first) {
+  constexpr int kMaxCand = 5;
+  __shared__ __align__(16) uint32_t s_win[4][120];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int per_frame = a.brows * a.bcols;
+  const int rest = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
+  if (rest >= per_frame) return;  // whole wave; no workgroup barrier follows
+  const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
+  const int bx = bcol * 16, by = brow * 16;
+  const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
+  const bool full = bw == 16 && bh == 16;
+  uint32_t* win = s_win[wave];
+  uint4 src[4];
+  {
+    const FrameDesc C = a.frames[a.center];
+    const int r = l >> 2, q = l & 3;
+    uint32_t v = 0;
+    if (r < bh) {
+      const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
+      if (full && a.aligned4) {
+        v = *reinterpret_cast<const uint32_t*>(row + 4 * q);
+      } else {
+        for (int e = 0; e < 4; e++)
+          if (4 * q + e < bw) v |= (uint32_t)row[4 * q + e] << (8 * e);
+      }
+    }
+    win[l] = v;
+    wave_lds_sync();
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) src[rr] = reinterpret_cast<const uint4*>(win)[4 * (l & 3) + rr];
+  }
+  const int nnb = a.n - 1;
+  // lane k < nnb: frame k's seed, own first-stage entry and its neighbours'
+  // vectors, all requested at once (an absent neighbour repeats the own
+  // vector)
+  int e_sx = 0, e_sy = 0, e_ox = 0, e_oy = 0, e_osad = 0, e_nx[4], e_ny[4];
+  const int nr[4] = {brow, brow, brow - 1, brow + 1}, ncl[4] = {bcol - 1, bcol + 1, bcol, bcol};
+  if (l < nnb) {
+    const int fi = l < a.center ? l : l + 1;
+    const int32_t* own = first + 3 * ((size_t)fi * per_frame + rest);
+    e_ox = own[0];
+    e_oy = own[1];
+    e_osad = own[2];
+    if (a.seeds) {
+      const int32_t* sd = a.seeds + 3 * ((size_t)fi * per_frame + rest);
+      e_sx = clampi(sd[0], -kMaxSeed, kMaxSeed);
+      e_sy = clampi(sd[1], -kMaxSeed, kMaxSeed);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const bool in = nr[t] >= 0 && nr[t] < a.brows && ncl[t] >= 0 && ncl[t] < a.bcols;
+      const int32_t* e = first + 3 * ((size_t)fi * per_frame + (size_t)(in ? nr[t] : brow) * a.bcols +
+                                      (in ? ncl[t] : bcol));
+      e_nx[t] = e[0];
+      e_ny[t] = e[1];
+    }
+  }
+  for (int k = 0; k < nnb; k++) {
+    const int fi = k < a.center ? k : k + 1;
+    const FrameDesc F = a.frames[fi];
+    const int ox = __builtin_amdgcn_readlane(e_ox, k), oy = __builtin_amdgcn_readlane(e_oy, k);
+    const uint32_t osad = (uint32_t)__builtin_amdgcn_readlane(e_osad, k);
+    const int sx = __builtin_amdgcn_readlane(e_sx, k), sy = __builtin_amdgcn_readlane(e_sy, k);
+    // slot 0: the own vector, searched again only when it lies on the first
+    // square's border (the minimum may continue past it); slots 1..4: left,
+    // right, upper, lower
+    int cx[kMaxCand], cy[kMaxCand];
+    bool use[kMaxCand];
+    cx[0] = ox;
+    cy[0] = oy;
+    use[0] = abs(ox - sx) == 2 || abs(oy - sy) == 2;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      cx[t + 1] = __builtin_amdgcn_readlane(e_nx[t], k);
+      cy[t + 1] = __builtin_amdgcn_readlane(e_ny[t], k);
+      use[t + 1] = true;
+    }
+#pragma unroll
+    for (int t = 0; t < kMaxCand; t++) {
+      cx[t] = clampi(cx[t], -kMaxSeed, kMaxSeed);
+      cy[t] = clampi(cy[t], -kMaxSeed, kMaxSeed);
+      use[t] = use[t] && !(cx[t] == sx && cy[t] == sy);  // that square was searched
+#pragma unroll
+      for (int u = 0; u < t; u++)
+        if (cx[u] == cx[t] && cy[u] == cy[t]) use[t] = false;  // the own vector or an earlier slot
+    }
+    int bxv = ox, byv = oy;
+    uint32_t bsad = osad;
+    uint64_t bkey = abs_key(ox, oy, osad);
+#pragma unroll
+    for (int t = 0; t < kMaxCand; t++) {
+      if (!use[t]) continue;
+      const R2Best r = r2_search_at(a, F, win, src, bx, by, bw, bh, full, cx[t], cy[t], l);
+      const uint64_t key = abs_key(r.mvx, r.mvy, r.sad);
+      if (key < bkey) {
+        bkey = key;
+        bxv = r.mvx;
+        byv = r.mvy;
+        bsad = r.sad;
+      }
+    }
+    if (l == 0) {
+      int32_t* o = a.out + 3 * ((size_t)fi * per_frame + rest);
+      o[0] = bxv;
+      o[1] = byv;
+      o[2] = (int32_t)bsad;
+    }
+  }
+}
+
+hipError_t launch_refine_candidates(const SearchArgs& a, const int32_t* first, hipStream_t stream) {
+  if (a.total_blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(refine_candidates_kernel, dim3((a.brows * a.bcols + 3) / 4), dim3(256), 0, stream, a, first);
+  return hipGetLastError();
+}
+
 bool block_search_supported(int radius) { return radius == 2 || radius == 16 || radius == 64; }
 
 hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream) {

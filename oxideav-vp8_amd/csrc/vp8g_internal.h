@@ -106,6 +106,7 @@ hipError_t launch_downsample2(const Downsample2Args& a, int frames, hipStream_t 
 hipError_t launch_upsample_table(const int32_t* coarse, int cbw, int cbh, int32_t* fine, int fbw, int fbh, int n,
                                  hipStream_t stream);
 bool block_search_supported(int radius);
+hipError_t launch_refine_candidates(const SearchArgs& a, const int32_t* first, hipStream_t stream);
 hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream);
 
 // Motion compensation / SAD scoring.

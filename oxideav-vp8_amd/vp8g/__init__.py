@@ -108,7 +108,7 @@ EXPORTS = (
     "vp8g_arnr_batch_create", "vp8g_arnr_batch_run", "vp8g_arnr_batch_shape",
     "vp8g_arnr_batch_destroy", "vp8g_ctx_load_tuning", "vp8g_tuning_lookup", "vp8g_arnr_shape",
     "vp8g_block_search_dev", "vp8g_build_arnr_altref_table_dev",
-    "vp8g_downsample2_i420_dev", "vp8g_upsample_table_dev",
+    "vp8g_downsample2_i420_dev", "vp8g_upsample_table_dev", "vp8g_refine_candidates_dev",
     "vp8g_proposer_param_count", "vp8g_proposer_create", "vp8g_proposer_destroy",
     "vp8g_proposer_set_params", "vp8g_proposer_get_params", "vp8g_proposer_get_grads",
     "vp8g_proposer_prepare_dev", "vp8g_proposer_forward_dev", "vp8g_proposer_propose_dev",
@@ -153,6 +153,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_arnr_shape": (i32, [vp, C.c_uint32, C.c_uint32, sz, u8, C.POINTER(i32), C.POINTER(i32)]),
         "vp8g_block_search_dev": (i32, [vp, C.POINTER(_I420), sz, sz, i32, vp, vp, C.POINTER(_Err)]),
         "vp8g_downsample2_i420_dev": (i32, [vp, C.POINTER(_I420), C.POINTER(_I420), sz]),
+        "vp8g_refine_candidates_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, vp, vp, C.POINTER(_Err)]),
         "vp8g_upsample_table_dev": (i32, [vp, vp, sz, i32, i32, vp, i32, i32]),
         "vp8g_build_arnr_altref_table_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, vp,
                                                    C.POINTER(_Err)]),
@@ -432,6 +433,14 @@ class Context:
         err = _Err()
         st = self.lib.vp8g_block_search_dev(self.h, descs, n, center, radius, seeds, out, C.byref(err))
         self._raise(st, "vp8g_block_search_dev")
+
+    def refine_candidates_dev(self, descs, n: int, center: int, seeds: int | None, first: int, out: int) -> None:
+        """Second stage of the seeded search: exact +-2 around each block's own
+        first-stage vector (when it lies on the first square's border) and its
+        4-neighbours' first-stage vectors, keeping the best.  Enqueues only."""
+        err = _Err()
+        st = self.lib.vp8g_refine_candidates_dev(self.h, descs, n, center, seeds, first, out, C.byref(err))
+        self._raise(st, "vp8g_refine_candidates_dev")
 
     def downsample2_dev(self, src, dst, n: int = 1) -> None:
         """One pyramid level of ``n`` frames: ``dst`` (``_I420`` descriptors of

@@ -188,3 +188,72 @@ def test_table_altref_large_vectors_ragged(gpu, oracle, geom):
     got = table_altref(gpu, Dev(tuples), 2, 5, t2)
     for name, a, b in zip("yuv", got, want):
         assert np.array_equal(a, b), name
+
+
+def abs_key(t):
+    """(sad, |mvx| + |mvy|, mvy, mvx) as one ordered integer key."""
+    t = t.astype(np.int64)
+    return (t[..., 2] << 48) + ((np.abs(t[..., 0]) + np.abs(t[..., 1])) << 32) + ((t[..., 1] + 32768) << 16) + \
+        (t[..., 0] + 32768)
+
+
+def expect_candidates(oracle, tuples, center, seeds, first):
+    """The second stage restated with the oracle's +-2 block search: slot 0
+    holds the own first-stage vector, searched again only when it lies on
+    the border of the first square around the seed; slots 1..4 hold the
+    left, right, upper and lower blocks' vectors (an absent neighbour repeats
+    the own vector), each searched when it differs from the seed and from
+    every earlier slot."""
+    n, bh, bw, _ = first.shape
+    own = first[..., :2]
+    slots = [own]
+    for dr, dc in ((0, -1), (0, 1), (-1, 0), (1, 0)):
+        nb = own.copy()
+        rs, cs = slice(max(0, -dr), bh - max(0, dr)), slice(max(0, -dc), bw - max(0, dc))
+        rd, cd = slice(max(0, dr), bh - max(0, -dr)), slice(max(0, dc), bw - max(0, -dc))
+        nb[:, rs, cs] = own[:, rd, cd]
+        slots.append(nb)
+    sd = seeds[..., :2]
+    best = first.copy()
+    for t, cand in enumerate(slots):
+        use = (cand != sd).any(-1)
+        if t == 0:
+            use &= (np.abs(cand - sd) == 2).any(-1)
+        for u in range(t):
+            use &= (slots[u] != cand).any(-1)
+        tab = np.zeros_like(first)
+        tab[..., :2] = cand
+        res = oracle.block_search_group(tuples, center, 2, tab)
+        better = use & (abs_key(res) < abs_key(best))
+        best[better] = res[better]
+    return best
+
+
+@pytest.mark.parametrize("geom", [(176, 144, 5, 0), (45, 37, 3, 3), (100, 70, 5, 1)], ids=lambda g: "x".join(map(str, g)))
+def test_refine_candidates_matches_oracle(gpu, oracle, geom):
+    import torch
+    w, h, n, pad = geom
+    tuples = tuples_of(w, h, n, 23, pad)
+    center = n // 2
+    dev = Dev(tuples)
+    rng = np.random.default_rng(w + h)
+    exact = expect_table(oracle, tuples, center, 16)
+    seeds = exact.copy()
+    seeds[..., :2] += rng.integers(-5, 6, seeds[..., :2].shape) * (rng.random(seeds.shape[:3]) < 0.5)[..., None]
+    seeds[..., 2] = 0
+    seeds[center] = 0
+    first = search(gpu, dev, center, 2, seeds)
+    assert_tables(first, expect_table(oracle, tuples, center, 2, seeds), "first stage")
+    ts = torch.from_numpy(np.ascontiguousarray(seeds, np.int32)).cuda()
+    tf = torch.from_numpy(np.ascontiguousarray(first, np.int32)).cuda()
+    out = dev.table(-7)
+    torch.cuda.synchronize()
+    gpu.refine_candidates_dev(dev.descs, n, center, ts.data_ptr(), tf.data_ptr(), out.data_ptr())
+    gpu.synchronize()
+    got = out.cpu().numpy()
+    others = [i for i in range(n) if i != center]
+    want = expect_candidates(oracle, tuples, center, seeds, first)
+    assert_tables(got[others], want[others], "candidates")
+    assert (got[center] == -7).all()
+    # the second stage never loses against the first
+    assert (abs_key(got[others]) <= abs_key(first[others])).all()
