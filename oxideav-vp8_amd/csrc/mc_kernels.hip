@@ -520,7 +520,10 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
     hpass_dot<16, 24>(win + (x & ~3u), WP, x & 3u, lt + t * 24 * 16, fh, lane);
   }
   wave_lds_sync();
-  uint32_t sads[9];
+  // Per-lane SADs of the 9 candidates; each whole-wave SAD is at most
+  // 64 * 1020 < 2^16, so two candidates share one wave reduction (low and
+  // high halves).
+  uint32_t part[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     // k = 0: centre; k = 1..8: (-st,-st), (-st,0), (-st,+st), (0,-st),
@@ -530,8 +533,16 @@ __global__ void __launch_bounds__(256) refine_ladder_kernel(const RefineArgs a) 
     const int16_t r8 = dbl16((int16_t)rowv[rk]);
     const DotTaps fv = sixtap_dot(r8 & 7, false);
     const uint8_t* tmp = lt + ck * 24 * 16 + ((r8 >> 3) - minr) * 16;
-    sads[k] = wave_sum_to_sgpr(sum16(sad4(s, vpass_dot<16>(tmp, fv, lane), 0)));
+    part[k] = sad4(s, vpass_dot<16>(tmp, fv, lane), 0);
   }
+  uint32_t sads[9];
+#pragma unroll
+  for (int k = 0; k < 8; k += 2) {
+    const uint32_t both = wave_sum_to_sgpr(sum16(part[k] | (part[k + 1] << 16)));
+    sads[k] = both & 0xFFFFu;
+    sads[k + 1] = both >> 16;
+  }
+  sads[8] = wave_sum_to_sgpr(sum16(part[8]));
   int best_r = crow, best_c = ccol;
   uint32_t best_sad = sads[0];
 #pragma unroll
