@@ -32,6 +32,7 @@
 // Chroma output clamping (src/arnr.rs:442-444) never fires from the block
 // grid: cbx + ceil(bw/2) = ceil(min(bx+16, w)/2) <= ceil(w/2).
 #include "vp8g_device.h"
+#include <type_traits>
 #include "vp8g_internal.h"
 
 namespace vp8g {
@@ -102,21 +103,32 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     rowmask0 = kk < bh ? ~0u : 0u;
     rowmask1 = kk + 8 < bh ? ~0u : 0u;
   }
-  // The lane's two rows at byte offset `off`: one alignment for both (the
-  // row pitch is a dword multiple), ten naturally aligned dword reads and
-  // byte funnel shifts (unaligned LDS reads stall the LDS pipe).
-  auto sad_at = [&](int off) -> uint32_t {
+  // The lane's two rows at byte offset `off`.  ALIGNED (off a dword
+  // multiple): four dword reads per row.  Otherwise one alignment for both
+  // rows (the row pitch is a dword multiple), five naturally aligned dword
+  // reads per row and byte funnel shifts (unaligned LDS reads stall the LDS
+  // pipe).  The window's centre column and pitch are dword multiples and the
+  // probes of steps 8 and 4 move by multiples of 4 columns, so those steps
+  // (and the partial block's (0, 0)) are ALIGNED.
+  auto sad_at = [&](int off, auto aligned_tag) -> uint32_t {
+    constexpr bool ALIGNED = decltype(aligned_tag)::value;
     const uint8_t* p = row0 + off;
-    const uint32_t sh = (uint32_t)(uintptr_t)p & 3;
-    // One opaque base for all ten reads: their offsets (at most 400 bytes)
-    // then fit the 8-bit dword offsets of ds_read2_b32.
+    const uint32_t sh = ALIGNED ? 0u : (uint32_t)(uintptr_t)p & 3;
+    // One opaque base for all reads: their offsets (at most 400 bytes) then
+    // fit the 8-bit dword offsets of ds_read2_b32.
     LdsWords w = (LdsWords)((LdsBytes)p - sh);
     asm volatile("" : "+v"(w));
     constexpr int P = 8 * G::lp >> 2;  // row kk + 8
-    const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3], a4 = w[4];
-    const uint32_t b0 = w[P], b1 = w[P + 1], b2 = w[P + 2], b3 = w[P + 3], b4 = w[P + 4];
-    uint4 q0 = make_uint4(funnel(a0, a1, sh), funnel(a1, a2, sh), funnel(a2, a3, sh), funnel(a3, a4, sh));
-    uint4 q1 = make_uint4(funnel(b0, b1, sh), funnel(b1, b2, sh), funnel(b2, b3, sh), funnel(b3, b4, sh));
+    uint4 q0, q1;
+    if (ALIGNED) {
+      q0 = make_uint4(w[0], w[1], w[2], w[3]);
+      q1 = make_uint4(w[P], w[P + 1], w[P + 2], w[P + 3]);
+    } else {
+      const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3], a4 = w[4];
+      const uint32_t b0 = w[P], b1 = w[P + 1], b2 = w[P + 2], b3 = w[P + 3], b4 = w[P + 4];
+      q0 = make_uint4(funnel(a0, a1, sh), funnel(a1, a2, sh), funnel(a2, a3, sh), funnel(a3, a4, sh));
+      q1 = make_uint4(funnel(b0, b1, sh), funnel(b1, b2, sh), funnel(b2, b3, sh), funnel(b3, b4, sh));
+    }
     if (!FULL) {
       q0.x &= colmask[0] & rowmask0; q0.y &= colmask[1] & rowmask0;
       q0.z &= colmask[2] & rowmask0; q0.w &= colmask[3] & rowmask0;
@@ -142,23 +154,28 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     uint32_t z = sad4(zsrc, *reinterpret_cast<const uint32_t*>(zrow), 0);
     bsad = wave_sum_to_sgpr(sum16(z));
   } else {
-    bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0));
+    bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0, std::true_type{}));
   }
   // Probe order is the group index and the incumbent keeps ties, so the key
   // sad << 11 | group << 7 | (probe offset + 64) against best << 11 yields
   // exactly the reference's winner as one unsigned minimum over the wave,
   // and carries the winner's window offset (|offset| <= lp + 1 < 64).
   static_assert(G::lp + 1 < 64, "probe offsets fit the key's 7-bit field");
+  static_assert((G::lp & 3) == 0, "aligned steps need a dword-multiple pitch");
   const uint32_t key_lo = ((uint32_t)gq << 7) | (uint32_t)(gd_off + 64);
-#pragma unroll
-  for (int sh = 3; sh >= 0; sh--) {
-    const uint32_t s = sad_at(boff + gd_off * (1 << sh));
+  auto step = [&](auto sh_tag) {
+    constexpr int sh = decltype(sh_tag)::value;
+    const uint32_t s = sad_at(boff + gd_off * (1 << sh), std::integral_constant<bool, (sh >= 2)>{});
     const uint32_t kmin = wave_min_to_sgpr((s << 11) | key_lo);
     if (kmin < (bsad << 11)) {
       bsad = kmin >> 11;
       boff += ((int)(kmin & 127) - 64) * (1 << sh);
     }
-  }
+  };
+  step(std::integral_constant<int, 3>{});
+  step(std::integral_constant<int, 2>{});
+  step(std::integral_constant<int, 1>{});
+  step(std::integral_constant<int, 0>{});
   // |dx| <= 15 < lp / 2: dy is boff / lp rounded to nearest, dx the rest
   besty = (boff + 16 * G::lp + G::lp / 2) / G::lp - 16;
   bestx = boff - besty * G::lp;
