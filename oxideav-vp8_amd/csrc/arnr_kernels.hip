@@ -617,6 +617,14 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   }
   if (!active || split != 0) return;
 
+  // The lane's pixel offsets are derived again here from the lane id (which
+  // the compiler cannot match with the ones computed before the frame loop),
+  // so that no address term is held across the loop.
+  {
+    const int lane2 = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const int lr = lane2 >> 2, lq = lane2 & 3;
+    const int cr = lane2 >> 3, cc = lane2 & 7;
+
   // Resolve: (16 * ctr + acc + ws / 2) / ws with ws = 16 + wsum.  The
   // numerator n is below 256 * ws and ws lies in [16, 1024] (at most 64
   // frames), so the quotient is a pixel and n / ws has a fractional part of
@@ -647,6 +655,7 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
     const size_t o = (size_t)(s.cby + cr) * a.cw + s.cbx + cc;
     a.out_u[o] = (uint8_t)resolve(s.au, s.cu);
     a.out_v[o] = (uint8_t)resolve(s.av, s.cv);
+  }
   }
 }
 
