@@ -573,20 +573,48 @@ __device__ __forceinline__ uint64_t abs_key(int mvx, int mvy, uint32_t sad) {
 // vector when it lies on the border of the first square around the seed
 // (the minimum may continue past it), then the first-stage vectors of the
 // left, right, upper and lower blocks, each taken once and only when it
-// differs from the seed and the own vector.  One wave per block position, 4
-// per workgroup.
+// differs from the seed and the own vector.  One wave per (neighbour frame,
+// block position), 4 per workgroup, frame-major: the searches of one block
+// are a chain of dependent LDS round trips, so the frames run in parallel
+// waves rather than one after another.
 __global__ void __launch_bounds__(256) refine_candidates_kernel(const SearchArgs a, const int32_t* first) {
   constexpr int kMaxCand = 5;
   __shared__ __align__(16) uint32_t s_win[4][120];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const int per_frame = a.brows * a.bcols;
-  const int rest = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
-  if (rest >= per_frame) return;  // whole wave; no workgroup barrier follows
+  const int item = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
+  if (item >= a.total_blocks) return;  // whole wave; no workgroup barrier follows
+  const int k = item / per_frame, rest = item - k * per_frame;
+  const int fi = k < a.center ? k : k + 1;
   const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
   const int bx = bcol * 16, by = brow * 16;
   const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
   const bool full = bw == 16 && bh == 16;
   uint32_t* win = s_win[wave];
+  // The frame's seed, own first-stage entry and neighbours' vectors (an
+  // absent neighbour repeats the own vector), requested before the centre
+  // block so that the loads overlap.
+  const int32_t* own = first + 3 * ((size_t)fi * per_frame + rest);
+  const int ox = (int)uniform_u32(own), oy = (int)uniform_u32(own + 1);
+  const uint32_t osad = uniform_u32(own + 2);
+  int sx = 0, sy = 0;
+  if (a.seeds) {
+    const int32_t* sd = a.seeds + 3 * ((size_t)fi * per_frame + rest);
+    sx = clampi((int)uniform_u32(sd), -kMaxSeed, kMaxSeed);
+    sy = clampi((int)uniform_u32(sd + 1), -kMaxSeed, kMaxSeed);
+  }
+  int cx[kMaxCand], cy[kMaxCand];
+  {
+    const int nr[4] = {brow, brow, brow - 1, brow + 1}, ncl[4] = {bcol - 1, bcol + 1, bcol, bcol};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const bool in = nr[t] >= 0 && nr[t] < a.brows && ncl[t] >= 0 && ncl[t] < a.bcols;
+      const int32_t* e = first + 3 * ((size_t)fi * per_frame + (size_t)(in ? nr[t] : brow) * a.bcols +
+                                      (in ? ncl[t] : bcol));
+      cx[t + 1] = (int)uniform_u32(e);
+      cy[t + 1] = (int)uniform_u32(e + 1);
+    }
+  }
   uint4 src[4];
   {
     const FrameDesc C = a.frames[a.center];
@@ -606,88 +634,51 @@ __global__ void __launch_bounds__(256) refine_candidates_kernel(const SearchArgs
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) src[rr] = reinterpret_cast<const uint4*>(win)[4 * (l & 3) + rr];
   }
-  const int nnb = a.n - 1;
-  // lane k < nnb: frame k's seed, own first-stage entry and its neighbours'
-  // vectors, all requested at once (an absent neighbour repeats the own
-  // vector)
-  int e_sx = 0, e_sy = 0, e_ox = 0, e_oy = 0, e_osad = 0, e_nx[4], e_ny[4];
-  const int nr[4] = {brow, brow, brow - 1, brow + 1}, ncl[4] = {bcol - 1, bcol + 1, bcol, bcol};
-  if (l < nnb) {
-    const int fi = l < a.center ? l : l + 1;
-    const int32_t* own = first + 3 * ((size_t)fi * per_frame + rest);
-    e_ox = own[0];
-    e_oy = own[1];
-    e_osad = own[2];
-    if (a.seeds) {
-      const int32_t* sd = a.seeds + 3 * ((size_t)fi * per_frame + rest);
-      e_sx = clampi(sd[0], -kMaxSeed, kMaxSeed);
-      e_sy = clampi(sd[1], -kMaxSeed, kMaxSeed);
-    }
+  const FrameDesc F = a.frames[fi];
+  // slot 0: the own vector, searched again only when it lies on the first
+  // square's border (the minimum may continue past it); slots 1..4: left,
+  // right, upper, lower
+  bool use[kMaxCand];
+  cx[0] = ox;
+  cy[0] = oy;
+  use[0] = abs(ox - sx) == 2 || abs(oy - sy) == 2;
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const bool in = nr[t] >= 0 && nr[t] < a.brows && ncl[t] >= 0 && ncl[t] < a.bcols;
-      const int32_t* e = first + 3 * ((size_t)fi * per_frame + (size_t)(in ? nr[t] : brow) * a.bcols +
-                                      (in ? ncl[t] : bcol));
-      e_nx[t] = e[0];
-      e_ny[t] = e[1];
+  for (int t = 1; t < kMaxCand; t++) use[t] = true;
+#pragma unroll
+  for (int t = 0; t < kMaxCand; t++) {
+    cx[t] = clampi(cx[t], -kMaxSeed, kMaxSeed);
+    cy[t] = clampi(cy[t], -kMaxSeed, kMaxSeed);
+    use[t] = use[t] && !(cx[t] == sx && cy[t] == sy);  // that square was searched
+#pragma unroll
+    for (int u = 0; u < t; u++)
+      if (cx[u] == cx[t] && cy[u] == cy[t]) use[t] = false;  // the own vector or an earlier slot
+  }
+  int bxv = ox, byv = oy;
+  uint32_t bsad = osad;
+  uint64_t bkey = abs_key(ox, oy, osad);
+#pragma unroll
+  for (int t = 0; t < kMaxCand; t++) {
+    if (!use[t]) continue;
+    const R2Best r = r2_search_at(a, F, win, src, bx, by, bw, bh, full, cx[t], cy[t], l);
+    const uint64_t key = abs_key(r.mvx, r.mvy, r.sad);
+    if (key < bkey) {
+      bkey = key;
+      bxv = r.mvx;
+      byv = r.mvy;
+      bsad = r.sad;
     }
   }
-  for (int k = 0; k < nnb; k++) {
-    const int fi = k < a.center ? k : k + 1;
-    const FrameDesc F = a.frames[fi];
-    const int ox = __builtin_amdgcn_readlane(e_ox, k), oy = __builtin_amdgcn_readlane(e_oy, k);
-    const uint32_t osad = (uint32_t)__builtin_amdgcn_readlane(e_osad, k);
-    const int sx = __builtin_amdgcn_readlane(e_sx, k), sy = __builtin_amdgcn_readlane(e_sy, k);
-    // slot 0: the own vector, searched again only when it lies on the first
-    // square's border (the minimum may continue past it); slots 1..4: left,
-    // right, upper, lower
-    int cx[kMaxCand], cy[kMaxCand];
-    bool use[kMaxCand];
-    cx[0] = ox;
-    cy[0] = oy;
-    use[0] = abs(ox - sx) == 2 || abs(oy - sy) == 2;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      cx[t + 1] = __builtin_amdgcn_readlane(e_nx[t], k);
-      cy[t + 1] = __builtin_amdgcn_readlane(e_ny[t], k);
-      use[t + 1] = true;
-    }
-#pragma unroll
-    for (int t = 0; t < kMaxCand; t++) {
-      cx[t] = clampi(cx[t], -kMaxSeed, kMaxSeed);
-      cy[t] = clampi(cy[t], -kMaxSeed, kMaxSeed);
-      use[t] = use[t] && !(cx[t] == sx && cy[t] == sy);  // that square was searched
-#pragma unroll
-      for (int u = 0; u < t; u++)
-        if (cx[u] == cx[t] && cy[u] == cy[t]) use[t] = false;  // the own vector or an earlier slot
-    }
-    int bxv = ox, byv = oy;
-    uint32_t bsad = osad;
-    uint64_t bkey = abs_key(ox, oy, osad);
-#pragma unroll
-    for (int t = 0; t < kMaxCand; t++) {
-      if (!use[t]) continue;
-      const R2Best r = r2_search_at(a, F, win, src, bx, by, bw, bh, full, cx[t], cy[t], l);
-      const uint64_t key = abs_key(r.mvx, r.mvy, r.sad);
-      if (key < bkey) {
-        bkey = key;
-        bxv = r.mvx;
-        byv = r.mvy;
-        bsad = r.sad;
-      }
-    }
-    if (l == 0) {
-      int32_t* o = a.out + 3 * ((size_t)fi * per_frame + rest);
-      o[0] = bxv;
-      o[1] = byv;
-      o[2] = (int32_t)bsad;
-    }
+  if (l == 0) {
+    int32_t* o = a.out + 3 * ((size_t)fi * per_frame + rest);
+    o[0] = bxv;
+    o[1] = byv;
+    o[2] = (int32_t)bsad;
   }
 }
 
 hipError_t launch_refine_candidates(const SearchArgs& a, const int32_t* first, hipStream_t stream) {
   if (a.total_blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(refine_candidates_kernel, dim3((a.brows * a.bcols + 3) / 4), dim3(256), 0, stream, a, first);
+  hipLaunchKernelGGL(refine_candidates_kernel, dim3((a.total_blocks + 3) / 4), dim3(256), 0, stream, a, first);
   return hipGetLastError();
 }
 
