@@ -257,3 +257,26 @@ def test_refine_candidates_matches_oracle(gpu, oracle, geom):
     assert (got[center] == -7).all()
     # the second stage never loses against the first
     assert (abs_key(got[others]) <= abs_key(first[others])).all()
+
+
+@pytest.mark.parametrize("geom", [(64, 48, 3, 0), (40, 24, 2, 1)], ids=lambda g: "x".join(map(str, g)))
+def test_refine_candidates_centre_first_without_seeds(gpu, oracle, geom):
+    """Centre frame 0 (every neighbour frame lies after it) and no seed
+    table: the seeds are then the zero vector, as a zero table gives."""
+    import torch
+    w, h, n, pad = geom
+    tuples = tuples_of(w, h, n, 29, pad)
+    center = 0
+    dev = Dev(tuples)
+    zero = np.zeros((n, (h + 15) // 16, (w + 15) // 16, 3), np.int32)
+    first = search(gpu, dev, center, 2)
+    assert_tables(first, expect_table(oracle, tuples, center, 2, zero), "first stage")
+    tf = torch.from_numpy(np.ascontiguousarray(first, np.int32)).cuda()
+    out = dev.table(-7)
+    torch.cuda.synchronize()
+    gpu.refine_candidates_dev(dev.descs, n, center, None, tf.data_ptr(), out.data_ptr())
+    gpu.synchronize()
+    got = out.cpu().numpy()
+    want = expect_candidates(oracle, tuples, center, zero, first)
+    assert_tables(got[1:], want[1:], "candidates")
+    assert (got[0] == -7).all()
