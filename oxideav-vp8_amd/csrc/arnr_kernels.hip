@@ -731,8 +731,8 @@ __device__ __forceinline__ ArnrGroup load_group(const ArnrGroup* p) {
 // Persistent batch kernel: every workgroup loops taking units (one strip of
 // one group) until all chunks are drained; a unit is processed exactly as
 // the single-group kernel processes that strip.
-// At most 72 VGPRs: 7 waves per SIMD.
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7, 7)))
+// At most 64 VGPRs: 8 waves per SIMD.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
 arnr_batch_kernel(const ArnrBatchArgs b) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_unit;
