@@ -234,163 +234,6 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
   }
 }
 
-// Radius 2 (the refinement around a proposal): one wave per block position,
-// 4 per workgroup, walking every neighbour frame.  The centre block sits in
-// registers; the next frame's 20 x 24 window (at its own seed) is loaded
-// into registers while the current one is scored out of LDS.  Lane l < 40
-// takes dy = l / 8, the dx quad j = (l / 4) & 1 and the source rows
-// 4h .. 4h + 3, h = l & 3: 16 v_qsad_pk_u16_u8 give its 4 candidates' partial
-// SADs, two DPP quad swaps add the row quarters.  Candidates outside the
-// +-2 square are forced to 0xFFFF (a real SAD is at most 65280), then two
-// wave minima pick the SAD and, among the candidates holding it, the tie
-// key (|mvx| + |mvy|, mvy, mvx).
-__global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
-  constexpr int R = 2, PD = 6, WROWS = 20, WDW = WROWS * PD;
-  __shared__ __align__(16) uint32_t s_win[4][WDW];
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
-  const int per_frame = a.brows * a.bcols;
-  const int rest = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
-  if (rest >= per_frame) return;  // whole wave; no workgroup barrier follows
-  const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
-  const int bx = bcol * 16, by = brow * 16;
-  const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
-  const bool full = bw == 16 && bh == 16;
-  const int nnb = a.n - 1;
-  uint32_t* win = s_win[wave];
-  const int dy = l >> 3, j = (l >> 2) & 1, h = l & 3;
-  // lane k < nnb holds neighbour k's seed
-  int my_sx = 0, my_sy = 0;
-  if (a.seeds && l < nnb) {
-    const int fi = l < a.center ? l : l + 1;
-    const size_t tab = 3 * ((size_t)fi * per_frame + rest);
-    my_sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
-    my_sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
-  }
-  // the centre block (zero outside it) staged through LDS once, one dword
-  // per lane, then the lane's source rows 4h .. 4h + 3 kept in registers
-  {
-    const FrameDesc C = a.frames[a.center];
-    const int r = l >> 2, q = l & 3;
-    uint32_t v = 0;
-    if (r < bh) {
-      const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
-      if (full && a.aligned4) {
-        v = *reinterpret_cast<const uint32_t*>(row + 4 * q);
-      } else {
-        for (int e = 0; e < 4; e++)
-          if (4 * q + e < bw) v |= (uint32_t)row[4 * q + e] << (8 * e);
-      }
-    }
-    win[l] = v;
-  }
-  wave_lds_sync();
-  uint4 src[4];
-#pragma unroll
-  for (int rr = 0; rr < 4; rr++) src[rr] = reinterpret_cast<const uint4*>(win)[4 * h + rr];
-  wave_lds_sync();  // the source is read before the first window overwrites it
-  // window dwords l and l + 64 (< 120): row s / 6, dword s % 6
-  const int wy0 = l / PD, wd0 = l - PD * wy0;
-  const int s1 = min(l + 64, WDW - 1), wy1 = s1 / PD, wd1 = s1 - PD * wy1;
-  auto floor4 = [](int x) { return x >= 0 ? (x & ~3) : -(((-x) + 3) & ~3); };
-  auto load_window = [&](int k, uint32_t& v0, uint32_t& v1) {
-    const FrameDesc F = a.frames[k < a.center ? k : k + 1];
-    const int sx = __builtin_amdgcn_readlane(my_sx, k), sy = __builtin_amdgcn_readlane(my_sy, k);
-    const int x0a = floor4(bx + sx - R), y0 = by + sy - R;
-    if (a.aligned4 && x0a >= 0 && x0a + 4 * PD <= a.w && y0 >= 0 && y0 + WROWS <= a.h) {
-      const uint8_t* base = F.y + (size_t)y0 * F.y_stride + x0a;
-      v0 = *reinterpret_cast<const uint32_t*>(base + (uint32_t)wy0 * F.y_stride + 4 * wd0);
-      v1 = *reinterpret_cast<const uint32_t*>(base + (uint32_t)wy1 * F.y_stride + 4 * wd1);
-    } else {
-      v0 = gather4_clamped(F.y + (size_t)clampi(y0 + wy0, 0, a.h - 1) * F.y_stride, x0a + 4 * wd0, a.w);
-      v1 = gather4_clamped(F.y + (size_t)clampi(y0 + wy1, 0, a.h - 1) * F.y_stride, x0a + 4 * wd1, a.w);
-    }
-  };
-  uint32_t v0 = 0, v1 = 0;
-  if (nnb > 0) load_window(0, v0, v1);
-  for (int k = 0; k < nnb; k++) {
-    wave_lds_sync();  // the previous frame's window reads are done
-    win[l] = v0;
-    if (l + 64 < WDW) win[l + 64] = v1;
-    wave_lds_sync();
-    const int sx = __builtin_amdgcn_readlane(my_sx, k), sy = __builtin_amdgcn_readlane(my_sy, k);
-    const int x0 = bx + sx - R;
-    const int shift0 = x0 - floor4(x0);
-    if (k + 1 < nnb) load_window(k + 1, v0, v1);  // in flight during the scoring
-    uint64_t acc = 0;
-    if (full) {
-#pragma unroll
-      for (int rr = 0; rr < 4; rr++) {
-        const uint32_t* wr = win + (dy + 4 * h + rr) * PD + j;
-        const uint32_t w0 = wr[0], w1 = wr[1], w2 = wr[2], w3 = wr[3], w4 = wr[4];
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w0 | ((uint64_t)w1 << 32), src[rr].x, acc);
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w1 | ((uint64_t)w2 << 32), src[rr].y, acc);
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w2 | ((uint64_t)w3 << 32), src[rr].z, acc);
-        acc = __builtin_amdgcn_qsad_pk_u16_u8((uint64_t)w3 | ((uint64_t)w4 << 32), src[rr].w, acc);
-      }
-    } else {
-      const int bwq = bw >> 2, rem = bw & 3;
-      const uint32_t remmask = rem ? (0xFFFFFFFFu >> (32 - 8 * rem)) : 0u;
-#pragma unroll
-      for (int rr = 0; rr < 4; rr++) {
-        if (4 * h + rr < bh) {
-          const uint32_t* wr = win + (dy + 4 * h + rr) * PD + j;
-          const uint32_t wv[5] = {wr[0], wr[1], wr[2], wr[3], wr[4]};
-          const uint32_t sv[4] = {src[rr].x, src[rr].y, src[rr].z, src[rr].w};
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const uint64_t pair = (uint64_t)wv[q] | ((uint64_t)wv[q + 1] << 32);
-            if (q < bwq) {
-              acc = __builtin_amdgcn_qsad_pk_u16_u8(pair, sv[q], acc);
-            } else if (q == bwq && rem) {
-              uint64_t add = 0;
-#pragma unroll
-              for (int i = 0; i < 4; i++)
-                add |= (uint64_t)sad4(sv[q], funnel(wv[q], wv[q + 1], i) & remmask, 0) << (16 * i);
-              acc += add;
-            }
-          }
-        }
-      }
-    }
-    // row quarters: the four lanes of a quad (no 16-bit field overflows, so
-    // the halves add independently)
-    uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
-    lo += dpp<kDppQuadXor1>(lo);
-    hi += dpp<kDppQuadXor1>(hi);
-    lo += dpp<kDppQuadXor2>(lo);
-    hi += dpp<kDppQuadXor2>(hi);
-    // candidate i of the lane: dxi = 4j + i - shift0, inside the square iff
-    // 0 <= dxi <= 4; outside ones and idle lanes read 0xFFFF
-    const uint32_t lo_bad = j == 0 ? (shift0 >= 2 ? 0xFFFFFFFFu : (shift0 == 1 ? 0x0000FFFFu : 0u))
-                                   : (shift0 >= 1 ? 0u : 0xFFFF0000u);
-    const uint32_t hi_bad = j == 0 ? (shift0 == 3 ? 0x0000FFFFu : 0u)
-                                   : (shift0 == 3 ? 0u : (shift0 == 2 ? 0xFFFF0000u : 0xFFFFFFFFu));
-    if (l < 40) {
-      lo |= lo_bad;
-      hi |= hi_bad;
-    } else {
-      lo = hi = 0xFFFFFFFFu;
-    }
-    const int mvy = sy - R + dy;
-    const int mvx0 = sx - R + 4 * j - shift0;  // mvx of candidate i is mvx0 + i
-    const int l1_base = abs(sx) + abs(sy) - 2 * R;
-    const uint32_t key_lo = (uint32_t)((dy << 3) + 4 * j - shift0);
-    uint32_t lane_key = 0xFFFFFFFFu;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const uint32_t sad = ((i < 2 ? lo : hi) >> (16 * (i & 1))) & 0xFFFFu;
-      const uint32_t rel = (uint32_t)(abs(mvx0 + i) + abs(mvy) - l1_base);
-      lane_key = min(lane_key, (sad << 10) | (rel << 6) | (key_lo + (uint32_t)i));
-    }
-    const uint32_t best = wave_min_any(lane_key);
-    if (l == 0) {
-      const size_t tab = 3 * ((size_t)(k < a.center ? k : k + 1) * per_frame + rest);
-      a.out[tab] = sx - R + (int)(best & 7);
-      a.out[tab + 1] = sy - R + (int)((best >> 3) & 7);
-      a.out[tab + 2] = (int32_t)(best >> 10);
-    }
-  }
-}
 
 // ---------------------------------------------------------------- pyramid
 // One level of a resolution pyramid for coarse-to-fine search: each output
@@ -560,6 +403,57 @@ __device__ __forceinline__ R2Best r2_search_at(const SearchArgs& a, const FrameD
   return R2Best{sx - R + (int)(best & 7), sy - R + (int)((best >> 3) & 7), best >> 10};
 }
 
+// Exact +-2 search around each block's seed: one wave per (neighbour frame,
+// block position), 4 per workgroup, frame-major.  The centre block is staged
+// through LDS into the lane's source registers, then r2_search_at loads the
+// frame's 20x24 window and scores the 25 vectors.
+__global__ void __launch_bounds__(256) refine2_kernel(const SearchArgs a) {
+  __shared__ __align__(16) uint32_t s_win[4][120];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int per_frame = a.brows * a.bcols;
+  const int item = xcd_band(blockIdx.x, gridDim.x) * 4 + wave;
+  if (item >= a.total_blocks) return;  // whole wave; no workgroup barrier follows
+  const int k = item / per_frame, rest = item - k * per_frame;
+  const int fi = k < a.center ? k : k + 1;
+  const int brow = rest / a.bcols, bcol = rest - brow * a.bcols;
+  const int bx = bcol * 16, by = brow * 16;
+  const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
+  const bool full = bw == 16 && bh == 16;
+  uint32_t* win = s_win[wave];
+  int sx = 0, sy = 0;
+  if (a.seeds) {
+    const int32_t* sd = a.seeds + 3 * ((size_t)fi * per_frame + rest);
+    sx = clampi((int)uniform_u32(sd), -kMaxSeed, kMaxSeed);
+    sy = clampi((int)uniform_u32(sd + 1), -kMaxSeed, kMaxSeed);
+  }
+  uint4 src[4];
+  {
+    const FrameDesc C = a.frames[a.center];
+    const int r = l >> 2, q = l & 3;
+    uint32_t v = 0;
+    if (r < bh) {
+      const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
+      if (full && a.aligned4) {
+        v = *reinterpret_cast<const uint32_t*>(row + 4 * q);
+      } else {
+        for (int e = 0; e < 4; e++)
+          if (4 * q + e < bw) v |= (uint32_t)row[4 * q + e] << (8 * e);
+      }
+    }
+    win[l] = v;
+    wave_lds_sync();
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) src[rr] = reinterpret_cast<const uint4*>(win)[4 * (l & 3) + rr];
+  }
+  const R2Best r = r2_search_at(a, a.frames[fi], win, src, bx, by, bw, bh, full, sx, sy, l);
+  if (l == 0) {
+    int32_t* o = a.out + 3 * ((size_t)fi * per_frame + rest);
+    o[0] = r.mvx;
+    o[1] = r.mvy;
+    o[2] = (int32_t)r.sad;
+  }
+}
+
 // (sad, |mvx| + |mvy|, mvy, mvx) as one ordered 64-bit key (components of at
 // most 16386 in magnitude).
 __device__ __forceinline__ uint64_t abs_key(int mvx, int mvy, uint32_t sad) {
@@ -695,7 +589,7 @@ hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stre
     break;                                                                                         \
   }
     case 2:
-      hipLaunchKernelGGL(refine2_kernel, dim3((a.brows * a.bcols + 3) / 4), dim3(256), 0, stream, a);
+      hipLaunchKernelGGL(refine2_kernel, dim3((a.total_blocks + 3) / 4), dim3(256), 0, stream, a);
       break;
     VP8G_SEARCH_CASE(16)
     VP8G_SEARCH_CASE(64)
