@@ -167,7 +167,8 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     cw, ch = (w + 1) // 2, (h + 1) // 2
     dev = torch.device("cuda:0")
     ex = torch.zeros((n, bh, bw, 3), dtype=torch.int32, device=dev)
-    seeds, first, ref = torch.zeros_like(ex), torch.zeros_like(ex), torch.zeros_like(ex)
+    seeds, first, ref, fb = torch.zeros_like(ex), torch.zeros_like(ex), torch.zeros_like(ex), torch.zeros_like(ex)
+    queued = torch.zeros(1, dtype=torch.int32, device=dev)
     oy = torch.empty(w * h, dtype=torch.uint8, device=dev)
     ou = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
     ov = torch.empty(cw * ch, dtype=torch.uint8, device=dev)
@@ -196,6 +197,14 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
         search_learned()
         ctx.refine_candidates_dev(descs, n, c, seeds.data_ptr(), first.data_ptr(), ref.data_ptr())
 
+    def search_learned_fallback():
+        search_learned()
+        ctx.search_fallback_dev(descs, n, c, first.data_ptr(), fb.data_ptr(), 16, True, queued.data_ptr())
+
+    def altref_learned_fallback():
+        search_learned_fallback()
+        ctx.build_arnr_altref_table_dev(descs, n, c, 3, fb.data_ptr(), oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
+
     def altref_learned():
         search_learned()
         ctx.build_arnr_altref_table_dev(descs, n, c, 3, first.data_ptr(), oy.data_ptr(), ou.data_ptr(), ov.data_ptr())
@@ -211,7 +220,7 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     clean = group.clean[c][0]
     out = {}
     for name, fn in (("descent", altref_descent), ("exhaustive16", altref_exhaustive), ("learned", altref_learned),
-                     ("learned_two_stage", altref_learned_two_stage)):
+                     ("learned_two_stage", altref_learned_two_stage), ("learned_fallback", altref_learned_fallback)):
         ms = timed(fn)
         ctx.synchronize()
         out[name] = {"ms": ms, "anchor_mb_per_s": anchors / (ms * 1e-3),
@@ -219,24 +228,34 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     t_search_ex = timed(lambda: ctx.block_search_dev(descs, n, c, 16, ex.data_ptr()))
     t_search_le = timed(search_learned)
     t_search_le2 = timed(search_learned_two_stage)
+    t_search_fb = timed(search_learned_fallback)
     ctx.synchronize()
     others = [i for i in range(n) if i != c]
     e, r1, r2 = ex.cpu().numpy()[others], first.cpu().numpy()[others], ref.cpu().numpy()[others]
+    r3 = fb.cpu().numpy()[others]
     area = (np.minimum(16, h - 16 * np.arange(bh))[:, None] * np.minimum(16, w - 16 * np.arange(bw))[None, :])
     matched = e[..., 2] < 13 * area[None]
     same = r1[..., 2] == e[..., 2]
     same2 = r2[..., 2] == e[..., 2]
+    same3 = r3[..., 2] == e[..., 2]
     out.update({
         "agreement": float(same.mean()),
         "agreement_matched": float(same[matched].mean()),
         "matched_fraction": float(matched.mean()),
         "psnr_loss_db": out["exhaustive16"]["psnr_y_vs_clean"] - out["learned"]["psnr_y_vs_clean"],
-        "search_ms": {"exhaustive16": t_search_ex, "learned": t_search_le, "learned_two_stage": t_search_le2},
+        "search_ms": {"exhaustive16": t_search_ex, "learned": t_search_le, "learned_two_stage": t_search_le2,
+                      "learned_fallback": t_search_fb},
         "search_speedup": t_search_ex / t_search_le,
         "two_stage": {"agreement_matched": float(same2[matched].mean()),
                       "psnr_loss_db": out["exhaustive16"]["psnr_y_vs_clean"] -
                       out["learned_two_stage"]["psnr_y_vs_clean"],
                       "search_speedup": t_search_ex / t_search_le2},
+        "fallback": {"agreement": float(same3.mean()), "agreement_matched": float(same3[matched].mean()),
+                     "queued_fraction": int(queued.item()) / float(e[..., 2].size),
+                     "psnr_loss_db": out["exhaustive16"]["psnr_y_vs_clean"] -
+                     out["learned_fallback"]["psnr_y_vs_clean"],
+                     "search_speedup": t_search_ex / t_search_fb,
+                     "rule": "exhaustive +-16 for entries failing the occlusion cut-off with a 4-neighbour under it"},
         "weights": str(wpath.name),
     })
     prop.close()

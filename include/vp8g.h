@@ -94,11 +94,24 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
  *   (0 = default 4; reduced automatically to fit 16 waves and 64 KiB of
  *   LDS).
  * VP8G_OPT_BATCH_WGS_PER_CU: persistent workgroups per CU of a batch,
- *   0..32 (0 = as many as fit). */
+ *   0..32 (0 = as many as fit).
+ * VP8G_OPT_FAULT: host-side fault injection for testing callers' error
+ *   paths (no counterpart in the reference).  VP8G_FAULT_ALLOC makes every
+ *   device allocation step of a host-pointer or batch entry point fail,
+ *   VP8G_FAULT_COPY every host<->device copy, VP8G_FAULT_LAUNCH every
+ *   kernel or graph launch.  The failing step reports VP8G_E_DEVICE
+ *   without running (nothing faulty reaches the device) and no output is
+ *   written.  A context starts with the mode named by $VP8G_FAULT
+ *   ("alloc", "copy" or "launch"; anything else = none). */
 #define VP8G_OPT_ARNR_WAVES 1
 #define VP8G_OPT_XCD_REMAP 2
 #define VP8G_OPT_ARNR_MBS 3
 #define VP8G_OPT_BATCH_WGS_PER_CU 4
+#define VP8G_OPT_FAULT 5
+#define VP8G_FAULT_NONE 0
+#define VP8G_FAULT_ALLOC 1
+#define VP8G_FAULT_COPY 2
+#define VP8G_FAULT_LAUNCH 3
 int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);
 
@@ -170,6 +183,24 @@ int vp8g_block_search_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frame
  * `first`; centre rows are left untouched. */
 int vp8g_refine_candidates_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames, size_t center,
                                const int32_t *seeds, const int32_t *first, int32_t *out, vp8g_err *err);
+/* Exhaustive fallback of a seeded search, no counterpart in the reference:
+ * `first` (device) is a seeded search's alignment table (e.g. the radius-2
+ * search around learned proposals).  Every (frame, block) entry whose SAD
+ * fails the occlusion cut-off (sad >= 13 * bw * bh, the reference's
+ * sad / (bw * bh) > 12, src/arnr.rs:603-608) is queued; with
+ * VP8G_FALLBACK_GATE_NEIGHBOUR only those with a left, right, upper or
+ * lower block of the same frame that passes it.  `out` (device, not
+ * `first`) receives `first`, where every queued entry is replaced by the
+ * better of itself and the exhaustive +-radius search around (0, 0)
+ * (vp8g_block_search_dev's semantics; radius 16 or 64) under (sad,
+ * |mvx| + |mvy|, mvy, mvx).  `queued` (device uint32, may be NULL) receives
+ * the number of queued entries.  Centre rows are left untouched.  The
+ * queue lives in a device workspace the context allocates (4 bytes per
+ * block and frame) and keeps.  Validation as vp8g_build_arnr_altref. */
+#define VP8G_FALLBACK_GATE_NEIGHBOUR 1
+int vp8g_search_fallback_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames, size_t center,
+                             const int32_t *first, int radius, int flags, int32_t *out, uint32_t *queued,
+                             vp8g_err *err);
 /* The altref of build_arnr_altref (src/arnr.rs:488-650) with the search
  * replaced by a given alignment table (device, the layout above; the
  * centre's rows are not read): occlusion guard on the table's SAD, blend

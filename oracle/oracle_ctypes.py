@@ -79,6 +79,8 @@ class Oracle:
         L.orc_block_search.restype = None
         L.orc_block_search_group.argtypes = [C.POINTER(I420), sz, sz, i32, vp, sz, vp]
         L.orc_block_search_group.restype = i32
+        L.orc_search_fallback.argtypes = [C.POINTER(I420), sz, sz, vp, i32, i32, vp, vp]
+        L.orc_search_fallback.restype = i32
         L.orc_build_arnr_altref_mv.argtypes = [C.POINTER(I420), sz, sz, u8, vp, vp, vp, vp,
                                                C.POINTER(Err)]
         L.orc_build_arnr_altref_mv.restype = i32
@@ -156,6 +158,18 @@ class Oracle:
                                              _p(sd) if sd is not None else None, threads, _p(out))
         assert st == 0, st
         return out
+
+    def search_fallback(self, frames, center, first, radius=16, gate=True):
+        """Exhaustive fallback over the table `first` (n, brows, bcols, 3):
+        returns (out table, queued count)."""
+        keep: list = []
+        arr = self._arr(frames, keep)
+        f = np.ascontiguousarray(first, np.int32)
+        out = f.copy()
+        q = C.c_uint32(0)
+        st = self.lib.orc_search_fallback(arr, len(frames), center, _p(f), radius, int(gate), _p(out), C.byref(q))
+        assert st == 0, st
+        return out, int(q.value)
 
     def build_arnr_altref_mv(self, frames, center, strength, table):
         """Altref from a given alignment table (n, brows, bcols, 3)."""

@@ -290,6 +290,66 @@ int orc_block_search_group(const orc_i420 *frames, size_t n, size_t center, int 
     return ORC_OK;
 }
 
+/* Exhaustive fallback of a seeded search (project-defined; the occlusion
+ * cut-off it keys on is the reference's sad / (bw * bh) > 12,
+ * src/arnr.rs:603-608).  Entries of `first` whose SAD fails the cut-off are
+ * queued (with `gate`, only when a left, right, upper or lower block of the
+ * same frame passes it) and take the better of themselves and the
+ * exhaustive +-radius search around (0, 0) under (sad, |mvx| + |mvy|, mvy,
+ * mvx); every other entry is copied.  `queued` receives the queue length. */
+static int entry_fails(const int32_t *e, int area) { return (uint32_t)e[2] >= 13u * (uint32_t)area; }
+
+static int key_less(const int32_t *a, const int32_t *b)
+{
+    if ((uint32_t)a[2] != (uint32_t)b[2]) return (uint32_t)a[2] < (uint32_t)b[2];
+    const int la = abs(a[0]) + abs(a[1]), lb = abs(b[0]) + abs(b[1]);
+    if (la != lb) return la < lb;
+    if (a[1] != b[1]) return a[1] < b[1];
+    return a[0] < b[0];
+}
+
+int orc_search_fallback(const orc_i420 *frames, size_t n, size_t center, const int32_t *first, int radius,
+                        int gate, int32_t *out, uint32_t *queued)
+{
+    if (center >= n || radius < 0 || !first || !out) return ORC_E_ARG;
+    const orc_i420 *ctr = &frames[center];
+    for (size_t i = 0; i < n; i++)
+        if (frames[i].width != ctr->width || frames[i].height != ctr->height) return ORC_E_ARG;
+    const int w = (int)ctr->width, h = (int)ctr->height, bcols = (w + 15) / 16, brows = (h + 15) / 16;
+    uint32_t count = 0;
+    for (size_t fi = 0; fi < n; fi++) {
+        if (fi == center) continue;
+        const size_t base = fi * (size_t)brows * (size_t)bcols;
+        for (int brow = 0; brow < brows; brow++) {
+            for (int bcol = 0; bcol < bcols; bcol++) {
+                const size_t at = 3 * (base + (size_t)brow * bcols + bcol);
+                const int bx = bcol * 16, by = brow * 16;
+                const int bw = w - bx < 16 ? w - bx : 16, bh = h - by < 16 ? h - by : 16;
+                memcpy(out + at, first + at, 3 * sizeof(int32_t));
+                int q = entry_fails(first + at, bw * bh);
+                if (q && gate) {
+                    const int nr[4] = {brow, brow, brow - 1, brow + 1}, nc[4] = {bcol - 1, bcol + 1, bcol, bcol};
+                    int any = 0;
+                    for (int t = 0; t < 4; t++) {
+                        if (nr[t] < 0 || nr[t] >= brows || nc[t] < 0 || nc[t] >= bcols) continue;
+                        const int nbw = w - 16 * nc[t] < 16 ? w - 16 * nc[t] : 16;
+                        const int nbh = h - 16 * nr[t] < 16 ? h - 16 * nr[t] : 16;
+                        if (!entry_fails(first + 3 * (base + (size_t)nr[t] * bcols + nc[t]), nbw * nbh)) any = 1;
+                    }
+                    q = any;
+                }
+                if (!q) continue;
+                count++;
+                int32_t res[3];
+                orc_block_search(ctr, &frames[fi], bx, by, bw, bh, 0, 0, radius, res);
+                if (key_less(res, first + at)) memcpy(out + at, res, sizeof(res));
+            }
+        }
+    }
+    if (queued) *queued = count;
+    return ORC_OK;
+}
+
 static int arnr_core(const orc_i420 *frames, size_t n, size_t center, uint8_t strength,
                      size_t threads, uint8_t *out_y, uint8_t *out_u, uint8_t *out_v, orc_err *err,
                      int32_t *aligned_out, const int32_t *given);

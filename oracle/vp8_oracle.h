@@ -63,6 +63,11 @@ void orc_block_search(const orc_i420 *src, const orc_i420 *ref, int bx, int by, 
                       int sx, int sy, int radius, int32_t out3[3]);
 int orc_block_search_group(const orc_i420 *frames, size_t n, size_t center, int radius,
                            const int32_t *seeds, size_t threads, int32_t *out);
+/* Exhaustive fallback of a seeded search over `first` (table layout);
+ * gate = 1 queues only failed entries with a 4-neighbour passing the
+ * occlusion cut-off. */
+int orc_search_fallback(const orc_i420 *frames, size_t n, size_t center, const int32_t *first, int radius,
+                        int gate, int32_t *out, uint32_t *queued);
 int orc_build_arnr_altref_mv(const orc_i420 *frames, size_t n, size_t center, uint8_t strength,
                              const int32_t *table, uint8_t *out_y, uint8_t *out_u,
                              uint8_t *out_v, orc_err *err);

@@ -1,12 +1,15 @@
 """Build the gfx950 library (libvp8g.so) in-tree with hipcc.
 
-Usage: python oxideav-vp8_amd/build.py
+Every translation unit is compiled in parallel into a temporary directory,
+then linked.  Usage: python oxideav-vp8_amd/build.py
 """
 from __future__ import annotations
 
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -30,12 +33,25 @@ SOURCES = ["capi.hip", "arnr_kernels.hip", "mc_kernels.hip", "search_kernels.hip
 
 
 def build(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / s) for s in SOURCES]
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), *srcs, "-ldl"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with tempfile.TemporaryDirectory(prefix="vp8g_build_") as tmpdir:
+        objs = [Path(tmpdir) / (Path(s).stem + ".o") for s in SOURCES]
+
+        def compile_one(i: int) -> None:
+            src = CSRC / SOURCES[i]
+            lang = ["-x", "hip"] if src.suffix == ".cpp" else []
+            cmd = [HIPCC, *HIP_FLAGS, *lang, "-c", str(src), "-o", str(objs[i])]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+
+        with ThreadPoolExecutor(jobs) as pool:
+            list(pool.map(compile_one, range(len(SOURCES))))
+        tmp = LIB.with_suffix(".so.tmp")
+        cmd = [HIPCC, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", str(tmp), *map(str, objs), "-ldl"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
 

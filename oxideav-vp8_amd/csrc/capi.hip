@@ -30,7 +30,26 @@ namespace {
 
 bool hip_ok(hipError_t e) { return e == hipSuccess; }
 
+// Host-side fault injection (VP8G_OPT_FAULT / $VP8G_FAULT): the chosen step
+// reports a HIP failure without running, so the error path is exercised
+// with no faulty device work.
+template <class F>
+int launch_status(const vp8g_ctx* c, F&& launch) {
+  if (c->fault == VP8G_FAULT_LAUNCH) return VP8G_E_DEVICE;
+  return launch() == hipSuccess ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int fault_from_env() {
+  const char* e = getenv("VP8G_FAULT");
+  if (!e) return VP8G_FAULT_NONE;
+  if (!strcmp(e, "alloc")) return VP8G_FAULT_ALLOC;
+  if (!strcmp(e, "copy")) return VP8G_FAULT_COPY;
+  if (!strcmp(e, "launch")) return VP8G_FAULT_LAUNCH;
+  return VP8G_FAULT_NONE;
+}
+
 int ensure_arena(vp8g_ctx* c, size_t bytes) {
+  if (c->fault == VP8G_FAULT_ALLOC) return VP8G_E_DEVICE;
   if (bytes <= c->arena_bytes) return VP8G_OK;
   if (c->arena) {
     if (!hip_ok(hipStreamSynchronize(c->stream))) return VP8G_E_DEVICE;
@@ -52,6 +71,44 @@ int ensure_arena(vp8g_ctx* c, size_t bytes) {
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+int ensure_work(vp8g_ctx* c, size_t bytes) {
+  if (c->fault == VP8G_FAULT_ALLOC) return VP8G_E_DEVICE;
+  if (bytes <= c->work_bytes) return VP8G_OK;
+  if (c->work) {
+    if (!hip_ok(hipStreamSynchronize(c->stream))) return VP8G_E_DEVICE;
+    (void)hipFree(c->work);
+    c->work = nullptr;
+    c->work_bytes = 0;
+  }
+  const size_t want = bytes + (bytes >> 2);
+  if (!hip_ok(hipMalloc(&c->work, want))) return VP8G_E_DEVICE;
+  c->work_bytes = want;
+  return VP8G_OK;
+}
+
+// Search arguments of a validated group (frames, geometry, seeds, output).
+SearchArgs make_search_args(const vp8g_i420* frames, size_t n, size_t center, const int32_t* seeds, int32_t* out) {
+  SearchArgs a;
+  memset(&a, 0, sizeof(a));
+  bool aligned4 = true;
+  for (size_t i = 0; i < n; i++) {
+    const vp8g_i420& f = frames[i];
+    a.frames[i] = FrameDesc{f.y, f.u, f.v, (uint32_t)f.y_stride, (uint32_t)f.uv_stride};
+    if (((uintptr_t)f.y & 3) || (f.y_stride & 3)) aligned4 = false;
+  }
+  a.seeds = seeds;
+  a.out = out;
+  a.w = (int)frames[center].width;
+  a.h = (int)frames[center].height;
+  a.bcols = (a.w + 15) / 16;
+  a.brows = (a.h + 15) / 16;
+  a.n = (int)n;
+  a.center = (int)center;
+  a.aligned4 = aligned4 ? 1 : 0;
+  a.total_blocks = (int)(n - 1) * a.bcols * a.brows;
+  return a;
+}
 
 // Rows of `w` bytes from a strided host plane into a packed run of pitch `p`.
 void pack_plane(uint8_t* dst, size_t p, const uint8_t* src, size_t stride, size_t w, size_t h) {
@@ -85,6 +142,40 @@ int validate_arnr(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* er
     if (f.y_stride > 0xFFFFFFFFu || f.uv_stride > 0xFFFFFFFFu) return VP8G_E_ARG;
   }
   return VP8G_OK;
+}
+
+// Validation shared by the fallback entry points (reference order first).
+int check_fallback(const vp8g_i420* frames, size_t n, size_t center, const int32_t* first, int radius, int flags,
+                   const int32_t* out, const uint32_t* queued, vp8g_err* err) {
+  const int st = validate_arnr(frames, n, center, err);
+  if (st != VP8G_OK) return st;
+  if (!first || !out || first == out || (radius != 16 && radius != 64)) return VP8G_E_ARG;
+  if ((flags & ~VP8G_FALLBACK_GATE_NEIGHBOUR) != 0) return VP8G_E_ARG;
+  if (((uintptr_t)first | (uintptr_t)out | (uintptr_t)queued) & 3) return VP8G_E_ARG;
+  return VP8G_OK;
+}
+
+// Enqueue the fallback with its queue and counter at `scratch` (device,
+// 256-byte aligned, 256 + 4 * total_blocks bytes).
+int enqueue_fallback(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, const int32_t* first, int radius,
+                     int flags, int32_t* out, uint32_t* queued, uint8_t* scratch) {
+  const SearchArgs a = make_search_args(frames, n, center, nullptr, nullptr);
+  FallbackArgs f;
+  memset(&f, 0, sizeof(f));
+  f.first = first;
+  f.out = out;
+  f.count = reinterpret_cast<uint32_t*>(scratch);
+  f.queue = reinterpret_cast<uint32_t*>(scratch + 256);
+  f.w = a.w;
+  f.h = a.h;
+  f.bcols = a.bcols;
+  f.brows = a.brows;
+  f.n = a.n;
+  f.center = a.center;
+  f.total_blocks = a.total_blocks;
+  f.gate = (flags & VP8G_FALLBACK_GATE_NEIGHBOUR) ? 1 : 0;
+  f.queued = queued;
+  return launch_status(c, [&] { return launch_search_fallback(a, f, radius, c->cus, c->stream); });
 }
 
 // Shrink (M, K) until M * K <= 16 waves and the workgroup's LDS fits:
@@ -197,7 +288,7 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &k, &m);
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
-  return hip_ok(launch_arnr_fused(a, k, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_arnr_fused(a, k, c->stream); });
 }
 
 int check_ref_planes(const vp8g_ref_planes* r) {
@@ -260,6 +351,7 @@ size_t plane_span(size_t stride, size_t width, size_t rows) { return rows ? (row
 // Host <-> arena copies through the pinned mirror (same offset as the
 // device pointer in the arena).  Outputs reach the caller in finish().
 int copy_out(vp8g_ctx* c, void* host, const void* dev, size_t bytes) {
+  if (c->fault == VP8G_FAULT_COPY) return VP8G_E_DEVICE;
   if (!bytes) return VP8G_OK;
   const size_t off = static_cast<const uint8_t*>(dev) - c->arena;
   if (!hip_ok(hipMemcpyAsync(c->pinned + off, dev, bytes, hipMemcpyDeviceToHost, c->stream))) return VP8G_E_DEVICE;
@@ -268,6 +360,7 @@ int copy_out(vp8g_ctx* c, void* host, const void* dev, size_t bytes) {
 }
 
 int copy_in(vp8g_ctx* c, void* dev, const void* host, size_t bytes) {
+  if (c->fault == VP8G_FAULT_COPY) return VP8G_E_DEVICE;
   if (!bytes) return VP8G_OK;
   const size_t off = static_cast<uint8_t*>(dev) - c->arena;
   memcpy(c->pinned + off, host, bytes);
@@ -323,6 +416,7 @@ int vp8g_ctx_create(int device, vp8g_ctx** out) {
     return VP8G_E_DEVICE;
   }
   c->stream = c->own;
+  c->fault = fault_from_env();
   // A missing or unreadable table leaves the built-in defaults.
   const std::string tp = default_tuning_path();
   if (!tp.empty()) (void)load_tuning_file(tp.c_str(), &c->tuning);
@@ -373,6 +467,7 @@ void vp8g_ctx_destroy(vp8g_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->arena) (void)hipFree(c->arena);
   if (c->pinned) (void)hipHostFree(c->pinned);  // allocated with the arena
+  if (c->work) (void)hipFree(c->work);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -408,6 +503,10 @@ int vp8g_ctx_set_option(vp8g_ctx* c, int option, int value) {
     case VP8G_OPT_BATCH_WGS_PER_CU:
       if (value < 0 || value > 32) return VP8G_E_ARG;
       c->batch_wgs_per_cu = value;
+      return VP8G_OK;
+    case VP8G_OPT_FAULT:
+      if (value < VP8G_FAULT_NONE || value > VP8G_FAULT_LAUNCH) return VP8G_E_ARG;
+      c->fault = value;
       return VP8G_OK;
     default:
       return VP8G_E_ARG;
@@ -453,7 +552,7 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
     pack_plane(pin, yp, f.y, f.y_stride, w, h);
     pack_plane(pin + ysz, cp, f.u, f.uv_stride, cw, ch);
     pack_plane(pin + ysz + csz, cp, f.v, f.uv_stride, cw, ch);
-    if (!hip_ok(hipMemcpyAsync(base, pin, per, hipMemcpyHostToDevice, c->stream)))
+    if (c->fault == VP8G_FAULT_COPY || !hip_ok(hipMemcpyAsync(base, pin, per, hipMemcpyHostToDevice, c->stream)))
       return finish(c, VP8G_E_DEVICE);
     dev[i] = vp8g_i420{f.width, f.height, base, base + ysz, base + ysz + csz, yp, cp};
   }
@@ -493,7 +592,19 @@ int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t
   a.center = (int)center;
   a.aligned4 = aligned4 ? 1 : 0;
   a.total_blocks = (int)(n - 1) * a.bcols * a.brows;
-  return hip_ok(launch_block_search(a, radius, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_block_search(a, radius, c->stream); });
+}
+
+int vp8g_search_fallback_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, const int32_t* first,
+                             int radius, int flags, int32_t* out, uint32_t* queued, vp8g_err* err) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_fallback(frames, n, center, first, radius, flags, out, queued, err);
+  if (st != VP8G_OK) return st;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const size_t per_frame = (size_t)((frames[center].width + 15) / 16) * ((frames[center].height + 15) / 16);
+  st = ensure_work(c, 256 + 4 * (n - 1) * per_frame);
+  if (st != VP8G_OK) return st;
+  return enqueue_fallback(c, frames, n, center, first, radius, flags, out, queued, c->work);
 }
 
 int vp8g_refine_candidates_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, const int32_t* seeds,
@@ -522,7 +633,7 @@ int vp8g_refine_candidates_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, s
   a.center = (int)center;
   a.aligned4 = aligned4 ? 1 : 0;
   a.total_blocks = (int)(n - 1) * a.bcols * a.brows;
-  return hip_ok(launch_refine_candidates(a, first, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_refine_candidates(a, first, c->stream); });
 }
 
 int vp8g_downsample2_i420_dev(vp8g_ctx* c, const vp8g_i420* src, const vp8g_i420* dst, size_t n) {
@@ -552,7 +663,7 @@ int vp8g_downsample2_i420_dev(vp8g_ctx* c, const vp8g_i420* src, const vp8g_i420
                                 const_cast<uint8_t*>(d.v), (uint32_t)s.y_stride, (uint32_t)s.uv_stride,
                                 (uint32_t)d.y_stride, (uint32_t)d.uv_stride};
     }
-    if (!hip_ok(launch_downsample2(a, cnt, c->stream))) return VP8G_E_DEVICE;
+    if (launch_status(c, [&] { return launch_downsample2(a, cnt, c->stream); }) != VP8G_OK) return VP8G_E_DEVICE;
   }
   return VP8G_OK;
 }
@@ -563,7 +674,7 @@ int vp8g_upsample_table_dev(vp8g_ctx* c, const int32_t* coarse, size_t n, int cw
   if (cw <= 0 || ch <= 0 || fw <= 0 || fh <= 0 || fw > 0x3FFF || fh > 0x3FFF) return VP8G_E_ARG;
   if (((uintptr_t)coarse | (uintptr_t)fine) & 3) return VP8G_E_ARG;
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
-  return hip_ok(launch_upsample_table(coarse, cw, ch, fine, fw, fh, (int)n, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_upsample_table(coarse, cw, ch, fine, fw, fh, (int)n, c->stream); });
 }
 
 int vp8g_build_arnr_altref_table_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
@@ -582,7 +693,7 @@ int vp8g_build_arnr_altref_table_dev(vp8g_ctx* c, const vp8g_i420* frames, size_
   pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &k, &m);
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
-  return hip_ok(launch_arnr_table(a, table, k, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_arnr_table(a, table, k, c->stream); });
 }
 
 int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
@@ -640,7 +751,7 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
   const size_t o_ticket = o_units + align_up(sizeof(uint16_t) * units, 256);
   const size_t bytes = o_ticket + 256;
   hipStream_t cs = nullptr;
-  bool ok = hip_ok(hipMalloc(&b->dev, bytes)) &&
+  bool ok = c->fault != VP8G_FAULT_ALLOC && hip_ok(hipMalloc(&b->dev, bytes)) && c->fault != VP8G_FAULT_COPY &&
             hip_ok(hipMemcpyAsync(b->dev + o_jobs, hj.data(), sizeof(ArnrBatchJob) * nj,
                                   hipMemcpyHostToDevice, c->stream)) &&
             hip_ok(hipMemcpyAsync(b->dev + o_frames, hf.data(), sizeof(FrameDesc) * total_frames,
@@ -661,7 +772,7 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
     ok = hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) &&
          hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     if (ok) {
-      const bool launched = hip_ok(launch_zero_u32(a.ticket, kBatchChunks, cs)) &&
+      const bool launched = launch_status(c, [&] { return launch_zero_u32(a.ticket, kBatchChunks, cs); }) == VP8G_OK &&
                             hip_ok(launch_arnr_batch(a, k, (int)wgs, cs));
       ok = hip_ok(hipStreamEndCapture(cs, &b->graph)) && launched;
     }
@@ -679,7 +790,7 @@ int vp8g_arnr_batch_create(vp8g_ctx* c, const vp8g_arnr_job* jobs, size_t nj,
 int vp8g_arnr_batch_run(vp8g_arnr_batch* b) {
   if (!b) return VP8G_E_ARG;
   if (!hip_ok(hipSetDevice(b->ctx->device))) return VP8G_E_DEVICE;
-  return hip_ok(hipGraphLaunch(b->exec, b->ctx->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(b->ctx, [&] { return hipGraphLaunch(b->exec, b->ctx->stream); });
 }
 
 int vp8g_arnr_batch_shape(const vp8g_arnr_batch* b, int* mbs, int* waves, int* workgroups,
@@ -710,7 +821,7 @@ int vp8g_predict_inter_mb_batch_dev(vp8g_ctx* c, const vp8g_ref_planes* ref,
   if (((uintptr_t)mb_xy | (uintptr_t)mv | (uintptr_t)out) & 3) return VP8G_E_ARG;
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
   McArgs a = make_mc_args(ref, mb_xy, mv, n, full_pixel, filters, out);
-  return hip_ok(launch_predict_inter_mb(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_predict_inter_mb(a, c->stream); });
 }
 
 int vp8g_predict_inter_mb_batch(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy,
@@ -744,7 +855,7 @@ int vp8g_predict_inter_mb_batch(vp8g_ctx* c, const vp8g_ref_planes* ref, const u
   McArgs a = make_mc_args(&d, reinterpret_cast<const uint32_t*>(b + o_xy),
                           reinterpret_cast<const vp8g_mv*>(b + o_mv), n, full_pixel, filters,
                           b + o_out);
-  st = hip_ok(launch_predict_inter_mb(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  st = launch_status(c, [&] { return launch_predict_inter_mb(a, c->stream); });
   if (st == VP8G_OK) st = copy_out(c, out, b + o_out, 384 * n);
   return finish(c, st);
 }
@@ -760,7 +871,7 @@ int vp8g_mb_luma_sad_at_mv_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref,
   if (((uintptr_t)mb_xy | (uintptr_t)src | (uintptr_t)mv | (uintptr_t)sad) & 3) return VP8G_E_ARG;
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
   SadArgs a = make_sad_args(ref, mb_xy, src, mv, n, sad);
-  return hip_ok(launch_mb_luma_sad_at_mv(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_mb_luma_sad_at_mv(a, c->stream); });
 }
 
 // Host staging shared by the SAD-family entry points: plane, mb_xy, src,
@@ -804,7 +915,7 @@ int vp8g_mb_luma_sad_at_mv_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const ui
   st = stage_luma_batch(c, ref, mb_xy, src, mv, n, 4 * n, &d, &dxy, &dsrc, &dmv, &dout);
   if (st != VP8G_OK) return finish(c, st);
   SadArgs a = make_sad_args(&d, dxy, dsrc, dmv, n, reinterpret_cast<uint32_t*>(dout));
-  st = hip_ok(launch_mb_luma_sad_at_mv(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  st = launch_status(c, [&] { return launch_mb_luma_sad_at_mv(a, c->stream); });
   if (st == VP8G_OK) st = copy_out(c, sad, dout, 4 * n);
   return finish(c, st);
 }
@@ -841,7 +952,7 @@ static int refine_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* m
   a.out = reinterpret_cast<uint32_t*>(dout);
   a.n = (int)n;
   a.step = step;
-  st = hip_ok(launch_refine_ladder(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  st = launch_status(c, [&] { return launch_refine_ladder(a, c->stream); });
   if (st == VP8G_OK) st = copy_out(c, out, dout, 8 * n);
   return finish(c, st);
 }
@@ -878,7 +989,7 @@ int vp8g_half_plane_sads_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const int3
   a.n = (int)n;
   a.diagonals = diagonals ? 1 : 0;
   memcpy(a.taps, taps, sizeof(a.taps));
-  st = hip_ok(launch_half_plane_sads(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  st = launch_status(c, [&] { return launch_half_plane_sads(a, c->stream); });
   if (st == VP8G_OK) st = copy_out(c, sad, b + o_out, 32 * n);
   return finish(c, st);
 }
@@ -904,7 +1015,7 @@ static int refine_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_
   a.out = reinterpret_cast<uint32_t*>(out);
   a.n = (int)n;
   a.step = step;
-  return hip_ok(launch_refine_ladder(a, c->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  return launch_status(c, [&] { return launch_refine_ladder(a, c->stream); });
 }
 
 int vp8g_half_pixel_refine_luma_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,

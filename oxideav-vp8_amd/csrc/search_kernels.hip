@@ -118,21 +118,26 @@ __device__ __forceinline__ void accumulate(const uint32_t* win, int j, int db, c
   }
 }
 
+// Shared memory of one block-search workgroup.
 template <int R>
-__global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const SearchArgs a) {
+struct SearchSmem {
+  uint32_t win[SearchGeom<R>::BPW][SearchGeom<R>::WIN_DW];
+  uint64_t red[SearchGeom<R>::G > 64 ? SearchGeom<R>::G / 64 : 1];
+  alignas(16) uint32_t src[SearchGeom<R>::G > 64 ? SearchGeom<R>::BPW * 64 : 4];
+};
+
+// The exhaustive +-R search of one (frame, block) item `b` (frame-major
+// over the non-centre frames) by the lane group of workgroup slot `slot`.
+// Every thread of the workgroup calls it (it holds workgroup barriers);
+// the returned key (sad, |mv|_1, mvy, mvx) is valid in every lane of the
+// group.  `*tab` receives the item's table offset.
+template <int R>
+__device__ __forceinline__ uint64_t search_item(const SearchArgs& a, SearchSmem<R>& sm, int b, int slot, int l,
+                                                size_t* tab_out) {
   using SG = SearchGeom<R>;
   constexpr int G = SG::G, K = SG::K, J = SG::J;
-  __shared__ uint32_t s_win[SG::BPW][SG::WIN_DW];
-  __shared__ uint64_t s_red[SG::G > 64 ? SG::G / 64 : 1];
-  __shared__ __align__(16) uint32_t s_src[SG::G > 64 ? SG::BPW * 64 : 4];
   const int tid = threadIdx.x;
-  const int slot = tid / G;         // block slot in the workgroup
-  const int l = tid - slot * G;     // lane within the block's group
   const int per_frame = a.brows * a.bcols;
-  const long long bi = (long long)blockIdx.x * SG::BPW + slot;
-  const bool live = bi < a.total_blocks;
-  // Out-of-range slots still run the staging/barriers with a valid block.
-  const int b = live ? (int)bi : 0;
   const int kf = b / per_frame;
   const int fi = kf < a.center ? kf : kf + 1;
   const int rest = b - kf * per_frame;
@@ -141,6 +146,7 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
   const int bw = min(16, a.w - bx), bh = min(16, a.h - by);
   int sx = 0, sy = 0;
   const size_t tab = 3 * ((size_t)fi * per_frame + rest);
+  *tab_out = tab;
   if (a.seeds) {
     sx = clampi(a.seeds[tab], -kMaxSeed, kMaxSeed);
     sy = clampi(a.seeds[tab + 1], -kMaxSeed, kMaxSeed);
@@ -153,7 +159,7 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
   const int y0 = by + sy - R;
 
   // Stage the clamped window.
-  uint32_t* win = s_win[slot];
+  uint32_t* win = sm.win[slot];
   const bool interior = a.aligned4 && x0a >= 0 && x0a + 4 * SG::PD <= a.w;
   for (int s = l; s < SG::WIN_DW; s += G) {
     const int wy = s / SG::PD, wd = s - wy * SG::PD;
@@ -178,7 +184,7 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
   };
   uint32_t S[16][4];
   if (G > 64) {
-    if (l < 64) s_src[slot * 64 + l] = src_dword(l >> 2, l & 3);
+    if (l < 64) sm.src[slot * 64 + l] = src_dword(l >> 2, l & 3);
   } else {
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -191,7 +197,7 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
     }
   }
   __syncthreads();
-  const SrcRows<R> src{S, s_src + (G > 64 ? slot * 64 : 0)};
+  const SrcRows<R> src{S, sm.src + (G > 64 ? slot * 64 : 0)};
 
   const int j = l % J, db = l / J;
   uint64_t best = ~0ull;
@@ -221,17 +227,135 @@ __global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const S
 #pragma unroll
   for (int m = 1; m < (G < 64 ? G : 64); m <<= 1) best = min_u64(best, shfl_xor_u64(best, m));
   if (G > 64) {
-    if ((tid & 63) == 0) s_red[tid >> 6] = best;
+    if ((tid & 63) == 0) sm.red[tid >> 6] = best;
     __syncthreads();
-    best = s_red[0];
+    best = sm.red[0];
 #pragma unroll
-    for (int wv = 1; wv < (G > 64 ? G / 64 : 1); wv++) best = min_u64(best, s_red[wv]);
+    for (int wv = 1; wv < (G > 64 ? G / 64 : 1); wv++) best = min_u64(best, sm.red[wv]);
   }
-  if (live && l == 0) {
-    a.out[tab] = (int32_t)(best & 0xFFFF) - 32768;
-    a.out[tab + 1] = (int32_t)((best >> 16) & 0xFFFF) - 32768;
-    a.out[tab + 2] = (int32_t)(best >> 48);
+  return best;
+}
+
+__device__ __forceinline__ void store_key(int32_t* o, uint64_t key) {
+  o[0] = (int32_t)(key & 0xFFFF) - 32768;
+  o[1] = (int32_t)((key >> 16) & 0xFFFF) - 32768;
+  o[2] = (int32_t)(key >> 48);
+}
+
+template <int R>
+__global__ void __launch_bounds__(SearchGeom<R>::WG) block_search_kernel(const SearchArgs a) {
+  using SG = SearchGeom<R>;
+  __shared__ SearchSmem<R> sm;
+  const int tid = threadIdx.x;
+  const int slot = tid / SG::G;         // block slot in the workgroup
+  const int l = tid - slot * SG::G;     // lane within the block's group
+  const long long bi = (long long)blockIdx.x * SG::BPW + slot;
+  const bool live = bi < a.total_blocks;
+  // Out-of-range slots still run the staging/barriers with a valid block.
+  size_t tab;
+  const uint64_t best = search_item<R>(a, sm, live ? (int)bi : 0, slot, l, &tab);
+  if (live && l == 0) store_key(a.out + tab, best);
+}
+
+// ---------------------------------------------------------------- fallback
+// Exhaustive fallback of a seeded search.  fallback_mark_kernel copies the
+// first-stage table to the output and queues every (frame, block) item whose
+// first-stage SAD fails the occlusion cut-off (sad >= 13 * bw * bh, the
+// reference's sad / (bw * bh) > 12, src/arnr.rs:603-608) and, with the
+// neighbour gate, has a 4-neighbour in the same frame that passes it.  One
+// atomic per wave appends the wave's queued items.  fallback_search_kernel
+// then runs the exhaustive +-R search around (0, 0) of every queued item
+// (persistent workgroups walking the queue) and keeps the better of that and
+// the first-stage entry under (sad, |mvx| + |mvy|, mvy, mvx).
+__device__ __forceinline__ bool entry_fails(const int32_t* e, int area) {
+  return (uint32_t)e[2] >= (kSadPerPixelCutoff + 1) * (uint32_t)area;
+}
+
+__global__ void __launch_bounds__(256) fallback_mark_kernel(const FallbackArgs f) {
+  const int item = blockIdx.x * 256 + threadIdx.x;
+  const int per_frame = f.brows * f.bcols;
+  bool queue = false;
+  if (item < f.total_blocks) {
+    const int k = item / per_frame, rest = item - k * per_frame;
+    const int fi = k < f.center ? k : k + 1;
+    const int brow = rest / f.bcols, bcol = rest - brow * f.bcols;
+    const size_t base = (size_t)fi * per_frame;
+    const int32_t* e = f.first + 3 * (base + rest);
+    int32_t* o = f.out + 3 * (base + rest);
+    o[0] = e[0];
+    o[1] = e[1];
+    o[2] = e[2];
+    auto area = [&](int r, int c) { return min(16, f.w - 16 * c) * min(16, f.h - 16 * r); };
+    queue = entry_fails(e, area(brow, bcol));
+    if (queue && f.gate) {
+      bool any_pass = false;
+      const int nr[4] = {brow, brow, brow - 1, brow + 1}, nc[4] = {bcol - 1, bcol + 1, bcol, bcol};
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        if (nr[t] < 0 || nr[t] >= f.brows || nc[t] < 0 || nc[t] >= f.bcols) continue;
+        if (!entry_fails(f.first + 3 * (base + (size_t)nr[t] * f.bcols + nc[t]), area(nr[t], nc[t])))
+          any_pass = true;
+      }
+      queue = any_pass;
+    }
   }
+  const uint64_t mask = __ballot(queue);
+  if (mask == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(f.count, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, leader, 64);
+  if (queue) f.queue[base + __popcll(mask & ((1ull << lane) - 1))] = (uint32_t)item;
+}
+
+template <int R>
+__global__ void __launch_bounds__(SearchGeom<R>::WG) fallback_search_kernel(const SearchArgs a, const FallbackArgs f) {
+  using SG = SearchGeom<R>;
+  __shared__ SearchSmem<R> sm;
+  const int tid = threadIdx.x;
+  const int slot = tid / SG::G;
+  const int l = tid - slot * SG::G;
+  const uint32_t cnt = uniform_u32(f.count);
+  if (f.queued && blockIdx.x == 0 && tid == 0) *f.queued = cnt;
+  for (uint32_t base = blockIdx.x * SG::BPW; base < cnt; base += gridDim.x * SG::BPW) {
+    const uint32_t qi = base + slot;
+    const bool live = qi < cnt;
+    // A dead slot repeats the workgroup's first item (base < cnt).
+    const int item = (int)f.queue[live ? qi : base];
+    size_t tab;
+    const uint64_t best = search_item<R>(a, sm, item, slot, l, &tab);
+    if (live && l == 0) {
+      const int32_t* e = f.first + tab;
+      const uint64_t prev = pack_key((uint32_t)e[2], e[0], e[1]);
+      store_key(f.out + tab, min_u64(best, prev));
+    }
+    __syncthreads();  // the next item's staging overwrites the windows
+  }
+}
+
+hipError_t launch_search_fallback(const SearchArgs& a, const FallbackArgs& f, int radius, int cus,
+                                  hipStream_t stream) {
+  if (a.total_blocks <= 0) return f.queued ? launch_zero_u32(f.queued, 1, stream) : hipSuccess;
+  hipError_t e = launch_zero_u32(f.count, 1, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fallback_mark_kernel, dim3((a.total_blocks + 255) / 256), dim3(256), 0, stream, f);
+  switch (radius) {
+#define VP8G_FALLBACK_CASE(RR)                                                                     \
+  case RR: {                                                                                       \
+    using SG = SearchGeom<RR>;                                                                     \
+    const int need = (a.total_blocks + SG::BPW - 1) / SG::BPW;                                     \
+    const int grid = min(need, max(1, cus) * (SG::WG >= 512 ? 2 : 8));                             \
+    hipLaunchKernelGGL(fallback_search_kernel<RR>, dim3(grid), dim3(SG::WG), 0, stream, a, f);     \
+    break;                                                                                         \
+  }
+    VP8G_FALLBACK_CASE(16)
+    VP8G_FALLBACK_CASE(64)
+#undef VP8G_FALLBACK_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 

@@ -15,6 +15,7 @@ struct vp8g_ctx {
   int xcd_remap = 1;
   int arnr_mbs = 0;
   int batch_wgs_per_cu = 0;
+  int fault = VP8G_FAULT_NONE;  // injected host-side failure (VP8G_OPT_FAULT)
   int cus = 0;
   std::vector<vp8g::TuneEntry> tuning;  // launch shapes by (resolution class, group length)
   // Device arena for the host-pointer entry points (grown, never shrunk).
@@ -29,6 +30,10 @@ struct vp8g_ctx {
     size_t off, bytes;
   };
   std::vector<PendingOut> pending;
+  // Device workspace of the `_dev` entry points that need scratch (the
+  // fallback queue); grown, never shrunk.
+  uint8_t* work = nullptr;
+  size_t work_bytes = 0;
 };
 
 // Group validation in the reference's order (src/arnr.rs:494-515) plus the

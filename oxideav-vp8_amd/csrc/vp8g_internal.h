@@ -105,6 +105,18 @@ struct Downsample2Args {
 hipError_t launch_downsample2(const Downsample2Args& a, int frames, hipStream_t stream);
 hipError_t launch_upsample_table(const int32_t* coarse, int cbw, int cbh, int32_t* fine, int fbw, int fbh, int n,
                                  hipStream_t stream);
+// Exhaustive fallback of a seeded search (queue of (frame, block) items).
+struct FallbackArgs {
+  const int32_t* first;   // first-stage table (table layout)
+  int32_t* out;           // output table (table layout); centre rows untouched
+  uint32_t* queue;        // capacity >= total_blocks items
+  uint32_t* count;        // queue length, cleared by the launcher
+  uint32_t* queued;       // optional: receives the queue length
+  int w, h, bcols, brows, n, center, total_blocks;
+  int gate;               // 1: queue only items with a 4-neighbour under the cut-off
+};
+hipError_t launch_search_fallback(const SearchArgs& a, const FallbackArgs& f, int radius, int cus,
+                                  hipStream_t stream);
 bool block_search_supported(int radius);
 hipError_t launch_refine_candidates(const SearchArgs& a, const int32_t* first, hipStream_t stream);
 hipError_t launch_block_search(const SearchArgs& a, int radius, hipStream_t stream);

@@ -40,6 +40,8 @@ OPT_ARNR_WAVES = 1
 OPT_XCD_REMAP = 2
 OPT_ARNR_MBS = 3
 OPT_BATCH_WGS_PER_CU = 4
+OPT_FAULT = 5
+FAULT_NONE, FAULT_ALLOC, FAULT_COPY, FAULT_LAUNCH = 0, 1, 2, 3
 
 # RFC 6386 section 18.3 tap tables, src/motion_comp.rs:294-303, 328-337.
 SIXTAP_FILTERS = np.array(
@@ -114,7 +116,9 @@ EXPORTS = (
     "vp8g_proposer_prepare_dev", "vp8g_proposer_forward_dev", "vp8g_proposer_propose_dev",
     "vp8g_proposer_backward_dev", "vp8g_proposer_train_step_dev", "vp8g_synth_luma_dev",
     "vp8g_half_pixel_refine_luma_batch_dev", "vp8g_quarter_pixel_refine_luma_batch_dev",
+    "vp8g_search_fallback_dev",
 )
+FALLBACK_GATE_NEIGHBOUR = 1
 
 
 def load_library(path: Path | None = None) -> C.CDLL:
@@ -154,6 +158,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_block_search_dev": (i32, [vp, C.POINTER(_I420), sz, sz, i32, vp, vp, C.POINTER(_Err)]),
         "vp8g_downsample2_i420_dev": (i32, [vp, C.POINTER(_I420), C.POINTER(_I420), sz]),
         "vp8g_refine_candidates_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, vp, vp, C.POINTER(_Err)]),
+        "vp8g_search_fallback_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp, C.POINTER(_Err)]),
         "vp8g_upsample_table_dev": (i32, [vp, vp, sz, i32, i32, vp, i32, i32]),
         "vp8g_build_arnr_altref_table_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, vp,
                                                    C.POINTER(_Err)]),
@@ -355,6 +360,12 @@ class Context:
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_BATCH_WGS_PER_CU, wgs),
                     "set_batch_wgs_per_cu")
 
+    def set_fault(self, mode: int) -> None:
+        """Host-side fault injection (FAULT_ALLOC / FAULT_COPY / FAULT_LAUNCH,
+        FAULT_NONE to clear): the chosen step of later calls reports
+        VP8G_E_DEVICE without running and writes no output."""
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_FAULT, int(mode)), "set_fault")
+
     def load_tuning(self, path: str | Path | None) -> None:
         """Load a tuning table (None clears it); see vp8g.autotune."""
         self._raise(self.lib.vp8g_ctx_load_tuning(self.h, None if path is None else str(path).encode()),
@@ -441,6 +452,19 @@ class Context:
         err = _Err()
         st = self.lib.vp8g_refine_candidates_dev(self.h, descs, n, center, seeds, first, out, C.byref(err))
         self._raise(st, "vp8g_refine_candidates_dev")
+
+    def search_fallback_dev(self, descs, n: int, center: int, first: int, out: int, radius: int = 16,
+                            gate: bool = True, queued: int | None = None) -> None:
+        """Exhaustive fallback of a seeded search: entries of the device table
+        ``first`` whose SAD fails the occlusion cut-off (with ``gate``, only
+        those with a 4-neighbour under it) are replaced in ``out`` by the
+        better of themselves and the exhaustive +-radius search; ``queued``
+        (device uint32) receives the count.  Enqueues only."""
+        err = _Err()
+        flags = FALLBACK_GATE_NEIGHBOUR if gate else 0
+        st = self.lib.vp8g_search_fallback_dev(self.h, descs, n, center, first, radius, flags, out, queued,
+                                               C.byref(err))
+        self._raise(st, "vp8g_search_fallback_dev")
 
     def downsample2_dev(self, src, dst, n: int = 1) -> None:
         """One pyramid level of ``n`` frames: ``dst`` (``_I420`` descriptors of

@@ -280,3 +280,59 @@ def test_refine_candidates_centre_first_without_seeds(gpu, oracle, geom):
     want = expect_candidates(oracle, tuples, center, zero, first)
     assert_tables(got[1:], want[1:], "candidates")
     assert (got[0] == -7).all()
+
+
+def fallback(gpu, dev, center, first, radius=16, gate=True):
+    import torch
+    tf = torch.from_numpy(np.ascontiguousarray(first, np.int32)).cuda()
+    out = dev.table(-7)
+    q = torch.full((1,), -1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    gpu.search_fallback_dev(dev.descs, len(dev.tuples), center, tf.data_ptr(), out.data_ptr(), radius, gate,
+                            q.data_ptr())
+    gpu.synchronize()
+    return out.cpu().numpy(), int(q.item())
+
+
+@pytest.mark.parametrize("radius", [16, 64])
+@pytest.mark.parametrize("gate", [True, False], ids=["gated", "all"])
+@pytest.mark.parametrize("geom", [(176, 144, 5, 0), (45, 37, 3, 3), (100, 70, 5, 1), (64, 48, 3, 0)],
+                         ids=lambda g: "x".join(map(str, g)))
+def test_search_fallback_matches_oracle(gpu, oracle, radius, gate, geom):
+    """First stage: +-2 around seeds that are exact on half of the blocks and
+    up to 12 px off elsewhere, so both passing and failing entries occur."""
+    w, h, n, pad = geom
+    tuples = tuples_of(w, h, n, 31 + w + radius, pad)
+    center = n // 2
+    dev = Dev(tuples)
+    rng = np.random.default_rng(w * h + radius)
+    seeds = expect_table(oracle, tuples, center, 16)
+    off = rng.integers(-12, 13, seeds[..., :2].shape) * (rng.random(seeds.shape[:3]) < 0.5)[..., None]
+    seeds[..., :2] += off
+    seeds[..., 2] = 0
+    seeds[center] = 0
+    first = search(gpu, dev, center, 2, seeds)
+    assert_tables(first, expect_table(oracle, tuples, center, 2, seeds), "first stage")
+    got, q = fallback(gpu, dev, center, first, radius, gate)
+    want, wq = oracle.search_fallback(tuples, center, first, radius, gate)
+    want[center] = -7
+    assert_tables(got, want, f"fallback R={radius} gate={gate}")
+    assert q == wq
+    others = [i for i in range(n) if i != center]
+    assert (abs_key(got[others]) <= abs_key(first[others])).all()
+
+
+def test_search_fallback_rejects_bad_arguments(gpu):
+    import torch
+    import vp8g
+    tuples = tuples_of(32, 32, 3, 5)
+    dev = Dev(tuples)
+    t = dev.table()
+    with pytest.raises(vp8g.Vp8gError):   # radius 2 is not a fallback radius
+        gpu.search_fallback_dev(dev.descs, 3, 1, t.data_ptr(), dev.table().data_ptr(), radius=2)
+    with pytest.raises(vp8g.Vp8gError):   # out aliases first
+        gpu.search_fallback_dev(dev.descs, 3, 1, t.data_ptr(), t.data_ptr())
+    with pytest.raises(vp8g.Vp8gError) as ei:   # centre index out of range, checked first
+        gpu.search_fallback_dev(dev.descs, 3, 5, t.data_ptr(), t.data_ptr(), radius=2)
+    assert ei.value.status == vp8g.VP8G_E_CENTER_OUT_OF_RANGE
+    torch.cuda.synchronize()

@@ -149,3 +149,72 @@ def test_altref_from_a_rejecting_table_is_the_centre(oracle):
     assert st == 0
     assert np.array_equal(got[0], g.y[g.center].reshape(-1))
     assert np.array_equal(got[1], g.u[g.center].reshape(-1))
+
+
+# ---------------------------------------------------------------- fallback
+FAILS = 65535  # above any 16x16 SAD (<= 65280): always fails the cut-off
+
+
+def _fallback_group(seed, w=64, h=48, n=3):
+    rng = np.random.default_rng(seed)
+    return [frame_tuple(rng.integers(0, 256, (h, w)).astype(np.uint8)) for _ in range(n)]
+
+
+def test_fallback_queue_rule_known_answer(oracle):
+    """A hand-set failure pattern on a 4 x 3 block grid: the gated queue is
+    the 7 failed blocks with a passing 4-neighbour, the ungated one all 9
+    failed blocks; queued entries take the exhaustive result (every first
+    SAD is above any real SAD), the others are copied."""
+    frames = _fallback_group(7)
+    n, center = len(frames), 1
+    pattern = ["FPFF", "FFFF", "PFFP"]
+    first = np.zeros((n, 3, 4, 3), np.int32)
+    for f in (0, 2):
+        for r, row in enumerate(pattern):
+            for c, ch in enumerate(row):
+                first[f, r, c] = (5, -3, FAILS if ch == "F" else 0)
+    exhaustive = oracle.block_search_group(frames, center, 4)
+    gated = {(0, 0), (0, 2), (1, 0), (1, 1), (1, 3), (2, 1), (2, 2)}
+    for gate, want_set in ((True, gated), (False, {(r, c) for r in range(3) for c in range(4)
+                                                     if pattern[r][c] == "F"})):
+        out, q = oracle.search_fallback(frames, center, first, radius=4, gate=gate)
+        assert q == 2 * len(want_set)
+        for f in (0, 2):
+            for r in range(3):
+                for c in range(4):
+                    want = exhaustive[f, r, c] if (r, c) in want_set else first[f, r, c]
+                    assert tuple(out[f, r, c]) == tuple(want), (gate, f, r, c)
+        assert (out[center] == first[center]).all()
+
+
+def test_fallback_keeps_the_better_entry_under_the_key_order(oracle):
+    """Random frames: every block fails, so with the gate off every entry
+    is queued.  A first entry equal to the exhaustive minimum in SAD wins
+    only by the (|mv|, mvy, mvx) order."""
+    frames = _fallback_group(11)
+    n, center = len(frames), 0
+    ex = oracle.block_search_group(frames, center, 3)
+    first = ex.copy()
+    cases = {}
+    for f in (1, 2):
+        for r in range(3):
+            for c in range(4):
+                mx, my, sad = (int(v) for v in ex[f, r, c])
+                k = (3 * f + r * 4 + c) % 4
+                if k == 0:    # same SAD, longer vector: the exhaustive entry wins
+                    first[f, r, c] = (mx + 40, my, sad)
+                    cases[(f, r, c)] = (mx, my, sad)
+                elif k == 1:  # same SAD and length, mvy = -length: the smaller (mvy, mvx) wins
+                    l1 = abs(mx) + abs(my)
+                    first[f, r, c] = (0, -l1, sad)
+                    cases[(f, r, c)] = (0, -l1, sad) if (-l1, 0) < (my, mx) else (mx, my, sad)
+                elif k == 2:  # smaller SAD than any real one: the first entry wins
+                    first[f, r, c] = (99, 99, max(0, sad - 1)) if sad >= 13 * 256 else (mx, my, sad)
+                    cases[(f, r, c)] = tuple(first[f, r, c])
+                else:         # larger SAD: the exhaustive entry wins
+                    first[f, r, c] = (1, 1, FAILS)
+                    cases[(f, r, c)] = (mx, my, sad)
+    out, q = oracle.search_fallback(frames, center, first, radius=3, gate=False)
+    assert q == sum(1 for (f, r, c) in cases if int(first[f, r, c, 2]) >= 13 * 256)
+    for (f, r, c), want in cases.items():
+        assert tuple(out[f, r, c]) == tuple(want), (f, r, c)
