@@ -201,6 +201,18 @@ int vp8g_refine_candidates_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_
 int vp8g_search_fallback_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames, size_t center,
                              const int32_t *first, int radius, int flags, int32_t *out, uint32_t *queued,
                              vp8g_err *err);
+/* vp8g_search_fallback_dev for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  Same arguments, checks and
+ * output, but the queue lives in a static device array the library holds
+ * on every GPU it runs on until the process ends (256 bytes + 4 MiB): at
+ * most 1048576 (frame, block) entries, (n_frames - 1) * ceil(w/16) *
+ * ceil(h/16); larger groups return VP8G_E_ARG.  The call allocates
+ * nothing.  Every call on one GPU shares that array, from any context:
+ * calls that overlap on one GPU (different streams or threads) race. */
+int vp8g_search_fallback_noalloc_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, size_t n_frames, size_t center,
+                                     const int32_t *first, int radius, int flags, int32_t *out,
+                                     uint32_t *queued, vp8g_err *err);
 /* The altref of build_arnr_altref (src/arnr.rs:488-650) with the search
  * replaced by a given alignment table (device, the layout above; the
  * centre's rows are not read): occlusion guard on the table's SAD, blend

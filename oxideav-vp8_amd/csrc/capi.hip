@@ -7,6 +7,7 @@
 
 #include <algorithm>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,37 @@
 #include "vp8g_internal.h"
 
 using namespace vp8g;
+
+// Scratch of vp8g_search_fallback_noalloc_dev: the queue counter in the
+// first 256 bytes, then up to kNoAllocQueueItems queued (frame, block)
+// items.  One copy per GPU, held until the process ends and shared by every
+// call on that GPU.  Host code never passes the array's name to a kernel:
+// it uses the device address hipGetSymbolAddress returns for the current
+// device (noalloc_scratch_address).
+namespace {
+constexpr size_t kNoAllocHeaderWords = 64;
+constexpr size_t kNoAllocQueueItems = size_t(1) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+static __device__ __align__(256) uint32_t scratch[kNoAllocHeaderWords + kNoAllocQueueItems];
+
+namespace {
+std::mutex g_scratch_mutex;
+void* g_scratch_by_device[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device; nullptr if the lookup fails.
+void* noalloc_scratch_address(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_by_device[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    g_scratch_by_device[device] = p;
+  }
+  return g_scratch_by_device[device];
+}
+}  // namespace
 
 struct vp8g_arnr_batch {
   vp8g_ctx* ctx = nullptr;
@@ -411,6 +443,9 @@ int vp8g_ctx_create(int device, vp8g_ctx** out) {
   vp8g_ctx* c = new vp8g_ctx();
   c->device = device;
   c->cus = prop.multiProcessorCount;
+  // The device is current: look up this device's copy of the no-alloc
+  // scratch (nullptr makes only vp8g_search_fallback_noalloc_dev fail).
+  c->scratch = static_cast<uint8_t*>(noalloc_scratch_address(device));
   if (!hip_ok(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking))) {
     delete c;
     return VP8G_E_DEVICE;
@@ -605,6 +640,19 @@ int vp8g_search_fallback_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, siz
   st = ensure_work(c, 256 + 4 * (n - 1) * per_frame);
   if (st != VP8G_OK) return st;
   return enqueue_fallback(c, frames, n, center, first, radius, flags, out, queued, c->work);
+}
+
+int vp8g_search_fallback_noalloc_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,
+                                     const int32_t* first, int radius, int flags, int32_t* out, uint32_t* queued,
+                                     vp8g_err* err) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_fallback(frames, n, center, first, radius, flags, out, queued, err);
+  if (st != VP8G_OK) return st;
+  const size_t per_frame = (size_t)((frames[center].width + 15) / 16) * ((frames[center].height + 15) / 16);
+  if ((n - 1) * per_frame > kNoAllocQueueItems) return VP8G_E_ARG;
+  if (!c->scratch) return VP8G_E_DEVICE;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  return enqueue_fallback(c, frames, n, center, first, radius, flags, out, queued, c->scratch);
 }
 
 int vp8g_refine_candidates_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, const int32_t* seeds,

@@ -34,6 +34,8 @@ struct vp8g_ctx {
   // fallback queue); grown, never shrunk.
   uint8_t* work = nullptr;
   size_t work_bytes = 0;
+  // Device address of this device's static no-alloc scratch (capi.hip).
+  uint8_t* scratch = nullptr;
 };
 
 // Group validation in the reference's order (src/arnr.rs:494-515) plus the

@@ -116,7 +116,7 @@ EXPORTS = (
     "vp8g_proposer_prepare_dev", "vp8g_proposer_forward_dev", "vp8g_proposer_propose_dev",
     "vp8g_proposer_backward_dev", "vp8g_proposer_train_step_dev", "vp8g_synth_luma_dev",
     "vp8g_half_pixel_refine_luma_batch_dev", "vp8g_quarter_pixel_refine_luma_batch_dev",
-    "vp8g_search_fallback_dev",
+    "vp8g_search_fallback_dev", "vp8g_search_fallback_noalloc_dev",
 )
 FALLBACK_GATE_NEIGHBOUR = 1
 
@@ -159,6 +159,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_downsample2_i420_dev": (i32, [vp, C.POINTER(_I420), C.POINTER(_I420), sz]),
         "vp8g_refine_candidates_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, vp, vp, C.POINTER(_Err)]),
         "vp8g_search_fallback_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp, C.POINTER(_Err)]),
+        "vp8g_search_fallback_noalloc_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp,
+                                                   C.POINTER(_Err)]),
         "vp8g_upsample_table_dev": (i32, [vp, vp, sz, i32, i32, vp, i32, i32]),
         "vp8g_build_arnr_altref_table_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, vp,
                                                    C.POINTER(_Err)]),
