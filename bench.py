@@ -262,66 +262,68 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     return out
 
 
-def bench_4k(args) -> int:
+def measure_4k(ctx, reps: int = 3) -> dict:
     """3840x2160 I420, 11-frame group (centre 5), region velocities in
-    [-48, 48] px/frame, 40 occluders per frame: exhaustive +-64 and
-    learned-seeded (+-2) whole-pel search, then half- and quarter-pel
-    refinement and six-tap MC of all 32400 macroblocks of each of the 10
-    neighbours.  Throughput = anchor MBs per second of the learned pipeline;
-    the residual proxy (SAD / SSE of the MC luma vs the centre) stands in for
-    encoded stream bytes."""
-    import torch
-
-    import vp8g
+    [-48, 48] px/frame, 40 occluders per frame: exhaustive +-64 and the
+    coarse-to-fine learned-seeded whole-pel search (proposals at quarter
+    resolution with the weights fine-tuned there, exact +-2 at every level),
+    then half- and quarter-pel refinement and six-tap MC of all 32400
+    macroblocks of each of the 10 neighbours.  Throughput = anchor MBs per
+    second of the learned pipeline; the residual proxy (SAD / SSE of the MC
+    luma vs the centre) stands in for encoded stream bytes."""
     import vp8g.synth as synth
     from vp8g.motion import evaluate_group
     from vp8g.proposer import WEIGHTS_DIR, Proposer
     t0 = time.perf_counter()
     g = synth.arnr_group_regions(3840, 2160, 11, vmax=48, occluders=40, clip=0, config=4)
     print(f"[bench] 4K group generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    torch.cuda.init()
-    ctx = vp8g.Context(0)
-    wpath = WEIGHTS_DIR / "proposer_4k.safetensors"
-    if not wpath.exists():
-        wpath = WEIGHTS_DIR / "proposer_1080p.safetensors"
-    prop = Proposer(ctx, params=Proposer.load_params(wpath))
     ppath = WEIGHTS_DIR / "proposer_4k_l2.safetensors"
-    pprop = Proposer(ctx, params=Proposer.load_params(ppath)) if ppath.exists() else None
-    r = evaluate_group(ctx, prop, g, radius=64, refine=2, reps=max(1, min(args.steps, 3)), pyramid_prop=pprop,
-                       pyramid_levels=2)
+    pprop = Proposer(ctx, params=Proposer.load_params(ppath))
+    r = evaluate_group(ctx, None, g, radius=64, refine=2, reps=reps, pyramid_prop=pprop, pyramid_levels=2)
+    pprop.close()
     anchors = 240 * 135
-    le = r["learned_pyramid"] if pprop is not None else r["learned"]
-    result = {
+    le, ex = r["learned_pyramid"], r["exhaustive"]
+    mc_gbs = le["mc_gb_per_s"]
+    return {
         "metric": "4K learned-seeded search + sub-pel refine + six-tap MC, anchor-MB/s (vs exhaustive +-64)",
         "value": le["anchor_mb_per_s"],
         "unit": "anchor-MB/s",
-        "n_gpus": 1,
-        "steps": r.get("reps", 1),
-        "warmup": 1,
-        "ms_per_step": le["ms_total"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic",
+        "ms_per_group": le["ms_total"],
+        "exhaustive_ms_per_group": ex["ms_total"],
+        "search_speedup": r["pyramid"]["search_speedup"],
+        "agreement": r["pyramid"]["agreement"],
+        "agreement_matched": r["pyramid"]["agreement_matched"],
+        "matched_fraction": r["matched_fraction"],
+        "residual_sad_per_px": {"learned": le["residual_sad_per_px"], "exhaustive": ex["residual_sad_per_px"]},
+        "roofline": {"bound": "hbm", "kernel": "predict_inter_mb_kernel", "achieved": mc_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": mc_gbs / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_launch": 32400 * (384 + 384 + 4)},
         "config": {"workload": "3840x2160 I420, 11-frame group (centre 5), 8x8 region motion [-48,48] px/frame, "
                                "40 occluders/frame, sigma 4 noise", "anchor_mbs": anchors,
-                   "search": ("coarse to fine: proposals at quarter resolution, exact +-2 at quarter, half and full"
-                              if pprop is not None else "proposals at full resolution, exact +-2"),
-                   "weights": ppath.name if pprop is not None else wpath.name,
-                   "single_level_weights": wpath.name, "parallelism": "single GPU"},
+                   "search": "coarse to fine: proposals at quarter resolution, exact +-2 at quarter, half and full",
+                   "weights": ppath.name},
         "report": r,
         "stream_bytes_qi44": "not measurable here (reference encoder not buildable); residual SAD/SSE reported",
     }
-    prop.close()
-    if pprop is not None:
-        pprop.close()
+
+
+def bench_4k(args) -> int:
+    import torch
+
+    import vp8g
+    torch.cuda.init()
+    ctx = vp8g.Context(0)
+    r = measure_4k(ctx, reps=max(1, min(args.steps, 3)))
     ctx.close()
+    result = {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": 1, "warmup": 1,
+              "ms_per_step": r["ms_per_group"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "u8", "data": "synthetic", "config": dict(r["config"], parallelism="single GPU")}
+    result.update({k: v for k, v in r.items() if k not in result})
     print(json.dumps(result), flush=True)
     return 0
 
 
-def bench_batch(args) -> int:
+def measure_batch(ctx, steps: int, warmup: int) -> dict:
     """64 clips (resolutions uniform over 176x144 .. 1920x1080, lengths over
     {3, 5, 7, 9, 11}, region motion + occluders) as one persistent,
     graph-captured launch, against the same clips as back-to-back
@@ -336,48 +338,39 @@ def bench_batch(args) -> int:
     specs = synth.batch_specs(64)
     groups = synth.batch_clips(64, specs=specs)
     print(f"[bench] 64 clips generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    torch.cuda.init()
-    ctx = vp8g.Context(0)
     clips = [DeviceClip(torch, g) for g in groups]
     torch.cuda.synchronize()
     batch = ctx.create_arnr_batch([c.job(vp8g) for c in clips])
     shape = batch.shape()
     stream = torch.cuda.ExternalStream(ctx.stream(), device=torch.device("cuda:0"))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batch.run()
     ctx.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     t_wall = time.perf_counter()
     ev[0].record(stream)
-    for k in range(args.steps):
+    for k in range(steps):
         batch.run()
         ev[k + 1].record(stream)
     ev[-1].synchronize()
     wall = time.perf_counter() - t_wall
-    per = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
-    ms_step = ev[0].elapsed_time(ev[-1]) / args.steps
-    print(f"[bench] batch: {args.steps} replays, {ms_step:.4f} ms/replay (wall {wall:.3f}s)",
-          file=sys.stderr, flush=True)
-    loop_ms = time_per_clip_loop(torch, vp8g, ctx, clips, reps=max(3, args.steps // 5))
+    per = [ev[k].elapsed_time(ev[k + 1]) for k in range(steps)]
+    ms_step = ev[0].elapsed_time(ev[-1]) / steps
+    print(f"[bench] batch: {steps} replays, {ms_step:.4f} ms/replay (wall {wall:.3f}s)", file=sys.stderr, flush=True)
+    loop_ms = time_per_clip_loop(torch, vp8g, ctx, clips, reps=max(3, steps // 5))
     print(f"[bench] per-clip launches: {loop_ms:.4f} ms per pass", file=sys.stderr, flush=True)
     anchors = sum(c.anchor_mbs for c in clips)
     nb = sum(c.anchor_mbs * (len(c.frames) - 1) for c in clips)
     bytes_alg = sum(arnr_algorithmic(g.width, g.height, g.n)[0] for g in groups)
     ms_kernel = float(np.mean(per))
     achieved = bytes_alg / (ms_kernel * 1e-3) / 1e9
-    result = {
+    batch.close()
+    return {
         "metric": "ARNR batched aggregate anchor-MB/s (64 uneven clips, one persistent launch)",
         "value": anchors / (ms_step * 1e-3),
         "unit": "anchor-MB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
         "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic",
         "config": {
             "workload": "64 clips, resolution uniform over {176x144, 352x288, 640x360, 1280x720, "
                         "1920x1080}, group length uniform over {3,5,7,9,11}, centre n/2, strength 3, "
@@ -385,17 +378,28 @@ def bench_batch(args) -> int:
             "anchor_mbs_per_step": anchors,
             "mb_neighbour_pairs_per_step": nb,
             "shape": shape,
-            "parallelism": "single GPU",
         },
         "per_clip_launches": {"ms_per_pass": loop_ms, "anchor_mb_per_s": anchors / (loop_ms * 1e-3),
                               "batch_speedup": loop_ms / ms_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "arnr_batch_kernel", "algorithmic_bytes_per_launch": bytes_alg},
-        "cpu_baseline": None,
     }
-    batch.close()
+
+
+def bench_batch(args) -> int:
+    import torch
+
+    import vp8g
+    torch.cuda.init()
+    ctx = vp8g.Context(0)
+    r = measure_batch(ctx, args.steps, args.warmup)
     ctx.close()
+    result = {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+              "config": dict(r["config"], parallelism="single GPU"), "cpu_baseline": None}
+    result.update({k: v for k, v in r.items() if k not in result})
     print(json.dumps(result), flush=True)
     return 0
 
@@ -412,6 +416,8 @@ def main() -> int:
     ap.add_argument("--arnr-mbs", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeded altref leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end (host buffers) leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched-clips (c2_batch) and 4K pipeline (c4_pipeline) sub-records")
     ap.add_argument("--workload", choices=("step", "batch", "4k"), default="step",
                     help="step: the 1080p group step (default); batch: 64 uneven clips in one "
                          "persistent graph-captured launch against per-clip launches; 4k: 3840x2160 "
@@ -643,6 +649,11 @@ def main() -> int:
     result["inputs"] = {"prng": "splitmix64", "seed_base": synth._seed(1, 0, 0), "group_sha256": h.hexdigest()}
     if not args.no_learned:
         result["learned_seeded"] = bench_learned(ctx, group, descs, stream, reps=max(5, args.steps // 5))
+    if not args.no_extra:
+        result["c2_batch"] = measure_batch(ctx, max(5, min(args.steps, 20)), 2)
+        c4 = measure_4k(ctx, reps=2)
+        c4.pop("report")
+        result["c4_pipeline"] = c4
     if not args.no_cpu:
         cores = max(1, min(os.cpu_count() or 1, 16))
         print(f"[bench] timing CPU oracle on {cores} threads", file=sys.stderr, flush=True)

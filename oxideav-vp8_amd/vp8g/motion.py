@@ -28,12 +28,16 @@ def centre_mbs(torch, y, width: int, height: int):
     return p.reshape(bh, 16, bw, 16).permute(0, 2, 1, 3).reshape(bh * bw, 256).contiguous()
 
 
-def subpel_and_mc(ctx: Context, planes, width: int, height: int, center: int, table, stream) -> dict:
+def subpel_and_mc(ctx: Context, planes, width: int, height: int, center: int, table, stream,
+                  record: bool = False) -> dict:
     """For every non-centre frame f: the table's whole-pel vectors refined to
     half then quarter pel against frame f, then the six-tap prediction of
     every macroblock from frame f at the refined vector.  Frames must be
     whole macroblocks (width, height multiples of 16), as ReferencePlanes
-    requires.  Returns timings and the residual proxy."""
+    requires.  Returns timings and the residual proxy; with `record`, also
+    "frames": per non-centre frame the host copies of the half- and
+    quarter-pel results ([n_mb] x {row, col, sad}) and the predictions
+    ([n_mb, 384])."""
     import torch
     assert width % 16 == 0 and height % 16 == 0
     dev = torch.device("cuda:0")
@@ -50,6 +54,12 @@ def subpel_and_mc(ctx: Context, planes, width: int, height: int, center: int, ta
         centers = torch.empty((n_mb, 2), dtype=torch.int16, device=dev)
     sad_total = sse_total = 0
     t_refine = t_mc = 0.0
+    recorded = []
+
+    def result_rows(t):
+        v = t.view(torch.int16).reshape(n_mb, 4).cpu().numpy()
+        sad = v[:, 2].astype(np.uint16).astype(np.int64) | (v[:, 3].astype(np.uint16).astype(np.int64) << 16)
+        return np.stack([v[:, 0].astype(np.int64), v[:, 1].astype(np.int64), sad], 1)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     for f in range(len(planes)):
         if f == center:
@@ -65,6 +75,10 @@ def subpel_and_mc(ctx: Context, planes, width: int, height: int, center: int, ta
         with torch.cuda.stream(stream):
             # vp8g_search_result = {int16 row, int16 col, uint32 sad}
             centers.copy_(half.view(torch.int16).reshape(n_mb, 4)[:, :2])
+        if record:
+            recorded.append({"frame": f, "half": None})
+            torch.cuda.synchronize()
+            recorded[-1]["half"] = result_rows(half)
         ctx.refine_batch_dev(y.data_ptr(), width, width, height, mb_xy.data_ptr(), src.data_ptr(),
                              centers.data_ptr(), n_mb, quarter.data_ptr(), quarter=True)
         with torch.cuda.stream(stream):
@@ -80,10 +94,25 @@ def subpel_and_mc(ctx: Context, planes, width: int, height: int, center: int, ta
             sse_total += int((d * d).sum().item())
         t_refine += ev[0].elapsed_time(ev[1])
         t_mc += ev[1].elapsed_time(ev[2])
+        if record:
+            recorded[-1]["quarter"] = result_rows(quarter)
+            recorded[-1]["pred"] = pred.cpu().numpy()
     pairs = len(planes) - 1
-    return {"ms_subpel": t_refine, "ms_mc": t_mc, "residual_sad": sad_total, "residual_sse": sse_total,
-            "residual_sad_per_px": sad_total / (pairs * n_mb * 256.0),
-            "mc_bytes": pairs * n_mb * (384 + 384 + 4)}
+    out = {"ms_subpel": t_refine, "ms_mc": t_mc, "residual_sad": sad_total, "residual_sse": sse_total,
+           "residual_sad_per_px": sad_total / (pairs * n_mb * 256.0),
+           "mc_bytes": pairs * n_mb * (384 + 384 + 4)}
+    if record:
+        out["frames"] = recorded
+    return out
+
+
+def block_area(width: int, height: int) -> np.ndarray:
+    """[brows, bcols] pixel count of every 16x16 block (partial edge blocks
+    smaller), for the occlusion cut-off sad / (bw * bh) > 12
+    (src/arnr.rs:603-608)."""
+    bw, bh = (width + 15) // 16, (height + 15) // 16
+    return (np.minimum(16, height - 16 * np.arange(bh))[:, None] *
+            np.minimum(16, width - 16 * np.arange(bw))[None, :])
 
 
 class Pyramid:
@@ -144,9 +173,10 @@ def evaluate_group(ctx: Context, prop, g, radius: int = 64, refine: int = 2, rep
                    pyramid_levels: int = 2) -> dict:
     """Exhaustive +-radius vs learned-seeded (+-refine) whole-pel search on a
     host group, each followed by half/quarter-pel refinement and six-tap MC
-    of every macroblock of every non-centre frame.  With `pyramid_prop` (a
-    proposer trained at that pyramid level) the learned search also runs
-    coarse to fine (Pyramid: building the levels is included in its time)."""
+    of every macroblock of every non-centre frame.  `prop` (may be None)
+    proposes at full resolution; with `pyramid_prop` (a proposer trained at
+    that pyramid level) the learned search runs coarse to fine (Pyramid:
+    building the levels is included in its time)."""
     import torch
     from .proposer import DeviceGroup, psnr
     stream = torch.cuda.ExternalStream(ctx.stream())
@@ -166,19 +196,22 @@ def evaluate_group(ctx: Context, prop, g, radius: int = 64, refine: int = 2, rep
         return e0.elapsed_time(e1) / reps
 
     t_ex = timed(lambda: ctx.block_search_dev(d.descs, n, c, radius, ex.data_ptr()))
-    t_pr = timed(lambda: prop.propose(d.descs, n, c, seeds.data_ptr(), radius - refine))
-    t_rf = timed(lambda: ctx.block_search_dev(d.descs, n, c, refine, ref.data_ptr(), seeds.data_ptr()))
     ctx.synchronize()
     others = [i for i in range(n) if i != c]
-    e, r = ex.cpu().numpy()[others], ref.cpu().numpy()[others]
-    matched = e[..., 2] < 13 * 256
-    same = r[..., 2] == e[..., 2]
+    e = ex.cpu().numpy()[others]
+    matched = e[..., 2] < 13 * block_area(g.width, g.height)[None]
     out = {"width": g.width, "height": g.height, "frames": n, "radius": radius,
-           "agreement": float(same.mean()), "agreement_matched": float(same[matched].mean()),
-           "matched_fraction": float(matched.mean()),
-           "ms_search_exhaustive": t_ex, "ms_search_learned": t_pr + t_rf, "ms_propose": t_pr, "ms_refine": t_rf,
-           "search_speedup": t_ex / (t_pr + t_rf)}
-    legs = [("exhaustive", ex, t_ex), ("learned", ref, t_pr + t_rf)]
+           "matched_fraction": float(matched.mean()), "ms_search_exhaustive": t_ex}
+    legs = [("exhaustive", ex, t_ex)]
+    if prop is not None:
+        t_pr = timed(lambda: prop.propose(d.descs, n, c, seeds.data_ptr(), radius - refine))
+        t_rf = timed(lambda: ctx.block_search_dev(d.descs, n, c, refine, ref.data_ptr(), seeds.data_ptr()))
+        ctx.synchronize()
+        same = ref.cpu().numpy()[others][..., 2] == e[..., 2]
+        out["single_level"] = {"agreement": float(same.mean()), "agreement_matched": float(same[matched].mean()),
+                               "ms_search": t_pr + t_rf, "ms_propose": t_pr, "ms_refine": t_rf,
+                               "search_speedup": t_ex / (t_pr + t_rf)}
+        legs.append(("learned", ref, t_pr + t_rf))
     if pyramid_prop is not None:
         pyr = Pyramid(torch, ctx, d, levels=pyramid_levels)
         t_build = timed(pyr.build)
