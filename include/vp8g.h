@@ -413,6 +413,92 @@ int vp8g_half_plane_sads_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const in
                                const uint8_t *src, size_t n, const int32_t taps[6],
                                int diagonals, uint32_t *sad);
 
+/* ---- encoder inter motion search ----
+ * The caller side of sub-pel SAD scoring: the rate-distortion motion search
+ * of the inter-frame encoder over a batch of macroblocks against one
+ * reference.
+ *
+ * MvCostTable (src/encoder/inter_search.rs:166-194): bits[component][diff +
+ * 1023] is the f32 bit cost of a motion-vector differential component
+ * (component 0 = row), as MvCostTable::new tables it from the frame's MV
+ * contexts.  The rate of a vector is lambda_sad * (bits[0][dr] + bits[1][dc])
+ * (f32 sum widened to f64), and vectors whose differential leaves
+ * [-1023, 1023] are never chosen. */
+typedef struct {
+    float bits[2][2047];
+} vp8g_mv_cost_table;
+/* The per-macroblock inputs of whole_pixel_search / sub_pixel_refine: the
+ * macroblock position, the census predictor the differential is coded
+ * against (RefSearch::census.best), the whole-pixel search range
+ * (RefSearch::range_x / range_y, min <= max, within +-8191), up to 8 seed
+ * vectors (quarter-pel), lambda_sad (finite, >= 0), good_enough
+ * (< 2^30) and whether to descend past the seeds. */
+typedef struct {
+    uint32_t mb_col, mb_row;
+    vp8g_mv best;
+    int32_t range_x[2], range_y[2];
+    uint32_t n_seeds;
+    vp8g_mv seeds[8];
+    double lambda_sad;
+    uint32_t good_enough;
+    int32_t descend;
+} vp8g_mb_search;
+/* SearchShape (src/encoder/inter_search.rs:66-93): the fields the two
+ * searches read. */
+typedef struct {
+    uint32_t descent_rounds;
+    int32_t wide_pattern, half_diagonals, quarter;
+} vp8g_search_shape;
+/* Found (src/encoder/inter_search.rs:349-355): the vector, its SAD and its
+ * cost sad + lambda_sad * bits. */
+typedef struct {
+    vp8g_mv mv;
+    uint32_t sad;
+    double cost;
+} vp8g_found;
+
+/* Replaces whole_pixel_search (src/encoder/inter_search.rs:380-495) over a
+ * batch: macroblock i of `jobs` with source src[256*i..] searched in `ref`
+ * (LumaRef: width/height are the clamp bounds).  The seeds, rounded to
+ * whole pixels and clamped into the range, are probed once each in order;
+ * unless the best SAD is already <= good_enough (or descend is 0), the
+ * two-pixel pattern (when shape->wide_pattern and the SAD exceeds 4 *
+ * good_enough) then the one-pixel pattern descend for up to
+ * descent_rounds rounds each.  out[i] receives the Found. */
+int vp8g_whole_pixel_search_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const vp8g_mv_cost_table *costs,
+                                  const vp8g_mb_search *jobs, const uint8_t *src, size_t n,
+                                  const vp8g_search_shape *shape, vp8g_found *out);
+/* Replaces sub_pixel_refine (src/encoder/inter_search.rs:499-552) over a
+ * batch: from start[i] (whole-pel, as whole_pixel_search returns it) the
+ * half-pel neighbours scored from HalfPlanes with filters[4] (the four
+ * diagonals only with shape->half_diagonals), then with shape->quarter the
+ * quarter-pel neighbours of the best through mb_luma_sad_at_mv; a
+ * neighbour replaces the best when its cost is lower.  filters taps lie
+ * within +-2^20. */
+int vp8g_sub_pixel_refine_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const vp8g_mv_cost_table *costs,
+                                const vp8g_mb_search *jobs, const uint8_t *src, const vp8g_found *start, size_t n,
+                                const int32_t filters[8][6], const vp8g_search_shape *shape, vp8g_found *out);
+/* Replaces small_diamond_search_luma (src/motion_search.rs:381-442) over a
+ * batch: from centers[i] (clamped to +-1023, snapped toward zero to whole
+ * pixels) up to max_iters rounds of the N, S, W, E one-pixel probes, strict
+ * improvement only. */
+int vp8g_small_diamond_search_luma_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const uint32_t *mb_xy,
+                                         const uint8_t *src, const vp8g_mv *centers, size_t n, uint32_t max_iters,
+                                         vp8g_search_result *out);
+/* The three over device arrays (costs, jobs, src, start, mb_xy, centers,
+ * out in device memory; `shape` and `filters` on the host); enqueue only.
+ * Device inputs out of the documented ranges are clamped into them. */
+int vp8g_whole_pixel_search_batch_dev(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const vp8g_mv_cost_table *costs,
+                                      const vp8g_mb_search *jobs, const uint8_t *src, size_t n,
+                                      const vp8g_search_shape *shape, vp8g_found *out);
+int vp8g_sub_pixel_refine_batch_dev(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const vp8g_mv_cost_table *costs,
+                                    const vp8g_mb_search *jobs, const uint8_t *src, const vp8g_found *start,
+                                    size_t n, const int32_t filters[8][6], const vp8g_search_shape *shape,
+                                    vp8g_found *out);
+int vp8g_small_diamond_search_luma_batch_dev(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const uint32_t *mb_xy,
+                                             const uint8_t *src, const vp8g_mv *centers, size_t n,
+                                             uint32_t max_iters, vp8g_search_result *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,14 @@ class SearchResult(C.Structure):
     _fields_ = [("mv", MvS), ("sad", C.c_uint32)]
 
 
+# orc_mb_search: vp8g_mb_search's fields plus the source pointer.
+ORC_MB_SEARCH = np.dtype([("mb_col", "<u4"), ("mb_row", "<u4"), ("best", "<i2", (2,)), ("range_x", "<i4", (2,)),
+                          ("range_y", "<i4", (2,)), ("n_seeds", "<u4"), ("seeds", "<i2", (8, 2)),
+                          ("lambda_sad", "<f8"), ("good_enough", "<u4"), ("descend", "<i4"), ("src", "<u8")],
+                         align=True)
+ORC_FOUND = np.dtype([("mv", "<i2", (2,)), ("sad", "<u4"), ("cost", "<f8")], align=True)
+
+
 class HalfPlanes(C.Structure):
     _fields_ = [("h", C.c_uint8 * (22 * 17)), ("v", C.c_uint8 * (17 * 16)),
                 ("d", C.c_uint8 * (17 * 17))]
@@ -124,6 +132,10 @@ class Oracle:
         L.orc_half_planes_new.argtypes = [vp, vp, i32, C.POINTER(HalfPlanes)]
         L.orc_half_planes_sad.argtypes = [C.POINTER(HalfPlanes), vp, i32, i32]
         L.orc_half_planes_sad.restype = u32
+        L.orc_small_diamond_search_luma.argtypes = [C.POINTER(LumaRef), sz, sz, vp, MvS, u32]
+        L.orc_small_diamond_search_luma.restype = SearchResult
+        L.orc_whole_pixel_search_batch.argtypes = [C.POINTER(LumaRef), vp, vp, sz, vp, vp]
+        L.orc_sub_pixel_refine_batch.argtypes = [C.POINTER(LumaRef), vp, vp, vp, sz, vp, vp, vp]
         self.SIXTAP = np.ctypeslib.as_array((C.c_int32 * 48).in_dll(L, "ORC_SIXTAP_FILTERS")).reshape(8, 6).copy()
         self.BILINEAR = np.ctypeslib.as_array((C.c_int32 * 48).in_dll(L, "ORC_BILINEAR_FILTERS")).reshape(8, 6).copy()
 
@@ -170,6 +182,41 @@ class Oracle:
         st = self.lib.orc_search_fallback(arr, len(frames), center, _p(f), radius, int(gate), _p(out), C.byref(q))
         assert st == 0, st
         return out, int(q.value)
+
+    # ---- encoder inter search
+    def _orc_jobs(self, jobs, src):
+        src = np.ascontiguousarray(src, np.uint8).reshape(-1, 256)
+        out = np.zeros(len(jobs), ORC_MB_SEARCH)
+        for name in jobs.dtype.names:
+            out[name] = jobs[name]
+        out["src"] = src.ctypes.data + 256 * np.arange(len(jobs), dtype=np.uint64)
+        return out, src
+
+    def whole_pixel_search_batch(self, plane, stride, width, height, bits, jobs, src, shape):
+        """whole_pixel_search per job (vp8g_mb_search structured array);
+        shape = (descent_rounds, wide_pattern, half_diagonals, quarter)."""
+        oj, keep = self._orc_jobs(jobs, src)
+        p = np.ascontiguousarray(plane, np.uint8)
+        lr = LumaRef(p.ctypes.data, stride, width, height)
+        b = np.ascontiguousarray(bits, np.float32)
+        sh = (C.c_uint32 * 4)(*[int(v) for v in shape])
+        out = np.zeros(len(jobs), ORC_FOUND)
+        self.lib.orc_whole_pixel_search_batch(C.byref(lr), b.ctypes.data, oj.ctypes.data, len(jobs), sh,
+                                              out.ctypes.data)
+        return out
+
+    def sub_pixel_refine_batch(self, plane, stride, width, height, bits, jobs, src, start, filters, shape):
+        oj, keep = self._orc_jobs(jobs, src)
+        p = np.ascontiguousarray(plane, np.uint8)
+        lr = LumaRef(p.ctypes.data, stride, width, height)
+        b = np.ascontiguousarray(bits, np.float32)
+        st = np.ascontiguousarray(start, ORC_FOUND)
+        f = np.ascontiguousarray(filters, np.int32)
+        sh = (C.c_uint32 * 4)(*[int(v) for v in shape])
+        out = np.zeros(len(jobs), ORC_FOUND)
+        self.lib.orc_sub_pixel_refine_batch(C.byref(lr), b.ctypes.data, oj.ctypes.data, st.ctypes.data, len(jobs),
+                                            f.ctypes.data, sh, out.ctypes.data)
+        return out
 
     def build_arnr_altref_mv(self, frames, center, strength, table):
         """Altref from a given alignment table (n, brows, bcols, 3)."""

@@ -13,6 +13,7 @@
 
 #include <pthread.h>
 #include <stdatomic.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -785,6 +786,211 @@ uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], i
     if (dy == 0) return window_sad(src, p->h, 17, 3 * 17 + cx);
     if (dx == 0) return window_sad(src, p->v, 16, cy * 16);
     return window_sad(src, p->d, 17, cy * 17 + cx);
+}
+
+/* ---------------- encoder inter search ---------------- */
+/* small_diamond_search_luma: src/motion_search.rs:381-442 (snap_to_whole_pixel
+ * :360-363 truncates toward zero; clamp first, then snap). */
+static inline int16_t snap_whole(int16_t v) { return (int16_t)((v / 4) * 4); }
+
+orc_search_result orc_small_diamond_search_luma(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
+                                                const uint8_t src[256], orc_mv center, uint32_t max_iters)
+{
+    orc_mv best = {snap_whole(clamp_component(center.row)), snap_whole(clamp_component(center.col))};
+    uint32_t best_sad = orc_mb_luma_sad_at_whole_mv(ref, mb_col, mb_row, src, best);
+    const int nb[4][2] = {{-4, 0}, {4, 0}, {0, -4}, {0, 4}};
+    for (uint32_t it = 0; it < max_iters; it++) {
+        int improved = 0;
+        for (int k = 0; k < 4; k++) {
+            orc_mv c = {snap_whole(clamp_component(best.row + nb[k][0])),
+                        snap_whole(clamp_component(best.col + nb[k][1]))};
+            if (c.row == best.row && c.col == best.col) continue;
+            uint32_t s = orc_mb_luma_sad_at_whole_mv(ref, mb_col, mb_row, src, c);
+            if (s < best_sad) {
+                best_sad = s;
+                best = c;
+                improved = 1;
+            }
+        }
+        if (!improved) break;
+    }
+    orc_search_result r = {best, best_sad};
+    return r;
+}
+
+/* MvCostTable::bits: src/encoder/inter_search.rs:180-193.  The two f32
+ * entries are added in f32, then widened. */
+static int mv_cost_bits(const orc_mv_cost_table *t, orc_mv mv, orc_mv base, double *out)
+{
+    const int dr = (int)mv.row - (int)base.row, dc = (int)mv.col - (int)base.col;
+    if (dr < -1023 || dr > 1023 || dc < -1023 || dc > 1023) return 0;
+    const float s = t->bits[0][dr + 1023] + t->bits[1][dc + 1023];
+    *out = (double)s;
+    return 1;
+}
+
+/* whole_sad: src/encoder/inter_search.rs:219-250 (gives up once the sum
+ * reaches `limit`, as the reference does). */
+static uint32_t enc_whole_sad(const orc_luma_ref *ref, int x, int y, const uint8_t src[256], uint32_t limit)
+{
+    const int w = (int)ref->width, h = (int)ref->height;
+    uint32_t acc = 0;
+    if (x >= 0 && y >= 0 && x + 16 <= w && y + 16 <= h) {
+        const uint8_t *base = ref->plane + (size_t)y * ref->stride + (size_t)x;
+        for (int band = 0; band < 4; band++) {
+            for (int r = band * 4; r < band * 4 + 4; r++)
+                for (int c = 0; c < 16; c++) acc += absdiff(src[r * 16 + c], base[(size_t)r * ref->stride + c]);
+            if (acc >= limit) return acc;
+        }
+        return acc;
+    }
+    for (int r = 0; r < 16; r++) {
+        const uint8_t *row = ref->plane + (size_t)clampi(y + r, 0, h - 1) * ref->stride;
+        for (int c = 0; c < 16; c++) acc += absdiff(src[r * 16 + c], row[clampi(x + c, 0, w - 1)]);
+        if (acc >= limit) return acc;
+    }
+    return acc;
+}
+
+/* whole_pixel_search: src/encoder/inter_search.rs:380-495. */
+static int wps_eval(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *job,
+                    int px, int py, double best_cost, orc_found *out)
+{
+    orc_mv mv = {(int16_t)(py * 4), (int16_t)(px * 4)};
+    double bits;
+    if (!mv_cost_bits(costs, mv, job->best, &bits)) return 0;
+    const double rate = job->lambda_sad * bits;
+    if (rate >= best_cost) return 0;
+    double lim = best_cost - rate;
+    if (lim > 4294967295.0) lim = 4294967295.0;
+    uint32_t limit = (uint32_t)lim;
+    limit = limit == UINT32_MAX ? limit : limit + 1;
+    const uint32_t sad = enc_whole_sad(ref, (int)job->mb_col * 16 + px, (int)job->mb_row * 16 + py, job->src, limit);
+    const double cost = (double)sad + rate;
+    if (!(cost < best_cost)) return 0;
+    out->mv = mv;
+    out->sad = sad;
+    out->cost = cost;
+    return 1;
+}
+
+orc_found orc_whole_pixel_search(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *job,
+                                 const orc_search_shape *shape)
+{
+    orc_found best = {{0, 0}, UINT32_MAX, INFINITY};
+    int tried[8][2];
+    size_t tried_len = 0;
+    for (uint32_t k = 0; k < job->n_seeds; k++) {
+        const orc_mv sd = job->seeds[k];
+        const int px = clampi(((int)sd.col + 2) >> 2, job->range_x[0], job->range_x[1]);
+        const int py = clampi(((int)sd.row + 2) >> 2, job->range_y[0], job->range_y[1]);
+        int seen = 0;
+        for (size_t t = 0; t < tried_len; t++)
+            if (tried[t][0] == px && tried[t][1] == py) seen = 1;
+        if (seen) continue;
+        if (tried_len < 8) {
+            tried[tried_len][0] = px;
+            tried[tried_len][1] = py;
+            tried_len++;
+        }
+        orc_found f;
+        if (wps_eval(ref, costs, job, px, py, best.cost, &f)) best = f;
+    }
+    if (!job->descend || best.sad <= job->good_enough) return best;
+    static const int wide[8][2] = {{-2, 0}, {2, 0}, {0, -2}, {0, 2}, {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+    static const int narrow[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+#define IN_RANGE(x, y) ((x) >= job->range_x[0] && (x) <= job->range_x[1] && (y) >= job->range_y[0] && \
+                        (y) <= job->range_y[1])
+    if (shape->wide_pattern && best.sad > 4 * job->good_enough) {
+        for (uint32_t round = 0; round < shape->descent_rounds; round++) {
+            const int cx = best.mv.col / 4, cy = best.mv.row / 4;
+            int improved = 0;
+            for (int k = 0; k < 8; k++) {
+                const int px = cx + wide[k][0], py = cy + wide[k][1];
+                orc_found f;
+                if (IN_RANGE(px, py) && wps_eval(ref, costs, job, px, py, best.cost, &f)) {
+                    best = f;
+                    improved = 1;
+                }
+            }
+            if (!improved || best.sad <= job->good_enough) break;
+        }
+    }
+    for (uint32_t round = 0; round < shape->descent_rounds; round++) {
+        if (best.sad <= job->good_enough) break;
+        const int cx = best.mv.col / 4, cy = best.mv.row / 4;
+        int improved = 0;
+        for (int k = 0; k < 4; k++) {
+            const int px = cx + narrow[k][0], py = cy + narrow[k][1];
+            orc_found f;
+            if (IN_RANGE(px, py) && wps_eval(ref, costs, job, px, py, best.cost, &f)) {
+                best = f;
+                improved = 1;
+            }
+        }
+        if (!improved) break;
+    }
+#undef IN_RANGE
+    return best;
+}
+
+/* sub_pixel_refine: src/encoder/inter_search.rs:499-552 (half-pel probes
+ * read from HalfPlanes with the caller's filters[4], quarter-pel probes
+ * through mb_luma_sad_at_mv). */
+static void spr_consider(const orc_mv_cost_table *costs, orc_mv base, double lambda_sad, orc_mv mv, uint32_t sad,
+                         orc_found *best)
+{
+    double bits;
+    if (!mv_cost_bits(costs, mv, base, &bits)) return;
+    const double cost = (double)sad + lambda_sad * bits;
+    if (cost < best->cost) {
+        best->mv = mv;
+        best->sad = sad;
+        best->cost = cost;
+    }
+}
+
+orc_found orc_sub_pixel_refine(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *job,
+                               orc_found start, const int32_t filters[8][6], const orc_search_shape *shape)
+{
+    const int px = start.mv.col / 4, py = start.mv.row / 4;
+    uint8_t win[22 * 22];
+    orc_fetch_window(ref, (int)job->mb_col * 16 + px, (int)job->mb_row * 16 + py, win);
+    orc_half_planes planes;
+    orc_half_planes_new(win, filters[4], shape->half_diagonals, &planes);
+    orc_found best = start;
+    for (int dy = -1; dy <= 1; dy++) {
+        for (int dx = -1; dx <= 1; dx++) {
+            if ((dx == 0 && dy == 0) || (!shape->half_diagonals && dx != 0 && dy != 0)) continue;
+            orc_mv mv = {(int16_t)(start.mv.row + 2 * dy), (int16_t)(start.mv.col + 2 * dx)};
+            if (abs(mv.row) > 1020 || abs(mv.col) > 1020) continue;
+            spr_consider(costs, job->best, job->lambda_sad, mv, orc_half_planes_sad(&planes, job->src, dx, dy), &best);
+        }
+    }
+    if (shape->quarter) {
+        const orc_mv centre = best.mv;
+        static const int q[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        for (int k = 0; k < 4; k++) {
+            orc_mv mv = {(int16_t)(centre.row + q[k][1]), (int16_t)(centre.col + q[k][0])};
+            if (abs(mv.row) > 1022 || abs(mv.col) > 1022) continue;
+            const uint32_t sad = orc_mb_luma_sad_at_mv(ref, job->mb_col, job->mb_row, job->src, mv);
+            spr_consider(costs, job->best, job->lambda_sad, mv, sad, &best);
+        }
+    }
+    return best;
+}
+
+void orc_whole_pixel_search_batch(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *jobs,
+                                  size_t n, const orc_search_shape *shape, orc_found *out)
+{
+    for (size_t i = 0; i < n; i++) out[i] = orc_whole_pixel_search(ref, costs, &jobs[i], shape);
+}
+
+void orc_sub_pixel_refine_batch(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *jobs,
+                                const orc_found *start, size_t n, const int32_t filters[8][6],
+                                const orc_search_shape *shape, orc_found *out)
+{
+    for (size_t i = 0; i < n; i++) out[i] = orc_sub_pixel_refine(ref, costs, &jobs[i], start[i], filters, shape);
 }
 
 /* ---------------- batch helpers ---------------- */

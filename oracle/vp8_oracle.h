@@ -161,6 +161,48 @@ void orc_half_planes_new(const uint8_t window[22 * 22], const int32_t taps[6], i
                          orc_half_planes *out);
 uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], int dx, int dy);
 
+/* ---------------- encoder inter search (src/encoder/inter_search.rs) ---------------- */
+orc_search_result orc_small_diamond_search_luma(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
+                                                const uint8_t src[256], orc_mv center, uint32_t max_iters);
+/* MvCostTable, src/encoder/inter_search.rs:166-194: bits[component][diff + 1023], 0 = row. */
+typedef struct {
+    float bits[2][2047];
+} orc_mv_cost_table;
+/* The per-macroblock inputs of whole_pixel_search / sub_pixel_refine:
+ * position, the census predictor (RefSearch::census.best), the whole-pixel
+ * range, seeds, lambda_sad, good_enough and the descend flag. */
+typedef struct {
+    uint32_t mb_col, mb_row;
+    orc_mv best;
+    int32_t range_x[2], range_y[2];
+    uint32_t n_seeds;
+    orc_mv seeds[8];
+    double lambda_sad;
+    uint32_t good_enough;
+    int32_t descend;
+    const uint8_t *src;  /* the macroblock's 16x16 source */
+} orc_mb_search;
+/* SearchShape fields the two searches read (src/encoder/inter_search.rs:66-93). */
+typedef struct {
+    uint32_t descent_rounds;
+    int32_t wide_pattern, half_diagonals, quarter;
+} orc_search_shape;
+/* Found, src/encoder/inter_search.rs:349-355. */
+typedef struct {
+    orc_mv mv;
+    uint32_t sad;
+    double cost;
+} orc_found;
+orc_found orc_whole_pixel_search(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *job,
+                                 const orc_search_shape *shape);
+orc_found orc_sub_pixel_refine(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *job,
+                               orc_found start, const int32_t filters[8][6], const orc_search_shape *shape);
+void orc_whole_pixel_search_batch(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *jobs,
+                                  size_t n, const orc_search_shape *shape, orc_found *out);
+void orc_sub_pixel_refine_batch(const orc_luma_ref *ref, const orc_mv_cost_table *costs, const orc_mb_search *jobs,
+                                const orc_found *start, size_t n, const int32_t filters[8][6],
+                                const orc_search_shape *shape, orc_found *out);
+
 /* ---------------- batch helpers (one call over many MBs) ---------------- */
 void orc_predict_inter_mb_batch(const orc_ref_planes *ref, const uint32_t *mb_xy,
                                 const orc_mv *mv, size_t n, int full_pixel,

@@ -2,6 +2,7 @@
 // device staging for the host-pointer entry points, kernel dispatch.
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1089,6 +1090,230 @@ int vp8g_quarter_pixel_refine_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref,
                                          const vp8g_mv* centers, size_t n,
                                          vp8g_search_result* out) {
   return refine_batch(c, ref, mb_xy, src, centers, n, out, 1);
+}
+
+// ---- encoder inter search
+}  // extern "C"
+
+namespace {
+
+bool jobs_valid(const vp8g_mb_search* jobs, size_t n) {
+  for (size_t i = 0; i < n; i++) {
+    const vp8g_mb_search& j = jobs[i];
+    if (j.mb_col > 0xFFFFFu || j.mb_row > 0xFFFFFu || j.n_seeds > 8) return false;
+    if (j.range_x[0] > j.range_x[1] || j.range_y[0] > j.range_y[1]) return false;
+    if (j.range_x[0] < -8191 || j.range_x[1] > 8191 || j.range_y[0] < -8191 || j.range_y[1] > 8191) return false;
+    if (!isfinite(j.lambda_sad) || j.lambda_sad < 0.0 || j.good_enough >= (1u << 30)) return false;
+  }
+  return true;
+}
+
+bool taps_valid(const int32_t filters[8][6]) {
+  for (int r = 0; r < 8; r++)
+    for (int k = 0; k < 6; k++)
+      if (filters[r][k] < -(1 << 20) || filters[r][k] > (1 << 20)) return false;
+  return true;
+}
+
+WpsArgs make_wps(const vp8g_luma_ref* r, const vp8g_mv_cost_table* costs, const vp8g_mb_search* jobs,
+                 const uint8_t* src, size_t n, const vp8g_search_shape* shape, vp8g_found* out) {
+  WpsArgs a;
+  memset(&a, 0, sizeof(a));
+  a.plane = r->plane;
+  a.stride = (uint32_t)r->stride;
+  a.w = (int)r->width;
+  a.h = (int)r->height;
+  a.bits = &costs->bits[0][0];
+  a.jobs = jobs;
+  a.src = src;
+  a.out = out;
+  a.n = (int)n;
+  a.descent_rounds = shape->descent_rounds;
+  a.wide_pattern = shape->wide_pattern ? 1 : 0;
+  return a;
+}
+
+SprArgs make_spr(const vp8g_luma_ref* r, const vp8g_mv_cost_table* costs, const vp8g_mb_search* jobs,
+                 const uint8_t* src, const vp8g_found* start, size_t n, const int32_t filters[8][6],
+                 const vp8g_search_shape* shape, vp8g_found* out) {
+  SprArgs a;
+  memset(&a, 0, sizeof(a));
+  a.plane = r->plane;
+  a.stride = (uint32_t)r->stride;
+  a.w = (int)r->width;
+  a.h = (int)r->height;
+  a.bits = &costs->bits[0][0];
+  a.jobs = jobs;
+  a.src = src;
+  a.start = start;
+  a.out = out;
+  a.n = (int)n;
+  a.half_diagonals = shape->half_diagonals ? 1 : 0;
+  a.quarter = shape->quarter ? 1 : 0;
+  memcpy(a.taps, filters[4], sizeof(a.taps));
+  return a;
+}
+
+DiamondArgs make_diamond(const vp8g_luma_ref* r, const uint32_t* mb_xy, const uint8_t* src, const vp8g_mv* centers,
+                         size_t n, uint32_t max_iters, vp8g_search_result* out) {
+  DiamondArgs a;
+  memset(&a, 0, sizeof(a));
+  a.plane = r->plane;
+  a.stride = (uint32_t)r->stride;
+  a.w = (int)r->width;
+  a.h = (int)r->height;
+  a.mb_xy = mb_xy;
+  a.src = src;
+  a.center = reinterpret_cast<const int16_t*>(centers);
+  a.out = reinterpret_cast<uint32_t*>(out);
+  a.n = (int)n;
+  a.max_iters = max_iters;
+  return a;
+}
+
+// Device offsets of a staged inter-search batch in the arena.
+struct InterStage {
+  size_t plane, costs, jobs, src, start, out, total;
+};
+
+InterStage inter_stage_layout(const vp8g_luma_ref* ref, size_t n, bool with_start, size_t out_bytes) {
+  InterStage L;
+  L.plane = 0;
+  L.costs = align_up(plane_span(ref->stride, ref->width, ref->height), 256);
+  L.jobs = L.costs + align_up(sizeof(vp8g_mv_cost_table), 256);
+  L.src = L.jobs + align_up(sizeof(vp8g_mb_search) * n, 256);
+  L.start = L.src + align_up(256 * n, 256);
+  L.out = L.start + (with_start ? align_up(sizeof(vp8g_found) * n, 256) : 0);
+  L.total = L.out + out_bytes;
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vp8g_whole_pixel_search_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const vp8g_mv_cost_table* costs,
+                                      const vp8g_mb_search* jobs, const uint8_t* src, size_t n,
+                                      const vp8g_search_shape* shape, vp8g_found* out) {
+  if (!c || !shape) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!costs || !jobs || !src || !out || n > 0x7FFFFFF) return VP8G_E_ARG;
+  if (((uintptr_t)costs & 3) || ((uintptr_t)src & 3) || (((uintptr_t)jobs | (uintptr_t)out) & 7)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const WpsArgs a = make_wps(ref, costs, jobs, src, n, shape, out);
+  return launch_status(c, [&] { return launch_whole_pixel_search(a, c->stream); });
+}
+
+int vp8g_whole_pixel_search_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const vp8g_mv_cost_table* costs,
+                                  const vp8g_mb_search* jobs, const uint8_t* src, size_t n,
+                                  const vp8g_search_shape* shape, vp8g_found* out) {
+  if (!c || !shape) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!costs || !jobs || !src || !out || n > 0x7FFFFFF || !jobs_valid(jobs, n)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const InterStage L = inter_stage_layout(ref, n, false, sizeof(vp8g_found) * n);
+  st = ensure_arena(c, L.total);
+  if (st != VP8G_OK) return st;
+  uint8_t* b = c->arena;
+  st = copy_in(c, b + L.plane, ref->plane, plane_span(ref->stride, ref->width, ref->height));
+  if (st == VP8G_OK) st = copy_in(c, b + L.costs, costs, sizeof(vp8g_mv_cost_table));
+  if (st == VP8G_OK) st = copy_in(c, b + L.jobs, jobs, sizeof(vp8g_mb_search) * n);
+  if (st == VP8G_OK) st = copy_in(c, b + L.src, src, 256 * n);
+  if (st != VP8G_OK) return finish(c, st);
+  vp8g_luma_ref d = *ref;
+  d.plane = b + L.plane;
+  const WpsArgs a = make_wps(&d, reinterpret_cast<const vp8g_mv_cost_table*>(b + L.costs),
+                             reinterpret_cast<const vp8g_mb_search*>(b + L.jobs), b + L.src, n, shape,
+                             reinterpret_cast<vp8g_found*>(b + L.out));
+  st = launch_status(c, [&] { return launch_whole_pixel_search(a, c->stream); });
+  if (st == VP8G_OK) st = copy_out(c, out, b + L.out, sizeof(vp8g_found) * n);
+  return finish(c, st);
+}
+
+int vp8g_sub_pixel_refine_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const vp8g_mv_cost_table* costs,
+                                    const vp8g_mb_search* jobs, const uint8_t* src, const vp8g_found* start,
+                                    size_t n, const int32_t filters[8][6], const vp8g_search_shape* shape,
+                                    vp8g_found* out) {
+  if (!c || !shape || !filters || !taps_valid(filters)) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!costs || !jobs || !src || !start || !out || n > 0x7FFFFFF) return VP8G_E_ARG;
+  if (((uintptr_t)costs & 3) || ((uintptr_t)src & 3) || (((uintptr_t)jobs | (uintptr_t)out | (uintptr_t)start) & 7))
+    return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const SprArgs a = make_spr(ref, costs, jobs, src, start, n, filters, shape, out);
+  return launch_status(c, [&] { return launch_sub_pixel_refine(a, c->stream); });
+}
+
+int vp8g_sub_pixel_refine_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const vp8g_mv_cost_table* costs,
+                                const vp8g_mb_search* jobs, const uint8_t* src, const vp8g_found* start, size_t n,
+                                const int32_t filters[8][6], const vp8g_search_shape* shape, vp8g_found* out) {
+  if (!c || !shape || !filters || !taps_valid(filters)) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!costs || !jobs || !src || !start || !out || n > 0x7FFFFFF || !jobs_valid(jobs, n)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const InterStage L = inter_stage_layout(ref, n, true, sizeof(vp8g_found) * n);
+  st = ensure_arena(c, L.total);
+  if (st != VP8G_OK) return st;
+  uint8_t* b = c->arena;
+  st = copy_in(c, b + L.plane, ref->plane, plane_span(ref->stride, ref->width, ref->height));
+  if (st == VP8G_OK) st = copy_in(c, b + L.costs, costs, sizeof(vp8g_mv_cost_table));
+  if (st == VP8G_OK) st = copy_in(c, b + L.jobs, jobs, sizeof(vp8g_mb_search) * n);
+  if (st == VP8G_OK) st = copy_in(c, b + L.src, src, 256 * n);
+  if (st == VP8G_OK) st = copy_in(c, b + L.start, start, sizeof(vp8g_found) * n);
+  if (st != VP8G_OK) return finish(c, st);
+  vp8g_luma_ref d = *ref;
+  d.plane = b + L.plane;
+  const SprArgs a = make_spr(&d, reinterpret_cast<const vp8g_mv_cost_table*>(b + L.costs),
+                             reinterpret_cast<const vp8g_mb_search*>(b + L.jobs), b + L.src,
+                             reinterpret_cast<const vp8g_found*>(b + L.start), n, filters, shape,
+                             reinterpret_cast<vp8g_found*>(b + L.out));
+  st = launch_status(c, [&] { return launch_sub_pixel_refine(a, c->stream); });
+  if (st == VP8G_OK) st = copy_out(c, out, b + L.out, sizeof(vp8g_found) * n);
+  return finish(c, st);
+}
+
+int vp8g_small_diamond_search_luma_batch_dev(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                                             const uint8_t* src, const vp8g_mv* centers, size_t n,
+                                             uint32_t max_iters, vp8g_search_result* out) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !src || !centers || !out || n > 0x7FFFFFF) return VP8G_E_ARG;
+  if (((uintptr_t)mb_xy | (uintptr_t)src | (uintptr_t)centers | (uintptr_t)out) & 3) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const DiamondArgs a = make_diamond(ref, mb_xy, src, centers, n, max_iters, out);
+  return launch_status(c, [&] { return launch_small_diamond(a, c->stream); });
+}
+
+int vp8g_small_diamond_search_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref, const uint32_t* mb_xy,
+                                         const uint8_t* src, const vp8g_mv* centers, size_t n, uint32_t max_iters,
+                                         vp8g_search_result* out) {
+  if (!c) return VP8G_E_ARG;
+  int st = check_luma_ref(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !src || !centers || !out || n > 0x7FFFFFF) return VP8G_E_ARG;
+  if (!mb_xy_in_range(mb_xy, n)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  vp8g_luma_ref d;
+  uint32_t* dxy;
+  uint8_t *dsrc, *dout;
+  vp8g_mv* dmv;
+  st = stage_luma_batch(c, ref, mb_xy, src, centers, n, 8 * n, &d, &dxy, &dsrc, &dmv, &dout);
+  if (st != VP8G_OK) return finish(c, st);
+  const DiamondArgs a = make_diamond(&d, dxy, dsrc, dmv, n, max_iters, reinterpret_cast<vp8g_search_result*>(dout));
+  st = launch_status(c, [&] { return launch_small_diamond(a, c->stream); });
+  if (st == VP8G_OK) st = copy_out(c, out, dout, 8 * n);
+  return finish(c, st);
 }
 
 }  // extern "C"

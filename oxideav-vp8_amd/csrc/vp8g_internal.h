@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/vp8g.h"
+
 namespace vp8g {
 
 constexpr uint32_t kWMax = 16;              // centre weight, src/arnr.rs:37
@@ -183,6 +185,51 @@ struct RefineArgs {
   int step;                // 2: half-pel ladder, 1: quarter-pel ladder
   int aligned4;            // set by the launcher: plane and stride 4-byte aligned
 };
+
+// Encoder inter search (whole_pixel_search, sub_pixel_refine,
+// small_diamond_search_luma).
+struct WpsArgs {
+  const uint8_t* plane;
+  uint32_t stride;
+  int w, h;                      // LumaRef clamp bounds
+  const float* bits;             // MvCostTable [2][2047]
+  const vp8g_mb_search* jobs;    // n
+  const uint8_t* src;            // n x 256
+  vp8g_found* out;               // n
+  int n;
+  uint32_t descent_rounds;
+  int wide_pattern;
+};
+
+struct SprArgs {
+  const uint8_t* plane;
+  uint32_t stride;
+  int w, h;
+  const float* bits;
+  const vp8g_mb_search* jobs;
+  const uint8_t* src;
+  const vp8g_found* start;
+  vp8g_found* out;
+  int n;
+  int half_diagonals, quarter;
+  int32_t taps[6];               // the caller's filters[4] (HalfPlanes)
+};
+
+struct DiamondArgs {
+  const uint8_t* plane;
+  uint32_t stride;
+  int w, h;
+  const uint32_t* mb_xy;         // n x {mb_col, mb_row}
+  const uint8_t* src;            // n x 256
+  const int16_t* center;         // n x {row, col}
+  uint32_t* out;                 // n x vp8g_search_result
+  int n;
+  uint32_t max_iters;
+};
+
+hipError_t launch_whole_pixel_search(const WpsArgs& a, hipStream_t stream);
+hipError_t launch_sub_pixel_refine(const SprArgs& a, hipStream_t stream);
+hipError_t launch_small_diamond(const DiamondArgs& a, hipStream_t stream);
 
 hipError_t launch_predict_inter_mb(const McArgs& a, hipStream_t stream);
 hipError_t launch_mb_luma_sad_at_mv(const SadArgs& a, hipStream_t stream);

@@ -117,6 +117,9 @@ EXPORTS = (
     "vp8g_proposer_backward_dev", "vp8g_proposer_train_step_dev", "vp8g_synth_luma_dev",
     "vp8g_half_pixel_refine_luma_batch_dev", "vp8g_quarter_pixel_refine_luma_batch_dev",
     "vp8g_search_fallback_dev", "vp8g_search_fallback_noalloc_dev",
+    "vp8g_whole_pixel_search_batch", "vp8g_whole_pixel_search_batch_dev", "vp8g_sub_pixel_refine_batch",
+    "vp8g_sub_pixel_refine_batch_dev", "vp8g_small_diamond_search_luma_batch",
+    "vp8g_small_diamond_search_luma_batch_dev",
 )
 FALLBACK_GATE_NEIGHBOUR = 1
 
@@ -161,6 +164,13 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_search_fallback_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp, C.POINTER(_Err)]),
         "vp8g_search_fallback_noalloc_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp,
                                                    C.POINTER(_Err)]),
+        "vp8g_whole_pixel_search_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp, vp]),
+        "vp8g_whole_pixel_search_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp, vp]),
+        "vp8g_sub_pixel_refine_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, vp, sz, vp, vp, vp]),
+        "vp8g_sub_pixel_refine_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, vp, sz, vp, vp, vp]),
+        "vp8g_small_diamond_search_luma_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, C.c_uint32, vp]),
+        "vp8g_small_diamond_search_luma_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, C.c_uint32,
+                                                           vp]),
         "vp8g_upsample_table_dev": (i32, [vp, vp, sz, i32, i32, vp, i32, i32]),
         "vp8g_build_arnr_altref_table_dev": (i32, [vp, C.POINTER(_I420), sz, sz, u8, vp, vp, vp, vp,
                                                    C.POINTER(_Err)]),
