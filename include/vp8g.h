@@ -413,6 +413,44 @@ int vp8g_half_plane_sads_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const in
                                const uint8_t *src, size_t n, const int32_t taps[6],
                                int diagonals, uint32_t *sad);
 
+/* ---- SPLITMV prediction and inter-macroblock reconstruction ----
+ * Replaces predict_split_mv (src/motion_comp.rs:2227-2320) over a batch:
+ * macroblock i at (mb_xy[2i], mb_xy[2i+1]) with its 16 quarter-pel luma
+ * sub-block vectors mvs[16*i .. 16*i+15] (raster order); every luma 4x4
+ * sub-block through filter_block_4x4 (:1693-1712) under its own doubled
+ * vector, every chroma 4x4 sub-block under split_chroma_mvs (:2162-2203),
+ * no secondary clamp.  `filters` taps lie within +-2^20.  out: n x 384
+ * (ReconstructedMb). */
+int vp8g_predict_split_mv_batch(vp8g_ctx *ctx, const vp8g_ref_planes *ref, const uint32_t *mb_xy,
+                                const vp8g_mv *mvs, size_t n, int full_pixel, const int32_t filters[8][6],
+                                uint8_t *out);
+/* Replaces reconstruct_inter_mb (src/motion_comp.rs:2098-2150) over a
+ * batch: predict_inter_mb at mv[i], then unless skip[i] (mb_skip_coeff) the
+ * residue of the dequantised coefficients coeffs[i] = y2[16], y[16][16],
+ * u[4][16], v[4][16] (int16, 400 per macroblock): the inverse WHT of Y2
+ * seeds each Y block's DC (src/inverse_transform.rs:170-240), and every
+ * block's inverse DCT is added with clamp255 (:417-480). */
+int vp8g_reconstruct_inter_mb_batch(vp8g_ctx *ctx, const vp8g_ref_planes *ref, const uint32_t *mb_xy,
+                                    const vp8g_mv *mv, size_t n, int full_pixel, const int32_t filters[8][6],
+                                    const uint8_t *skip, const int16_t *coeffs, uint8_t *out);
+/* Replaces reconstruct_split_mv_mb (src/motion_comp.rs:2340-2385): as
+ * predict_split_mv, then the residue of coeffs[i] = y[16][16], u[4][16],
+ * v[4][16] (384 per macroblock; no Y2). */
+int vp8g_reconstruct_split_mv_mb_batch(vp8g_ctx *ctx, const vp8g_ref_planes *ref, const uint32_t *mb_xy,
+                                       const vp8g_mv *mvs, size_t n, int full_pixel, const int32_t filters[8][6],
+                                       const uint8_t *skip, const int16_t *coeffs, uint8_t *out);
+/* The three over device arrays (enqueue only). */
+int vp8g_predict_split_mv_batch_dev(vp8g_ctx *ctx, const vp8g_ref_planes *ref, const uint32_t *mb_xy,
+                                    const vp8g_mv *mvs, size_t n, int full_pixel, const int32_t filters[8][6],
+                                    uint8_t *out);
+int vp8g_reconstruct_inter_mb_batch_dev(vp8g_ctx *ctx, const vp8g_ref_planes *ref, const uint32_t *mb_xy,
+                                        const vp8g_mv *mv, size_t n, int full_pixel, const int32_t filters[8][6],
+                                        const uint8_t *skip, const int16_t *coeffs, uint8_t *out);
+int vp8g_reconstruct_split_mv_mb_batch_dev(vp8g_ctx *ctx, const vp8g_ref_planes *ref, const uint32_t *mb_xy,
+                                           const vp8g_mv *mvs, size_t n, int full_pixel,
+                                           const int32_t filters[8][6], const uint8_t *skip, const int16_t *coeffs,
+                                           uint8_t *out);
+
 /* ---- encoder inter motion search ----
  * The caller side of sub-pel SAD scoring: the rate-distortion motion search
  * of the inter-frame encoder over a batch of macroblocks against one

@@ -132,6 +132,13 @@ class Oracle:
         L.orc_half_planes_new.argtypes = [vp, vp, i32, C.POINTER(HalfPlanes)]
         L.orc_half_planes_sad.argtypes = [C.POINTER(HalfPlanes), vp, i32, i32]
         L.orc_half_planes_sad.restype = u32
+        L.orc_filter_block_4x4.argtypes = [vp, sz, sz, sz, sz, sz, MvS, vp, vp]
+        L.orc_predict_split_mv.argtypes = [C.POINTER(RefPlanes), sz, sz, vp, i32, vp, vp]
+        L.orc_reconstruct_inter_mb.argtypes = [C.POINTER(RefPlanes), sz, sz, MvS, i32, vp, i32, vp, vp]
+        L.orc_reconstruct_split_mv_mb.argtypes = [C.POINTER(RefPlanes), sz, sz, vp, i32, vp, i32, vp, vp]
+        L.orc_predict_split_mv_batch.argtypes = [C.POINTER(RefPlanes), vp, vp, sz, i32, vp, vp]
+        L.orc_reconstruct_inter_mb_batch.argtypes = [C.POINTER(RefPlanes), vp, vp, sz, i32, vp, vp, vp, vp]
+        L.orc_reconstruct_split_mv_mb_batch.argtypes = [C.POINTER(RefPlanes), vp, vp, sz, i32, vp, vp, vp, vp]
         L.orc_small_diamond_search_luma.argtypes = [C.POINTER(LumaRef), sz, sz, vp, MvS, u32]
         L.orc_small_diamond_search_luma.restype = SearchResult
         L.orc_whole_pixel_search_batch.argtypes = [C.POINTER(LumaRef), vp, vp, sz, vp, vp]

@@ -788,6 +788,203 @@ uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], i
     return window_sad(src, p->d, 17, cy * 17 + cx);
 }
 
+/* ---------------- SPLITMV prediction and reconstruction ---------------- */
+/* filter_block_4x4: src/motion_comp.rs:1693-1712 (whole-pixel copy when both
+ * fractions are zero, fetch_block_whole_pixel :1442-1487; else the 9x9
+ * halo through sixtap_2d). */
+void orc_filter_block_4x4(const uint8_t *plane, size_t stride, size_t w, size_t h, size_t blk_x, size_t blk_y,
+                          orc_mv mv, const int32_t filters[8][6], uint8_t out[16])
+{
+    const int mx = mv.col & 7, my = mv.row & 7;
+    if (mx == 0 && my == 0) {
+        gather(plane, stride, w, h, (long)blk_x + (mv.col >> 3), (long)blk_y + (mv.row >> 3), 4, 4, out);
+        return;
+    }
+    uint8_t halo[81];
+    orc_fetch_block_halo(plane, stride, w, h, blk_x, blk_y, mv, halo);
+    orc_sixtap_2d(halo, mx, my, filters, out);
+}
+
+/* chroma_idx_for_luma_subblock + split_chroma_mvs: src/motion_comp.rs:2162-2203. */
+void orc_split_chroma_mvs(const orc_mv luma[16], orc_mv out[4])
+{
+    int32_t sr[4] = {0, 0, 0, 0}, sc[4] = {0, 0, 0, 0};
+    for (int b = 0; b < 16; b++) {
+        const int c = ((b >> 1) & 1) + ((b >> 2) & 2);
+        sr[c] = (int32_t)((uint32_t)sr[c] + (uint32_t)(2 * (int32_t)luma[b].row));
+        sc[c] = (int32_t)((uint32_t)sc[c] + (uint32_t)(2 * (int32_t)luma[b].col));
+    }
+    for (int c = 0; c < 4; c++) {
+        const int32_t r = sr[c] >= 0 ? (sr[c] + 4) >> 3 : -((-sr[c] + 4) >> 3);
+        const int32_t q = sc[c] >= 0 ? (sc[c] + 4) >> 3 : -((-sc[c] + 4) >> 3);
+        out[c].row = (int16_t)r;
+        out[c].col = (int16_t)q;
+    }
+}
+
+/* predict_split_mv: src/motion_comp.rs:2227-2320. */
+void orc_predict_split_mv(const orc_ref_planes *ref, size_t mb_col, size_t mb_row, const orc_mv mvs[16],
+                          int full_pixel, const int32_t filters[8][6], uint8_t out[384])
+{
+    const size_t lw = ref->mb_cols * 16, lh = ref->mb_rows * 16, cw = ref->mb_cols * 8, ch = ref->mb_rows * 8;
+    uint8_t blk[16];
+    for (int sb = 0; sb < 4; sb++)
+        for (int sc = 0; sc < 4; sc++) {
+            orc_mv m = orc_stored_luma_mv(mvs[sb * 4 + sc]);
+            if (full_pixel) m = orc_apply_full_pixel(m);
+            orc_filter_block_4x4(ref->y, ref->y_stride, lw, lh, mb_col * 16 + sc * 4, mb_row * 16 + sb * 4, m, filters,
+                                 blk);
+            for (int r = 0; r < 4; r++) memcpy(out + (sb * 4 + r) * 16 + sc * 4, blk + r * 4, 4);
+        }
+    orc_mv cm[4];
+    orc_split_chroma_mvs(mvs, cm);
+    for (int sb = 0; sb < 2; sb++)
+        for (int sc = 0; sc < 2; sc++) {
+            orc_mv m = cm[sb * 2 + sc];
+            if (full_pixel) m = orc_apply_full_pixel(m);
+            for (int p = 0; p < 2; p++) {
+                orc_filter_block_4x4(p ? ref->v : ref->u, ref->uv_stride, cw, ch, mb_col * 8 + sc * 4,
+                                     mb_row * 8 + sb * 4, m, filters, blk);
+                for (int r = 0; r < 4; r++) memcpy(out + 256 + 64 * p + (sb * 4 + r) * 8 + sc * 4, blk + r * 4, 4);
+            }
+        }
+}
+
+/* inverse_wht_4x4 (the scalar listing): src/inverse_transform.rs:190-240. */
+void orc_inverse_wht_4x4(const int16_t in[16], int16_t out[16])
+{
+    int32_t t[16];
+    for (int c = 0; c < 4; c++) {
+        const int32_t a1 = in[c] + in[12 + c], b1 = in[4 + c] + in[8 + c];
+        const int32_t c1 = in[4 + c] - in[8 + c], d1 = in[c] - in[12 + c];
+        t[c] = a1 + b1;
+        t[4 + c] = c1 + d1;
+        t[8 + c] = a1 - b1;
+        t[12 + c] = d1 - c1;
+    }
+    for (int r = 0; r < 4; r++) {
+        const int32_t *q = t + 4 * r;
+        const int32_t a1 = q[0] + q[3], b1 = q[1] + q[2], c1 = q[1] - q[2], d1 = q[0] - q[3];
+        out[4 * r] = (int16_t)((a1 + b1 + 3) >> 3);
+        out[4 * r + 1] = (int16_t)((c1 + d1 + 3) >> 3);
+        out[4 * r + 2] = (int16_t)((a1 - b1 + 3) >> 3);
+        out[4 * r + 3] = (int16_t)((d1 - c1 + 3) >> 3);
+    }
+}
+
+/* inverse_dct_4x4_scalar: src/inverse_transform.rs:609-678, in the wrapping
+ * 32-bit arithmetic of the reference listing (constants :361-363). */
+static inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+static inline int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+static inline int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+
+static void idct_1d(int32_t i0, int32_t i4, int32_t i8, int32_t i12, int32_t o[4])
+{
+    enum { kCos = 20091, kSin = 35468 };
+    const int32_t a1 = wadd(i0, i8), b1 = wsub(i0, i8);
+    const int32_t c1 = wsub(wmul(i4, kSin) >> 16, wadd(i12, wmul(i12, kCos) >> 16));
+    const int32_t d1 = wadd(wadd(i4, wmul(i4, kCos) >> 16), wmul(i12, kSin) >> 16);
+    o[0] = wadd(a1, d1);
+    o[1] = wadd(b1, c1);
+    o[2] = wsub(b1, c1);
+    o[3] = wsub(a1, d1);
+}
+
+void orc_inverse_dct_4x4(const int16_t in[16], int16_t out[16])
+{
+    int32_t t[16], o[4];
+    for (int c = 0; c < 4; c++) {
+        idct_1d(in[c], in[4 + c], in[8 + c], in[12 + c], o);
+        for (int k = 0; k < 4; k++) t[4 * k + c] = o[k];
+    }
+    for (int r = 0; r < 4; r++) {
+        idct_1d(t[4 * r], t[4 * r + 1], t[4 * r + 2], t[4 * r + 3], o);
+        for (int k = 0; k < 4; k++) out[4 * r + k] = (int16_t)(wadd(o[k], 4) >> 3);
+    }
+}
+
+/* inverse_dct_4x4_add_into: src/inverse_transform.rs:417-480 (with its
+ * DC-only path) and add_residue_4x4 :871-875. */
+void orc_inverse_dct_4x4_add_into(const int16_t in[16], uint8_t *plane, size_t stride, int sub_row, int sub_col)
+{
+    int dc_only = 1;
+    for (int k = 1; k < 16; k++)
+        if (in[k]) dc_only = 0;
+    uint8_t *base = plane + (size_t)sub_row * 4 * stride + (size_t)sub_col * 4;
+    if (dc_only) {
+        const int dc = ((int)in[0] + 4) >> 3;
+        if (dc)
+            for (int r = 0; r < 4; r++)
+                for (int c = 0; c < 4; c++) base[r * stride + c] = (uint8_t)clampi(base[r * stride + c] + dc, 0, 255);
+        return;
+    }
+    int16_t res[16];
+    orc_inverse_dct_4x4(in, res);
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) base[r * stride + c] = (uint8_t)clampi(base[r * stride + c] + res[4 * r + c], 0, 255);
+}
+
+static void add_mb_residue(const int16_t (*yc)[16], const int16_t (*uc)[16], const int16_t (*vc)[16], uint8_t out[384])
+{
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) orc_inverse_dct_4x4_add_into(yc[i * 4 + j], out, 16, i, j);
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            orc_inverse_dct_4x4_add_into(uc[i * 2 + j], out + 256, 8, i, j);
+            orc_inverse_dct_4x4_add_into(vc[i * 2 + j], out + 320, 8, i, j);
+        }
+}
+
+/* reconstruct_inter_mb: src/motion_comp.rs:2098-2150.  coeffs = y2[16],
+ * y[16][16], u[4][16], v[4][16] (dequantised). */
+void orc_reconstruct_inter_mb(const orc_ref_planes *ref, size_t mb_col, size_t mb_row, orc_mv mv, int full_pixel,
+                              const int32_t filters[8][6], int skip, const int16_t coeffs[25][16], uint8_t out[384])
+{
+    orc_predict_inter_mb(ref, mb_col, mb_row, mv, full_pixel, filters, out);
+    if (skip) return;
+    int16_t y2r[16], yc[16][16];
+    orc_inverse_wht_4x4(coeffs[0], y2r);
+    memcpy(yc, coeffs[1], sizeof(yc));
+    for (int b = 0; b < 16; b++) yc[b][0] = y2r[b];
+    add_mb_residue((const int16_t(*)[16])yc, coeffs + 17, coeffs + 21, out);
+}
+
+/* reconstruct_split_mv_mb: src/motion_comp.rs:2340-2385 (no Y2).  coeffs =
+ * y[16][16], u[4][16], v[4][16]. */
+void orc_reconstruct_split_mv_mb(const orc_ref_planes *ref, size_t mb_col, size_t mb_row, const orc_mv mvs[16],
+                                 int full_pixel, const int32_t filters[8][6], int skip, const int16_t coeffs[24][16],
+                                 uint8_t out[384])
+{
+    orc_predict_split_mv(ref, mb_col, mb_row, mvs, full_pixel, filters, out);
+    if (skip) return;
+    add_mb_residue(coeffs, coeffs + 16, coeffs + 20, out);
+}
+
+void orc_predict_split_mv_batch(const orc_ref_planes *ref, const uint32_t *mb_xy, const orc_mv *mvs, size_t n,
+                                int full_pixel, const int32_t filters[8][6], uint8_t *out)
+{
+    for (size_t i = 0; i < n; i++)
+        orc_predict_split_mv(ref, mb_xy[2 * i], mb_xy[2 * i + 1], mvs + 16 * i, full_pixel, filters, out + 384 * i);
+}
+
+void orc_reconstruct_inter_mb_batch(const orc_ref_planes *ref, const uint32_t *mb_xy, const orc_mv *mv, size_t n,
+                                    int full_pixel, const int32_t filters[8][6], const uint8_t *skip,
+                                    const int16_t *coeffs, uint8_t *out)
+{
+    for (size_t i = 0; i < n; i++)
+        orc_reconstruct_inter_mb(ref, mb_xy[2 * i], mb_xy[2 * i + 1], mv[i], full_pixel, filters, skip[i],
+                                 (const int16_t(*)[16])(coeffs + 400 * i), out + 384 * i);
+}
+
+void orc_reconstruct_split_mv_mb_batch(const orc_ref_planes *ref, const uint32_t *mb_xy, const orc_mv *mvs, size_t n,
+                                       int full_pixel, const int32_t filters[8][6], const uint8_t *skip,
+                                       const int16_t *coeffs, uint8_t *out)
+{
+    for (size_t i = 0; i < n; i++)
+        orc_reconstruct_split_mv_mb(ref, mb_xy[2 * i], mb_xy[2 * i + 1], mvs + 16 * i, full_pixel, filters, skip[i],
+                                    (const int16_t(*)[16])(coeffs + 384 * i), out + 384 * i);
+}
+
 /* ---------------- encoder inter search ---------------- */
 /* small_diamond_search_luma: src/motion_search.rs:381-442 (snap_to_whole_pixel
  * :360-363 truncates toward zero; clamp first, then snap). */

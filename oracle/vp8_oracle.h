@@ -161,6 +161,29 @@ void orc_half_planes_new(const uint8_t window[22 * 22], const int32_t taps[6], i
                          orc_half_planes *out);
 uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], int dx, int dy);
 
+/* ---------------- SPLITMV and reconstruction (src/motion_comp.rs, src/inverse_transform.rs) ---------------- */
+void orc_filter_block_4x4(const uint8_t *plane, size_t stride, size_t w, size_t h, size_t blk_x, size_t blk_y,
+                          orc_mv mv, const int32_t filters[8][6], uint8_t out[16]);
+void orc_split_chroma_mvs(const orc_mv luma[16], orc_mv out[4]);
+void orc_predict_split_mv(const orc_ref_planes *ref, size_t mb_col, size_t mb_row, const orc_mv mvs[16],
+                          int full_pixel, const int32_t filters[8][6], uint8_t out[384]);
+void orc_inverse_wht_4x4(const int16_t in[16], int16_t out[16]);
+void orc_inverse_dct_4x4(const int16_t in[16], int16_t out[16]);
+void orc_inverse_dct_4x4_add_into(const int16_t in[16], uint8_t *plane, size_t stride, int sub_row, int sub_col);
+void orc_reconstruct_inter_mb(const orc_ref_planes *ref, size_t mb_col, size_t mb_row, orc_mv mv, int full_pixel,
+                              const int32_t filters[8][6], int skip, const int16_t coeffs[25][16], uint8_t out[384]);
+void orc_reconstruct_split_mv_mb(const orc_ref_planes *ref, size_t mb_col, size_t mb_row, const orc_mv mvs[16],
+                                 int full_pixel, const int32_t filters[8][6], int skip, const int16_t coeffs[24][16],
+                                 uint8_t out[384]);
+void orc_predict_split_mv_batch(const orc_ref_planes *ref, const uint32_t *mb_xy, const orc_mv *mvs, size_t n,
+                                int full_pixel, const int32_t filters[8][6], uint8_t *out);
+void orc_reconstruct_inter_mb_batch(const orc_ref_planes *ref, const uint32_t *mb_xy, const orc_mv *mv, size_t n,
+                                    int full_pixel, const int32_t filters[8][6], const uint8_t *skip,
+                                    const int16_t *coeffs, uint8_t *out);
+void orc_reconstruct_split_mv_mb_batch(const orc_ref_planes *ref, const uint32_t *mb_xy, const orc_mv *mvs, size_t n,
+                                       int full_pixel, const int32_t filters[8][6], const uint8_t *skip,
+                                       const int16_t *coeffs, uint8_t *out);
+
 /* ---------------- encoder inter search (src/encoder/inter_search.rs) ---------------- */
 orc_search_result orc_small_diamond_search_luma(const orc_luma_ref *ref, size_t mb_col, size_t mb_row,
                                                 const uint8_t src[256], orc_mv center, uint32_t max_iters);

@@ -324,6 +324,14 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   return launch_status(c, [&] { return launch_arnr_fused(a, k, c->stream); });
 }
 
+// Caller tap tables whose taps keep every six-tap sum inside int32.
+bool taps_valid(const int32_t filters[8][6]) {
+  for (int r = 0; r < 8; r++)
+    for (int k = 0; k < 6; k++)
+      if (filters[r][k] < -(1 << 20) || filters[r][k] > (1 << 20)) return false;
+  return true;
+}
+
 int check_ref_planes(const vp8g_ref_planes* r) {
   if (!r || !r->y || !r->u || !r->v || r->mb_cols == 0 || r->mb_rows == 0) return VP8G_E_ARG;
   if (r->y_stride < r->mb_cols * 16 || r->uv_stride < r->mb_cols * 8) return VP8G_E_ARG;
@@ -1092,6 +1100,143 @@ int vp8g_quarter_pixel_refine_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref,
   return refine_batch(c, ref, mb_xy, src, centers, n, out, 1);
 }
 
+// ---- SPLITMV and reconstruction
+}  // extern "C"
+
+namespace {
+
+SplitArgs make_split(const vp8g_ref_planes* r, const uint32_t* mb_xy, const vp8g_mv* mvs, size_t n, int full_pixel,
+                     const int32_t filters[8][6], uint8_t* out) {
+  SplitArgs a;
+  memset(&a, 0, sizeof(a));
+  a.ref = McPlanes{r->y, r->u, r->v, (uint32_t)r->y_stride, (uint32_t)r->uv_stride, (int)(r->mb_cols * 16),
+                   (int)(r->mb_rows * 16), (int)(r->mb_cols * 8), (int)(r->mb_rows * 8)};
+  a.mb_xy = mb_xy;
+  a.mvs = reinterpret_cast<const int16_t*>(mvs);
+  a.out = out;
+  a.n = (int)n;
+  a.full_pixel = full_pixel ? 1 : 0;
+  memcpy(a.filters, filters, sizeof(a.filters));
+  return a;
+}
+
+// Enqueue prediction (split or whole-MB) and, with coefficients, the
+// residual, on device arrays.  mvs_per_mb: 16 (SPLITMV) or 1.
+int enqueue_recon(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy, const vp8g_mv* mvs, size_t n,
+                  int full_pixel, const int32_t filters[8][6], const uint8_t* skip, const int16_t* coeffs,
+                  uint8_t* out, int mvs_per_mb) {
+  int st;
+  if (mvs_per_mb == 16) {
+    const SplitArgs a = make_split(ref, mb_xy, mvs, n, full_pixel, filters, out);
+    st = launch_status(c, [&] { return launch_split_predict(a, c->stream); });
+  } else {
+    const McArgs a = make_mc_args(ref, mb_xy, mvs, n, full_pixel, filters, out);
+    st = launch_status(c, [&] { return launch_predict_inter_mb(a, c->stream); });
+  }
+  if (st != VP8G_OK || !coeffs) return st;
+  ResidualArgs r;
+  r.coeffs = coeffs;
+  r.skip = skip;
+  r.out = out;
+  r.n = (int)n;
+  r.y2 = mvs_per_mb == 16 ? 0 : 1;
+  return launch_status(c, [&] { return launch_mb_residual(r, c->stream); });
+}
+
+int recon_dev(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy, const vp8g_mv* mvs, size_t n,
+              int full_pixel, const int32_t filters[8][6], const uint8_t* skip, const int16_t* coeffs, uint8_t* out,
+              int mvs_per_mb, bool with_residual) {
+  if (!c || !filters || !taps_valid(filters)) return VP8G_E_ARG;
+  int st = check_ref_planes(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !mvs || !out || n > 0x7FFFFFF || (with_residual && (!skip || !coeffs))) return VP8G_E_ARG;
+  if (((uintptr_t)mb_xy | (uintptr_t)mvs | (uintptr_t)out | (uintptr_t)coeffs) & 3) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  return enqueue_recon(c, ref, mb_xy, mvs, n, full_pixel, filters, with_residual ? skip : nullptr,
+                       with_residual ? coeffs : nullptr, out, mvs_per_mb);
+}
+
+int recon_host(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy, const vp8g_mv* mvs, size_t n,
+               int full_pixel, const int32_t filters[8][6], const uint8_t* skip, const int16_t* coeffs, uint8_t* out,
+               int mvs_per_mb, bool with_residual) {
+  if (!c || !filters || !taps_valid(filters)) return VP8G_E_ARG;
+  int st = check_ref_planes(ref);
+  if (st != VP8G_OK) return st;
+  if (n == 0) return VP8G_OK;
+  if (!mb_xy || !mvs || !out || n > 0x7FFFFFF || (with_residual && (!skip || !coeffs))) return VP8G_E_ARG;
+  if (!mb_xy_in_range(mb_xy, n)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const size_t ys = plane_span(ref->y_stride, ref->mb_cols * 16, ref->mb_rows * 16);
+  const size_t cs = plane_span(ref->uv_stride, ref->mb_cols * 8, ref->mb_rows * 8);
+  const size_t cf_bytes = with_residual ? (size_t)(mvs_per_mb == 16 ? 384 : 400) * 2 * n : 0;
+  const size_t o_y = 0, o_u = align_up(ys, 256), o_v = o_u + align_up(cs, 256);
+  const size_t o_xy = o_v + align_up(cs, 256), o_mv = o_xy + align_up(8 * n, 256);
+  const size_t o_skip = o_mv + align_up(4 * mvs_per_mb * n, 256);
+  const size_t o_cf = o_skip + align_up(with_residual ? n : 0, 256);
+  const size_t o_out = o_cf + align_up(cf_bytes, 256), total = o_out + 384 * n;
+  st = ensure_arena(c, total);
+  if (st != VP8G_OK) return st;
+  uint8_t* b = c->arena;
+  st = copy_in(c, b + o_y, ref->y, ys);
+  if (st == VP8G_OK) st = copy_in(c, b + o_u, ref->u, cs);
+  if (st == VP8G_OK) st = copy_in(c, b + o_v, ref->v, cs);
+  if (st == VP8G_OK) st = copy_in(c, b + o_xy, mb_xy, 8 * n);
+  if (st == VP8G_OK) st = copy_in(c, b + o_mv, mvs, 4 * mvs_per_mb * n);
+  if (st == VP8G_OK && with_residual) st = copy_in(c, b + o_skip, skip, n);
+  if (st == VP8G_OK && with_residual) st = copy_in(c, b + o_cf, coeffs, cf_bytes);
+  if (st != VP8G_OK) return finish(c, st);
+  vp8g_ref_planes d = *ref;
+  d.y = b + o_y;
+  d.u = b + o_u;
+  d.v = b + o_v;
+  st = enqueue_recon(c, &d, reinterpret_cast<const uint32_t*>(b + o_xy), reinterpret_cast<const vp8g_mv*>(b + o_mv),
+                     n, full_pixel, filters, with_residual ? b + o_skip : nullptr,
+                     with_residual ? reinterpret_cast<const int16_t*>(b + o_cf) : nullptr, b + o_out, mvs_per_mb);
+  if (st == VP8G_OK) st = copy_out(c, out, b + o_out, 384 * n);
+  return finish(c, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vp8g_predict_split_mv_batch(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy, const vp8g_mv* mvs,
+                                size_t n, int full_pixel, const int32_t filters[8][6], uint8_t* out) {
+  return recon_host(c, ref, mb_xy, mvs, n, full_pixel, filters, nullptr, nullptr, out, 16, false);
+}
+
+int vp8g_predict_split_mv_batch_dev(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy,
+                                    const vp8g_mv* mvs, size_t n, int full_pixel, const int32_t filters[8][6],
+                                    uint8_t* out) {
+  return recon_dev(c, ref, mb_xy, mvs, n, full_pixel, filters, nullptr, nullptr, out, 16, false);
+}
+
+int vp8g_reconstruct_inter_mb_batch(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy,
+                                    const vp8g_mv* mv, size_t n, int full_pixel, const int32_t filters[8][6],
+                                    const uint8_t* skip, const int16_t* coeffs, uint8_t* out) {
+  return recon_host(c, ref, mb_xy, mv, n, full_pixel, filters, skip, coeffs, out, 1, true);
+}
+
+int vp8g_reconstruct_inter_mb_batch_dev(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy,
+                                        const vp8g_mv* mv, size_t n, int full_pixel, const int32_t filters[8][6],
+                                        const uint8_t* skip, const int16_t* coeffs, uint8_t* out) {
+  return recon_dev(c, ref, mb_xy, mv, n, full_pixel, filters, skip, coeffs, out, 1, true);
+}
+
+int vp8g_reconstruct_split_mv_mb_batch(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy,
+                                       const vp8g_mv* mvs, size_t n, int full_pixel, const int32_t filters[8][6],
+                                       const uint8_t* skip, const int16_t* coeffs, uint8_t* out) {
+  return recon_host(c, ref, mb_xy, mvs, n, full_pixel, filters, skip, coeffs, out, 16, true);
+}
+
+int vp8g_reconstruct_split_mv_mb_batch_dev(vp8g_ctx* c, const vp8g_ref_planes* ref, const uint32_t* mb_xy,
+                                           const vp8g_mv* mvs, size_t n, int full_pixel,
+                                           const int32_t filters[8][6], const uint8_t* skip, const int16_t* coeffs,
+                                           uint8_t* out) {
+  return recon_dev(c, ref, mb_xy, mvs, n, full_pixel, filters, skip, coeffs, out, 16, true);
+}
+
 // ---- encoder inter search
 }  // extern "C"
 
@@ -1105,13 +1250,6 @@ bool jobs_valid(const vp8g_mb_search* jobs, size_t n) {
     if (j.range_x[0] < -8191 || j.range_x[1] > 8191 || j.range_y[0] < -8191 || j.range_y[1] > 8191) return false;
     if (!isfinite(j.lambda_sad) || j.lambda_sad < 0.0 || j.good_enough >= (1u << 30)) return false;
   }
-  return true;
-}
-
-bool taps_valid(const int32_t filters[8][6]) {
-  for (int r = 0; r < 8; r++)
-    for (int k = 0; k < 6; k++)
-      if (filters[r][k] < -(1 << 20) || filters[r][k] > (1 << 20)) return false;
   return true;
 }
 

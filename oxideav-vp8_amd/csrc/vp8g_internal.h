@@ -231,6 +231,28 @@ hipError_t launch_whole_pixel_search(const WpsArgs& a, hipStream_t stream);
 hipError_t launch_sub_pixel_refine(const SprArgs& a, hipStream_t stream);
 hipError_t launch_small_diamond(const DiamondArgs& a, hipStream_t stream);
 
+// SPLITMV prediction and the residual of the inter reconstructions.
+struct SplitArgs {
+  McPlanes ref;
+  const uint32_t* mb_xy;         // n x {mb_col, mb_row}
+  const int16_t* mvs;            // n x 16 x {row, col} quarter-pel
+  uint8_t* out;                  // n x 384
+  int n;
+  int full_pixel;
+  int32_t filters[8][6];
+};
+
+struct ResidualArgs {
+  const int16_t* coeffs;         // n x (y2 ? 25 : 24) x 16, dequantised
+  const uint8_t* skip;           // n (mb_skip_coeff)
+  uint8_t* out;                  // n x 384, holding the prediction
+  int n;
+  int y2;                        // 1: a Y2 block comes first (non-SPLITMV)
+};
+
+hipError_t launch_split_predict(const SplitArgs& a, hipStream_t stream);
+hipError_t launch_mb_residual(const ResidualArgs& a, hipStream_t stream);
+
 hipError_t launch_predict_inter_mb(const McArgs& a, hipStream_t stream);
 hipError_t launch_mb_luma_sad_at_mv(const SadArgs& a, hipStream_t stream);
 hipError_t launch_refine_ladder(const RefineArgs& a, hipStream_t stream);

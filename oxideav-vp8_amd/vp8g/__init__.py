@@ -120,6 +120,9 @@ EXPORTS = (
     "vp8g_whole_pixel_search_batch", "vp8g_whole_pixel_search_batch_dev", "vp8g_sub_pixel_refine_batch",
     "vp8g_sub_pixel_refine_batch_dev", "vp8g_small_diamond_search_luma_batch",
     "vp8g_small_diamond_search_luma_batch_dev",
+    "vp8g_predict_split_mv_batch", "vp8g_predict_split_mv_batch_dev", "vp8g_reconstruct_inter_mb_batch",
+    "vp8g_reconstruct_inter_mb_batch_dev", "vp8g_reconstruct_split_mv_mb_batch",
+    "vp8g_reconstruct_split_mv_mb_batch_dev",
 )
 FALLBACK_GATE_NEIGHBOUR = 1
 
@@ -164,6 +167,13 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_search_fallback_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp, C.POINTER(_Err)]),
         "vp8g_search_fallback_noalloc_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp,
                                                    C.POINTER(_Err)]),
+        "vp8g_predict_split_mv_batch": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp]),
+        "vp8g_predict_split_mv_batch_dev": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp]),
+        "vp8g_reconstruct_inter_mb_batch": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp, vp, vp]),
+        "vp8g_reconstruct_inter_mb_batch_dev": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp, vp, vp]),
+        "vp8g_reconstruct_split_mv_mb_batch": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp, vp, vp]),
+        "vp8g_reconstruct_split_mv_mb_batch_dev": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp, vp,
+                                                         vp]),
         "vp8g_whole_pixel_search_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp, vp]),
         "vp8g_whole_pixel_search_batch_dev": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, sz, vp, vp]),
         "vp8g_sub_pixel_refine_batch": (i32, [vp, C.POINTER(_LumaRef), vp, vp, vp, vp, sz, vp, vp, vp]),
@@ -556,6 +566,59 @@ class Context:
         self._raise(st, "vp8g_predict_inter_mb_batch")
         return out[:n]
 
+    def _ref_planes(self, reference: ReferencePlanes):
+        ys = reference.y_stride
+        y = _check_plane("y", reference.y, (reference.mb_rows * 16 - 1) * ys + reference.mb_cols * 16)
+        cneed = (reference.mb_rows * 8 - 1) * reference.uv_stride + reference.mb_cols * 8
+        u = _check_plane("u", reference.u, cneed)
+        v = _check_plane("v", reference.v, cneed)
+        return (y, u, v), _RefPlanes(_ptr(y), _ptr(u), _ptr(v), ys, reference.uv_stride, reference.mb_cols,
+                                     reference.mb_rows)
+
+    def _recon(self, fn, what, reference, mb_xy, mvs, per_mb, full_pixel, filters, skip=None, coeffs=None,
+               blocks=0):
+        mb_xy = np.ascontiguousarray(mb_xy, np.uint32).reshape(-1, 2)
+        n = mb_xy.shape[0]
+        mvs = np.ascontiguousarray(mvs, np.int16).reshape(n, per_mb, 2)
+        filt = np.ascontiguousarray(filters, np.int32).reshape(8, 6)
+        keep, rp = self._ref_planes(reference)
+        out = np.empty((max(n, 1), 384), np.uint8)
+        args = [self.h, C.byref(rp), _ptr(mb_xy), _ptr(mvs), n, 1 if full_pixel else 0, _ptr(filt)]
+        if blocks:
+            sk = np.ascontiguousarray(skip, np.uint8).reshape(n)
+            cf = np.ascontiguousarray(coeffs, np.int16).reshape(n, blocks, 16)
+            args += [_ptr(sk), _ptr(cf)]
+        self._raise(fn(*args, _ptr(out)), what)
+        return out[:n]
+
+    def predict_split_mv_batch(self, reference: ReferencePlanes, mb_xy, split_mvs, full_pixel: bool,
+                               filters) -> np.ndarray:
+        """predict_split_mv per macroblock: split_mvs is (n, 16, 2) of (row,
+        col) quarter-pel sub-block vectors; returns (n, 384)."""
+        return self._recon(self.lib.vp8g_predict_split_mv_batch, "vp8g_predict_split_mv_batch", reference, mb_xy,
+                           split_mvs, 16, full_pixel, filters)
+
+    def reconstruct_inter_mb_batch(self, reference: ReferencePlanes, mb_xy, mvs, full_pixel: bool, filters, skip,
+                                   coeffs) -> np.ndarray:
+        """reconstruct_inter_mb per macroblock; coeffs (n, 25, 16) = y2, y[16],
+        u[4], v[4] (dequantised)."""
+        return self._recon(self.lib.vp8g_reconstruct_inter_mb_batch, "vp8g_reconstruct_inter_mb_batch", reference,
+                           mb_xy, mvs, 1, full_pixel, filters, skip, coeffs, 25)
+
+    def reconstruct_split_mv_mb_batch(self, reference: ReferencePlanes, mb_xy, split_mvs, full_pixel: bool, filters,
+                                      skip, coeffs) -> np.ndarray:
+        """reconstruct_split_mv_mb per macroblock; coeffs (n, 24, 16) = y[16],
+        u[4], v[4] (no Y2)."""
+        return self._recon(self.lib.vp8g_reconstruct_split_mv_mb_batch, "vp8g_reconstruct_split_mv_mb_batch",
+                           reference, mb_xy, split_mvs, 16, full_pixel, filters, skip, coeffs, 24)
+
+    def predict_split_mv(self, reference: ReferencePlanes, mb_col: int, mb_row: int, split_luma_mvs,
+                         full_pixel: bool, filters) -> ReconstructedMb:
+        """src/motion_comp.rs:2227-2320."""
+        mvs = [[m.row, m.col] if isinstance(m, Mv) else list(m) for m in split_luma_mvs]
+        o = self.predict_split_mv_batch(reference, [[mb_col, mb_row]], [mvs], full_pixel, filters)[0]
+        return ReconstructedMb(o[:256].copy(), o[256:320].copy(), o[320:].copy())
+
     def predict_inter_mb(self, reference: ReferencePlanes, mb_col: int, mb_row: int, luma_mv: Mv,
                          full_pixel: bool, filters) -> ReconstructedMb:
         o = self.predict_inter_mb_batch(reference, [[mb_col, mb_row]], [[luma_mv.row, luma_mv.col]],
@@ -649,6 +712,23 @@ class Context:
         r = self.quarter_pixel_refine_luma_batch(reference, [[mb_col, mb_row]], src_y,
                                                  [[center.row, center.col]])[0]
         return SearchResult(Mv(int(r[0]), int(r[1])), int(r[2]))
+
+
+def split_chroma_mvs(split_luma_mvs) -> list:
+    """src/motion_comp.rs:2162-2203: the four chroma sub-block vectors
+    (eighth-pel) of sixteen quarter-pel luma sub-block vectors, each the
+    avg() of its four doubled luma vectors (host arithmetic)."""
+    mvs = [(m.row, m.col) if isinstance(m, Mv) else tuple(int(v) for v in m) for m in split_luma_mvs]
+    out = []
+    for c in range(4):
+        s = [0, 0]
+        for b in range(16):
+            if ((b >> 1) & 1) + ((b >> 2) & 2) == c:
+                s[0] += 2 * mvs[b][0]
+                s[1] += 2 * mvs[b][1]
+        r = [(v + 4) >> 3 if v >= 0 else -((-v + 4) >> 3) for v in s]
+        out.append(Mv(int(np.int16(r[0])), int(np.int16(r[1]))))
+    return out
 
 
 # ------------------------------------------------ module-level reference API
