@@ -413,6 +413,16 @@ int vp8g_half_plane_sads_batch(vp8g_ctx *ctx, const vp8g_luma_ref *ref, const in
                                const uint8_t *src, size_t n, const int32_t taps[6],
                                int diagonals, uint32_t *sad);
 
+/* ---- stream driver: the scene-cut metric ----
+ * Replaces luma_mad (src/stream.rs:2654-2666) over a batch of frame pairs:
+ * mad[i] = (sum over the visible w x h luma of |prev[i] - cur[i]|) as f64
+ * / (w * h) as f64.  Every frame of a call has the dims of prev[0]
+ * (VP8G_E_REF_DIMS_MISMATCH otherwise: the stream driver locks them);
+ * w, h in 1..16383.  The _dev form takes device planes and writes the
+ * integer sums sad[i] (device) instead; enqueue only. */
+int vp8g_luma_mad_batch(vp8g_ctx *ctx, const vp8g_i420 *prev, const vp8g_i420 *cur, size_t n, double *mad);
+int vp8g_luma_sad_batch_dev(vp8g_ctx *ctx, const vp8g_i420 *prev, const vp8g_i420 *cur, size_t n, uint64_t *sad);
+
 /* ---- SPLITMV prediction and inter-macroblock reconstruction ----
  * Replaces predict_split_mv (src/motion_comp.rs:2227-2320) over a batch:
  * macroblock i at (mb_xy[2i], mb_xy[2i+1]) with its 16 quarter-pel luma

@@ -132,6 +132,8 @@ class Oracle:
         L.orc_half_planes_new.argtypes = [vp, vp, i32, C.POINTER(HalfPlanes)]
         L.orc_half_planes_sad.argtypes = [C.POINTER(HalfPlanes), vp, i32, i32]
         L.orc_half_planes_sad.restype = u32
+        L.orc_luma_mad.argtypes = [C.POINTER(I420), C.POINTER(I420)]
+        L.orc_luma_mad.restype = C.c_double
         L.orc_filter_block_4x4.argtypes = [vp, sz, sz, sz, sz, sz, MvS, vp, vp]
         L.orc_predict_split_mv.argtypes = [C.POINTER(RefPlanes), sz, sz, vp, i32, vp, vp]
         L.orc_reconstruct_inter_mb.argtypes = [C.POINTER(RefPlanes), sz, sz, MvS, i32, vp, i32, vp, vp]

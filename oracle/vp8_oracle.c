@@ -788,6 +788,17 @@ uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], i
     return window_sad(src, p->d, 17, cy * 17 + cx);
 }
 
+/* luma_mad: src/stream.rs:2654-2666 (u64 SAD over the visible luma, then
+ * an f64 division by the pixel count). */
+double orc_luma_mad(const orc_i420 *prev, const orc_i420 *frame)
+{
+    const size_t w = prev->width, h = prev->height;
+    uint64_t sad = 0;
+    for (size_t r = 0; r < h; r++)
+        for (size_t c = 0; c < w; c++) sad += absdiff(prev->y[r * prev->y_stride + c], frame->y[r * frame->y_stride + c]);
+    return (double)sad / (double)(w * h);
+}
+
 /* ---------------- SPLITMV prediction and reconstruction ---------------- */
 /* filter_block_4x4: src/motion_comp.rs:1693-1712 (whole-pixel copy when both
  * fractions are zero, fetch_block_whole_pixel :1442-1487; else the 9x9

@@ -161,6 +161,9 @@ void orc_half_planes_new(const uint8_t window[22 * 22], const int32_t taps[6], i
                          orc_half_planes *out);
 uint32_t orc_half_planes_sad(const orc_half_planes *p, const uint8_t src[256], int dx, int dy);
 
+/* luma_mad, src/stream.rs:2654-2666. */
+double orc_luma_mad(const orc_i420 *prev, const orc_i420 *frame);
+
 /* ---------------- SPLITMV and reconstruction (src/motion_comp.rs, src/inverse_transform.rs) ---------------- */
 void orc_filter_block_4x4(const uint8_t *plane, size_t stride, size_t w, size_t h, size_t blk_x, size_t blk_y,
                           orc_mv mv, const int32_t filters[8][6], uint8_t out[16]);

@@ -1100,6 +1100,96 @@ int vp8g_quarter_pixel_refine_luma_batch(vp8g_ctx* c, const vp8g_luma_ref* ref,
   return refine_batch(c, ref, mb_xy, src, centers, n, out, 1);
 }
 
+// ---- stream driver scene-cut metric
+}  // extern "C"
+
+namespace {
+
+int check_mad_pairs(const vp8g_i420* prev, const vp8g_i420* cur, size_t n) {
+  if (!prev || !cur) return VP8G_E_ARG;
+  const uint32_t w = prev[0].width, h = prev[0].height;
+  if (w == 0 || h == 0 || w > 0x3FFF || h > 0x3FFF) return VP8G_E_INVALID_DIMENSIONS;
+  for (size_t i = 0; i < n; i++) {
+    const vp8g_i420* fs[2] = {prev + i, cur + i};
+    for (const vp8g_i420* f : fs) {
+      if (f->width != w || f->height != h) return VP8G_E_REF_DIMS_MISMATCH;
+      if (!f->y || f->y_stride < w || f->y_stride > 0xFFFFFFFFu) return VP8G_E_ARG;
+    }
+  }
+  return VP8G_OK;
+}
+
+int enqueue_luma_sad(vp8g_ctx* c, const vp8g_i420* prev, const vp8g_i420* cur, size_t n, uint64_t* sad) {
+  for (size_t i0 = 0; i0 < n; i0 += kLumaSadPairsPerLaunch) {
+    const int cnt = (int)std::min<size_t>(kLumaSadPairsPerLaunch, n - i0);
+    LumaSadArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int k = 0; k < cnt; k++)
+      a.pairs[k] = LumaSadPair{prev[i0 + k].y, cur[i0 + k].y, (uint32_t)prev[i0 + k].y_stride,
+                               (uint32_t)cur[i0 + k].y_stride};
+    a.sad = sad + i0;
+    a.w = (int)prev[0].width;
+    a.h = (int)prev[0].height;
+    const int st = launch_status(c, [&] { return launch_luma_sad_pairs(a, cnt, c->stream); });
+    if (st != VP8G_OK) return st;
+  }
+  return VP8G_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vp8g_luma_sad_batch_dev(vp8g_ctx* c, const vp8g_i420* prev, const vp8g_i420* cur, size_t n, uint64_t* sad) {
+  if (!c) return VP8G_E_ARG;
+  if (n == 0) return VP8G_OK;
+  int st = check_mad_pairs(prev, cur, n);
+  if (st != VP8G_OK) return st;
+  if (!sad || ((uintptr_t)sad & 7)) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  return enqueue_luma_sad(c, prev, cur, n, sad);
+}
+
+int vp8g_luma_mad_batch(vp8g_ctx* c, const vp8g_i420* prev, const vp8g_i420* cur, size_t n, double* mad) {
+  if (!c) return VP8G_E_ARG;
+  if (n == 0) return VP8G_OK;
+  int st = check_mad_pairs(prev, cur, n);
+  if (st != VP8G_OK) return st;
+  if (!mad) return VP8G_E_ARG;
+  if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
+  const size_t w = prev[0].width, h = prev[0].height, per = align_up(w * h, 256);
+  const size_t o_sad = 2 * n * per;
+  st = ensure_arena(c, o_sad + 8 * n);
+  if (st != VP8G_OK) return st;
+  std::vector<vp8g_i420> dp(n), dc(n);
+  for (size_t i = 0; i < n && st == VP8G_OK; i++) {
+    const vp8g_i420* fs[2] = {prev + i, cur + i};
+    for (int k = 0; k < 2 && st == VP8G_OK; k++) {
+      uint8_t* dst = c->arena + (2 * i + k) * per;
+      if (c->fault == VP8G_FAULT_COPY) {
+        st = VP8G_E_DEVICE;
+        break;
+      }
+      pack_plane(c->pinned + (dst - c->arena), w, fs[k]->y, fs[k]->y_stride, w, h);
+      if (!hip_ok(hipMemcpyAsync(dst, c->pinned + (dst - c->arena), w * h, hipMemcpyHostToDevice, c->stream)))
+        st = VP8G_E_DEVICE;
+      vp8g_i420 d = *fs[k];
+      d.y = dst;
+      d.y_stride = w;
+      (k ? dc : dp)[i] = d;
+    }
+  }
+  if (st != VP8G_OK) return finish(c, st);
+  uint64_t* dsad = reinterpret_cast<uint64_t*>(c->arena + o_sad);
+  st = enqueue_luma_sad(c, dp.data(), dc.data(), n, dsad);
+  std::vector<uint64_t> sums(n);
+  if (st == VP8G_OK) st = copy_out(c, sums.data(), dsad, 8 * n);
+  st = finish(c, st);
+  if (st == VP8G_OK)
+    for (size_t i = 0; i < n; i++) mad[i] = (double)sums[i] / (double)(w * h);
+  return st;
+}
+
 // ---- SPLITMV and reconstruction
 }  // extern "C"
 

@@ -253,6 +253,20 @@ struct ResidualArgs {
 hipError_t launch_split_predict(const SplitArgs& a, hipStream_t stream);
 hipError_t launch_mb_residual(const ResidualArgs& a, hipStream_t stream);
 
+// Luma SAD of frame pairs (the stream driver's scene-cut metric).
+constexpr int kLumaSadPairsPerLaunch = 64;
+struct LumaSadPair {
+  const uint8_t* a;
+  const uint8_t* b;
+  uint32_t a_stride, b_stride;
+};
+struct LumaSadArgs {
+  LumaSadPair pairs[kLumaSadPairsPerLaunch];
+  uint64_t* sad;                 // one total per pair
+  int w, h;
+};
+hipError_t launch_luma_sad_pairs(const LumaSadArgs& a, int pairs, hipStream_t stream);
+
 hipError_t launch_predict_inter_mb(const McArgs& a, hipStream_t stream);
 hipError_t launch_mb_luma_sad_at_mv(const SadArgs& a, hipStream_t stream);
 hipError_t launch_refine_ladder(const RefineArgs& a, hipStream_t stream);

@@ -122,7 +122,7 @@ EXPORTS = (
     "vp8g_small_diamond_search_luma_batch_dev",
     "vp8g_predict_split_mv_batch", "vp8g_predict_split_mv_batch_dev", "vp8g_reconstruct_inter_mb_batch",
     "vp8g_reconstruct_inter_mb_batch_dev", "vp8g_reconstruct_split_mv_mb_batch",
-    "vp8g_reconstruct_split_mv_mb_batch_dev",
+    "vp8g_reconstruct_split_mv_mb_batch_dev", "vp8g_luma_mad_batch", "vp8g_luma_sad_batch_dev",
 )
 FALLBACK_GATE_NEIGHBOUR = 1
 
@@ -167,6 +167,8 @@ def load_library(path: Path | None = None) -> C.CDLL:
         "vp8g_search_fallback_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp, C.POINTER(_Err)]),
         "vp8g_search_fallback_noalloc_dev": (i32, [vp, C.POINTER(_I420), sz, sz, vp, i32, i32, vp, vp,
                                                    C.POINTER(_Err)]),
+        "vp8g_luma_mad_batch": (i32, [vp, C.POINTER(_I420), C.POINTER(_I420), sz, vp]),
+        "vp8g_luma_sad_batch_dev": (i32, [vp, C.POINTER(_I420), C.POINTER(_I420), sz, vp]),
         "vp8g_predict_split_mv_batch": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp]),
         "vp8g_predict_split_mv_batch_dev": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp]),
         "vp8g_reconstruct_inter_mb_batch": (i32, [vp, C.POINTER(_RefPlanes), vp, vp, sz, i32, vp, vp, vp, vp]),
