@@ -262,6 +262,212 @@ def bench_learned(ctx, group, descs, stream, reps: int = 10) -> dict:
     return out
 
 
+def measure_next_rows(ctx, group, cores: int, cpu_seconds: float = 2.0, reps: int = 10) -> dict:
+    """The rows beside the hot path, each over every macroblock of a 1080p
+    frame (the bench group's centre against its next frame, edge-padded to
+    1920x1088), GPU time from HIP events on the library stream (device
+    inputs), the oracle on `cores` host threads beside it:
+
+    * encoder inter search: whole_pixel_search then sub_pixel_refine
+      (Balanced shape, zero and census seeds, lambda_sad 8, good_enough 896,
+      +-32 px range, default MV contexts);
+    * SPLITMV: predict_split_mv (16 vectors per MB in [-64, 64] quarter
+      pel), reconstruct_split_mv_mb and reconstruct_inter_mb with sparse
+      dequantised coefficients (a quarter of the MBs skipped);
+    * luma_mad of 64 consecutive 1080p frame pairs in one call."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+
+    import vp8g
+    import vp8g.inter_search as IS
+    import vp8g.synth as synth
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle_ctypes import I420, LumaRef, Oracle, RefPlanes
+
+    orc = Oracle()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.ExternalStream(ctx.stream(), device=dev)
+    c = group.center
+    cur = np.ascontiguousarray(synth.pad_to_mb(group.y[c], 16))
+    ref = np.ascontiguousarray(synth.pad_to_mb(group.y[c + 1], 16))
+    ru = np.ascontiguousarray(synth.pad_to_mb(group.u[c + 1], 8))
+    rv = np.ascontiguousarray(synth.pad_to_mb(group.v[c + 1], 8))
+    h, w = ref.shape
+    bw, bh = w // 16, h // 16
+    n = bw * bh
+    xy = np.stack(np.meshgrid(np.arange(bw), np.arange(bh)), -1).reshape(-1, 2).astype(np.uint32)
+    src = np.ascontiguousarray(cur.reshape(bh, 16, bw, 16).transpose(0, 2, 1, 3).reshape(n, 256))
+    rng = np.random.default_rng(8160)
+    costs = IS.MvCostTable.new()
+    jobs = np.zeros(n, IS.MB_SEARCH)
+    jobs["mb_col"], jobs["mb_row"] = xy[:, 0], xy[:, 1]
+    jobs["range_x"], jobs["range_y"] = (-32, 32), (-32, 32)
+    jobs["n_seeds"] = 2
+    jobs["seeds"][:, 1] = rng.integers(-24, 25, (n, 2)) * 4
+    jobs["best"] = jobs["seeds"][:, 1]
+    jobs["lambda_sad"] = 8.0
+    jobs["good_enough"] = 896
+    jobs["descend"] = 1
+    shape = IS.SHAPES["balanced"]
+    split_mvs = rng.integers(-64, 65, (n, 16, 2)).astype(np.int16)
+    mvs = rng.integers(-64, 65, (n, 2)).astype(np.int16)
+    skip = (rng.random(n) < 0.25).astype(np.uint8)
+    cf25 = (rng.integers(-40, 41, (n, 25, 16)) * (rng.random((n, 25, 16)) < 0.1)).astype(np.int16)
+    cf24 = np.ascontiguousarray(cf25[:, 1:])
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1).view(np.uint8).copy()).to(dev)
+    d = {k: t(v) for k, v in (("ref", ref), ("u", ru), ("v", rv), ("xy", xy), ("src", src), ("costs", costs.bits),
+                              ("jobs", jobs), ("smv", split_mvs), ("mv", mvs), ("skip", skip), ("cf25", cf25),
+                              ("cf24", cf24))}
+    found = torch.zeros(n * 16, dtype=torch.uint8, device=dev)
+    found2 = torch.zeros(n * 16, dtype=torch.uint8, device=dev)
+    out = torch.zeros(n * 384, dtype=torch.uint8, device=dev)
+    lr = vp8g._LumaRef(d["ref"].data_ptr(), w, w, h)
+    rp = vp8g._RefPlanes(d["ref"].data_ptr(), d["u"].data_ptr(), d["v"].data_ptr(), w, w // 2, bw, bh)
+    f6 = np.ascontiguousarray(vp8g.SIXTAP_FILTERS, np.int32)
+    lib = ctx.lib
+
+    def ok(st):
+        if st:
+            raise RuntimeError(f"vp8g launch failed: {st}")
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    wps = lambda: ok(lib.vp8g_whole_pixel_search_batch_dev(ctx.h, C.byref(lr), d["costs"].data_ptr(),
+                                                           d["jobs"].data_ptr(), d["src"].data_ptr(), n, shape.c(),
+                                                           found.data_ptr()))
+    spr = lambda: ok(lib.vp8g_sub_pixel_refine_batch_dev(ctx.h, C.byref(lr), d["costs"].data_ptr(),
+                                                         d["jobs"].data_ptr(), d["src"].data_ptr(), found.data_ptr(),
+                                                         n, f6.ctypes.data, shape.c(), found2.data_ptr()))
+    ms_wps, ms_spr = timed(wps), timed(spr)
+    split = lambda: ok(lib.vp8g_predict_split_mv_batch_dev(ctx.h, C.byref(rp), d["xy"].data_ptr(),
+                                                           d["smv"].data_ptr(), n, 0, f6.ctypes.data, out.data_ptr()))
+    rsplit = lambda: ok(lib.vp8g_reconstruct_split_mv_mb_batch_dev(ctx.h, C.byref(rp), d["xy"].data_ptr(),
+                                                                   d["smv"].data_ptr(), n, 0, f6.ctypes.data,
+                                                                   d["skip"].data_ptr(), d["cf24"].data_ptr(),
+                                                                   out.data_ptr()))
+    rinter = lambda: ok(lib.vp8g_reconstruct_inter_mb_batch_dev(ctx.h, C.byref(rp), d["xy"].data_ptr(),
+                                                                d["mv"].data_ptr(), n, 0, f6.ctypes.data,
+                                                                d["skip"].data_ptr(), d["cf25"].data_ptr(),
+                                                                out.data_ptr()))
+    ms_split, ms_rsplit, ms_rinter = timed(split), timed(rsplit), timed(rinter)
+    # luma_mad of a 256-frame 1080p stream (255 consecutive pairs, 530 MB of
+    # luma: more than the last-level cache holds), one call
+    gw, gh = group.width, group.height
+    n_mad = 255
+    stream_y = torch.randint(0, 256, (n_mad + 1, gh * gw), dtype=torch.uint8, device=dev)
+    pa, ca = (vp8g._I420 * n_mad)(), (vp8g._I420 * n_mad)()
+    for i in range(n_mad):
+        a, b = stream_y[i].data_ptr(), stream_y[i + 1].data_ptr()
+        pa[i] = vp8g._I420(gw, gh, a, a, a, gw, (gw + 1) // 2)
+        ca[i] = vp8g._I420(gw, gh, b, b, b, gw, (gw + 1) // 2)
+    sads = torch.zeros(n_mad, dtype=torch.int64, device=dev)
+    mad = lambda: ok(lib.vp8g_luma_sad_batch_dev(ctx.h, pa, ca, n_mad, sads.data_ptr()))
+    ms_mad = timed(mad)
+    torch.cuda.synchronize()
+
+    # CPU: the oracle over the same inputs on `cores` threads
+    pool = ThreadPoolExecutor(cores)
+    chunks = [np.arange(k, n, cores) for k in range(cores)]
+    olr = LumaRef(ref.ctypes.data, w, w, h)
+    orp = RefPlanes(ref.ctypes.data, ru.ctypes.data, rv.ctypes.data, w, w // 2, bw, bh)
+    oj, keep = orc._orc_jobs(jobs, src)
+    ofound = np.zeros(n, IS.FOUND)
+    ofound2 = np.zeros_like(ofound)
+    oout = np.zeros((n, 384), np.uint8)
+    bits = np.ascontiguousarray(costs.bits)
+    sh = (C.c_uint32 * 4)(shape.descent_rounds, 1, 1, 1)
+    contig = [np.ascontiguousarray(ix) for ix in np.array_split(np.arange(n), cores)]
+
+    def cpu_time(fn):
+        reps_c, t0 = 0, time.perf_counter()
+        while True:
+            list(pool.map(fn, contig))
+            reps_c += 1
+            if time.perf_counter() - t0 >= cpu_seconds:
+                break
+        return (time.perf_counter() - t0) / reps_c * 1e3
+
+    def o_search(ix):
+        s0, k = int(ix[0]), len(ix)
+        orc.lib.orc_whole_pixel_search_batch(C.byref(olr), bits.ctypes.data, oj[s0:].ctypes.data, k, sh,
+                                             ofound[s0:].ctypes.data)
+        orc.lib.orc_sub_pixel_refine_batch(C.byref(olr), bits.ctypes.data, oj[s0:].ctypes.data,
+                                           ofound[s0:].ctypes.data, k, f6.ctypes.data, sh, ofound2[s0:].ctypes.data)
+
+    def o_split(ix):
+        s0, k = int(ix[0]), len(ix)
+        orc.lib.orc_reconstruct_split_mv_mb_batch(C.byref(orp), xy[s0:].ctypes.data, split_mvs[s0:].ctypes.data, k,
+                                                  0, f6.ctypes.data, skip[s0:].ctypes.data, cf24[s0:].ctypes.data,
+                                                  oout[s0:].ctypes.data)
+
+    def o_inter(ix):
+        s0, k = int(ix[0]), len(ix)
+        orc.lib.orc_reconstruct_inter_mb_batch(C.byref(orp), xy[s0:].ctypes.data, mvs[s0:].ctypes.data, k, 0,
+                                               f6.ctypes.data, skip[s0:].ctypes.data, cf25[s0:].ctypes.data,
+                                               oout[s0:].ctypes.data)
+
+    cpu_search, cpu_split, cpu_inter = cpu_time(o_search), cpu_time(o_split), cpu_time(o_inter)
+    # the oracle over the first 33 frames of the same stream, scaled to 255 pairs
+    ys = stream_y[:33].cpu().numpy()
+    i420 = [I420(gw, gh, y.ctypes.data, y.ctypes.data, y.ctypes.data, gw, (gw + 1) // 2) for y in ys]
+    wmad = np.array([orc.lib.orc_luma_mad(C.byref(i420[i]), C.byref(i420[i + 1])) for i in range(32)])
+    got_mad = sads[:32].cpu().numpy().astype(np.float64) / float(gw * gh)
+
+    def o_mad(ix):
+        for i in ix:
+            orc.lib.orc_luma_mad(C.byref(i420[i]), C.byref(i420[i + 1]))
+
+    reps_c, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < cpu_seconds:
+        list(pool.map(o_mad, np.array_split(np.arange(32), cores)))
+        reps_c += 1
+    cpu_mad = (time.perf_counter() - t0) / reps_c * 1e3 * n_mad / 32
+    pool.shutdown()
+    # the GPU results equal the oracle's on these inputs
+    got2 = found2.cpu().numpy().view(ofound.dtype)
+    match = {"inter_search": bool(np.array_equal(got2["mv"], ofound2["mv"]) and
+                                  np.array_equal(got2["cost"], ofound2["cost"]))}
+    rinter()
+    torch.cuda.synchronize()
+    match["reconstruct_inter_mb"] = bool(np.array_equal(out.cpu().numpy().reshape(n, 384), oout))
+    match["luma_mad"] = bool(np.array_equal(got_mad, wmad))
+
+    def rec(ms, cpu_ms, units, bytes_per_unit, what):
+        gbs = bytes_per_unit * units / (ms * 1e-3) / 1e9
+        return {"ms": ms, "units": units, "unit": what, "per_s": units / (ms * 1e-3),
+                "cpu_ms": cpu_ms, "cpu_cores": cores, "gpu_over_cpu": cpu_ms / ms if cpu_ms else None,
+                "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_unit": bytes_per_unit}}
+
+    return {
+        "workload": "1920x1088 frame (8160 MBs) of the bench group against its next frame; luma_mad over a "
+                    "256-frame 1080p stream (255 pairs, one call)",
+        # per MB: the 256-byte source, its 20x20 reference neighbourhood and
+        # the 16-byte result (inter search); 384 B reference + 384 B
+        # output + MVs + coefficients (reconstructions)
+        "inter_search": dict(rec(ms_wps + ms_spr, cpu_search, n, 256 + 400 + 16 + 80, "MBs"),
+                             ms_whole_pixel=ms_wps, ms_sub_pixel=ms_spr, shape="balanced"),
+        "predict_split_mv": rec(ms_split, None, n, 384 + 384 + 64, "MBs"),
+        "reconstruct_split_mv_mb": rec(ms_rsplit, cpu_split, n, 384 + 384 + 64 + 769, "MBs"),
+        "reconstruct_inter_mb": rec(ms_rinter, cpu_inter, n, 384 + 384 + 4 + 801, "MBs"),
+        # each frame of the stream read once
+        "luma_mad": rec(ms_mad, cpu_mad, n_mad, gw * gh, "frame pairs"),
+        "bit_exact_vs_oracle": match,
+    }
+
+
 def measure_4k(ctx, reps: int = 3) -> dict:
     """3840x2160 I420, 11-frame group (centre 5), region velocities in
     [-48, 48] px/frame, 40 occluders per frame: exhaustive +-64 and the
@@ -650,6 +856,8 @@ def main() -> int:
     if not args.no_learned:
         result["learned_seeded"] = bench_learned(ctx, group, descs, stream, reps=max(5, args.steps // 5))
     if not args.no_extra:
+        cores = max(1, min(os.cpu_count() or 1, 16))
+        result["next_rows"] = measure_next_rows(ctx, group, cores)
         result["c2_batch"] = measure_batch(ctx, max(5, min(args.steps, 20)), 2)
         c4 = measure_4k(ctx, reps=2)
         c4.pop("report")
