@@ -235,8 +235,8 @@ int vp8g_build_arnr_altref_table_dev(vp8g_ctx *ctx, const vp8g_i420 *frames, siz
  * table of n_frames x coarse_h x coarse_w blocks onto n_frames x fine_h x
  * fine_w blocks: fine block (r, c) takes coarse block (min(r/2, coarse_h-1),
  * min(c/2, coarse_w-1)) with both vector components doubled and sad 0 (the
- * seeds of vp8g_block_search_dev one level finer).  All on the context's
- * stream. */
+ * seeds of vp8g_block_search_dev one level finer); every grid side is
+ * 1..1024 blocks (VP8G_E_ARG otherwise).  All on the context's stream. */
 int vp8g_downsample2_i420_dev(vp8g_ctx *ctx, const vp8g_i420 *src, const vp8g_i420 *dst, size_t n_frames);
 int vp8g_upsample_table_dev(vp8g_ctx *ctx, const int32_t *coarse, size_t n_frames, int coarse_w,
                             int coarse_h, int32_t *fine, int fine_w, int fine_h);

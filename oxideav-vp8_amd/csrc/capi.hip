@@ -728,7 +728,11 @@ int vp8g_downsample2_i420_dev(vp8g_ctx* c, const vp8g_i420* src, const vp8g_i420
 int vp8g_upsample_table_dev(vp8g_ctx* c, const int32_t* coarse, size_t n, int cw, int ch, int32_t* fine, int fw,
                             int fh) {
   if (!c || !coarse || !fine || n == 0 || n > (size_t)VP8G_MAX_FRAMES) return VP8G_E_ARG;
-  if (cw <= 0 || ch <= 0 || fw <= 0 || fh <= 0 || fw > 0x3FFF || fh > 0x3FFF) return VP8G_E_ARG;
+  // block grids of frames up to 0x3FFF pixels a side: at most 1024 blocks,
+  // so n * fine_w * fine_h stays below 2^26
+  constexpr int kMaxBlocks = (0x3FFF + 15) / 16;
+  if (cw <= 0 || ch <= 0 || fw <= 0 || fh <= 0) return VP8G_E_ARG;
+  if (cw > kMaxBlocks || ch > kMaxBlocks || fw > kMaxBlocks || fh > kMaxBlocks) return VP8G_E_ARG;
   if (((uintptr_t)coarse | (uintptr_t)fine) & 3) return VP8G_E_ARG;
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
   return launch_status(c, [&] { return launch_upsample_table(coarse, cw, ch, fine, fw, fh, (int)n, c->stream); });

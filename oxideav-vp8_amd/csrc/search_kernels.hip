@@ -431,10 +431,11 @@ hipError_t launch_downsample2(const Downsample2Args& a, int frames, hipStream_t 
 
 hipError_t launch_upsample_table(const int32_t* coarse, int cbw, int cbh, int32_t* fine, int fbw, int fbh, int n,
                                  hipStream_t stream) {
-  const int total = n * fbw * fbh;
-  if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(upsample_table_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, coarse, cbw, cbh, fine,
-                     fbw, fbh, total);
+  if (n <= 0 || fbw <= 0 || fbh <= 0 || cbw <= 0 || cbh <= 0) return hipErrorInvalidValue;
+  const int64_t total = (int64_t)n * fbw * fbh;
+  if (total > INT32_MAX - 255) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(upsample_table_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, coarse, cbw,
+                     cbh, fine, fbw, fbh, (int)total);
   return hipGetLastError();
 }
 

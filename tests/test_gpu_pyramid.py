@@ -119,3 +119,19 @@ def test_upsample_table_matches_numpy(gpu, dims):
     want = np.zeros((n, fh, fw, 3), np.int32)
     want[..., :2] = 2 * coarse[:, r][:, :, c, :2]
     assert np.array_equal(tf.cpu().numpy(), want)
+
+
+def test_upsample_table_rejects_grids_beyond_1024_blocks(gpu):
+    """Grid sides are block counts of frames up to 0x3FFF pixels: 1..1024;
+    anything else is refused before a launch (the fine table is untouched)."""
+    import torch
+    import vp8g
+    tc = torch.zeros(3 * 4, dtype=torch.int32, device="cuda")
+    tf = torch.full((3 * 4,), 7, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    for cw, ch, fw, fh in ((1025, 1, 2, 2), (1, 1025, 2, 2), (1, 1, 1025, 2), (1, 1, 2, 1025), (1, 1, 0, 2),
+                           (0, 1, 2, 2), (1, 1, 16383, 16383)):
+        with pytest.raises(vp8g.Vp8gError):
+            gpu.upsample_table_dev(tc.data_ptr(), 64, cw, ch, tf.data_ptr(), fw, fh)
+    gpu.synchronize()
+    assert (tf.cpu().numpy() == 7).all()
