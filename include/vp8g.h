@@ -102,7 +102,13 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
  *   kernel or graph launch.  The failing step reports VP8G_E_DEVICE
  *   without running (nothing faulty reaches the device) and no output is
  *   written.  A context starts with the mode named by $VP8G_FAULT
- *   ("alloc", "copy" or "launch"; anything else = none). */
+ *   ("alloc", "copy" or "launch"; anything else = none).
+ * VP8G_OPT_HOST_THREADS: host threads that pack the caller's planes into
+ *   pinned staging and copy results out in vp8g_build_arnr_altref, 1..64
+ *   (0 = default, min(8, half the hardware threads)).
+ * VP8G_OPT_COPY_STREAMS: streams that carry vp8g_build_arnr_altref's
+ *   host<->device copies, 1..4 (0 = default 2); the frames go out
+ *   round-robin, each copy engine taking a share. */
 #define VP8G_OPT_ARNR_WAVES 1
 #define VP8G_OPT_XCD_REMAP 2
 #define VP8G_OPT_ARNR_MBS 3
@@ -112,6 +118,8 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
 #define VP8G_FAULT_ALLOC 1
 #define VP8G_FAULT_COPY 2
 #define VP8G_FAULT_LAUNCH 3
+#define VP8G_OPT_HOST_THREADS 6
+#define VP8G_OPT_COPY_STREAMS 7
 int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);
 

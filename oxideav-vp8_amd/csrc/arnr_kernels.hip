@@ -84,22 +84,30 @@ constexpr uint32_t kProbeDy1 = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) | (
 // `row0` points at this lane's first source row's (0,0)-displaced reference
 // row.  The probes of
 // a step never leave +-15 (8+4+2+1), so the reference's range check never
-// fires and is not evaluated.  FULL: 16x16 block; otherwise columns >= bw
-// and rows >= bh are masked out of every SAD (the centre strip holds zeros
-// there, the reference row is masked to zero).
-template <bool FULL, class G>
+// fires and is not evaluated.  MASK: which of columns >= bw (kMaskCols) and
+// rows >= bh (kMaskRows) a partial block masks out of every SAD (the centre
+// strip holds zeros there; the reference bytes are masked to zero, rows on
+// the lane's row sums).  WAVE_ZERO (16x16 only): the (0, 0) SAD as one
+// wave-wide sum over `zrow` / `zsrc` (row lane / 4, dword lane % 4) instead
+// of the probe layout's eight group sums.
+enum { kMaskNone = 0, kMaskRows = 1, kMaskCols = 2, kMaskBoth = 3 };
+template <int MASK, bool WAVE_ZERO, class G>
 __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, const uint4 src1,
                                         int gd_off, int gq, int kk, int bw, int bh, int& bestx,
                                         int& besty, uint32_t& best_sad, const uint8_t* zrow, uint32_t zsrc) {
   static_assert((G::lp & 3) == 0, "window rows are dword aligned");
+  constexpr bool kCols = (MASK & kMaskCols) != 0, kRows = (MASK & kMaskRows) != 0;
   uint32_t colmask[4] = {~0u, ~0u, ~0u, ~0u};
   uint32_t rowmask0 = ~0u, rowmask1 = ~0u;
-  if (!FULL) {
+  if (kCols) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int left = bw - 4 * q;
       colmask[q] = left >= 4 ? 0xFFFFFFFFu : (left <= 0 ? 0u : (0xFFFFFFFFu >> (32 - 8 * left)));
     }
+  }
+  if (kRows) {
+    // row kk + 8 lies inside the block only if row kk does
     rowmask0 = kk < bh ? ~0u : 0u;
     rowmask1 = kk + 8 < bh ? ~0u : 0u;
   }
@@ -129,26 +137,27 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
       q0 = make_uint4(funnel(a0, a1, sh), funnel(a1, a2, sh), funnel(a2, a3, sh), funnel(a3, a4, sh));
       q1 = make_uint4(funnel(b0, b1, sh), funnel(b1, b2, sh), funnel(b2, b3, sh), funnel(b3, b4, sh));
     }
-    if (!FULL) {
-      q0.x &= colmask[0] & rowmask0; q0.y &= colmask[1] & rowmask0;
-      q0.z &= colmask[2] & rowmask0; q0.w &= colmask[3] & rowmask0;
-      q1.x &= colmask[0] & rowmask1; q1.y &= colmask[1] & rowmask1;
-      q1.z &= colmask[2] & rowmask1; q1.w &= colmask[3] & rowmask1;
+    if (kCols) {
+      q0.x &= colmask[0]; q0.y &= colmask[1]; q0.z &= colmask[2]; q0.w &= colmask[3];
+      q1.x &= colmask[0]; q1.y &= colmask[1]; q1.z &= colmask[2]; q1.w &= colmask[3];
     }
-    uint32_t acc = sad4(src0.x, q0.x, 0);
-    acc = sad4(src0.y, q0.y, acc);
-    acc = sad4(src0.z, q0.z, acc);
-    acc = sad4(src0.w, q0.w, acc);
-    acc = sad4(src1.x, q1.x, acc);
+    uint32_t acc = sad4(src1.x, q1.x, 0);
     acc = sad4(src1.y, q1.y, acc);
     acc = sad4(src1.z, q1.z, acc);
     acc = sad4(src1.w, q1.w, acc);
+    if (kRows) acc &= rowmask1;
+    acc = sad4(src0.x, q0.x, acc);
+    acc = sad4(src0.y, q0.y, acc);
+    acc = sad4(src0.z, q0.z, acc);
+    acc = sad4(src0.w, q0.w, acc);
+    if (kRows) acc &= rowmask0;
     return sum8(acc);
   };
   // boff: window offset of the current best vector (dy * lp + dx)
   int boff = 0;
   uint32_t bsad;
-  if (FULL) {
+  static_assert(MASK == kMaskNone || !WAVE_ZERO, "the wave-wide (0, 0) SAD covers a whole block");
+  if constexpr (WAVE_ZERO) {
     // SAD at (0, 0) over the whole wave: lane = (row l / 4, dword l % 4),
     // both operands dword aligned.
     uint32_t z = sad4(zsrc, *reinterpret_cast<const uint32_t*>(zrow), 0);
@@ -212,6 +221,7 @@ __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::
 __device__ __forceinline__ void stage_gather(uint8_t* win, const FrameDesc& f, const ArnrGroup& a, int bx, int by,
                                              int lane) {
   constexpr int ldw = kLumaPitch / 4;  // dwords per window row
+#pragma unroll
   for (int s = lane; s < kWinRows * ldw; s += 64) {
     const int wy = s / ldw, dq = s - wy * ldw;
     const int y = clampi(by - 15 + wy, 0, a.h - 1);
@@ -220,15 +230,24 @@ __device__ __forceinline__ void stage_gather(uint8_t* win, const FrameDesc& f, c
   }
 }
 
-// Row r of the anchor MB as 16 bytes, zero beyond (bw, bh).
-__device__ __forceinline__ uint4 centre_row16(const FrameDesc& C, int bx, int by, int r, int bw, int bh, bool fast16) {
-  if (fast16) return *reinterpret_cast<const uint4*>(C.y + (size_t)(by + r) * C.y_stride + bx);
+// Row r of the anchor MB as 16 bytes, zero beyond (bw, bh).  WHOLE: the MB
+// is 16 pixels wide and its rows are 16-byte aligned (one load per row; a
+// byte-wise gather, whose loads the compiler may serialise, is the
+// fallback).
+__device__ __forceinline__ uint4 centre_row16(const FrameDesc& C, int bx, int by, int r, int bw, int bh, bool whole) {
+  if (whole) {
+    if (r >= bh) return make_uint4(0u, 0u, 0u, 0u);
+    return *reinterpret_cast<const uint4*>(C.y + (size_t)(by + r) * C.y_stride + bx);
+  }
+  // Every load is issued (row and column clamped into the block) and the
+  // bytes outside the block are zeroed afterwards, so the loads go out
+  // together instead of one branch at a time.
   uint32_t d[4] = {0u, 0u, 0u, 0u};
-  if (r < bh) {
-    const uint8_t* row = C.y + (size_t)(by + r) * C.y_stride + bx;
+  const uint8_t* row = C.y + (size_t)(by + min(r, bh - 1)) * C.y_stride + bx;
 #pragma unroll
-    for (int j = 0; j < 16; j++)
-      if (j < bw) d[j >> 2] |= (uint32_t)row[j] << (8 * (j & 3));
+  for (int j = 0; j < 16; j++) {
+    const uint32_t keep = j < bw && r < bh ? 0xFFu : 0u;
+    d[j >> 2] |= ((uint32_t)row[min(j, bw - 1)] & keep) << (8 * (j & 3));
   }
   return make_uint4(d[0], d[1], d[2], d[3]);
 }
@@ -251,18 +270,22 @@ __device__ __forceinline__ void blend_px(uint32_t& acc, uint32_t ref, uint32_t c
 
 // Frame loops of one wave, one per kind of MB, so that each loop keeps only
 // its own lane constants live:
-//   kPathFast:   interior 16x16 MB: the unmasked search, chroma unclamped.
-//   kPathDma:    MB in the top or bottom rows (or partial), columns
-//                interior: the masked search (identical to the unmasked one
-//                for a whole MB), chroma clamped.
-//   kPathGather: MB at the left or right edge, or planes not 16-byte
-//                aligned or without one stride: clamped gathers.
-// Fast and Dma MBs (16-byte aligned luma, one stride for all frames,
-// columns bx-16..bx+31 inside the frame) take their window by DMA with
-// row-clamped piece offsets; the next frame's window arrives during this
-// frame's search.  A frame's chroma bytes are loaded after its search and
-// blended after the next frame's wait, so that their latency overlaps the
-// next search.
+//   kPathFast:    interior 16x16 MB (window and chroma reach inside the
+//                 frame): the unmasked search, chroma unclamped.
+//   kPathDma:     any other MB on the DMA path (top and bottom rows, left
+//                 and right columns; always 16 pixels wide): the search
+//                 with rows >= bh masked on the row-clamped, edge-filled
+//                 window, chroma clamped.
+//   kPathGather:  planes not 16-byte aligned or without one stride, or a
+//                 window piece straddling the frame's right edge (width not
+//                 a multiple of 16): clamped dword gathers.
+// DMA MBs (16-byte aligned luma, one stride for all frames, every window
+// piece wholly inside or wholly outside the frame's columns) take their
+// window by DMA with row-clamped piece offsets; pieces outside the columns
+// are filled with the row's edge pixel.  The next frame's window arrives
+// during this frame's search.  A frame's chroma bytes are loaded after its
+// search and blended after the next frame's wait, so that their latency
+// overlaps the next search.
 enum { kPathFast = 1, kPathDma = 2, kPathGather = 3 };
 
 // `lut_sync` fills the weight LUT and synchronises the workgroup; every wave
@@ -275,6 +298,9 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   const int nnb = a.filter_on ? a.n - 1 : 0;
   // search: lane = probe group gq x row pair kk (rows kk, kk + 8)
   const int gq = lane >> 3, kk = lane & 7;
+  constexpr bool kDma = PATH == kPathFast || PATH == kPathDma;
+  // window pieces may lie outside the frame's columns (filled with the edge pixel)
+  constexpr bool kMasked = PATH == kPathDma;
   uint4 src0, src1;
   // DMA paths: plane offset of lane piece i (window row p / 3 clamped to the
   // frame, 16-byte column p % 3, p = lane + 64 i); kPathDma: bit i of
@@ -283,14 +309,15 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   uint32_t off0 = 0, off1 = 0, off2 = 0, coff = 0, outside = 0;
   {
     const FrameDesc C = frame_at(a.center);
-    src0 = centre_row16(C, s.bx, s.by, kk, s.bw, s.bh, PATH == kPathFast);
-    src1 = centre_row16(C, s.bx, s.by, kk + 8, s.bw, s.bh, PATH == kPathFast);
-    if (PATH == kPathFast || PATH == kPathDma) {
+    // DMA MBs are 16 pixels wide with 16-byte aligned rows
+    src0 = centre_row16(C, s.bx, s.by, kk, s.bw, s.bh, kDma);
+    src1 = centre_row16(C, s.bx, s.by, kk + 8, s.bw, s.bh, kDma);
+    if (kDma) {
       auto piece = [&](int p, int i) -> uint32_t {
         const int row = p / 3;
         const uint32_t y = (uint32_t)clampi(s.by - 15 + row, 0, a.h - 1) * C.y_stride;
         const int x0 = s.bx - 16 + 16 * (p - 3 * row);
-        if (PATH == kPathDma && (x0 < 0 || x0 + 16 > a.w)) {
+        if (kMasked && (x0 < 0 || x0 + 16 > a.w)) {
           outside |= 1u << i;
           return y + (uint32_t)(x0 < 0 ? 0 : a.w - 1);
         }
@@ -302,9 +329,7 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     }
     if (PATH == kPathFast) coff = (uint32_t)cr * C.uv_stride + (uint32_t)cc;
   }
-  constexpr bool kDma = PATH == kPathFast || PATH == kPathDma;
-  constexpr bool kMasked = PATH == kPathDma;
-  // edge pixels of the outside pieces of the frame in flight (kPathDma)
+  // edge pixels of the outside pieces of the frame in flight (kMasked)
   uint32_t e0 = 0, e1 = 0, e2 = 0;
   auto fetch_edges = [&](const uint8_t* plane) {
     if (outside & 1) e0 = plane[off0];
@@ -364,12 +389,19 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     {
       const int gd_off = (((kProbeDy1 >> (2 * gq)) & 3) - 1) * kLumaPitch + (((kProbeDx1 >> (2 * gq)) & 3) - 1);
       const uint8_t* row0 = w + (kk + 15) * kLumaPitch + 16;
-      if (PATH == kPathFast) {
+      if constexpr (PATH == kPathFast) {
         // (0, 0) operands of the whole-wave SAD: row lane / 4, dword lane % 4
         const uint8_t* zrow = w + (lr + 15) * kLumaPitch + 16 + 4 * lq;
-        descend<true, MbGeom>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad, zrow, s.ctr4);
+        descend<kMaskNone, true, MbGeom>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad, zrow,
+                                         s.ctr4);
+      } else if constexpr (PATH == kPathDma) {
+        // a DMA MB is 16 pixels wide: a partial-width MB has a window piece
+        // straddling the right edge, which sends it to the gather path
+        descend<kMaskRows, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, 16, s.bh, bestx, besty, best_sad,
+                                          nullptr, 0);
       } else {
-        descend<false, MbGeom>(row0, src0, src1, gd_off, gq, kk, s.bw, s.bh, bestx, besty, best_sad, nullptr, 0);
+        descend<kMaskBoth, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, s.bw, s.bh, bestx, besty, best_sad,
+                                          nullptr, 0);
       }
       if (a.mv_out && lane == 0) {
         int32_t* o = a.mv_out + 3 * (((size_t)fi * a.brows + s.brow) * a.bcols + (s.bx >> 4));
@@ -548,13 +580,18 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   if (active) {
     const FrameDesc C = frame_at(a.center);
     // centre pixels of this lane's blend positions (zero outside the frame)
-    if (s.bw == 16 && s.bh == 16 && a.aligned4) {
-      s.ctr4 = *reinterpret_cast<const uint32_t*>(C.y + (size_t)(s.by + lr) * C.y_stride + s.bx + 4 * lq);
-    } else if (lr < s.bh) {
-      const uint8_t* row = C.y + (size_t)(s.by + lr) * C.y_stride + s.bx + 4 * lq;
+    if (s.bw == 16 && a.aligned4) {
+      if (lr < s.bh)
+        s.ctr4 = *reinterpret_cast<const uint32_t*>(C.y + (size_t)(s.by + lr) * C.y_stride + s.bx + 4 * lq);
+    } else {
+      // all four loads issued (clamped into the block), outside bytes zeroed
+      const uint8_t* row = C.y + (size_t)(s.by + min(lr, s.bh - 1)) * C.y_stride + s.bx;
 #pragma unroll
-      for (int j = 0; j < 4; j++)
-        if (4 * lq + j < s.bw) s.ctr4 |= (uint32_t)row[j] << (8 * j);
+      for (int j = 0; j < 4; j++) {
+        const int c = 4 * lq + j;
+        const uint32_t keep = c < s.bw && lr < s.bh ? 0xFFu : 0u;
+        s.ctr4 |= ((uint32_t)row[min(c, s.bw - 1)] & keep) << (8 * j);
+      }
     }
     if (cr < cbh && cc < cbw) {
       const size_t o = (size_t)(s.cby + cr) * C.uv_stride + s.cbx + cc;
@@ -595,6 +632,11 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   }
 
   if (K > 1) {
+    // lane offsets derived again from the lane id, so that none is held
+    // across the frame loops
+    const int lane2 = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const int lr = lane2 >> 2, lq = lane2 & 3;
+    const int cr = lane2 >> 3, cc = lane2 & 7;
     uint32_t* acc = s_acc + 384 * mi;
     if (active) {
       lds_add(&acc[lr * 16 + 4 * lq + 0], s.ay0);
@@ -664,7 +706,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)
 arnr_fused_kernel(const ArnrArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int total = a.g.strips_per_row * a.g.brows;
-  const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
+  // Strips go out bottom row first: the last MB row, partial when the height
+  // is not a multiple of 16, takes the masked search (the slowest path) and
+  // so starts with the launch instead of closing it.
+  const int strip = total - 1 - (a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x);
   const int brow = strip / a.g.strips_per_row;
   const int scol = strip - brow * a.g.strips_per_row;
   process_mbs<false>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, a.mbs_per_wg, smem);

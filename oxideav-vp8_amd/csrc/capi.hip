@@ -418,6 +418,79 @@ int finish(vp8g_ctx* c, int st) {
   return st;
 }
 
+int copy_stream_count(const vp8g_ctx* c) { return c->copy_streams > 0 ? c->copy_streams : 2; }
+
+// finish() after work on the copy streams: they are drained first, so no
+// copy still reads the staging or writes the pinned mirror on return.
+int finish_io(vp8g_ctx* c, int st) {
+  for (int k = 0; k < vp8g_ctx::kMaxCopyStreams; k++)
+    if (c->copy[k] && !hip_ok(hipStreamSynchronize(c->copy[k]))) st = VP8G_E_DEVICE;
+  return finish(c, st);
+}
+
+// The host threads, copy streams and events of the host-pointer ARNR entry
+// point, and `bytes` of write-combined pinned staging (grown, never shrunk).
+int ensure_host_io(vp8g_ctx* c, size_t bytes) {
+  if (c->fault == VP8G_FAULT_ALLOC) return VP8G_E_DEVICE;
+  if (!c->pool) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int def = (int)std::min(8u, std::max(1u, hw / 2));
+    const int threads = c->host_threads > 0 ? c->host_threads : def;
+    c->pool = new vp8g::HostPool(threads - 1);  // the calling thread works too
+  }
+  for (int k = 0; k < copy_stream_count(c); k++) {
+    if (!c->copy[k] && !hip_ok(hipStreamCreateWithFlags(&c->copy[k], hipStreamNonBlocking))) return VP8G_E_DEVICE;
+    if (!c->copy_done[k] && !hip_ok(hipEventCreateWithFlags(&c->copy_done[k], hipEventDisableTiming)))
+      return VP8G_E_DEVICE;
+  }
+  if (!c->ready && !hip_ok(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming))) return VP8G_E_DEVICE;
+  if (bytes <= c->staging_bytes) return VP8G_OK;
+  if (c->staging) {
+    (void)hipHostFree(c->staging);
+    c->staging = nullptr;
+    c->staging_bytes = 0;
+  }
+  const size_t want = bytes + (bytes >> 2);
+  if (!hip_ok(hipHostMalloc(reinterpret_cast<void**>(&c->staging), want, hipHostMallocWriteCombined)))
+    return VP8G_E_DEVICE;
+  c->staging_bytes = want;
+  return VP8G_OK;
+}
+
+// Rows [r0, r1) of a plane of w-byte rows from stride `ss` to pitch `dp`.
+void copy_rows(uint8_t* dst, size_t dp, const uint8_t* src, size_t ss, size_t w, size_t r0, size_t r1) {
+  if (ss == dp && w == dp) {
+    memcpy(dst + r0 * dp, src + r0 * ss, (r1 - r0) * dp);
+    return;
+  }
+  for (size_t r = r0; r < r1; r++) memcpy(dst + r * dp, src + r * ss, w);
+}
+
+// A plane copy cut into row bands of about 256 KiB, for the host threads.
+struct PlaneJob {
+  uint8_t* dst;
+  size_t dp;
+  const uint8_t* src;
+  size_t ss, w, h, rows_per_band;
+};
+
+void add_plane(std::vector<std::pair<const PlaneJob*, size_t>>& tasks, const PlaneJob& j) {
+  for (size_t r = 0; r < j.h; r += j.rows_per_band) tasks.push_back({&j, r});
+}
+
+PlaneJob plane_job(uint8_t* dst, size_t dp, const uint8_t* src, size_t ss, size_t w, size_t h) {
+  const size_t band = std::max<size_t>(1, (256u << 10) / std::max<size_t>(1, w));
+  return PlaneJob{dst, dp, src, ss, w, h, band};
+}
+
+void run_plane_tasks(vp8g_ctx* c, const std::vector<std::pair<const PlaneJob*, size_t>>& tasks) {
+  c->pool->run(tasks.size(), [&](size_t i) {
+    const PlaneJob& j = *tasks[i].first;
+    const size_t r0 = tasks[i].second, r1 = std::min(j.h, r0 + j.rows_per_band);
+    copy_rows(j.dst, j.dp, j.src, j.ss, j.w, r0, r1);
+  });
+}
+
 }  // namespace
 
 int vp8g_validate_group(const vp8g_i420* frames, size_t n, size_t center, vp8g_err* err) {
@@ -512,6 +585,14 @@ void vp8g_ctx_destroy(vp8g_ctx* c) {
   if (c->arena) (void)hipFree(c->arena);
   if (c->pinned) (void)hipHostFree(c->pinned);  // allocated with the arena
   if (c->work) (void)hipFree(c->work);
+  for (int k = 0; k < vp8g_ctx::kMaxCopyStreams; k++) {
+    if (c->copy[k]) (void)hipStreamSynchronize(c->copy[k]);
+    if (c->copy[k]) (void)hipStreamDestroy(c->copy[k]);
+    if (c->copy_done[k]) (void)hipEventDestroy(c->copy_done[k]);
+  }
+  if (c->ready) (void)hipEventDestroy(c->ready);
+  if (c->staging) (void)hipHostFree(c->staging);
+  delete c->pool;
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -552,6 +633,18 @@ int vp8g_ctx_set_option(vp8g_ctx* c, int option, int value) {
       if (value < VP8G_FAULT_NONE || value > VP8G_FAULT_LAUNCH) return VP8G_E_ARG;
       c->fault = value;
       return VP8G_OK;
+    case VP8G_OPT_HOST_THREADS:
+      if (value < 0 || value > 64) return VP8G_E_ARG;
+      if (value != c->host_threads) {
+        delete c->pool;  // rebuilt with the new count on next use
+        c->pool = nullptr;
+      }
+      c->host_threads = value;
+      return VP8G_OK;
+    case VP8G_OPT_COPY_STREAMS:
+      if (value < 0 || value > vp8g_ctx::kMaxCopyStreams) return VP8G_E_ARG;
+      c->copy_streams = value;
+      return VP8G_OK;
     default:
       return VP8G_E_ARG;
   }
@@ -583,31 +676,87 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   const size_t yp = align_up(w, 64), cp = align_up(cw, 64);
   const size_t ysz = align_up(yp * h, 256), csz = align_up(cp * ch, 256);
   const size_t per = ysz + 2 * csz;
-  const size_t out_bytes = align_up(w * h, 256) + 2 * align_up(cw * ch, 256);
-  st = ensure_arena(c, per * n + out_bytes);
+  const size_t oyb = align_up(w * h, 256), ocb = align_up(cw * ch, 256);
+  st = ensure_arena(c, per * n + oyb + 2 * ocb);
+  if (st == VP8G_OK) st = ensure_host_io(c, per * n);
   if (st != VP8G_OK) return st;
-  // Each frame is packed into pinned memory in the device layout and its
-  // copy issued at once, so packing the next frame overlaps the transfer.
+  if (c->fault == VP8G_FAULT_COPY) return VP8G_E_DEVICE;
+  const int S = copy_stream_count(c);
+  // The copies follow whatever the context's stream already holds.
+  if (!hip_ok(hipEventRecord(c->ready, c->stream))) return finish_io(c, VP8G_E_DEVICE);
+  for (int k = 0; k < S; k++)
+    if (!hip_ok(hipStreamWaitEvent(c->copy[k], c->ready, 0))) return finish_io(c, VP8G_E_DEVICE);
+  // Each frame goes out in two pieces, its Y plane and then its U and V
+  // planes (adjacent in the device layout): the host threads pack a piece
+  // into write-combined staging and its copy is issued at once on the copy
+  // stream with the fewest bytes queued, so packing the next piece overlaps
+  // the transfers and the last transfer after the last pack is short.
   std::vector<vp8g_i420> dev(n);
+  std::vector<std::pair<const PlaneJob*, size_t>> tasks;
+  size_t queued[vp8g_ctx::kMaxCopyStreams] = {};
+  int failed = 0;
+  auto send = [&](uint8_t* dst, const uint8_t* src, size_t bytes) {
+    int k = 0;
+    for (int j = 1; j < S; j++)
+      if (queued[j] < queued[k]) k = j;
+    queued[k] += bytes;
+    if (!failed && !hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->copy[k]))) failed = 1;
+  };
   for (size_t i = 0; i < n; i++) {
     uint8_t* base = c->arena + per * i;
-    uint8_t* pin = c->pinned + per * i;
+    uint8_t* pin = c->staging + per * i;
     const vp8g_i420& f = frames[i];
-    pack_plane(pin, yp, f.y, f.y_stride, w, h);
-    pack_plane(pin + ysz, cp, f.u, f.uv_stride, cw, ch);
-    pack_plane(pin + ysz + csz, cp, f.v, f.uv_stride, cw, ch);
-    if (c->fault == VP8G_FAULT_COPY || !hip_ok(hipMemcpyAsync(base, pin, per, hipMemcpyHostToDevice, c->stream)))
-      return finish(c, VP8G_E_DEVICE);
+    const PlaneJob y = plane_job(pin, yp, f.y, f.y_stride, w, h);
+    tasks.clear();
+    add_plane(tasks, y);
+    run_plane_tasks(c, tasks);
+    send(base, pin, ysz);
+    const PlaneJob uv[2] = {plane_job(pin + ysz, cp, f.u, f.uv_stride, cw, ch),
+                            plane_job(pin + ysz + csz, cp, f.v, f.uv_stride, cw, ch)};
+    tasks.clear();
+    for (const PlaneJob& j : uv) add_plane(tasks, j);
+    run_plane_tasks(c, tasks);
+    send(base + ysz, pin + ysz, 2 * csz);
     dev[i] = vp8g_i420{f.width, f.height, base, base + ysz, base + ysz + csz, yp, cp};
   }
+  for (int k = 0; k < S; k++) {
+    if (!hip_ok(hipEventRecord(c->copy_done[k], c->copy[k])) ||
+        !hip_ok(hipStreamWaitEvent(c->stream, c->copy_done[k], 0)))
+      failed = 1;
+  }
+  if (failed) return finish_io(c, VP8G_E_DEVICE);
   uint8_t* dy = c->arena + per * n;
-  uint8_t* du = dy + align_up(w * h, 256);
-  uint8_t* dv = du + align_up(cw * ch, 256);
+  uint8_t* du = dy + oyb;
+  uint8_t* dv = du + ocb;
   st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr);
-  if (st == VP8G_OK) st = copy_out(c, oy, dy, w * h);
-  if (st == VP8G_OK) st = copy_out(c, ou, du, cw * ch);
-  if (st == VP8G_OK) st = copy_out(c, ov, dv, cw * ch);
-  return finish(c, st);
+  if (st != VP8G_OK) return finish_io(c, st);
+  // The altref comes back on the copy streams (luma in S bands, U and V on
+  // two of them) into the arena's pinned mirror.
+  if (!hip_ok(hipEventRecord(c->ready, c->stream))) return finish_io(c, VP8G_E_DEVICE);
+  const size_t off = (size_t)(dy - c->arena);
+  const size_t yb = w * h, band = align_up((yb + S - 1) / S, 4096);
+  for (int k = 0; k < S && !failed; k++) {
+    if (!hip_ok(hipStreamWaitEvent(c->copy[k], c->ready, 0))) failed = 1;
+    const size_t b0 = std::min(yb, (size_t)k * band), b1 = std::min(yb, b0 + band);
+    if (!failed && b1 > b0 &&
+        !hip_ok(hipMemcpyAsync(c->pinned + off + b0, dy + b0, b1 - b0, hipMemcpyDeviceToHost, c->copy[k])))
+      failed = 1;
+  }
+  if (!failed && !hip_ok(hipMemcpyAsync(c->pinned + off + oyb, du, cw * ch, hipMemcpyDeviceToHost, c->copy[0])))
+    failed = 1;
+  if (!failed &&
+      !hip_ok(hipMemcpyAsync(c->pinned + off + oyb + ocb, dv, cw * ch, hipMemcpyDeviceToHost, c->copy[1 % S])))
+    failed = 1;
+  st = finish_io(c, failed ? VP8G_E_DEVICE : VP8G_OK);
+  if (st != VP8G_OK) return st;
+  // Outputs are written only after every copy has succeeded.
+  const PlaneJob outs[3] = {plane_job(oy, w, c->pinned + off, w, w, h),
+                            plane_job(ou, cw, c->pinned + off + oyb, cw, cw, ch),
+                            plane_job(ov, cw, c->pinned + off + oyb + ocb, cw, cw, ch)};
+  tasks.clear();
+  for (const PlaneJob& j : outs) add_plane(tasks, j);
+  run_plane_tasks(c, tasks);
+  return VP8G_OK;
 }
 
 int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "../../include/vp8g.h"
+#include "host_pool.h"
 #include "tuning.h"
 
 struct vp8g_ctx {
@@ -36,6 +37,18 @@ struct vp8g_ctx {
   size_t work_bytes = 0;
   // Device address of this device's static no-alloc scratch (capi.hip).
   uint8_t* scratch = nullptr;
+  // Host-pointer ARNR entry point: host threads that pack the caller's
+  // planes, copy streams that carry them (both created on first use), and
+  // the write-combined pinned staging the planes are packed into.
+  static constexpr int kMaxCopyStreams = 4;
+  int host_threads = 0;  // VP8G_OPT_HOST_THREADS (0 = default)
+  int copy_streams = 0;  // VP8G_OPT_COPY_STREAMS (0 = default)
+  vp8g::HostPool* pool = nullptr;
+  hipStream_t copy[kMaxCopyStreams] = {};
+  hipEvent_t copy_done[kMaxCopyStreams] = {};
+  hipEvent_t ready = nullptr;
+  uint8_t* staging = nullptr;
+  size_t staging_bytes = 0;
 };
 
 // Group validation in the reference's order (src/arnr.rs:494-515) plus the

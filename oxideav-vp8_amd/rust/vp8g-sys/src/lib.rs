@@ -125,6 +125,10 @@ pub const VP8G_FAULT_NONE: c_int = 0;
 pub const VP8G_FAULT_ALLOC: c_int = 1;
 pub const VP8G_FAULT_COPY: c_int = 2;
 pub const VP8G_FAULT_LAUNCH: c_int = 3;
+/// Host threads of `vp8g_build_arnr_altref`'s packing and copy-out.
+pub const VP8G_OPT_HOST_THREADS: c_int = 6;
+/// Streams carrying `vp8g_build_arnr_altref`'s host<->device copies.
+pub const VP8G_OPT_COPY_STREAMS: c_int = 7;
 
 /// `vp8g_search_fallback[_noalloc]_dev` flag: queue only failed entries
 /// with a passing left, right, upper or lower neighbour.

@@ -41,6 +41,8 @@ OPT_XCD_REMAP = 2
 OPT_ARNR_MBS = 3
 OPT_BATCH_WGS_PER_CU = 4
 OPT_FAULT = 5
+OPT_HOST_THREADS = 6
+OPT_COPY_STREAMS = 7
 FAULT_NONE, FAULT_ALLOC, FAULT_COPY, FAULT_LAUNCH = 0, 1, 2, 3
 
 # RFC 6386 section 18.3 tap tables, src/motion_comp.rs:294-303, 328-337.
@@ -383,6 +385,14 @@ class Context:
     def set_batch_wgs_per_cu(self, wgs: int) -> None:
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_BATCH_WGS_PER_CU, wgs),
                     "set_batch_wgs_per_cu")
+
+    def set_host_threads(self, threads: int) -> None:
+        """Host threads of build_arnr_altref's packing and copy-out (0 = default)."""
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_HOST_THREADS, threads), "set_host_threads")
+
+    def set_copy_streams(self, streams: int) -> None:
+        """Copy streams of build_arnr_altref's transfers, 1..4 (0 = default 2)."""
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_COPY_STREAMS, streams), "set_copy_streams")
 
     def set_fault(self, mode: int) -> None:
         """Host-side fault injection (FAULT_ALLOC / FAULT_COPY / FAULT_LAUNCH,

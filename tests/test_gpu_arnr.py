@@ -226,3 +226,33 @@ def test_max_group_length(gpu, oracle):
     import vp8g.synth as synth
     g = synth.arnr_group_cif(48, 32, 64, clip=5)
     check(gpu, oracle, g.frames(), 40, 3)
+
+
+@pytest.mark.parametrize("threads,streams", [(1, 1), (3, 2), (8, 4), (0, 3)])
+def test_host_copy_shapes_leave_bytes_unchanged(gpu, oracle, threads, streams):
+    """The host threads and copy streams of the host-pointer entry change no
+    output byte: strided planes of odd sizes (row bands of every length),
+    a group longer than the copy streams, and a call after a smaller one
+    that reuses the staging."""
+    import vp8g
+    import vp8g.synth as synth
+    try:
+        gpu.set_host_threads(threads)
+        gpu.set_copy_streams(streams)
+        for (w, h, n, c) in ((98, 66, 5, 2), (353, 289, 7, 3), (37, 21, 2, 0)):
+            g = synth.arnr_group_cif(w, h, n, clip=w)
+            tuples = []
+            for (fw, fh, y, u, v, _, _) in g.frames():
+                cw, ch = (fw + 1) // 2, (fh + 1) // 2
+                ys = np.full((fh, fw + 7), 3, np.uint8); ys[:, :fw] = y.reshape(fh, fw)
+                us = np.full((ch, cw + 3), 5, np.uint8); us[:, :cw] = u.reshape(ch, cw)
+                vs = np.full((ch, cw + 3), 6, np.uint8); vs[:, :cw] = v.reshape(ch, cw)
+                tuples.append((fw, fh, ys.reshape(-1), us.reshape(-1), vs.reshape(-1), fw + 7, cw + 3))
+            check(gpu, oracle, tuples, c, 3)
+        with pytest.raises(vp8g.Vp8gError):
+            gpu.set_copy_streams(5)
+        with pytest.raises(vp8g.Vp8gError):
+            gpu.set_host_threads(65)
+    finally:
+        gpu.set_host_threads(0)
+        gpu.set_copy_streams(0)
