@@ -556,7 +556,8 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   asm volatile("" : "+v"(tid));
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int mi = wave % M, split = wave / M;
+  // M is 1, 2, 4 or 8
+  const int mi = wave & (M - 1), split = wave >> __builtin_ctz((unsigned)M);
 
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
@@ -710,7 +711,7 @@ arnr_fused_kernel(const ArnrArgs a) {
   // is not a multiple of 16, takes the masked search (the slowest path) and
   // so starts with the launch instead of closing it.
   const int strip = total - 1 - (a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x);
-  const int brow = strip / a.g.strips_per_row;
+  const int brow = a.spr_mul ? (int)__umulhi((uint32_t)strip, a.spr_mul) : strip;
   const int scol = strip - brow * a.g.strips_per_row;
   process_mbs<false>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, a.mbs_per_wg, smem);
 }
@@ -720,7 +721,7 @@ __global__ void __launch_bounds__(1024) arnr_table_kernel(const ArnrArgs a, cons
   extern __shared__ __align__(16) uint8_t smem[];
   const int total = a.g.strips_per_row * a.g.brows;
   const int strip = a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x;
-  const int brow = strip / a.g.strips_per_row;
+  const int brow = a.spr_mul ? (int)__umulhi((uint32_t)strip, a.spr_mul) : strip;
   const int scol = strip - brow * a.g.strips_per_row;
   process_mbs<true>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, a.mbs_per_wg, smem, table);
 }

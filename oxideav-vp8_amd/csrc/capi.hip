@@ -310,6 +310,14 @@ void fill_group(const vp8g_i420* frames, size_t n, size_t center, uint8_t streng
   memcpy(wt4, wt, sizeof(wt));
 }
 
+// ceil(2^32 / d) for d >= 2, 0 for d == 1.  umulhi(n, ceil(2^32 / d)) is
+// n / d + n * e / 2^32 rounded down, e < 1, so it equals n / d whenever
+// n * d < 2^32: every strip index (< 2^20) of every row width (<= 1024
+// strips) qualifies.
+uint32_t strip_divider(int d) {
+  return d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
+}
+
 // Enqueue the fused ARNR kernel over device-resident frames.
 int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, uint8_t strength,
                  uint8_t* oy, uint8_t* ou, uint8_t* ov, int32_t* mv_out) {
@@ -321,6 +329,7 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &k, &m);
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
+  a.spr_mul = strip_divider(a.g.strips_per_row);
   return launch_status(c, [&] { return launch_arnr_fused(a, k, c->stream); });
 }
 
@@ -903,6 +912,7 @@ int vp8g_build_arnr_altref_table_dev(vp8g_ctx* c, const vp8g_i420* frames, size_
   pick_arnr_shape(c, a.g.w, a.g.h, (int)n, a.g.filter_on ? (int)n - 1 : 1, &k, &m);
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
+  a.spr_mul = strip_divider(a.g.strips_per_row);
   return launch_status(c, [&] { return launch_arnr_table(a, table, k, c->stream); });
 }
 

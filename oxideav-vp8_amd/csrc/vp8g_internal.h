@@ -45,7 +45,10 @@ struct ArnrArgs {
   ArnrGroup g;
   uint32_t wt4[64]; // blend weight per |ref - ctr| as 256 bytes (values <= 16)
   int xcd_remap;    // map workgroups to contiguous MB bands per XCD
-  int mbs_per_wg;   // M: anchor MBs per workgroup (one strip of an MB row)
+  int mbs_per_wg;   // M: anchor MBs per workgroup (one strip of an MB row): 1, 2, 4 or 8
+  // strip / strips_per_row == umulhi(strip, spr_mul) for strips below 2^20
+  // (spr_mul = ceil(2^32 / strips_per_row); 0 when strips_per_row is 1)
+  uint32_t spr_mul;
 };
 // Launch shape: a workgroup holds M anchor MBs x K waves per MB (K splits the
 // neighbour frames), 64 * M * K threads, M * K <= 16.
