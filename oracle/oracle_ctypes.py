@@ -53,6 +53,46 @@ ORC_MB_SEARCH = np.dtype([("mb_col", "<u4"), ("mb_row", "<u4"), ("best", "<i2", 
 ORC_FOUND = np.dtype([("mv", "<i2", (2,)), ("sad", "<u4"), ("cost", "<f8")], align=True)
 
 
+class LfParams(C.Structure):
+    _fields_ = [("interior_limit", C.c_uint8), ("hev_threshold", C.c_uint8), ("mbedge_limit", C.c_uint8),
+                ("sub_bedge_limit", C.c_uint8)]
+
+
+class LfConfig(C.Structure):
+    """orc_lf_config: FrameFilterConfig with ref_delta [CURRENT, LAST, GOLDEN,
+    ALTREF] and mode_delta [B_PRED, ZEROMV, NEAREST/NEAR/NEW, SPLITMV]."""
+    _fields_ = [("simple", C.c_int32), ("key_frame", C.c_int32), ("loop_filter_level", C.c_int32),
+                ("sharpness_level", C.c_int32), ("segmentation_enabled", C.c_int32), ("segment_abs", C.c_int32),
+                ("delta_enabled", C.c_int32), ("segment_lf_level", C.c_int16 * 4), ("ref_delta", C.c_int16 * 4),
+                ("mode_delta", C.c_int16 * 4)]
+
+
+class LfMb(C.Structure):
+    _fields_ = [("segment_id", C.c_uint8), ("ref_frame", C.c_uint8), ("inter_mode", C.c_uint8),
+                ("flags", C.c_uint8)]
+
+
+class LfPlanes(C.Structure):
+    _fields_ = [("y", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("y_stride", C.c_size_t),
+                ("uv_stride", C.c_size_t), ("mb_cols", C.c_size_t), ("mb_rows", C.c_size_t)]
+
+
+class LfSource(C.Structure):
+    _fields_ = [("width", C.c_size_t), ("height", C.c_size_t), ("y", C.c_void_p), ("u", C.c_void_p),
+                ("v", C.c_void_p), ("y_stride", C.c_size_t), ("uv_stride", C.c_size_t)]
+
+
+def lf_config(**kw) -> LfConfig:
+    """An LfConfig from keyword fields; list values fill the array fields."""
+    c = LfConfig()
+    for k, v in kw.items():
+        if k in ("segment_lf_level", "ref_delta", "mode_delta"):
+            getattr(c, k)[:] = list(v)
+        else:
+            setattr(c, k, int(v))
+    return c
+
+
 class HalfPlanes(C.Structure):
     _fields_ = [("h", C.c_uint8 * (22 * 17)), ("v", C.c_uint8 * (17 * 16)),
                 ("d", C.c_uint8 * (17 * 17))]
@@ -145,6 +185,26 @@ class Oracle:
         L.orc_small_diamond_search_luma.restype = SearchResult
         L.orc_whole_pixel_search_batch.argtypes = [C.POINTER(LumaRef), vp, vp, sz, vp, vp]
         L.orc_sub_pixel_refine_batch.argtypes = [C.POINTER(LumaRef), vp, vp, vp, sz, vp, vp, vp]
+        L.orc_lf_derive.argtypes = [u8, u8, i32]
+        L.orc_lf_derive.restype = LfParams
+        L.orc_lf_common_adjust.argtypes = [i32, vp]
+        L.orc_lf_common_adjust.restype = i32
+        L.orc_lf_simple_segment.argtypes = [u8, vp]
+        L.orc_lf_subblock_filter.argtypes = [u8, u8, u8, vp]
+        L.orc_lf_mb_filter.argtypes = [u8, u8, u8, vp]
+        L.orc_lf_mb_level.argtypes = [C.POINTER(LfConfig), LfMb]
+        L.orc_lf_mb_level.restype = u8
+        L.orc_lf_filter_frame.argtypes = [C.POINTER(LfPlanes), vp, C.POINTER(LfConfig)]
+        L.orc_lf_ssd.argtypes = [C.POINTER(LfPlanes), C.POINTER(LfSource)]
+        L.orc_lf_ssd.restype = C.c_uint64
+        L.orc_lf_score.argtypes = [C.POINTER(LfPlanes), vp, C.POINTER(LfConfig), C.POINTER(LfSource), u8, u8]
+        L.orc_lf_score.restype = C.c_uint64
+        L.orc_lf_score_candidates.argtypes = [C.POINTER(LfPlanes), vp, C.POINTER(LfConfig), C.POINTER(LfSource),
+                                              vp, vp, sz, sz, vp]
+        L.orc_lf_select_params.argtypes = [C.POINTER(LfPlanes), vp, C.POINTER(LfConfig), C.POINTER(LfSource), sz,
+                                           vp, vp]
+        L.orc_lf_select_params_sampled.argtypes = [C.POINTER(LfPlanes), vp, C.POINTER(LfConfig),
+                                                   C.POINTER(LfSource), sz, i32, sz, vp, vp]
         self.SIXTAP = np.ctypeslib.as_array((C.c_int32 * 48).in_dll(L, "ORC_SIXTAP_FILTERS")).reshape(8, 6).copy()
         self.BILINEAR = np.ctypeslib.as_array((C.c_int32 * 48).in_dll(L, "ORC_BILINEAR_FILTERS")).reshape(8, 6).copy()
 
@@ -270,3 +330,66 @@ class Oracle:
         if st != 0:
             return st, None, err, None
         return st, (oy[:w * h], ou[:cw * ch], ov[:cw * ch]), err, mvs
+
+    # ---- loop filter.  Planes are (y, u, v) 2-D uint8 arrays of the
+    # MB-padded reconstruction (mb_rows*16 x y_stride, mb_rows*8 x uv_stride);
+    # mbs is an (mb_rows*mb_cols, 4) uint8 array of orc_lf_mb rows.
+    @staticmethod
+    def _lf_planes(y, u, v, keep) -> LfPlanes:
+        for a in (y, u, v):
+            assert a.dtype == np.uint8 and a.flags.c_contiguous
+        keep += [y, u, v]
+        return LfPlanes(_p(y), _p(u), _p(v), y.shape[1], u.shape[1], y.shape[1] // 16, y.shape[0] // 16)
+
+    @staticmethod
+    def _lf_source(src, keep) -> LfSource:
+        w, h, y, u, v = src
+        keep += [y, u, v]
+        return LfSource(w, h, _p(y), _p(u), _p(v), y.shape[1], u.shape[1])
+
+    @staticmethod
+    def _lf_mbs(mbs, keep):
+        m = np.ascontiguousarray(mbs, np.uint8).reshape(-1, 4)
+        keep.append(m)
+        return _p(m)
+
+    def lf_filter_frame(self, y, u, v, mbs, cfg: LfConfig, mb_cols=None):
+        """Filtered copies of (y, u, v); mb_cols defaults to y_stride // 16."""
+        keep = []
+        y, u, v = (np.array(a, np.uint8, order="C", copy=True) for a in (y, u, v))
+        pl = self._lf_planes(y, u, v, keep)
+        if mb_cols is not None:
+            pl.mb_cols = mb_cols
+        self.lib.orc_lf_filter_frame(C.byref(pl), self._lf_mbs(mbs, keep), C.byref(cfg))
+        return y, u, v
+
+    def lf_ssd(self, y, u, v, src) -> int:
+        keep = []
+        return int(self.lib.orc_lf_ssd(C.byref(self._lf_planes(y, u, v, keep)), C.byref(self._lf_source(src, keep))))
+
+    def lf_score_candidates(self, y, u, v, mbs, cfg, src, levels, sharps, threads=8):
+        keep = []
+        lv = np.ascontiguousarray(levels, np.uint8)
+        sh = np.ascontiguousarray(sharps, np.uint8)
+        out = np.zeros(lv.size, np.uint64)
+        self.lib.orc_lf_score_candidates(C.byref(self._lf_planes(y, u, v, keep)), self._lf_mbs(mbs, keep),
+                                         C.byref(cfg), C.byref(self._lf_source(src, keep)), _p(lv), _p(sh),
+                                         lv.size, threads, _p(out))
+        return out
+
+    def lf_select(self, y, u, v, mbs, cfg, src, threads=8):
+        keep = []
+        lv = np.zeros(1, np.uint8)
+        sh = np.zeros(1, np.uint8)
+        self.lib.orc_lf_select_params(C.byref(self._lf_planes(y, u, v, keep)), self._lf_mbs(mbs, keep),
+                                      C.byref(cfg), C.byref(self._lf_source(src, keep)), threads, _p(lv), _p(sh))
+        return int(lv[0]), int(sh[0])
+
+    def lf_select_sampled(self, y, u, v, mbs, cfg, src, every_rows, sweep_sharpness, threads=8):
+        keep = []
+        lv = np.zeros(1, np.uint8)
+        sh = np.zeros(1, np.uint8)
+        self.lib.orc_lf_select_params_sampled(C.byref(self._lf_planes(y, u, v, keep)), self._lf_mbs(mbs, keep),
+                                              C.byref(cfg), C.byref(self._lf_source(src, keep)), every_rows,
+                                              int(bool(sweep_sharpness)), threads, _p(lv), _p(sh))
+        return int(lv[0]), int(sh[0])

@@ -237,6 +237,62 @@ void orc_mb_luma_sad_at_mv_batch(const orc_luma_ref *ref, const uint32_t *mb_xy,
                                  const uint8_t *src, const orc_mv *mv, size_t n,
                                  uint32_t *sad);
 
+/* ---------------- loop filter (src/loop_filter.rs) ---------------- */
+/* LoopFilterParams, src/loop_filter.rs:71-84. */
+typedef struct {
+    uint8_t interior_limit, hev_threshold, mbedge_limit, sub_bedge_limit;
+} orc_lf_params;
+/* FrameFilterConfig, src/loop_filter.rs:891-944; ref_delta is
+ * [CURRENT, LAST, GOLDEN, ALTREF] and mode_delta [B_PRED, ZEROMV,
+ * NEAREST/NEAR/NEW, SPLITMV] (src/loop_filter.rs:1193-1209).  Same layout
+ * as vp8g_lf_config. */
+typedef struct {
+    int32_t simple, key_frame, loop_filter_level, sharpness_level;
+    int32_t segmentation_enabled, segment_abs, delta_enabled;
+    int16_t segment_lf_level[4];
+    int16_t ref_delta[4];
+    int16_t mode_delta[4];
+} orc_lf_config;
+/* One macroblock's filter inputs: MacroblockModes::segment_id / y_mode,
+ * the ref frame, the inter mode and mb_has_coeffs (src/loop_filter.rs:1364-1372).
+ * Same layout as vp8g_lf_mb. */
+typedef struct {
+    uint8_t segment_id;  /* 0..3, ORC_LF_NO_SEGMENT for None */
+    uint8_t ref_frame;   /* 0 intra (CURRENT), 1 LAST, 2 GOLDEN, 3 ALTREF */
+    uint8_t inter_mode;  /* ORC_LF_ZEROMV, ORC_LF_SPLITMV, anything else NEAREST/NEAR/NEW */
+    uint8_t flags;       /* ORC_LF_BPRED | ORC_LF_HAS_COEFFS */
+} orc_lf_mb;
+enum { ORC_LF_NO_SEGMENT = 255, ORC_LF_ZEROMV = 1, ORC_LF_SPLITMV = 2, ORC_LF_BPRED = 1, ORC_LF_HAS_COEFFS = 2 };
+/* KeyframePlanes (MB-padded reconstruction), src/frame.rs. */
+typedef struct {
+    uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride, mb_cols, mb_rows;
+} orc_lf_planes;
+/* SourcePlanes, src/loop_filter.rs:1916-1932. */
+typedef struct {
+    size_t width, height;
+    const uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride;
+} orc_lf_source;
+orc_lf_params orc_lf_derive(uint8_t level, uint8_t sharpness, int key_frame);
+int orc_lf_common_adjust(int use_outer_taps, uint8_t *seg);
+void orc_lf_simple_segment(uint8_t edge_limit, uint8_t *seg);
+void orc_lf_subblock_filter(uint8_t hev_threshold, uint8_t interior_limit, uint8_t edge_limit, uint8_t *seg);
+void orc_lf_mb_filter(uint8_t hev_threshold, uint8_t interior_limit, uint8_t edge_limit, uint8_t *seg);
+uint8_t orc_lf_mb_level(const orc_lf_config *cfg, orc_lf_mb mb);
+void orc_lf_filter_frame(orc_lf_planes *pl, const orc_lf_mb *mbs, const orc_lf_config *cfg);
+uint64_t orc_lf_ssd(const orc_lf_planes *recon, const orc_lf_source *src);
+uint64_t orc_lf_score(const orc_lf_planes *recon, const orc_lf_mb *mbs, const orc_lf_config *base,
+                      const orc_lf_source *src, uint8_t level, uint8_t sharpness);
+void orc_lf_score_candidates(const orc_lf_planes *recon, const orc_lf_mb *mbs, const orc_lf_config *base,
+                             const orc_lf_source *src, const uint8_t *levels, const uint8_t *sharps, size_t n,
+                             size_t threads, uint64_t *out);
+void orc_lf_select_params(const orc_lf_planes *recon, const orc_lf_mb *mbs, const orc_lf_config *base,
+                          const orc_lf_source *src, size_t threads, uint8_t *out_level, uint8_t *out_sharp);
+void orc_lf_select_params_sampled(const orc_lf_planes *recon, const orc_lf_mb *mbs, const orc_lf_config *base,
+                                  const orc_lf_source *src, size_t every_rows, int sweep_sharpness, size_t threads,
+                                  uint8_t *out_level, uint8_t *out_sharp);
+
 #ifdef __cplusplus
 }
 #endif
