@@ -555,6 +555,102 @@ int vp8g_small_diamond_search_luma_batch_dev(vp8g_ctx *ctx, const vp8g_luma_ref 
                                              const uint8_t *src, const vp8g_mv *centers, size_t n,
                                              uint32_t max_iters, vp8g_search_result *out);
 
+/* ---- loop filter (RFC 6386 section 15) ----
+ * FrameFilterConfig (src/loop_filter.rs:891-944) with the section 20.6
+ * arrays: ref_delta[CURRENT, LAST, GOLDEN, ALTREF] and mode_delta[B_PRED,
+ * ZEROMV, NEAREST/NEAR/NEW, SPLITMV] (FrameFilterConfig::ref_deltas /
+ * mode_deltas, src/loop_filter.rs:1193-1209).  Boolean fields are 0 / 1;
+ * loop_filter_level and sharpness_level take the u8 range. */
+typedef struct {
+    int32_t simple, key_frame, loop_filter_level, sharpness_level;
+    int32_t segmentation_enabled, segment_abs, delta_enabled;
+    int16_t segment_lf_level[4];
+    int16_t ref_delta[4];
+    int16_t mode_delta[4];
+} vp8g_lf_config;
+
+/* One macroblock's filter inputs, raster order: MacroblockModes::segment_id
+ * (VP8G_LF_NO_SEGMENT for None) and y_mode (VP8G_LF_BPRED when B_PRED; the
+ * decoder records SPLITMV MBs as B_PRED, src/loop_filter.rs:1301-1308), the
+ * reference frame (0 intra, 1 LAST, 2 GOLDEN, 3 ALTREF), the inter mode of
+ * an inter MB (VP8G_LF_ZEROMV, VP8G_LF_SPLITMV, anything else is
+ * NEAREST/NEAR/NEW) and mb_has_coeffs (src/loop_filter.rs:1500-1505) as
+ * VP8G_LF_HAS_COEFFS.  Four bytes, 4-byte aligned arrays. */
+typedef struct {
+    uint8_t segment_id, ref_frame, inter_mode, flags;
+} vp8g_lf_mb;
+#define VP8G_LF_NO_SEGMENT 255
+#define VP8G_LF_ZEROMV 1
+#define VP8G_LF_SPLITMV 2
+#define VP8G_LF_BPRED 1
+#define VP8G_LF_HAS_COEFFS 2
+
+/* KeyframePlanes (src/frame.rs): the MB-padded reconstruction, mb_rows*16
+ * rows of y_stride >= mb_cols*16 bytes, mb_rows*8 rows of uv_stride >=
+ * mb_cols*8 for u and v; at most 1024 x 1024 MBs.  In device memory y and
+ * y_stride are multiples of 16, u, v and uv_stride multiples of 8. */
+typedef struct {
+    uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride, mb_cols, mb_rows;
+} vp8g_lf_planes;
+
+/* SourcePlanes (src/loop_filter.rs:1916-1932): the source picture and its
+ * visible width x height window (at most the padded frame); chroma covers
+ * ceil(w/2) x ceil(h/2).  Any alignment. */
+typedef struct {
+    size_t width, height;
+    const uint8_t *y, *u, *v;
+    size_t y_stride, uv_stride;
+} vp8g_lf_source;
+
+/* Replaces filter_inter_frame_flags (src/loop_filter.rs:1364-1496) and, with
+ * every MB intra, filter_frame_flags (src/loop_filter.rs:2506-2636), the
+ * loops filter_frame_flags_threaded / filter_inter_frame_flags_threaded
+ * (src/loop_filter.rs:2435-2504, src/loop_filter/bands.rs) run on several
+ * threads: the whole frame filtered in place, byte-identical to the raster
+ * pass.  `mbs` has mb_cols * mb_rows entries.  _dev: planes and mbs in
+ * device memory, enqueue only; the host form copies planes and mbs in and
+ * the filtered planes back (synchronous). */
+int vp8g_loop_filter_frame_dev(vp8g_ctx *ctx, const vp8g_lf_planes *planes, const vp8g_lf_mb *mbs,
+                               const vp8g_lf_config *cfg);
+int vp8g_loop_filter_frame(vp8g_ctx *ctx, const vp8g_lf_planes *planes, const vp8g_lf_mb *mbs,
+                           const vp8g_lf_config *cfg);
+
+/* Replaces score_filter_candidate over score_candidates
+ * (src/loop_filter.rs:2062-2131): ssd[i] = the visible-window SSD against
+ * `src` of the reconstruction filtered with cfg's level and sharpness
+ * replaced by levels[i], sharps[i] (level 0: the unfiltered SSD, whatever
+ * the segment overrides).  every_rows 0 scores the whole frame; every_rows
+ * >= 2 scores the bands of select_filter_params_sampled
+ * (src/loop_filter.rs:2303-2365): two MB rows starting every every_rows
+ * rows, each filtered as a frame of its own.  recon, mbs, src and ssd are
+ * device memory (recon is only read); levels and sharps are host arrays of
+ * n <= 256 entries.  Enqueue only. */
+int vp8g_loop_filter_score_dev(vp8g_ctx *ctx, const vp8g_lf_planes *recon, const vp8g_lf_mb *mbs,
+                               const vp8g_lf_config *cfg, const vp8g_lf_source *src, const uint8_t *levels,
+                               const uint8_t *sharps, size_t n, size_t every_rows, uint64_t *ssd);
+
+/* Replaces select_filter_params_threaded (src/loop_filter.rs:2133-2200) when
+ * every_rows is 0, and select_filter_params_sampled
+ * (src/loop_filter.rs:2288-2412, with sweep_sharpness; without it
+ * select_filter_level_sampled, :2224-2258) when every_rows >= 2: the
+ * (level, sharpness) the reference's search picks, first strict minimum at
+ * every stage.  The searches' level stages are scored in one pass over all
+ * 64 levels, the sharpness sweep in a second.  _dev: recon, mbs and src in
+ * device memory; the host form copies them in.  Synchronous. */
+int vp8g_loop_filter_select_dev(vp8g_ctx *ctx, const vp8g_lf_planes *recon, const vp8g_lf_mb *mbs,
+                                const vp8g_lf_config *cfg, const vp8g_lf_source *src, size_t every_rows,
+                                int sweep_sharpness, uint8_t *level, uint8_t *sharpness);
+int vp8g_loop_filter_select(vp8g_ctx *ctx, const vp8g_lf_planes *recon, const vp8g_lf_mb *mbs,
+                            const vp8g_lf_config *cfg, const vp8g_lf_source *src, size_t every_rows,
+                            int sweep_sharpness, uint8_t *level, uint8_t *sharpness);
+
+/* Waits for the context's stream and reports whether any loop-filter launch
+ * since the last call gave up on a row wait (VP8G_E_DEVICE; its output is
+ * then incomplete), then clears the flag.  The row schedule cannot stall,
+ * so this reports a broken invariant, not a normal outcome. */
+int vp8g_loop_filter_status(vp8g_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,7 @@ HIP_FLAGS = [
 
 SOURCES = ["capi.hip", "arnr_kernels.hip", "mc_kernels.hip", "search_kernels.hip", "proposer_kernels.hip",
            "proposer_host.hip", "synth_kernels.hip", "synth_host.hip", "tuning.cpp", "inter_search_kernels.hip", "split_kernels.hip",
-           "stream_kernels.hip"]
+           "stream_kernels.hip", "lf_kernels.hip", "lf_host.hip"]
 
 
 def build(verbose: bool = False) -> Path:

@@ -601,6 +601,11 @@ void vp8g_ctx_destroy(vp8g_ctx* c) {
   }
   if (c->ready) (void)hipEventDestroy(c->ready);
   if (c->staging) (void)hipHostFree(c->staging);
+  if (c->lf_work) (void)hipFree(c->lf_work);
+  if (c->lf_io) (void)hipFree(c->lf_io);
+  if (c->lf_pinned) (void)hipHostFree(c->lf_pinned);
+  if (c->lf_ssd) (void)hipFree(c->lf_ssd);
+  if (c->lf_err) (void)hipFree(c->lf_err);
   delete c->pool;
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;

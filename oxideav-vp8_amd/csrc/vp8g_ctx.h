@@ -49,6 +49,16 @@ struct vp8g_ctx {
   hipEvent_t ready = nullptr;
   uint8_t* staging = nullptr;
   size_t staging_bytes = 0;
+  // Loop filter (lf_host.hip): row rings and per-instance SSDs, the device
+  // copy of host-pointer inputs with its pinned mirror, the candidate SSDs
+  // of the level search, and the flag a launch sets when a row wait gives up.
+  uint8_t* lf_work = nullptr;
+  size_t lf_work_bytes = 0;
+  uint8_t* lf_io = nullptr;
+  uint8_t* lf_pinned = nullptr;
+  size_t lf_io_bytes = 0;
+  unsigned long long* lf_ssd = nullptr;  // kLfSsdSlots entries
+  int* lf_err = nullptr;
 };
 
 // Group validation in the reference's order (src/arnr.rs:494-515) plus the

@@ -274,4 +274,39 @@ hipError_t launch_predict_inter_mb(const McArgs& a, hipStream_t stream);
 hipError_t launch_mb_luma_sad_at_mv(const SadArgs& a, hipStream_t stream);
 hipError_t launch_refine_ladder(const RefineArgs& a, hipStream_t stream);
 
+// Loop filter: one workgroup per instance (candidate x band); see
+// lf_kernels.hip.
+constexpr int kLfMaxCands = 256;
+struct LfArgs {
+  const uint8_t* y;              // reconstruction (read)
+  const uint8_t* u;
+  const uint8_t* v;
+  uint8_t* oy;                   // frame mode: filtered output (may alias y/u/v)
+  uint8_t* ou;
+  uint8_t* ov;
+  const uint8_t* sy;             // score mode: source picture
+  const uint8_t* su;
+  const uint8_t* sv;
+  size_t y_stride, uv_stride, sy_stride, suv_stride;
+  int mb_cols, mb_rows;
+  int vis_w, vis_h, vis_cw, vis_ch;  // visible window of the source
+  int src_aligned4;              // source planes and strides are multiples of 4
+  const uint32_t* mbs;           // vp8g_lf_mb per MB, raster order
+  int key_frame, segmentation_enabled, segment_abs, delta_enabled;
+  int segment_lf_level[4], ref_delta[4], mode_delta[4];
+  int n_bands, every_rows, band_rows;  // instance i: candidate i / n_bands, rows from (i % n_bands) * every_rows
+  uint8_t cand_level[kLfMaxCands], cand_sharp[kLfMaxCands];
+  uint8_t* ring;                 // row rings between waves, ring_instance_bytes per instance
+  size_t ring_instance_bytes;
+  unsigned long long* inst_ssd;  // score mode: one SSD per instance
+  int* err;                      // set when a row wait gives up
+  int spin_limit;                // polls before a row wait gives up
+};
+size_t lf_lds_bytes(int waves);
+size_t lf_ring_bytes(int waves, int mb_cols);
+hipError_t launch_loop_filter(const LfArgs& a, int instances, int waves, bool simple, bool score,
+                              hipStream_t stream);
+hipError_t launch_lf_sum_bands(const unsigned long long* inst, int n_cands, int n_bands, unsigned long long* out,
+                               hipStream_t stream);
+
 }  // namespace vp8g

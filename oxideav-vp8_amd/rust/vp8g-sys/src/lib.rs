@@ -239,6 +239,67 @@ pub struct vp8g_found {
     pub cost: f64,
 }
 
+/// `FrameFilterConfig` (src/loop_filter.rs:891-944) with `ref_delta` =
+/// [CURRENT, LAST, GOLDEN, ALTREF] and `mode_delta` = [B_PRED, ZEROMV,
+/// NEAREST/NEAR/NEW, SPLITMV] (src/loop_filter.rs:1193-1209).
+#[repr(C)]
+#[derive(Debug, Clone, Copy, Default, PartialEq, Eq)]
+pub struct vp8g_lf_config {
+    pub simple: i32,
+    pub key_frame: i32,
+    pub loop_filter_level: i32,
+    pub sharpness_level: i32,
+    pub segmentation_enabled: i32,
+    pub segment_abs: i32,
+    pub delta_enabled: i32,
+    pub segment_lf_level: [i16; 4],
+    pub ref_delta: [i16; 4],
+    pub mode_delta: [i16; 4],
+}
+
+/// One macroblock's loop-filter inputs (segment id, ref frame, inter mode,
+/// `VP8G_LF_BPRED | VP8G_LF_HAS_COEFFS`).
+#[repr(C)]
+#[derive(Debug, Clone, Copy, Default, PartialEq, Eq)]
+pub struct vp8g_lf_mb {
+    pub segment_id: u8,
+    pub ref_frame: u8,
+    pub inter_mode: u8,
+    pub flags: u8,
+}
+
+/// `KeyframePlanes` (src/frame.rs): the MB-padded reconstruction.
+#[repr(C)]
+#[derive(Debug, Clone, Copy)]
+pub struct vp8g_lf_planes {
+    pub y: *mut u8,
+    pub u: *mut u8,
+    pub v: *mut u8,
+    pub y_stride: usize,
+    pub uv_stride: usize,
+    pub mb_cols: usize,
+    pub mb_rows: usize,
+}
+
+/// `SourcePlanes` (src/loop_filter.rs:1916-1932).
+#[repr(C)]
+#[derive(Debug, Clone, Copy)]
+pub struct vp8g_lf_source {
+    pub width: usize,
+    pub height: usize,
+    pub y: *const u8,
+    pub u: *const u8,
+    pub v: *const u8,
+    pub y_stride: usize,
+    pub uv_stride: usize,
+}
+
+pub const VP8G_LF_NO_SEGMENT: c_int = 255;
+pub const VP8G_LF_ZEROMV: c_int = 1;
+pub const VP8G_LF_SPLITMV: c_int = 2;
+pub const VP8G_LF_BPRED: c_int = 1;
+pub const VP8G_LF_HAS_COEFFS: c_int = 2;
+
 extern "C" {
     // ------------------------------------------------------------ context
     pub fn vp8g_ctx_create(device: c_int, out: *mut *mut vp8g_ctx) -> c_int;
@@ -427,4 +488,19 @@ extern "C" {
                                                     mb_xy: *const u32, src: *const u8, centers: *const vp8g_mv,
                                                     n: usize, max_iters: u32,
                                                     out: *mut vp8g_search_result) -> c_int;
+    // ------------------------------------------------------------ loop filter
+    pub fn vp8g_loop_filter_frame_dev(ctx: *mut vp8g_ctx, planes: *const vp8g_lf_planes, mbs: *const vp8g_lf_mb,
+                                      cfg: *const vp8g_lf_config) -> c_int;
+    pub fn vp8g_loop_filter_frame(ctx: *mut vp8g_ctx, planes: *const vp8g_lf_planes, mbs: *const vp8g_lf_mb,
+                                  cfg: *const vp8g_lf_config) -> c_int;
+    pub fn vp8g_loop_filter_score_dev(ctx: *mut vp8g_ctx, recon: *const vp8g_lf_planes, mbs: *const vp8g_lf_mb,
+                                      cfg: *const vp8g_lf_config, src: *const vp8g_lf_source, levels: *const u8,
+                                      sharps: *const u8, n: usize, every_rows: usize, ssd: *mut u64) -> c_int;
+    pub fn vp8g_loop_filter_select_dev(ctx: *mut vp8g_ctx, recon: *const vp8g_lf_planes, mbs: *const vp8g_lf_mb,
+                                       cfg: *const vp8g_lf_config, src: *const vp8g_lf_source, every_rows: usize,
+                                       sweep_sharpness: c_int, level: *mut u8, sharpness: *mut u8) -> c_int;
+    pub fn vp8g_loop_filter_select(ctx: *mut vp8g_ctx, recon: *const vp8g_lf_planes, mbs: *const vp8g_lf_mb,
+                                   cfg: *const vp8g_lf_config, src: *const vp8g_lf_source, every_rows: usize,
+                                   sweep_sharpness: c_int, level: *mut u8, sharpness: *mut u8) -> c_int;
+    pub fn vp8g_loop_filter_status(ctx: *mut vp8g_ctx) -> c_int;
 }

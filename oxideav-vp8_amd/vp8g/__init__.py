@@ -26,6 +26,8 @@ from pathlib import Path
 
 import numpy as np
 
+from . import loop_filter as _lf
+
 _LIB_PATH = Path(__file__).resolve().parent.parent / "libvp8g.so"
 
 VP8G_OK = 0
@@ -125,7 +127,7 @@ EXPORTS = (
     "vp8g_predict_split_mv_batch", "vp8g_predict_split_mv_batch_dev", "vp8g_reconstruct_inter_mb_batch",
     "vp8g_reconstruct_inter_mb_batch_dev", "vp8g_reconstruct_split_mv_mb_batch",
     "vp8g_reconstruct_split_mv_mb_batch_dev", "vp8g_luma_mad_batch", "vp8g_luma_sad_batch_dev",
-)
+) + _lf.EXPORTS
 FALLBACK_GATE_NEIGHBOUR = 1
 
 
@@ -207,6 +209,7 @@ def load_library(path: Path | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    _lf.bind(lib)
     return lib
 
 
