@@ -468,6 +468,188 @@ def measure_next_rows(ctx, group, cores: int, cpu_seconds: float = 2.0, reps: in
     }
 
 
+def measure_loop_filter(ctx, group, cores: int, reps: int = 10, cpu_seconds: float = 2.0) -> dict:
+    """The section 15 loop filter over a 1920x1088 frame (8160 MBs): the
+    bench group's centre frame is the source, the reconstruction is that
+    frame plus a +-9 step per 4x4 block (chroma +-7), and every MB carries
+    a seeded inter-frame record (segment, ref frame, inter mode, B_PRED /
+    coefficient flags).  GPU legs on the library stream with device inputs:
+
+    * frame: filter_inter_frame_flags over the frame, in place (level 36,
+      sharpness 2, ref / mode deltas on);
+    * score64: 64 candidates (every level at the base sharpness, the level
+      stages of the search in one launch);
+    * select / select_sampled: select_filter_params and
+      select_filter_params_sampled (key-frame sampling, sweep on), wall
+      clock of the synchronous calls.
+
+    CPU: the oracle's raster pass on one thread (frame) and its searches
+    scoring candidates on `cores` threads, as the reference's threaded
+    search does."""
+    import ctypes as C
+
+    import torch
+
+    import vp8g
+    from vp8g import loop_filter as L
+    import vp8g.synth as synth
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle_ctypes import Oracle, lf_config
+
+    orc = Oracle()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.ExternalStream(ctx.stream(), device=dev)
+    c = group.center
+    sy = np.ascontiguousarray(synth.pad_to_mb(group.y[c], 16))
+    su = np.ascontiguousarray(synth.pad_to_mb(group.u[c], 8))
+    sv = np.ascontiguousarray(synth.pad_to_mb(group.v[c], 8))
+    H, W = sy.shape
+    bw, bh = W // 16, H // 16
+    n = bw * bh
+    rng = np.random.default_rng(1536)
+    ry = np.clip(sy.astype(np.int32) + rng.integers(-9, 10, (H // 4, W // 4)).repeat(4, 0).repeat(4, 1), 0,
+                 255).astype(np.uint8)
+    cb = rng.integers(-7, 8, (2, H // 8, W // 8)).repeat(4, 1).repeat(4, 2)
+    ru = np.clip(su.astype(np.int32) + cb[0], 0, 255).astype(np.uint8)
+    rv = np.clip(sv.astype(np.int32) + cb[1], 0, 255).astype(np.uint8)
+    mbs = np.zeros((n, 4), np.uint8)
+    mbs[:, 0] = rng.choice([255, 0, 1, 2, 3], n)
+    mbs[:, 1] = rng.choice([0, 1, 2, 3], n)
+    mbs[:, 2] = np.where(mbs[:, 1] > 0, rng.choice([1, 2, 3], n), 0)
+    mbs[:, 3] = rng.choice([0, 1], n) | np.where(rng.random(n) < 0.6, 2, 0)
+    fcfg = dict(key_frame=False, loop_filter_level=36, sharpness_level=2, delta_enabled=True, ref_delta_last=-4,
+                ref_delta_golden=6, zero_mv_mode_delta=3, split_mv_mode_delta=-2, segmentation_enabled=True,
+                segment_lf_level=(0, 4, -6, 10))
+    g = L.FrameFilterConfig(**fcfg)
+    og = lf_config(simple=0, key_frame=0, loop_filter_level=36, sharpness_level=2, delta_enabled=1,
+                   segmentation_enabled=1, segment_abs=0, segment_lf_level=[0, 4, -6, 10], ref_delta=g.ref_deltas(),
+                   mode_delta=g.mode_deltas())
+    sel_g = L.FrameFilterConfig(key_frame=True, loop_filter_level=20, sharpness_level=0)
+    sel_og = lf_config(simple=0, key_frame=1, loop_filter_level=20, sharpness_level=0)
+    vis_w, vis_h = group.width, group.height
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1).copy()).to(dev)
+    d = {k: t(v) for k, v in (("ry", ry), ("ru", ru), ("rv", rv), ("sy", sy), ("su", su), ("sv", sv),
+                              ("mbs", mbs))}
+    wy, wu, wv = d["ry"].clone(), d["ru"].clone(), d["rv"].clone()
+    work = L._Planes(wy.data_ptr(), wu.data_ptr(), wv.data_ptr(), W, W // 2, bw, bh)
+    recon = L._Planes(d["ry"].data_ptr(), d["ru"].data_ptr(), d["rv"].data_ptr(), W, W // 2, bw, bh)
+    src = L._Source(vis_w, vis_h, d["sy"].data_ptr(), d["su"].data_ptr(), d["sv"].data_ptr(), W, W // 2)
+    lib = ctx.lib
+    torch.cuda.synchronize()
+
+    def ok(st):
+        if st:
+            raise RuntimeError(f"vp8g loop filter call failed: {st}")
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    gc = g.c()
+    frame = lambda: ok(lib.vp8g_loop_filter_frame_dev(ctx.h, C.byref(work), d["mbs"].data_ptr(), C.byref(gc)))
+    # correctness first: one pass from the reconstruction against the oracle
+    frame()
+    ok(lib.vp8g_loop_filter_status(ctx.h))
+    want = orc.lf_filter_frame(ry, ru, rv, mbs, og)
+    exact = {"frame": bool(all(np.array_equal(a.cpu().numpy().reshape(b.shape), b)
+                               for a, b in zip((wy, wu, wv), want)))}
+    ms_frame = timed(frame)
+    lv = np.arange(64, dtype=np.uint8)
+    sh = np.zeros(64, np.uint8)
+    ssd = torch.zeros(64, dtype=torch.int64, device=dev)
+    sgc = sel_g.c()
+    score = lambda: ok(lib.vp8g_loop_filter_score_dev(ctx.h, C.byref(recon), d["mbs"].data_ptr(), C.byref(sgc),
+                                                      C.byref(src), lv.ctypes.data, sh.ctypes.data, 64, 0,
+                                                      ssd.data_ptr()))
+    ms_score = timed(score)
+    every = L.sampling_for_key_frame(bh)
+    lo, so = C.c_uint8(), C.c_uint8()
+
+    def select(every_rows, sweep):
+        ok(lib.vp8g_loop_filter_select_dev(ctx.h, C.byref(recon), d["mbs"].data_ptr(), C.byref(sgc),
+                                           C.byref(src), every_rows, sweep, C.byref(lo), C.byref(so)))
+        return lo.value, so.value
+
+    def wall(fn, k=reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            r = fn()
+        return (time.perf_counter() - t0) / k * 1e3, r
+
+    ms_select, pick = wall(lambda: select(0, 1))
+    ms_sampled, pick_s = wall(lambda: select(every, 1))
+    torch.cuda.synchronize()
+
+    # CPU: the oracle
+    osrc = (vis_w, vis_h, sy, su, sv)
+
+    def cpu_wall(fn):
+        k, t0 = 0, time.perf_counter()
+        while True:
+            r = fn()
+            k += 1
+            if time.perf_counter() - t0 >= cpu_seconds:
+                break
+        return (time.perf_counter() - t0) / k * 1e3, r
+
+    cpu_frame, _ = cpu_wall(lambda: orc.lf_filter_frame(ry, ru, rv, mbs, og))
+    cpu_select, want_pick = cpu_wall(lambda: orc.lf_select(ry, ru, rv, mbs, sel_og, osrc, threads=cores))
+    cpu_sampled, want_pick_s = cpu_wall(
+        lambda: orc.lf_select_sampled(ry, ru, rv, mbs, sel_og, osrc, every, True, threads=cores))
+    exact["select"] = pick == want_pick
+    exact["select_sampled"] = pick_s == want_pick_s
+    wscore = orc.lf_score_candidates(ry, ru, rv, mbs, sel_og, osrc, lv, sh, threads=cores)
+    exact["score64"] = bool(np.array_equal(ssd.cpu().numpy().astype(np.uint64), wscore))
+    # bytes a pass must move: the frame read once and written once (384 B
+    # per MB each way) plus the 4-byte MB record; scoring reads the source
+    # once instead of writing
+    frame_bytes = n * (384 + 384 + 4)
+    gbs = frame_bytes / (ms_frame * 1e-3) / 1e9
+    return {
+        "workload": f"1920x1088 reconstruction (8160 MBs) of the bench group's centre frame with 4x4 block steps, "
+                    f"seeded inter MB records; visible {vis_w}x{vis_h}; sampled search every {every} MB rows",
+        "frame": {"ms": ms_frame, "frames_per_s": 1e3 / ms_frame, "mbs_per_s": n / (ms_frame * 1e-3),
+                  "cpu_ms": cpu_frame, "cpu_cores": 1, "gpu_over_cpu": cpu_frame / ms_frame,
+                  "roofline": {"bound": "row wavefront (latency)", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                               "algorithmic_bytes_per_unit": 384 + 384 + 4, "units": n,
+                               "critical_path_mb_steps": bw + 2 * (bh - 1),
+                               "us_per_critical_step": ms_frame * 1e3 / (bw + 2 * (bh - 1))}},
+        "score64": {"ms": ms_score, "candidates": 64, "frames_per_s": 64e3 / ms_score},
+        "select": {"ms": ms_select, "pick": list(pick), "cpu_ms": cpu_select, "cpu_cores": cores,
+                   "gpu_over_cpu": cpu_select / ms_select},
+        "select_sampled": {"ms": ms_sampled, "every_rows": every, "pick": list(pick_s), "cpu_ms": cpu_sampled,
+                           "cpu_cores": cores, "gpu_over_cpu": cpu_sampled / ms_sampled},
+        "bit_exact_vs_oracle": exact,
+    }
+
+
+def bench_loop_filter(args) -> int:
+    import torch
+
+    import vp8g
+    import vp8g.synth as synth
+
+    group = synth.arnr_group_regions()
+    torch.cuda.init()
+    ctx = vp8g.Context(0)
+    cores = max(1, min(os.cpu_count() or 1, 16))
+    r = measure_loop_filter(ctx, group, cores, reps=max(3, args.steps // 5),
+                            cpu_seconds=0.0 if args.no_cpu else 2.0)
+    ctx.close()
+    print(json.dumps({"loop_filter": r}), flush=True)
+    return 0
+
+
 def measure_4k(ctx, reps: int = 3) -> dict:
     """3840x2160 I420, 11-frame group (centre 5), region velocities in
     [-48, 48] px/frame, 40 occluders per frame: exhaustive +-64 and the
@@ -624,11 +806,12 @@ def main() -> int:
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end (host buffers) leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched-clips (c2_batch) and 4K pipeline (c4_pipeline) sub-records")
-    ap.add_argument("--workload", choices=("step", "batch", "4k"), default="step",
+    ap.add_argument("--workload", choices=("step", "batch", "4k", "loop_filter"), default="step",
                     help="step: the 1080p group step (default); batch: 64 uneven clips in one "
                          "persistent graph-captured launch against per-clip launches; 4k: 3840x2160 "
                          "11-frame group, exhaustive +-64 vs learned-seeded search, then half/quarter-pel "
-                         "refinement and six-tap MC of every macroblock")
+                         "refinement and six-tap MC of every macroblock; loop_filter: the section 15 "
+                         "loop filter legs alone")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (5 times), the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counter readings")
@@ -653,6 +836,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "4k":
         return bench_4k(args)
+    if args.workload == "loop_filter":
+        return bench_loop_filter(args)
 
     import torch
 
@@ -858,6 +1043,7 @@ def main() -> int:
     if not args.no_extra:
         cores = max(1, min(os.cpu_count() or 1, 16))
         result["next_rows"] = measure_next_rows(ctx, group, cores)
+        result["next_rows"]["loop_filter"] = measure_loop_filter(ctx, group, cores)
         result["c2_batch"] = measure_batch(ctx, max(5, min(args.steps, 20)), 2)
         c4 = measure_4k(ctx, reps=2)
         c4.pop("report")

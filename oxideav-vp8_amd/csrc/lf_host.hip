@@ -124,7 +124,7 @@ LfArgs base_args(vp8g_ctx* c, const vp8g_lf_planes* p, const vp8g_lf_mb* mbs, co
   return a;
 }
 
-int waves_for(int rows) { return std::min(16, (rows + 1) / 2); }
+int waves_for(int rows) { return std::min(kLfMaxWaves, (rows + 1) / 2); }
 
 int enqueue_frame(vp8g_ctx* c, const vp8g_lf_planes* p, const vp8g_lf_mb* mbs, const vp8g_lf_config* g) {
   int st = ensure_small(c);

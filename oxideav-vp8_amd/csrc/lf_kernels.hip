@@ -58,61 +58,57 @@ __device__ __forceinline__ int c8(int v) { return min(max(v, -128), 127); }
 __device__ __forceinline__ int s2u(int v) { return c8(v) + 128; }
 __device__ __forceinline__ int absd(int a, int b) { return a > b ? a - b : b - a; }
 
+__device__ __forceinline__ uint32_t ad(uint32_t a, uint32_t b) { return __builtin_amdgcn_sad_u16(a, b, 0u); }
+__device__ __forceinline__ uint32_t px(int v) { return (uint32_t)min(max(v, 0), 255); }
+
 // Section 15.3 MB-edge (MB) or sub-block-edge filter on the 8 taps
-// p3 p2 p1 p0 | q0 q1 q2 q3 = bytes of lo then hi (src/loop_filter.rs:292-393).
+// p3 p2 p1 p0 | q0 q1 q2 q3 = bytes of lo then hi (src/loop_filter.rs:292-393),
+// both outcomes computed and selected.  In unsigned pixels: the signed
+// differences of the section 15.2 arithmetic are the unsigned ones, and
+// s2u(u2s(p) + a) = clamp(p + a, 0, 255).
 template <bool MB>
 __device__ __forceinline__ void normal_edge(uint32_t& lo, uint32_t& hi, int I, int E, int T, bool on) {
-  const int p3 = lo & 255, p2 = (lo >> 8) & 255, p1 = (lo >> 16) & 255, p0 = lo >> 24;
-  const int q0 = hi & 255, q1 = (hi >> 8) & 255, q2 = (hi >> 16) & 255, q3 = hi >> 24;
-  const int m = max(max(max(absd(p3, p2), absd(p2, p1)), max(absd(p1, p0), absd(q3, q2))),
-                    max(absd(q2, q1), absd(q1, q0)));
-  const bool yes = on && absd(p0, q0) * 2 + (absd(p1, q1) >> 1) <= E && m <= I;
-  const bool hv = max(absd(p1, p0), absd(q1, q0)) > T;
-  const int sp2 = p2 - 128, sp1 = p1 - 128, sp0 = p0 - 128, sq0 = q0 - 128, sq1 = q1 - 128, sq2 = q2 - 128;
-  const int outer = c8(sp1 - sq1), step = 3 * (sq0 - sp0);
-  int np2 = p2, np1 = p1, np0, nq0, nq1 = q1, nq2 = q2;
+  const uint32_t p3 = lo & 255, p2 = (lo >> 8) & 255, p1 = (lo >> 16) & 255, p0 = lo >> 24;
+  const uint32_t q0 = hi & 255, q1 = (hi >> 8) & 255, q2 = (hi >> 16) & 255, q3 = hi >> 24;
+  const uint32_t dp = ad(p1, p0), dq = ad(q1, q0);
+  const uint32_t m = max(max(max(ad(p3, p2), ad(p2, p1)), dp), max(max(ad(q3, q2), ad(q2, q1)), dq));
+  const bool yes = on && ad(p0, q0) * 2 + (ad(p1, q1) >> 1) <= (uint32_t)E && m <= (uint32_t)I;
+  const bool hv = max(dp, dq) > (uint32_t)T;
+  const int P0 = p0, P1 = p1, P2 = p2, Q0 = q0, Q1 = q1, Q2 = q2;
+  const int outer = c8(P1 - Q1), step = 3 * (Q0 - P0);
+  uint32_t np0, nq0, np1, nq1, np2 = p2, nq2 = q2;
   if (MB) {
     const int w = c8(outer + step);
-    if (!hv) {
-      const int a0 = c8((27 * w + 63) >> 7), a1 = c8((18 * w + 63) >> 7), a2 = c8((9 * w + 63) >> 7);
-      np0 = s2u(sp0 + a0);
-      nq0 = s2u(sq0 - a0);
-      np1 = s2u(sp1 + a1);
-      nq1 = s2u(sq1 - a1);
-      np2 = s2u(sp2 + a2);
-      nq2 = s2u(sq2 - a2);
-    } else {
-      np0 = s2u(sp0 + (c8(w + 3) >> 3));
-      nq0 = s2u(sq0 - (c8(w + 4) >> 3));
-    }
+    const int a0 = c8((27 * w + 63) >> 7), a1 = c8((18 * w + 63) >> 7), a2 = c8((9 * w + 63) >> 7);
+    const int fa = c8(w + 4) >> 3, fb = c8(w + 3) >> 3;
+    np0 = px(P0 + (hv ? fb : a0));
+    nq0 = px(Q0 - (hv ? fa : a0));
+    np1 = hv ? p1 : px(P1 + a1);
+    nq1 = hv ? q1 : px(Q1 - a1);
+    np2 = hv ? p2 : px(P2 + a2);
+    nq2 = hv ? q2 : px(Q2 - a2);
   } else {
     const int a = c8((hv ? outer : 0) + step);
-    const int f1 = c8(a + 4) >> 3, f2 = c8(a + 3) >> 3;
-    np0 = s2u(sp0 + f2);
-    nq0 = s2u(sq0 - f1);
-    if (!hv) {
-      const int h = (f1 + 1) >> 1;
-      np1 = s2u(sp1 + h);
-      nq1 = s2u(sq1 - h);
-    }
+    const int f1 = c8(a + 4) >> 3, f2 = c8(a + 3) >> 3, h = (f1 + 1) >> 1;
+    np0 = px(P0 + f2);
+    nq0 = px(Q0 - f1);
+    np1 = hv ? p1 : px(P1 + h);
+    nq1 = hv ? q1 : px(Q1 - h);
   }
-  if (yes) {
-    lo = (uint32_t)p3 | (uint32_t)np2 << 8 | (uint32_t)np1 << 16 | (uint32_t)np0 << 24;
-    hi = (uint32_t)nq0 | (uint32_t)nq1 << 8 | (uint32_t)nq2 << 16 | (uint32_t)q3 << 24;
-  }
+  const uint32_t nlo = p3 | np2 << 8 | np1 << 16 | np0 << 24, nhi = nq0 | nq1 << 8 | nq2 << 16 | q3 << 24;
+  lo = yes ? nlo : lo;
+  hi = yes ? nhi : hi;
 }
 
 // Section 15.2 simple filter on p1 p0 | q0 q1 = the top two bytes of lo and
 // the bottom two of hi (src/loop_filter.rs:200-245).
 __device__ __forceinline__ void simple_edge(uint32_t& lo, uint32_t& hi, int E, bool on) {
-  const int p1 = (lo >> 16) & 255, p0 = lo >> 24, q0 = hi & 255, q1 = (hi >> 8) & 255;
-  const bool yes = on && absd(p0, q0) * 2 + (absd(p1, q1) >> 1) <= E;
-  const int a = c8(c8(p1 - q1) + 3 * (q0 - p0));
-  const int np0 = s2u(p0 - 128 + (c8(a + 3) >> 3)), nq0 = s2u(q0 - 128 - (c8(a + 4) >> 3));
-  if (yes) {
-    lo = (lo & 0x00FFFFFFu) | (uint32_t)np0 << 24;
-    hi = (hi & 0xFFFFFF00u) | (uint32_t)nq0;
-  }
+  const uint32_t p1 = (lo >> 16) & 255, p0 = lo >> 24, q0 = hi & 255, q1 = (hi >> 8) & 255;
+  const bool yes = on && ad(p0, q0) * 2 + (ad(p1, q1) >> 1) <= (uint32_t)E;
+  const int a = c8(c8((int)p1 - (int)q1) + 3 * ((int)q0 - (int)p0));
+  const uint32_t np0 = px((int)p0 + (c8(a + 3) >> 3)), nq0 = px((int)q0 - (c8(a + 4) >> 3));
+  lo = yes ? (lo & 0x00FFFFFFu) | np0 << 24 : lo;
+  hi = yes ? (hi & 0xFFFFFF00u) | nq0 : hi;
 }
 
 // calculate_mb_filter_level_inter, src/loop_filter.rs:1280-1334 (with an
@@ -151,38 +147,89 @@ __device__ __forceinline__ uint32_t ssd4(uint32_t a, uint32_t b, uint32_t keep) 
   return s;
 }
 
-// One output dword: strip row `row`, dword `dw` of the tile (plane column
-// x0 - 4 + 4 dw), for plane `pl`.  Frame mode stores it; score mode adds its
-// squared error against the source inside the visible window.
-template <bool SCORE>
-__device__ __forceinline__ void emit(const LfArgs& a, const uint8_t* strip, int pl, int row, int dw, int x0,
-                                     int y0, uint32_t& acc) {
-  const int S = pl == 0 ? 32 : 16;
-  const int x = x0 - 4 + 4 * dw, y = y0 - 4 + row;
-  const uint32_t v = *reinterpret_cast<const uint32_t*>(strip + row * S + (x & (S - 1)));
-  if (!SCORE) {
-    uint8_t* base = pl == 0 ? a.oy : (pl == 1 ? a.ou : a.ov);
-    const size_t stride = pl == 0 ? a.y_stride : a.uv_stride;
-    *reinterpret_cast<uint32_t*>(base + (size_t)y * stride + x) = v;
+// Output pass q of a lane (one dword of the final pixels of a step):
+// q = 0, 1: luma strip rows 1..16 by dwords 0..3; q = 2: chroma strip rows
+// 1..8 by dwords 0..1 of U (lanes 0-15) or V (lanes 16-31).
+__device__ __forceinline__ void out_entry(int li, int q, int& epl, int& row, int& dw) {
+  if (q < 2) {
+    epl = 0, row = 1 + (li >> 2) + 8 * q, dw = li & 3;
   } else {
-    const int vw = pl == 0 ? a.vis_w : a.vis_cw, vh = pl == 0 ? a.vis_h : a.vis_ch;
-    if (y >= vh || x >= vw) return;
-    const uint8_t* sb = pl == 0 ? a.sy : (pl == 1 ? a.su : a.sv);
-    const uint8_t* sp = sb + (size_t)y * (pl == 0 ? a.sy_stride : a.suv_stride) + x;
-    uint32_t s;
-    if (x + 4 <= vw && a.src_aligned4) {
-      s = *reinterpret_cast<const uint32_t*>(sp);
-    } else {
-      s = 0;
-      for (int j = 0; j < 4 && x + j < vw; j++) s |= (uint32_t)sp[j] << (8 * j);
-    }
-    const uint32_t keep = x + 4 <= vw ? 0xFu : ((1u << (vw - x)) - 1u);
-    acc += ssd4(v, s, keep);
+    epl = li < 16 ? 1 : 2, row = 1 + ((li & 15) >> 1), dw = li & 1;
   }
 }
 
+__device__ __forceinline__ int strip_off(int pl) { return pl == 0 ? 0 : (pl == 1 ? 640 : 832); }
+
+// Plane base pointers as uniform values: the kernels address planes as
+// base + 32-bit offset (a plane is below 4 GiB), with one instruction per
+// plane under lane masks so that the base stays in scalar registers.
+__device__ __forceinline__ const uint8_t* src_plane(const LfArgs& a, int pl) { return pl == 0 ? a.sy : (pl == 1 ? a.su : a.sv); }
+
+// The source dword at plane (x, y) for scoring, 0 outside the visible
+// window; bytes past its right edge are not read.
+__device__ __forceinline__ uint32_t src_word(const LfArgs& a, int pl, int x, int y) {
+  const int vw = pl == 0 ? a.vis_w : a.vis_cw, vh = pl == 0 ? a.vis_h : a.vis_ch;
+  if (y >= vh || x >= vw) return 0;
+  const uint32_t off = (uint32_t)y * (uint32_t)(pl == 0 ? a.sy_stride : a.suv_stride) + (uint32_t)x;
+  const bool whole = x + 4 <= vw && a.src_aligned4;
+  uint32_t s = 0;
+  if (whole) {
+    if (pl == 0) s = *reinterpret_cast<const uint32_t*>(a.sy + off);
+    if (pl == 1) s = *reinterpret_cast<const uint32_t*>(a.su + off);
+    if (pl == 2) s = *reinterpret_cast<const uint32_t*>(a.sv + off);
+  } else {
+    const uint8_t* sp = src_plane(a, pl) + off;
+    for (int j = 0; j < 4 && x + j < vw; j++) s |= (uint32_t)sp[j] << (8 * j);
+  }
+  return s;
+}
+
+// Squared error of the 4 bytes of v against the source word, inside the
+// visible window.
+__device__ __forceinline__ uint32_t src_ssd(const LfArgs& a, int pl, int x, int y, uint32_t v, uint32_t sw) {
+  const int vw = pl == 0 ? a.vis_w : a.vis_cw, vh = pl == 0 ? a.vis_h : a.vis_ch;
+  if (y >= vh || x >= vw) return 0;
+  return ssd4(v, sw, x + 4 <= vw ? 0xFu : ((1u << (vw - x)) - 1u));
+}
+
+__device__ __forceinline__ void store_out(const LfArgs& a, int pl, int x, int y, uint32_t v) {
+  const uint32_t off = (uint32_t)y * (uint32_t)(pl == 0 ? a.y_stride : a.uv_stride) + (uint32_t)x;
+  if (pl == 0) *reinterpret_cast<uint32_t*>(a.oy + off) = v;
+  if (pl == 1) *reinterpret_cast<uint32_t*>(a.ou + off) = v;
+  if (pl == 2) *reinterpret_cast<uint32_t*>(a.ov + off) = v;
+}
+
+// One final dword straight from the strip (the rare right / bottom margins).
+template <bool SCORE>
+__device__ __forceinline__ void emit(const LfArgs& a, const uint8_t* hs, int pl, int row, int dw, int x0, int y0,
+                                     uint32_t& acc) {
+  const int S = pl == 0 ? 32 : 16;
+  const int x = x0 - 4 + 4 * dw, y = y0 - 4 + row;
+  const uint32_t v = *reinterpret_cast<const uint32_t*>(hs + strip_off(pl) + row * S + (x & (S - 1)));
+  if (!SCORE)
+    store_out(a, pl, x, y, v);
+  else
+    acc += src_ssd(a, pl, x, y, v, src_word(a, pl, x, y));
+}
+
+// The n pristine bytes of one MB row of plane pl at offset `off` (chroma
+// in .x .y).
+__device__ __forceinline__ uint4 load_mb_row(const LfArgs& a, int pl, uint32_t off) {
+  uint4 m = make_uint4(0, 0, 0, 0);
+  if (pl == 0) m = *reinterpret_cast<const uint4*>(a.y + off);
+  if (pl == 1) {
+    const uint2 v = *reinterpret_cast<const uint2*>(a.u + off);
+    m.x = v.x, m.y = v.y;
+  }
+  if (pl == 2) {
+    const uint2 v = *reinterpret_cast<const uint2*>(a.v + off);
+    m.x = v.x, m.y = v.y;
+  }
+  return m;
+}
+
 template <bool SIMPLE, bool SCORE>
-__global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
+__global__ void __launch_bounds__(64 * kLfMaxWaves) lf_kernel(LfArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int half = lane >> 5, li = lane & 31;
@@ -198,6 +245,8 @@ __global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
   const int base_level = a.cand_level[cand], sharp = a.cand_sharp[cand];
   const bool skip = SCORE && base_level == 0;  // level 0 scores the unfiltered picture
   const int P = (rows + 1) >> 1;
+  // interior limit of a level (src/loop_filter.rs:99-112) depends on it and the sharpness only
+  const int il_shift = sharp ? (sharp > 4 ? 2 : 1) : 0, il_cap = sharp ? (sharp >= 9 ? 0 : 9 - sharp) : 255;
 
   int* prog = reinterpret_cast<int*>(lds + nw * kWaveLds);
   int* abort_flag = prog + (nw + 1);
@@ -208,14 +257,15 @@ __global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
   __syncthreads();
 
   uint8_t* wl = lds + wave * kWaveLds;
-  uint8_t* strip = wl + half * kStripBytes + (pl == 0 ? 0 : (pl == 1 ? 640 : 832));
+  uint8_t* hs = wl + half * kStripBytes;
+  uint8_t* strip = hs + strip_off(pl);
   uint8_t* ringa = wl + 2 * kStripBytes;
   uint8_t* ringb = a.ring + (size_t)blockIdx.x * a.ring_instance_bytes;
-  const uint8_t* src = pl == 0 ? a.y : (pl == 1 ? a.u : a.v);
-  const size_t stride = pl == 0 ? a.y_stride : a.uv_stride;
-  const int rows_here = n + 4;
+  const uint32_t stride = (uint32_t)(pl == 0 ? a.y_stride : a.uv_stride);
   // ring dword li <-> (plane, row m in 0..3, dword j) of a slot
   const int rm = pl == 0 ? li >> 2 : ((li & 7) >> 1), rj = pl == 0 ? (li & 3) : (li & 1);
+  // the strip row read down a column, clamped into this plane's strip
+  const int hrow_last = n + 3;
 
   unsigned long long total = 0;
   bool aborted = false;
@@ -224,7 +274,16 @@ __global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
     const bool row_ok = r < rows;
     const bool has_below = r + 1 < rows;     // this row's ring has a consumer
     const int y0 = (row0 + r) * n;
+    const uint32_t* mbrow = a.mbs + (size_t)(row0 + r) * C;
+    const uint32_t grow = (uint32_t)(y0 + k) * stride;  // offset of this lane's pixel row
     uint32_t acc = 0;
+    // this half's current MB: pristine row k and the MB record, one column ahead
+    uint4 cur = make_uint4(0, 0, 0, 0), nxt = cur;
+    uint32_t mbc = 0, mbn = 0;
+    if (row_ok && half == 0) {
+      cur = load_mb_row(a, pl, grow);
+      mbc = mbrow[0];
+    }
     for (int s = 0; s < C + 2; s++) {
       // -- wait for the row above when it lives in another wave
       if (p > 0 && s < C) {
@@ -250,49 +309,42 @@ __global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
       const int x0 = c * n;
       wave_sync();
 
-      // -- 1. the 4 rows above from the ring of the row above
-      if (act && r > 0) {
-        uint32_t v;
-        if (half)
-          v = *reinterpret_cast<const uint32_t*>(ringa + (c % kRingA) * kSlotBytes + 4 * li);
-        else
-          v = *reinterpret_cast<const uint32_t*>(ringb + ((size_t)((p - 1) % (nw + 1)) * C + c) * kSlotBytes +
-                                                 4 * li);
-        *reinterpret_cast<uint32_t*>(strip + rm * S + ((x0 + 4 * rj) & (S - 1))) = v;
+      // -- A. loads for later in this step and for the next one
+      uint32_t ringv = 0;
+      if (act && r > 0 && half == 0)  // the row above is the previous pair's lower half
+        ringv = *reinterpret_cast<const uint32_t*>(ringb + (uint32_t)((((p - 1) % (nw + 1)) * C + c) * kSlotBytes +
+                                                                   4 * li));
+      if (row_ok && c + 1 >= 0 && c + 1 < C) {
+        nxt = load_mb_row(a, pl, grow + (uint32_t)(x0 + n));
+        mbn = mbrow[c + 1];
+      }
+      uint32_t sw[3] = {0, 0, 0};
+      if (SCORE && act) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          int epl, row, dw;
+          out_entry(li, q, epl, row, dw);
+          const int en = epl == 0 ? 16 : 8;
+          if ((row >= 4 || r > 0) && (dw > 0 || c > 0))
+            sw[q] = src_word(a, epl, c * en - 4 + 4 * dw, (row0 + r) * en - 4 + row);
+        }
       }
 
-      // -- this half's MB: level and limits
-      int level = 0;
-      bool sub = false;
-      if (act) {
-        const uint32_t mb = a.mbs[(size_t)(row0 + r) * C + c];
-        level = skip ? 0 : mb_level(a, base_level, mb);
-        sub = (mb >> 24) & 3;
-      }
-      int il = level;
-      if (sharp) {
-        il >>= sharp > 4 ? 2 : 1;
-        il = min(il, sharp >= 9 ? 0 : 9 - sharp);
-      }
-      il = max(il, 1);
+      // -- B. this half's MB: level and limits
+      const int level = act && !skip ? mb_level(a, base_level, mbc) : 0;
+      const bool sub = (mbc >> 24) & 3;
+      const int il = max(min(level >> il_shift, il_cap), 1);
       const int hev = a.key_frame ? (level >= 40 ? 2 : (level >= 15 ? 1 : 0))
                                   : (level >= 40 ? 3 : (level >= 20 ? 2 : (level >= 15 ? 1 : 0)));
       const int mbe = min((level + 2) * 2 + il, 255), sbe = min(level * 2 + il, 255);
       const bool lanes_on = level > 0 && (!SIMPLE || pl == 0);
 
-      // -- 2. vertical edges along this lane's row (row k of the MB)
+      // -- C. vertical edges along this lane's row (row k of the MB)
       if (act) {
         uint32_t d[5];
         uint8_t* rowp = strip + (4 + k) * S;
         d[0] = *reinterpret_cast<const uint32_t*>(rowp + ((x0 - 4) & (S - 1)));
-        const uint8_t* g = src + (size_t)(y0 + k) * stride + x0;
-        if (pl == 0) {
-          const uint4 m = *reinterpret_cast<const uint4*>(g);
-          d[1] = m.x, d[2] = m.y, d[3] = m.z, d[4] = m.w;
-        } else {
-          const uint2 m = *reinterpret_cast<const uint2*>(g);
-          d[1] = m.x, d[2] = m.y, d[3] = 0, d[4] = 0;
-        }
+        d[1] = cur.x, d[2] = cur.y, d[3] = cur.z, d[4] = cur.w;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           const bool on = lanes_on && (e == 0 ? c > 0 : sub) && (e < n / 4);
@@ -308,26 +360,31 @@ __global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
           *reinterpret_cast<uint4*>(rowp + (x0 & (S - 1))) = make_uint4(d[1], d[2], d[3], d[4]);
         else
           *reinterpret_cast<uint2*>(rowp + (x0 & (S - 1))) = make_uint2(d[1], d[2]);
+        // -- D. the 4 rows above from the ring of the row above
+        if (r > 0) {
+          if (half)
+            ringv = *reinterpret_cast<const uint32_t*>(ringa + (c % kRingA) * kSlotBytes + 4 * li);
+          *reinterpret_cast<uint32_t*>(strip + rm * S + ((x0 + 4 * rj) & (S - 1))) = ringv;
+        }
       }
       wave_sync();
 
-      // -- 3. publish this row's ring slot c-1 (bottom 4 rows, MB c-1)
+      // -- E. publish this row's ring slot c-1 (bottom 4 rows, MB c-1)
       if (act && has_below && c > 0) {
         const uint32_t v =
             *reinterpret_cast<const uint32_t*>(strip + (n + rm) * S + ((x0 - n + 4 * rj) & (S - 1)));
         if (half == 0)
           *reinterpret_cast<uint32_t*>(ringa + ((c - 1) % kRingA) * kSlotBytes + 4 * li) = v;
         else
-          *reinterpret_cast<uint32_t*>(ringb + ((size_t)(p % (nw + 1)) * C + (c - 1)) * kSlotBytes + 4 * li) = v;
+          *reinterpret_cast<uint32_t*>(ringb + (uint32_t)(((p % (nw + 1)) * C + (c - 1)) * kSlotBytes + 4 * li)) = v;
       }
 
-      // -- 4. horizontal edges down this lane's column (column k of the MB)
+      // -- F. horizontal edges down this lane's column (column k of the MB)
       if (act) {
         uint8_t* colp = strip + ((x0 + k) & (S - 1));
         uint32_t d[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-        for (int i = 0; i < 20; i++)
-          if (i < rows_here) d[i >> 2] |= (uint32_t)colp[i * S] << (8 * (i & 3));
+        for (int i = 0; i < 20; i++) d[i >> 2] |= (uint32_t)colp[min(i, hrow_last) * S] << (8 * (i & 3));
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           const bool on = lanes_on && (e == 0 ? r > 0 : sub) && (e < n / 4);
@@ -338,33 +395,49 @@ __global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
           else
             normal_edge<false>(d[e], d[e + 1], il, sbe, hev, on);
         }
+        // rows 1..n+1 can change; chroma lanes repeat their row 9 past it
 #pragma unroll
-        for (int i = 1; i < 18; i++)
-          if (i < n + 2) colp[i * S] = (uint8_t)(d[i >> 2] >> (8 * (i & 3)));
+        for (int i = 1; i < 18; i++) {
+          const uint8_t vb = (i < 10 || pl == 0) ? (uint8_t)(d[i >> 2] >> (8 * (i & 3))) : (uint8_t)(d[2] >> 8);
+          colp[((i < 10 || pl == 0) ? i : 9) * S] = vb;
+        }
       }
       wave_sync();
 
-      // -- 5. final pixels: strip rows 1..n by dwords 0..n/4-1
+      // -- G. the last column's ring slot is final now
+      const bool last_col = c == C - 1;
+      if (act && last_col && has_below) {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(strip + (n + rm) * S + ((x0 + 4 * rj) & (S - 1)));
+        if (half == 0)
+          *reinterpret_cast<uint32_t*>(ringa + (c % kRingA) * kSlotBytes + 4 * li) = v;
+        else
+          *reinterpret_cast<uint32_t*>(ringb + (uint32_t)(((p % (nw + 1)) * C + c) * kSlotBytes + 4 * li)) = v;
+      }
+      // -- H. progress of this pair (every lane stores the same value)
+      __hip_atomic_store(prog + p % (nw + 1), p * 65536 + s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+
+      // -- I. final pixels: strip rows 1..n by dwords 0..n/4-1, plus the
+      //    right / bottom margins at the instance's last column / row
       if (act) {
-        const bool last_row = r == rows - 1, last_col = c == C - 1;
-        // luma: two passes of 8 rows x 4 dwords; chroma: 8 rows x 2 dwords per plane
+        const bool last_row = r == rows - 1;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
           int epl, row, dw;
-          if (q < 2) {
-            epl = 0, row = 1 + (li >> 2) + 8 * q, dw = li & 3;
-          } else {
-            epl = li < 16 ? 1 : 2, row = 1 + ((li & 15) >> 1), dw = li & 1;
-          }
+          out_entry(li, q, epl, row, dw);
           if ((row >= 4 || r > 0) && (dw > 0 || c > 0)) {
-            const int en = epl == 0 ? 16 : 8;
-            uint8_t* es = wl + half * kStripBytes + (epl == 0 ? 0 : (epl == 1 ? 640 : 832));
-            emit<SCORE>(a, es, epl, row, dw, c * en, (row0 + r) * en, acc);
+            const int en = epl == 0 ? 16 : 8, Se = 2 * en;
+            const int x = c * en - 4 + 4 * dw, y = (row0 + r) * en - 4 + row;
+            const uint32_t v = *reinterpret_cast<const uint32_t*>(hs + strip_off(epl) + row * Se + (x & (Se - 1)));
+            if (!SCORE) {
+              store_out(a, epl, x, y, v);
+            } else {
+              acc += src_ssd(a, epl, x, y, v, sw[q]);
+            }
           }
         }
         if (last_col) {  // right margin: dword n/4 of rows 1..n
           const int row = 1 + k;
-          if (row >= 4 || r > 0) emit<SCORE>(a, strip, pl, row, n / 4, x0, y0, acc);
+          if (row >= 4 || r > 0) emit<SCORE>(a, hs, pl, row, n / 4, x0, y0, acc);
         }
         if (last_row) {  // bottom margin: rows n+1..n+3
           // luma 3 rows x 4 dwords (lanes 0-11), chroma 3 x 2 per plane (12-17, 18-23)
@@ -373,28 +446,16 @@ __global__ void __launch_bounds__(1024) lf_kernel(LfArgs a) {
             const int e = li < 12 ? li : (li < 18 ? li - 12 : li - 18);
             const int en = epl == 0 ? 16 : 8, nd = en / 4;
             const int row = en + 1 + e / nd, dw = e % nd;
-            if (dw > 0 || c > 0) {
-              uint8_t* es = wl + half * kStripBytes + (epl == 0 ? 0 : (epl == 1 ? 640 : 832));
-              emit<SCORE>(a, es, epl, row, dw, c * en, (row0 + r) * en, acc);
-            }
+            if (dw > 0 || c > 0) emit<SCORE>(a, hs, epl, row, dw, c * en, (row0 + r) * en, acc);
           }
           if (last_col && li < 9) {  // bottom-right corner dword
             const int epl = li / 3, en = epl == 0 ? 16 : 8;
-            uint8_t* es = wl + half * kStripBytes + (epl == 0 ? 0 : (epl == 1 ? 640 : 832));
-            emit<SCORE>(a, es, epl, en + 1 + li % 3, en / 4, c * en, (row0 + r) * en, acc);
+            emit<SCORE>(a, hs, epl, en + 1 + li % 3, en / 4, c * en, (row0 + r) * en, acc);
           }
         }
-        // the last column's ring slot is final now
-        if (last_col && has_below) {
-          const uint32_t v = *reinterpret_cast<const uint32_t*>(strip + (n + rm) * S + ((x0 + 4 * rj) & (S - 1)));
-          if (half == 0)
-            *reinterpret_cast<uint32_t*>(ringa + (c % kRingA) * kSlotBytes + 4 * li) = v;
-          else
-            *reinterpret_cast<uint32_t*>(ringb + ((size_t)(p % (nw + 1)) * C + c) * kSlotBytes + 4 * li) = v;
-        }
       }
-      // -- progress of this pair (every lane stores the same value)
-      __hip_atomic_store(prog + p % (nw + 1), p * 65536 + s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+      cur = nxt;
+      mbc = mbn;
     }
     total += acc;
   }

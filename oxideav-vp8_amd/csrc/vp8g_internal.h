@@ -277,6 +277,7 @@ hipError_t launch_refine_ladder(const RefineArgs& a, hipStream_t stream);
 // Loop filter: one workgroup per instance (candidate x band); see
 // lf_kernels.hip.
 constexpr int kLfMaxCands = 256;
+constexpr int kLfMaxWaves = 12;  // waves per instance workgroup (168 VGPRs each)
 struct LfArgs {
   const uint8_t* y;              // reconstruction (read)
   const uint8_t* u;
