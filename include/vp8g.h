@@ -587,8 +587,9 @@ typedef struct {
 
 /* KeyframePlanes (src/frame.rs): the MB-padded reconstruction, mb_rows*16
  * rows of y_stride >= mb_cols*16 bytes, mb_rows*8 rows of uv_stride >=
- * mb_cols*8 for u and v; at most 1024 x 1024 MBs.  In device memory y and
- * y_stride are multiples of 16, u, v and uv_stride multiples of 8. */
+ * mb_cols*8 for u and v; at most 1024 x 1024 MBs and below 4 GiB per
+ * plane.  In device memory y and y_stride are multiples of 16, u, v and
+ * uv_stride multiples of 8. */
 typedef struct {
     uint8_t *y, *u, *v;
     size_t y_stride, uv_stride, mb_cols, mb_rows;

@@ -80,6 +80,8 @@ int check_planes(const vp8g_lf_planes* p, bool device) {
   if (!p->y || !p->u || !p->v || p->mb_cols > 1024 || p->mb_rows > 1024) return VP8G_E_ARG;
   if (p->y_stride < p->mb_cols * 16 || p->uv_stride < p->mb_cols * 8) return VP8G_E_ARG;
   if (p->y_stride > 0x7FFFFFFF || p->uv_stride > 0x7FFFFFFF) return VP8G_E_ARG;
+  // the kernels address a plane with 32-bit offsets
+  if (p->y_stride * p->mb_rows * 16 > 0xFFFFFFFFull || p->uv_stride * p->mb_rows * 8 > 0xFFFFFFFFull) return VP8G_E_ARG;
   if (device && (((uintptr_t)p->y | p->y_stride) & 15)) return VP8G_E_ARG;
   if (device && (((uintptr_t)p->u | (uintptr_t)p->v | p->uv_stride) & 7)) return VP8G_E_ARG;
   return VP8G_OK;
@@ -98,6 +100,8 @@ int check_source(const vp8g_lf_source* s, const vp8g_lf_planes* p) {
   if (s->width == 0 || s->height == 0) return VP8G_OK;
   if (!s->y || !s->u || !s->v || s->y_stride < s->width || s->uv_stride < (s->width + 1) / 2) return VP8G_E_ARG;
   if (s->y_stride > 0x7FFFFFFF || s->uv_stride > 0x7FFFFFFF) return VP8G_E_ARG;
+  if (s->y_stride * s->height > 0xFFFFFFFFull || s->uv_stride * ((s->height + 1) / 2) > 0xFFFFFFFFull)
+    return VP8G_E_ARG;
   return VP8G_OK;
 }
 

@@ -589,3 +589,239 @@ pub fn luma_mad_batch(prev: &[I420Frame<'_>], cur: &[I420Frame<'_>]) -> Result<V
     })?;
     Ok(mad)
 }
+
+// ---------------------------------------------------------------- loop filter
+
+/// `FrameFilterConfig` (src/loop_filter.rs:891-944).
+#[derive(Debug, Clone, Copy, Default, PartialEq, Eq)]
+pub struct FrameFilterConfig {
+    pub simple: bool,
+    pub key_frame: bool,
+    pub loop_filter_level: u8,
+    pub sharpness_level: u8,
+    pub segmentation_enabled: bool,
+    pub segment_abs: bool,
+    pub segment_lf_level: [i16; 4],
+    pub delta_enabled: bool,
+    pub ref_delta_current: i16,
+    pub bpred_mode_delta: i16,
+    pub ref_delta_last: i16,
+    pub ref_delta_golden: i16,
+    pub ref_delta_altref: i16,
+    pub zero_mv_mode_delta: i16,
+    pub other_mv_mode_delta: i16,
+    pub split_mv_mode_delta: i16,
+}
+
+impl FrameFilterConfig {
+    fn c(&self) -> sys::vp8g_lf_config {
+        sys::vp8g_lf_config {
+            simple: self.simple as i32,
+            key_frame: self.key_frame as i32,
+            loop_filter_level: self.loop_filter_level as i32,
+            sharpness_level: self.sharpness_level as i32,
+            segmentation_enabled: self.segmentation_enabled as i32,
+            segment_abs: self.segment_abs as i32,
+            delta_enabled: self.delta_enabled as i32,
+            segment_lf_level: self.segment_lf_level,
+            // FrameFilterConfig::ref_deltas / mode_deltas (src/loop_filter.rs:1193-1209)
+            ref_delta: [self.ref_delta_current, self.ref_delta_last, self.ref_delta_golden, self.ref_delta_altref],
+            mode_delta: [self.bpred_mode_delta, self.zero_mv_mode_delta, self.other_mv_mode_delta,
+                         self.split_mv_mode_delta],
+        }
+    }
+}
+
+/// `KeyframePlanes` (src/frame.rs): the MB-padded reconstruction.
+#[derive(Debug, Clone, PartialEq, Eq)]
+pub struct KeyframePlanes {
+    pub y: Vec<u8>,
+    pub u: Vec<u8>,
+    pub v: Vec<u8>,
+    pub y_stride: usize,
+    pub uv_stride: usize,
+    pub mb_cols: usize,
+    pub mb_rows: usize,
+}
+
+/// `SourcePlanes` (src/loop_filter.rs:1916-1932).
+#[derive(Debug, Clone, Copy)]
+pub struct SourcePlanes<'a> {
+    pub width: usize,
+    pub height: usize,
+    pub y: &'a [u8],
+    pub u: &'a [u8],
+    pub v: &'a [u8],
+    pub y_stride: usize,
+    pub uv_stride: usize,
+}
+
+/// `motion_comp::RefFrame` of an inter macroblock.
+#[derive(Debug, Clone, Copy, PartialEq, Eq)]
+pub enum RefFrame {
+    Last,
+    Golden,
+    AltRef,
+}
+
+/// `near_mv::InterMode` of an inter macroblock.
+#[derive(Debug, Clone, Copy, PartialEq, Eq)]
+pub enum InterMode {
+    Nearest,
+    Near,
+    Zero,
+    New,
+    Split,
+}
+
+/// What the section 15 pass reads of one macroblock: `MacroblockModes::segment_id`,
+/// `y_mode == B_PRED` (SPLITMV macroblocks are recorded as B_PRED,
+/// src/loop_filter.rs:1301-1308), `mb_has_coeffs`, and for an inter
+/// macroblock its reference frame and inter mode.
+#[derive(Debug, Clone, Copy, Default, PartialEq, Eq)]
+pub struct MbFilterInfo {
+    pub segment_id: Option<u8>,
+    pub bpred: bool,
+    pub has_coeffs: bool,
+    pub inter: Option<(RefFrame, InterMode)>,
+}
+
+fn lf_records(mbs: &[MbFilterInfo]) -> Vec<sys::vp8g_lf_mb> {
+    mbs.iter()
+        .map(|m| {
+            let (ref_frame, inter_mode) = match m.inter {
+                None => (0u8, 0u8),
+                Some((r, i)) => (
+                    match r {
+                        RefFrame::Last => 1,
+                        RefFrame::Golden => 2,
+                        RefFrame::AltRef => 3,
+                    },
+                    match i {
+                        InterMode::Zero => sys::VP8G_LF_ZEROMV as u8,
+                        InterMode::Split => sys::VP8G_LF_SPLITMV as u8,
+                        _ => 3,
+                    },
+                ),
+            };
+            sys::vp8g_lf_mb {
+                segment_id: m.segment_id.unwrap_or(sys::VP8G_LF_NO_SEGMENT as u8),
+                ref_frame,
+                inter_mode,
+                flags: (if m.bpred { sys::VP8G_LF_BPRED as u8 } else { 0 })
+                    | (if m.has_coeffs { sys::VP8G_LF_HAS_COEFFS as u8 } else { 0 }),
+            }
+        })
+        .collect()
+}
+
+fn check_lf_planes(p: &KeyframePlanes) {
+    let (c, r) = (p.mb_cols, p.mb_rows);
+    assert!(p.y_stride >= c * 16 && p.uv_stride >= c * 8, "plane stride shorter than the MB columns");
+    assert!(p.y.len() >= plane_span(p.y_stride, c * 16, r * 16), "luma plane shorter than its MB rows");
+    assert!(p.u.len() >= plane_span(p.uv_stride, c * 8, r * 8) && p.v.len() >= plane_span(p.uv_stride, c * 8, r * 8),
+            "chroma plane shorter than its MB rows");
+}
+
+fn lf_planes(p: &mut KeyframePlanes) -> sys::vp8g_lf_planes {
+    sys::vp8g_lf_planes {
+        y: p.y.as_mut_ptr(),
+        u: p.u.as_mut_ptr(),
+        v: p.v.as_mut_ptr(),
+        y_stride: p.y_stride,
+        uv_stride: p.uv_stride,
+        mb_cols: p.mb_cols,
+        mb_rows: p.mb_rows,
+    }
+}
+
+fn lf_source(s: &SourcePlanes<'_>) -> sys::vp8g_lf_source {
+    let (cw, ch) = (s.width.div_ceil(2), s.height.div_ceil(2));
+    assert!(s.y.len() >= plane_span(s.y_stride, s.width, s.height), "source luma shorter than its window");
+    assert!(s.u.len() >= plane_span(s.uv_stride, cw, ch) && s.v.len() >= plane_span(s.uv_stride, cw, ch),
+            "source chroma shorter than its window");
+    sys::vp8g_lf_source {
+        width: s.width,
+        height: s.height,
+        y: s.y.as_ptr(),
+        u: s.u.as_ptr(),
+        v: s.v.as_ptr(),
+        y_stride: s.y_stride,
+        uv_stride: s.uv_stride,
+    }
+}
+
+/// `filter_inter_frame_flags` (src/loop_filter.rs:1364-1496): the frame
+/// filtered in place, byte for byte the raster pass.  Like the reference, a
+/// record count that is not mb_cols * mb_rows leaves the planes alone.
+pub fn filter_inter_frame_flags(planes: &mut KeyframePlanes, mbs: &[MbFilterInfo], config: &FrameFilterConfig)
+                                -> Result<(), DeviceError> {
+    if planes.mb_cols == 0 || planes.mb_rows == 0 || mbs.len() != planes.mb_cols * planes.mb_rows {
+        return Ok(());
+    }
+    check_lf_planes(planes);
+    let rec = lf_records(mbs);
+    let cfg = config.c();
+    let p = lf_planes(planes);
+    with_context(|ctx| {
+        // SAFETY: planes checked above; one record per MB.
+        check(unsafe { sys::vp8g_loop_filter_frame(ctx.as_ptr(), &p, rec.as_ptr(), &cfg) }, "vp8g_loop_filter_frame")
+    })
+}
+
+/// `filter_frame_flags` (src/loop_filter.rs:2506-2636): a key frame, every
+/// macroblock intra.
+pub fn filter_frame_flags(planes: &mut KeyframePlanes, segment_ids: &[Option<u8>], bpred: &[bool],
+                          has_coeffs: &[bool], config: &FrameFilterConfig) -> Result<(), DeviceError> {
+    if bpred.len() != has_coeffs.len() || segment_ids.len() != bpred.len() {
+        return Ok(());
+    }
+    let mbs: Vec<MbFilterInfo> = (0..bpred.len())
+        .map(|i| MbFilterInfo { segment_id: segment_ids[i], bpred: bpred[i], has_coeffs: has_coeffs[i], inter: None })
+        .collect();
+    filter_inter_frame_flags(planes, &mbs, config)
+}
+
+fn select(planes: &KeyframePlanes, mbs: &[MbFilterInfo], base: &FrameFilterConfig, src: &SourcePlanes<'_>,
+          every_rows: usize, sweep: bool) -> Result<(u8, u8), DeviceError> {
+    assert_eq!(mbs.len(), planes.mb_cols * planes.mb_rows, "one record per macroblock");
+    check_lf_planes(planes);
+    let rec = lf_records(mbs);
+    let cfg = base.c();
+    let s = lf_source(src);
+    // the call only reads the planes
+    let p = sys::vp8g_lf_planes {
+        y: planes.y.as_ptr() as *mut u8,
+        u: planes.u.as_ptr() as *mut u8,
+        v: planes.v.as_ptr() as *mut u8,
+        y_stride: planes.y_stride,
+        uv_stride: planes.uv_stride,
+        mb_cols: planes.mb_cols,
+        mb_rows: planes.mb_rows,
+    };
+    let (mut level, mut sharp) = (0u8, 0u8);
+    with_context(|ctx| {
+        // SAFETY: planes and source checked above; one record per MB.
+        check(unsafe {
+            sys::vp8g_loop_filter_select(ctx.as_ptr(), &p, rec.as_ptr(), &cfg, &s, every_rows, sweep as i32,
+                                         &mut level, &mut sharp)
+        }, "vp8g_loop_filter_select")
+    })?;
+    Ok((level, sharp))
+}
+
+/// `select_filter_params_threaded` (src/loop_filter.rs:2133-2200): the
+/// (level, sharpness) of the reference's search; the candidates are
+/// scored on the device, whatever `threads` says.
+pub fn select_filter_params_threaded(planes: &KeyframePlanes, mbs: &[MbFilterInfo], base: &FrameFilterConfig,
+                                     src: &SourcePlanes<'_>, _threads: usize) -> (u8, u8) {
+    or_panic(select(planes, mbs, base, src, 0, true))
+}
+
+/// `select_filter_params_sampled` (src/loop_filter.rs:2288-2412) with
+/// `FilterSampling { every_rows, sweep_sharpness }`.
+pub fn select_filter_params_sampled(planes: &KeyframePlanes, mbs: &[MbFilterInfo], base: &FrameFilterConfig,
+                                    src: &SourcePlanes<'_>, every_rows: usize, sweep_sharpness: bool,
+                                    _threads: usize) -> (u8, u8) {
+    or_panic(select(planes, mbs, base, src, every_rows.max(2), sweep_sharpness))
+}
