@@ -120,6 +120,12 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
 #define VP8G_FAULT_LAUNCH 3
 #define VP8G_OPT_HOST_THREADS 6
 #define VP8G_OPT_COPY_STREAMS 7
+/* Row bands of vp8g_build_arnr_altref (host buffers): the frames go out a
+ * band of MB rows at a time (each band's rows plus the 15 luma / 7 chroma
+ * rows its search reaches below), the kernel runs on each band as soon as
+ * its rows are in, and each band's altref rows come back while later bands
+ * are still in flight.  1..8; 0 restores the default (2). */
+#define VP8G_OPT_HOST_BANDS 8
 int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);
 

@@ -706,11 +706,11 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
 arnr_fused_kernel(const ArnrArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
-  const int total = a.g.strips_per_row * a.g.brows;
+  const int total = a.strip_count;
   // Strips go out bottom row first: the last MB row, partial when the height
   // is not a multiple of 16, takes the masked search (the slowest path) and
   // so starts with the launch instead of closing it.
-  const int strip = total - 1 - (a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x);
+  const int strip = a.strip_base + total - 1 - (a.xcd_remap ? xcd_band(blockIdx.x, total) : (int)blockIdx.x);
   const int brow = a.spr_mul ? (int)__umulhi((uint32_t)strip, a.spr_mul) : strip;
   const int scol = strip - brow * a.g.strips_per_row;
   process_mbs<false>(a.g, a.wt4, [&](int i) { return a.frames[i]; }, brow, scol, a.mbs_per_wg, smem);
@@ -831,7 +831,9 @@ size_t arnr_lds_bytes(int k, int m, bool table) {
 int arnr_supported_mbs(int m) { return m >= 8 ? 8 : (m >= 4 ? 4 : (m >= 2 ? 2 : 1)); }
 
 hipError_t launch_arnr_fused(const ArnrArgs& a, int k, hipStream_t stream) {
-  const int blocks = a.g.strips_per_row * a.g.brows;
+  const int blocks = a.strip_count;
+  if (blocks < 1 || a.strip_base < 0 || a.strip_base + blocks > a.g.strips_per_row * a.g.brows)
+    return hipErrorInvalidValue;
   const int m = a.mbs_per_wg;
   if (m < 1 || k < 1 || m * k > 16) return hipErrorInvalidValue;
   hipLaunchKernelGGL(arnr_fused_kernel, dim3(blocks), dim3(64 * m * k), arnr_dynamic_lds_bytes(k, m, false), stream, a);

@@ -319,8 +319,9 @@ uint32_t strip_divider(int d) {
 }
 
 // Enqueue the fused ARNR kernel over device-resident frames.
+// MB rows [row0, row1) of the group (all of them by default).
 int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, uint8_t strength,
-                 uint8_t* oy, uint8_t* ou, uint8_t* ov, int32_t* mv_out) {
+                 uint8_t* oy, uint8_t* ou, uint8_t* ov, int32_t* mv_out, int row0 = 0, int row1 = -1) {
   ArnrArgs a;
   memset(&a, 0, sizeof(a));
   fill_group(frames, n, center, strength, oy, ou, ov, mv_out, a.frames, &a.g, a.wt4);
@@ -330,6 +331,9 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   a.mbs_per_wg = m;
   a.g.strips_per_row = (a.g.bcols + m - 1) / m;
   a.spr_mul = strip_divider(a.g.strips_per_row);
+  if (row1 < 0) row1 = a.g.brows;
+  a.strip_base = row0 * a.g.strips_per_row;
+  a.strip_count = (row1 - row0) * a.g.strips_per_row;
   return launch_status(c, [&] { return launch_arnr_fused(a, k, c->stream); });
 }
 
@@ -453,6 +457,9 @@ int ensure_host_io(vp8g_ctx* c, size_t bytes) {
       return VP8G_E_DEVICE;
   }
   if (!c->ready && !hip_ok(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming))) return VP8G_E_DEVICE;
+  for (int b = 0; b < vp8g_ctx::kMaxBands; b++)
+    if (!c->band_done[b] && !hip_ok(hipEventCreateWithFlags(&c->band_done[b], hipEventDisableTiming)))
+      return VP8G_E_DEVICE;
   if (bytes <= c->staging_bytes) return VP8G_OK;
   if (c->staging) {
     (void)hipHostFree(c->staging);
@@ -600,6 +607,8 @@ void vp8g_ctx_destroy(vp8g_ctx* c) {
     if (c->copy_done[k]) (void)hipEventDestroy(c->copy_done[k]);
   }
   if (c->ready) (void)hipEventDestroy(c->ready);
+  for (int b = 0; b < vp8g_ctx::kMaxBands; b++)
+    if (c->band_done[b]) (void)hipEventDestroy(c->band_done[b]);
   if (c->staging) (void)hipHostFree(c->staging);
   if (c->lf_work) (void)hipFree(c->lf_work);
   if (c->lf_io) (void)hipFree(c->lf_io);
@@ -659,6 +668,10 @@ int vp8g_ctx_set_option(vp8g_ctx* c, int option, int value) {
       if (value < 0 || value > vp8g_ctx::kMaxCopyStreams) return VP8G_E_ARG;
       c->copy_streams = value;
       return VP8G_OK;
+    case VP8G_OPT_HOST_BANDS:
+      if (value < 0 || value > vp8g_ctx::kMaxBands) return VP8G_E_ARG;
+      c->host_bands = value;
+      return VP8G_OK;
     default:
       return VP8G_E_ARG;
   }
@@ -685,92 +698,132 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
   const size_t w = frames[center].width, h = frames[center].height;
   const size_t cw = (w + 1) / 2, ch = (h + 1) / 2;
-  // Device layout: each frame's planes at a 256-byte aligned offset with
-  // pitch rounded to 64 bytes, then the packed outputs.
+  // Device layout: the frames' rows interleaved, one plane each of Y, U and
+  // V for the whole group: row r of frame i's luma at (r n + i) yp, so frame
+  // i's plane starts at i yp with stride n yp (pitches rounded to 64 bytes),
+  // and rows a..b-1 of every frame are one contiguous range.  Then the
+  // packed outputs.
   const size_t yp = align_up(w, 64), cp = align_up(cw, 64);
-  const size_t ysz = align_up(yp * h, 256), csz = align_up(cp * ch, 256);
-  const size_t per = ysz + 2 * csz;
+  const size_t yrow = yp * n, crow = cp * n;  // bytes of one row of every frame
+  const size_t ysz = align_up(yrow * h, 256), csz = align_up(crow * ch, 256);
+  const size_t in_bytes = ysz + 2 * csz;
   const size_t oyb = align_up(w * h, 256), ocb = align_up(cw * ch, 256);
-  st = ensure_arena(c, per * n + oyb + 2 * ocb);
-  if (st == VP8G_OK) st = ensure_host_io(c, per * n);
+  if (yrow > 0x7FFFFFFFu || ysz > 0x7FFFFFFFu) return VP8G_E_ARG;
+  st = ensure_arena(c, in_bytes + oyb + 2 * ocb);
+  if (st == VP8G_OK) st = ensure_host_io(c, in_bytes);
   if (st != VP8G_OK) return st;
   if (c->fault == VP8G_FAULT_COPY) return VP8G_E_DEVICE;
   const int S = copy_stream_count(c);
-  // The copies follow whatever the context's stream already holds.
-  if (!hip_ok(hipEventRecord(c->ready, c->stream))) return finish_io(c, VP8G_E_DEVICE);
-  for (int k = 0; k < S; k++)
-    if (!hip_ok(hipStreamWaitEvent(c->copy[k], c->ready, 0))) return finish_io(c, VP8G_E_DEVICE);
-  // Each frame goes out in two pieces, its Y plane and then its U and V
-  // planes (adjacent in the device layout): the host threads pack a piece
-  // into write-combined staging and its copy is issued at once on the copy
-  // stream with the fewest bytes queued, so packing the next piece overlaps
-  // the transfers and the last transfer after the last pack is short.
+  // The copies need not wait for the context's stream: the arena and the
+  // staging are only used by the host-pointer entry points, which all
+  // return synchronised, so no queued work reads them.
+  uint8_t* const dY = c->arena;
+  uint8_t* const dU = dY + ysz;
+  uint8_t* const dV = dU + csz;
+  uint8_t* const pY = c->staging;
+  uint8_t* const pU = pY + ysz;
+  uint8_t* const pV = pU + csz;
   std::vector<vp8g_i420> dev(n);
+  for (size_t i = 0; i < n; i++)
+    dev[i] = vp8g_i420{frames[i].width, frames[i].height, dY + i * yp, dU + i * cp, dV + i * cp, yrow, crow};
+  uint8_t* dy = c->arena + in_bytes;
+  uint8_t* du = dy + oyb;
+  uint8_t* dv = du + ocb;
+  const size_t off = (size_t)(dy - c->arena);
+  // Row bands.  The search of an MB row reaches 15 luma and 7 chroma rows
+  // below it (and as many above), clamped to the frame, so band b (MB rows
+  // r0..r1-1) can run once every frame's rows up to 16 r1 + 15 and 8 r1 + 7
+  // are in.  Each band sends only the rows the previous ones did not: host
+  // threads pack them into write-combined staging, then one copy each for
+  // the Y, U and V rows, on the copy stream with the fewest bytes queued;
+  // the band's kernel waits for them on the context's stream, and the
+  // band's altref rows come back on that stream while the copy streams
+  // carry the next band.
+  const int brows = (int)((h + 15) / 16);
+  const int B = std::max(1, std::min(brows, c->host_bands > 0 ? c->host_bands : 2));
+  std::vector<PlaneJob> jobs;
+  jobs.reserve(3 * n);
   std::vector<std::pair<const PlaneJob*, size_t>> tasks;
   size_t queued[vp8g_ctx::kMaxCopyStreams] = {};
   int failed = 0;
   auto send = [&](uint8_t* dst, const uint8_t* src, size_t bytes) {
+    if (!bytes) return;
     int k = 0;
     for (int j = 1; j < S; j++)
       if (queued[j] < queued[k]) k = j;
     queued[k] += bytes;
     if (!failed && !hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->copy[k]))) failed = 1;
   };
-  for (size_t i = 0; i < n; i++) {
-    uint8_t* base = c->arena + per * i;
-    uint8_t* pin = c->staging + per * i;
-    const vp8g_i420& f = frames[i];
-    const PlaneJob y = plane_job(pin, yp, f.y, f.y_stride, w, h);
-    tasks.clear();
-    add_plane(tasks, y);
-    run_plane_tasks(c, tasks);
-    send(base, pin, ysz);
-    const PlaneJob uv[2] = {plane_job(pin + ysz, cp, f.u, f.uv_stride, cw, ch),
-                            plane_job(pin + ysz + csz, cp, f.v, f.uv_stride, cw, ch)};
-    tasks.clear();
-    for (const PlaneJob& j : uv) add_plane(tasks, j);
-    run_plane_tasks(c, tasks);
-    send(base + ysz, pin + ysz, 2 * csz);
-    dev[i] = vp8g_i420{f.width, f.height, base, base + ysz, base + ysz + csz, yp, cp};
-  }
-  for (int k = 0; k < S; k++) {
-    if (!hip_ok(hipEventRecord(c->copy_done[k], c->copy[k])) ||
-        !hip_ok(hipStreamWaitEvent(c->stream, c->copy_done[k], 0)))
+  auto recv = [&](size_t dev_off, size_t bytes) {
+    if (!failed && bytes &&
+        !hip_ok(hipMemcpyAsync(c->pinned + off + dev_off, dy + dev_off, bytes, hipMemcpyDeviceToHost, c->stream)))
       failed = 1;
+  };
+  size_t sent_y = 0, sent_c = 0;  // rows of every frame already sent
+  int bands = 0;
+  for (int b = 0; b < B && !failed; b++) {
+    const int r0 = (int)((int64_t)brows * b / B), r1 = (int)((int64_t)brows * (b + 1) / B);
+    if (r1 <= r0) continue;
+    const bool last = b == B - 1;
+    const size_t need_y = last ? h : std::min(h, (size_t)16 * r1 + 15);
+    const size_t need_c = last ? ch : std::min(ch, (size_t)8 * r1 + 7);
+    // luma first, sent while the chroma rows are packed
+    jobs.clear();
+    tasks.clear();
+    for (size_t i = 0; i < n && need_y > sent_y; i++)
+      jobs.push_back(plane_job(pY + sent_y * yrow + i * yp, yrow, frames[i].y + sent_y * frames[i].y_stride,
+                               frames[i].y_stride, w, need_y - sent_y));
+    for (const PlaneJob& j : jobs) add_plane(tasks, j);
+    run_plane_tasks(c, tasks);
+    send(dY + sent_y * yrow, pY + sent_y * yrow, (need_y - sent_y) * yrow);
+    jobs.clear();
+    tasks.clear();
+    for (size_t i = 0; i < n && need_c > sent_c; i++) {
+      const vp8g_i420& f = frames[i];
+      jobs.push_back(plane_job(pU + sent_c * crow + i * cp, crow, f.u + sent_c * f.uv_stride, f.uv_stride, cw,
+                               need_c - sent_c));
+      jobs.push_back(plane_job(pV + sent_c * crow + i * cp, crow, f.v + sent_c * f.uv_stride, f.uv_stride, cw,
+                               need_c - sent_c));
+    }
+    for (const PlaneJob& j : jobs) add_plane(tasks, j);
+    run_plane_tasks(c, tasks);
+    send(dU + sent_c * crow, pU + sent_c * crow, (need_c - sent_c) * crow);
+    send(dV + sent_c * crow, pV + sent_c * crow, (need_c - sent_c) * crow);
+    sent_y = need_y, sent_c = need_c;
+    for (int k = 0; k < S; k++) {
+      if (!failed && (!hip_ok(hipEventRecord(c->copy_done[k], c->copy[k])) ||
+                      !hip_ok(hipStreamWaitEvent(c->stream, c->copy_done[k], 0))))
+        failed = 1;
+    }
+    if (failed) break;
+    st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr, r0, r1);
+    if (st != VP8G_OK) return finish_io(c, st);
+    const size_t y0 = (size_t)16 * r0, y1 = std::min(h, (size_t)16 * r1);
+    const size_t c0 = (size_t)8 * r0, c1 = std::min(ch, (size_t)8 * r1);
+    recv(y0 * w, (y1 - y0) * w);
+    recv(oyb + c0 * cw, (c1 - c0) * cw);
+    recv(oyb + ocb + c0 * cw, (c1 - c0) * cw);
+    if (!failed && !hip_ok(hipEventRecord(c->band_done[bands], c->stream))) failed = 1;
+    bands++;
   }
   if (failed) return finish_io(c, VP8G_E_DEVICE);
-  uint8_t* dy = c->arena + per * n;
-  uint8_t* du = dy + oyb;
-  uint8_t* dv = du + ocb;
-  st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr);
-  if (st != VP8G_OK) return finish_io(c, st);
-  // The altref comes back on the copy streams (luma in S bands, U and V on
-  // two of them) into the arena's pinned mirror.
-  if (!hip_ok(hipEventRecord(c->ready, c->stream))) return finish_io(c, VP8G_E_DEVICE);
-  const size_t off = (size_t)(dy - c->arena);
-  const size_t yb = w * h, band = align_up((yb + S - 1) / S, 4096);
-  for (int k = 0; k < S && !failed; k++) {
-    if (!hip_ok(hipStreamWaitEvent(c->copy[k], c->ready, 0))) failed = 1;
-    const size_t b0 = std::min(yb, (size_t)k * band), b1 = std::min(yb, b0 + band);
-    if (!failed && b1 > b0 &&
-        !hip_ok(hipMemcpyAsync(c->pinned + off + b0, dy + b0, b1 - b0, hipMemcpyDeviceToHost, c->copy[k])))
-      failed = 1;
+  // Each band's rows go to the caller as soon as they are back; if a later
+  // band fails on the device, the rows of the bands before it are written.
+  for (int b = 0, done = 0; b < B; b++) {
+    const int r0 = (int)((int64_t)brows * b / B), r1 = (int)((int64_t)brows * (b + 1) / B);
+    if (r1 <= r0) continue;
+    if (!hip_ok(hipEventSynchronize(c->band_done[done++]))) return finish_io(c, VP8G_E_DEVICE);
+    const size_t y0 = (size_t)16 * r0, y1 = std::min(h, (size_t)16 * r1);
+    const size_t c0 = (size_t)8 * r0, c1 = std::min(ch, (size_t)8 * r1);
+    const PlaneJob outs[3] = {plane_job(oy + y0 * w, w, c->pinned + off + y0 * w, w, w, y1 - y0),
+                              plane_job(ou + c0 * cw, cw, c->pinned + off + oyb + c0 * cw, cw, cw, c1 - c0),
+                              plane_job(ov + c0 * cw, cw, c->pinned + off + oyb + ocb + c0 * cw, cw, cw, c1 - c0)};
+    tasks.clear();
+    for (const PlaneJob& j : outs) add_plane(tasks, j);
+    run_plane_tasks(c, tasks);
   }
-  if (!failed && !hip_ok(hipMemcpyAsync(c->pinned + off + oyb, du, cw * ch, hipMemcpyDeviceToHost, c->copy[0])))
-    failed = 1;
-  if (!failed &&
-      !hip_ok(hipMemcpyAsync(c->pinned + off + oyb + ocb, dv, cw * ch, hipMemcpyDeviceToHost, c->copy[1 % S])))
-    failed = 1;
-  st = finish_io(c, failed ? VP8G_E_DEVICE : VP8G_OK);
-  if (st != VP8G_OK) return st;
-  // Outputs are written only after every copy has succeeded.
-  const PlaneJob outs[3] = {plane_job(oy, w, c->pinned + off, w, w, h),
-                            plane_job(ou, cw, c->pinned + off + oyb, cw, cw, ch),
-                            plane_job(ov, cw, c->pinned + off + oyb + ocb, cw, cw, ch)};
-  tasks.clear();
-  for (const PlaneJob& j : outs) add_plane(tasks, j);
-  run_plane_tasks(c, tasks);
-  return VP8G_OK;
+  (void)bands;
+  return finish_io(c, VP8G_OK);
 }
 
 int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,

@@ -7,8 +7,8 @@
 // calling thread and returns when every call has returned.  Task indices
 // are claimed with a compare-and-swap on (generation, index), so a worker
 // that wakes late never takes an index of a later run.  Workers spin
-// briefly on the generation counter before sleeping, since the entry
-// points issue runs back to back.
+// on the generation counter for up to 2 ms before sleeping, since an entry
+// point issues its runs within that.
 #pragma once
 #include <atomic>
 #include <chrono>
@@ -83,8 +83,8 @@ class HostPool {
   void loop() {
     uint32_t seen = 0;
     for (;;) {
-      // spin about 200 us for the next run before sleeping
-      const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(200);
+      // spin up to 2 ms for the next run before sleeping
+      const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(2000);
       while (gen_.load(std::memory_order_acquire) == seen && std::chrono::steady_clock::now() < until)
         std::this_thread::yield();
       const std::function<void(size_t)>* fn;

@@ -47,6 +47,11 @@ struct vp8g_ctx {
   hipStream_t copy[kMaxCopyStreams] = {};
   hipEvent_t copy_done[kMaxCopyStreams] = {};
   hipEvent_t ready = nullptr;
+  // Row bands of the host-pointer ARNR entry point: band b's altref rows
+  // are back in pinned memory when band_done[b] has fired.
+  static constexpr int kMaxBands = 8;
+  int host_bands = 0;  // VP8G_OPT_HOST_BANDS (0 = default)
+  hipEvent_t band_done[kMaxBands] = {};
   uint8_t* staging = nullptr;
   size_t staging_bytes = 0;
   // Loop filter (lf_host.hip): row rings and per-instance SSDs, the device

@@ -49,6 +49,9 @@ struct ArnrArgs {
   // strip / strips_per_row == umulhi(strip, spr_mul) for strips below 2^20
   // (spr_mul = ceil(2^32 / strips_per_row); 0 when strips_per_row is 1)
   uint32_t spr_mul;
+  // The launch covers strips [strip_base, strip_base + strip_count) (whole
+  // MB rows of the group, for the banded host entry point).
+  int strip_base, strip_count;
 };
 // Launch shape: a workgroup holds M anchor MBs x K waves per MB (K splits the
 // neighbour frames), 64 * M * K threads, M * K <= 16.

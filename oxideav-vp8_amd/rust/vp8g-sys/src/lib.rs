@@ -129,6 +129,8 @@ pub const VP8G_FAULT_LAUNCH: c_int = 3;
 pub const VP8G_OPT_HOST_THREADS: c_int = 6;
 /// Streams carrying `vp8g_build_arnr_altref`'s host<->device copies.
 pub const VP8G_OPT_COPY_STREAMS: c_int = 7;
+/// Row bands of `vp8g_build_arnr_altref` (1..8; 0 = default).
+pub const VP8G_OPT_HOST_BANDS: c_int = 8;
 
 /// `vp8g_search_fallback[_noalloc]_dev` flag: queue only failed entries
 /// with a passing left, right, upper or lower neighbour.

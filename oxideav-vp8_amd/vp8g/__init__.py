@@ -45,6 +45,7 @@ OPT_BATCH_WGS_PER_CU = 4
 OPT_FAULT = 5
 OPT_HOST_THREADS = 6
 OPT_COPY_STREAMS = 7
+OPT_HOST_BANDS = 8
 FAULT_NONE, FAULT_ALLOC, FAULT_COPY, FAULT_LAUNCH = 0, 1, 2, 3
 
 # RFC 6386 section 18.3 tap tables, src/motion_comp.rs:294-303, 328-337.
@@ -396,6 +397,10 @@ class Context:
     def set_copy_streams(self, streams: int) -> None:
         """Copy streams of build_arnr_altref's transfers, 1..4 (0 = default 2)."""
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_COPY_STREAMS, streams), "set_copy_streams")
+
+    def set_host_bands(self, bands: int) -> None:
+        """Row bands of the host-buffer ARNR entry point (1..8; 0 = default 2)."""
+        self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_HOST_BANDS, int(bands)), "set_host_bands")
 
     def set_fault(self, mode: int) -> None:
         """Host-side fault injection (FAULT_ALLOC / FAULT_COPY / FAULT_LAUNCH,

@@ -228,17 +228,19 @@ def test_max_group_length(gpu, oracle):
     check(gpu, oracle, g.frames(), 40, 3)
 
 
-@pytest.mark.parametrize("threads,streams", [(1, 1), (3, 2), (8, 4), (0, 3)])
-def test_host_copy_shapes_leave_bytes_unchanged(gpu, oracle, threads, streams):
-    """The host threads and copy streams of the host-pointer entry change no
-    output byte: strided planes of odd sizes (row bands of every length),
-    a group longer than the copy streams, and a call after a smaller one
-    that reuses the staging."""
+@pytest.mark.parametrize("threads,streams,bands", [(1, 1, 1), (3, 2, 2), (8, 4, 8), (0, 3, 0), (2, 2, 5)])
+def test_host_copy_shapes_leave_bytes_unchanged(gpu, oracle, threads, streams, bands):
+    """The host threads, copy streams and row bands of the host-pointer
+    entry change no output byte: strided planes of odd sizes (partial last
+    MB rows, bands of uneven MB-row counts, more bands than MB rows), a
+    group longer than the copy streams, and a call after a smaller one that
+    reuses the staging."""
     import vp8g
     import vp8g.synth as synth
     try:
         gpu.set_host_threads(threads)
         gpu.set_copy_streams(streams)
+        gpu.set_host_bands(bands)
         for (w, h, n, c) in ((98, 66, 5, 2), (353, 289, 7, 3), (37, 21, 2, 0)):
             g = synth.arnr_group_cif(w, h, n, clip=w)
             tuples = []
@@ -253,6 +255,9 @@ def test_host_copy_shapes_leave_bytes_unchanged(gpu, oracle, threads, streams):
             gpu.set_copy_streams(5)
         with pytest.raises(vp8g.Vp8gError):
             gpu.set_host_threads(65)
+        with pytest.raises(vp8g.Vp8gError):
+            gpu.set_host_bands(9)
     finally:
         gpu.set_host_threads(0)
         gpu.set_copy_streams(0)
+        gpu.set_host_bands(0)
