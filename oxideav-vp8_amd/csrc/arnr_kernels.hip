@@ -230,26 +230,29 @@ __device__ __forceinline__ void stage_gather(uint8_t* win, const FrameDesc& f, c
   }
 }
 
-// Row r of the anchor MB as 16 bytes, zero beyond (bw, bh).  WHOLE: the MB
-// is 16 pixels wide and its rows are 16-byte aligned (one load per row; a
-// byte-wise gather, whose loads the compiler may serialise, is the
-// fallback).
-__device__ __forceinline__ uint4 centre_row16(const FrameDesc& C, int bx, int by, int r, int bw, int bh, bool whole) {
-  if (whole) {
-    if (r >= bh) return make_uint4(0u, 0u, 0u, 0u);
-    return *reinterpret_cast<const uint4*>(C.y + (size_t)(by + r) * C.y_stride + bx);
-  }
-  // Every load is issued (row and column clamped into the block) and the
-  // bytes outside the block are zeroed afterwards, so the loads go out
-  // together instead of one branch at a time.
-  uint32_t d[4] = {0u, 0u, 0u, 0u};
-  const uint8_t* row = C.y + (size_t)(by + min(r, bh - 1)) * C.y_stride + bx;
-#pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const uint32_t keep = j < bw && r < bh ? 0xFFu : 0u;
-    d[j >> 2] |= ((uint32_t)row[min(j, bw - 1)] & keep) << (8 * (j & 3));
-  }
-  return make_uint4(d[0], d[1], d[2], d[3]);
+// Row r of the anchor MB as 16 bytes (zero beyond (bw, bh)) from the lanes'
+// blend pixels: lane 4r + q holds dword q of row r in `ctr4`.  A crossbar
+// exchange (no LDS memory, whole wave active) instead of a second global
+// load of the centre, which every wave would issue at launch.
+__device__ __forceinline__ uint4 centre_row16(uint32_t ctr4, int r) {
+  auto from = [&](int l) { return (uint32_t)__builtin_amdgcn_ds_bpermute(4 * l, (int)ctr4); };
+  return make_uint4(from(4 * r), from(4 * r + 1), from(4 * r + 2), from(4 * r + 3));
+}
+
+// Issue priority of a wave at iteration `it` of `nf` frames: 3 down to 0 as
+// it progresses, so that the SIMD's arbiter (highest priority first, then
+// oldest) lets waves that are behind catch up.  Waves then finish together
+// instead of leaving the SIMD to a few latency-bound stragglers.
+__device__ __forceinline__ void pace(int it, int nf) {
+  const int p = 3 - (it * 4) / nf;
+  if (p >= 3)
+    __builtin_amdgcn_s_setprio(3);
+  else if (p == 2)
+    __builtin_amdgcn_s_setprio(2);
+  else if (p == 1)
+    __builtin_amdgcn_s_setprio(1);
+  else
+    __builtin_amdgcn_s_setprio(0);
 }
 
 // One anchor MB as a wave sees it, and the lane's packed accumulators
@@ -309,9 +312,6 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   uint32_t off0 = 0, off1 = 0, off2 = 0, coff = 0, outside = 0;
   {
     const FrameDesc C = frame_at(a.center);
-    // DMA MBs are 16 pixels wide with 16-byte aligned rows
-    src0 = centre_row16(C, s.bx, s.by, kk, s.bw, s.bh, kDma);
-    src1 = centre_row16(C, s.bx, s.by, kk + 8, s.bw, s.bh, kDma);
     if (kDma) {
       auto piece = [&](int p, int i) -> uint32_t {
         const int row = p / 3;
@@ -355,14 +355,18 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     if (kMasked) fetch_edges(F.y);
   }
   lut_sync();
+  src0 = centre_row16(s.ctr4, kk);
+  src1 = centre_row16(s.ctr4, kk + 8);
   bool pend = false;  // chroma bytes of the previous frame await their blend
   uint32_t ru = 0, rv = 0;
-  for (; k < nnb; k += K) {
+  const int nf = (nnb - split + K - 1) / K;
+  for (int it = 0; k < nnb; k += K, it++) {
     const int fi = k < a.center ? k : k + 1;
     const int kn = k + K;
     FrameDesc Fn{};
     if (kn < nnb) Fn = frame_at(kn < a.center ? kn : kn + 1);
     uint8_t* w = win + cur * kLumaWin;
+    pace(it, nf);
     if (kDma) wait_vm();  // this frame's window and the previous frame's chroma have landed
     // Registers loaded for this frame are read here (unconditionally: the
     // compiler's own waits for them then sit here), before the next DMA goes
@@ -603,11 +607,18 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
   // The LUT and the split accumulators are written, and the workgroup
   // synchronised, by each wave's frame loop once its first window DMA is
   // issued, so that the DMA's latency overlaps the centre and LUT loads.
+  // With K = 1 no wave reads another's LDS: each wave writes the whole LUT
+  // itself (every wave the same values) and needs only its own writes done,
+  // so no wave waits at a barrier for the workgroup's slowest prologue.
   auto lut_sync = [&]() {
-    for (int i = tid; i < 64; i += blockDim.x) s_arnr_lut[i] = wt4[i];
-    if (K > 1)
+    if (K > 1) {
+      for (int i = tid; i < 64; i += blockDim.x) s_arnr_lut[i] = wt4[i];
       for (int i = tid; i < 384 * M; i += blockDim.x) s_acc[i] = 0;
-    __syncthreads();
+      __syncthreads();
+    } else {
+      s_arnr_lut[lane] = wt4[lane];
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    }
   };
 
   if (active) {
@@ -705,6 +716,7 @@ __device__ __forceinline__ void process_mbs(const ArnrGroup& a, const uint32_t* 
 // At most 64 VGPRs: 8 waves per SIMD.
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
 arnr_fused_kernel(const ArnrArgs a) {
+  __builtin_amdgcn_s_setprio(3);  // the prologue's loads go out ahead of running searches
   extern __shared__ __align__(16) uint8_t smem[];
   const int total = a.strip_count;
   // Strips go out bottom row first: the last MB row, partial when the height
