@@ -79,10 +79,28 @@ constexpr uint32_t kProbeDx1 = (0u << 0) | (1u << 2) | (2u << 4) | (0u << 6) | (
 constexpr uint32_t kProbeDy1 = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) | (1u << 8) | (2u << 10) |
                                (2u << 12) | (2u << 14);
 
+// Window reads and LDS banks.  A read instruction is served 32 lanes at a
+// time by 32 banks of one dword; lanes of one 32-lane half that hit one bank
+// at different dwords cost a cycle each.  The window is built from 16-byte
+// pieces, so a dword's bank is its column dword mod 4 plus 4 times a row
+// term, and 8 lanes of a probe (8 rows) cover all 8 row terms of one bank
+// class: two probes of one half conflict whenever their dword columns agree
+// mod 4.  Lane group j (lanes 8j .. 8j + 7) holds probe (kLaneProbe >> 4j)
+// & 7, which puts probes 0, 1, 2, 6 in the first half and 3, 4, 5, 7 in the
+// second, and probes 0..4 read their two dword pairs in the order (2, 3),
+// (0, 1) (the source dwords are held in that order too).  With that, steps 2
+// and 1 read without conflicts and step 4 with a quarter of them, against
+// 20, 20 and 16 cycles per block for plain lane order (a 32-bank model,
+// which also reproduces the conflict counter of the plain order).
+constexpr uint32_t kLaneProbe = 0x75436210u;
+__device__ __forceinline__ int lane_probe(int lane) { return (kLaneProbe >> (4 * (lane >> 3))) & 7; }
+__device__ __forceinline__ int probe_pair_order(int probe) { return probe < 5 ? 1 : 0; }
+
 // refine_search (src/arnr.rs:264-300) for one block against the staged
 // window.  Lanes are 8 probe groups x 8 row pairs (rows kk and kk + 8);
 // `row0` points at this lane's first source row's (0,0)-displaced reference
-// row.  The probes of
+// row; `rh` = 1 when the lane reads its dword pairs swapped (src0 / src1 are
+// held in the same order).  The probes of
 // a step never leave +-15 (8+4+2+1), so the reference's range check never
 // fires and is not evaluated.  MASK: which of columns >= bw (kMaskCols) and
 // rows >= bh (kMaskRows) a partial block masks out of every SAD (the centre
@@ -93,16 +111,17 @@ constexpr uint32_t kProbeDy1 = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) | (
 enum { kMaskNone = 0, kMaskRows = 1, kMaskCols = 2, kMaskBoth = 3 };
 template <int MASK, bool WAVE_ZERO, class G>
 __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, const uint4 src1,
-                                        int gd_off, int gq, int kk, int bw, int bh, int& bestx,
+                                        int gd_off, int gq, int kk, int rh, int bw, int bh, int& bestx,
                                         int& besty, uint32_t& best_sad, const uint8_t* zrow, uint32_t zsrc) {
   static_assert((G::lp & 3) == 0, "window rows are dword aligned");
   constexpr bool kCols = (MASK & kMaskCols) != 0, kRows = (MASK & kMaskRows) != 0;
+  const int dab = 8 - 16 * rh;  // pair B's offset from pair A
   uint32_t colmask[4] = {~0u, ~0u, ~0u, ~0u};
   uint32_t rowmask0 = ~0u, rowmask1 = ~0u;
   if (kCols) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int left = bw - 4 * q;
+      const int left = bw - 4 * (q ^ (2 * rh));  // the lane's q-th dword is column dword q ^ 2 rh
       colmask[q] = left >= 4 ? 0xFFFFFFFFu : (left <= 0 ? 0u : (0xFFFFFFFFu >> (32 - 8 * left)));
     }
   }
@@ -111,31 +130,49 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     rowmask0 = kk < bh ? ~0u : 0u;
     rowmask1 = kk + 8 < bh ? ~0u : 0u;
   }
-  // The lane's two rows at byte offset `off`.  ALIGNED (off a dword
-  // multiple): four dword reads per row.  Otherwise one alignment for both
-  // rows (the row pitch is a dword multiple), five naturally aligned dword
-  // reads per row and byte funnel shifts (unaligned LDS reads stall the LDS
-  // pipe).  The window's centre column and pitch are dword multiples and the
-  // probes of steps 8 and 4 move by multiples of 4 columns, so those steps
-  // (and the partial block's (0, 0)) are ALIGNED.
-  auto sad_at = [&](int off, auto aligned_tag) -> uint32_t {
-    constexpr bool ALIGNED = decltype(aligned_tag)::value;
-    const uint8_t* p = row0 + off;
+  // The lane's two rows at byte offset `off`, whose alignment (in bytes) is
+  // the tag: 8, two 8-byte reads per row; 4, four dword reads per row;
+  // otherwise one alignment for both rows (the row pitch is a dword
+  // multiple), six naturally aligned dword reads per row and byte funnel
+  // shifts (unaligned LDS reads stall the LDS pipe): the pair A = dwords
+  // (2 rh, 2 rh + 1) and the dword after it, then the pair B = (2 - 2 rh,
+  // 3 - 2 rh) and the dword after it, which gives the four funnels in the
+  // lane's dword order for either rh.  The window rows start
+  // 16-byte aligned and the pitch is a multiple of 16, so step 8's probes
+  // (dx a multiple of 8 columns, from the centre) are 8-byte aligned and
+  // step 4's and the partial block's (0, 0) dword aligned.  An 8-byte read
+  // puts a lane on two adjacent banks in one access, which spreads step 8's
+  // probes (whose dword offsets otherwise share one bank class for every
+  // horizontal direction) over more banks: fewer conflict cycles.
+  auto sad_at = [&](int off, auto align_tag) -> uint32_t {
+    constexpr int kAlign = decltype(align_tag)::value;
+    constexpr bool ALIGNED = kAlign >= 4;
+    // Pair A starts at row0 + 8 rh + off rounded down to a dword (8 rh
+    // keeps the byte shift), pair B at a lane constant (8 - 16 rh) from it.
+    // The bases are opaque: all read offsets (at most 400 bytes) then fit
+    // the 8-bit offsets of ds_read2.
+    const uint8_t* p = row0 + (8 * rh + off);
     const uint32_t sh = ALIGNED ? 0u : (uint32_t)(uintptr_t)p & 3;
-    // One opaque base for all reads: their offsets (at most 400 bytes) then
-    // fit the 8-bit dword offsets of ds_read2_b32.
-    LdsWords w = (LdsWords)((LdsBytes)p - sh);
-    asm volatile("" : "+v"(w));
+    LdsWords wa = (LdsWords)((LdsBytes)p - sh);
+    asm volatile("" : "+v"(wa));
+    LdsWords wb = (LdsWords)((LdsBytes)wa + dab);
+    asm volatile("" : "+v"(wb));
     constexpr int P = 8 * G::lp >> 2;  // row kk + 8
     uint4 q0, q1;
-    if (ALIGNED) {
-      q0 = make_uint4(w[0], w[1], w[2], w[3]);
-      q1 = make_uint4(w[P], w[P + 1], w[P + 2], w[P + 3]);
+    if (kAlign >= 8) {
+      typedef uint32_t u32x2a8 __attribute__((ext_vector_type(2), aligned(8)));
+      using LdsV = __attribute__((address_space(3))) const u32x2a8*;
+      const u32x2a8 a0 = *(LdsV)wa, b0 = *(LdsV)wb, a1 = *(LdsV)(wa + P), b1 = *(LdsV)(wb + P);
+      q0 = make_uint4(a0.x, a0.y, b0.x, b0.y);
+      q1 = make_uint4(a1.x, a1.y, b1.x, b1.y);
+    } else if (ALIGNED) {
+      q0 = make_uint4(wa[0], wa[1], wb[0], wb[1]);
+      q1 = make_uint4(wa[P], wa[P + 1], wb[P], wb[P + 1]);
     } else {
-      const uint32_t a0 = w[0], a1 = w[1], a2 = w[2], a3 = w[3], a4 = w[4];
-      const uint32_t b0 = w[P], b1 = w[P + 1], b2 = w[P + 2], b3 = w[P + 3], b4 = w[P + 4];
-      q0 = make_uint4(funnel(a0, a1, sh), funnel(a1, a2, sh), funnel(a2, a3, sh), funnel(a3, a4, sh));
-      q1 = make_uint4(funnel(b0, b1, sh), funnel(b1, b2, sh), funnel(b2, b3, sh), funnel(b3, b4, sh));
+      const uint32_t a0 = wa[0], a1 = wa[1], ax = wa[2], b0 = wb[0], b1 = wb[1], bx = wb[2];
+      const uint32_t c0 = wa[P], c1 = wa[P + 1], cx = wa[P + 2], d0 = wb[P], d1 = wb[P + 1], dx = wb[P + 2];
+      q0 = make_uint4(funnel(a0, a1, sh), funnel(a1, ax, sh), funnel(b0, b1, sh), funnel(b1, bx, sh));
+      q1 = make_uint4(funnel(c0, c1, sh), funnel(c1, cx, sh), funnel(d0, d1, sh), funnel(d1, dx, sh));
     }
     if (kCols) {
       q0.x &= colmask[0]; q0.y &= colmask[1]; q0.z &= colmask[2]; q0.w &= colmask[3];
@@ -163,7 +200,7 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     uint32_t z = sad4(zsrc, *reinterpret_cast<const uint32_t*>(zrow), 0);
     bsad = wave_sum_to_sgpr(sum16(z));
   } else {
-    bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0, std::true_type{}));
+    bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0, std::integral_constant<int, 4>{}));
   }
   // Probe order is the group index and the incumbent keeps ties, so the key
   // sad << 11 | group << 7 | (probe offset + 64) against best << 11 yields
@@ -174,7 +211,10 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
   const uint32_t key_lo = ((uint32_t)gq << 7) | (uint32_t)(gd_off + 64);
   auto step = [&](auto sh_tag) {
     constexpr int sh = decltype(sh_tag)::value;
-    const uint32_t s = sad_at(boff + gd_off * (1 << sh), std::integral_constant<bool, (sh >= 2)>{});
+    // alignment of the probe offsets: step 8 from the centre (boff = 0),
+    // step 4 by whole dwords, steps 2 and 1 by bytes
+    constexpr int kStepAlign = sh == 3 ? 8 : (sh == 2 ? 4 : 1);
+    const uint32_t s = sad_at(boff + gd_off * (1 << sh), std::integral_constant<int, kStepAlign>{});
     const uint32_t kmin = wave_min_to_sgpr((s << 11) | key_lo);
     if (kmin < (bsad << 11)) {
       bsad = kmin >> 11;
@@ -193,22 +233,31 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
 
 
 
-// DMA of one frame's luma window into `win` (global_load_lds_dwordx4: lane
-// l of an instruction lands at win + 16 l): piece p = lane + 64 i is window
-// row p / 3, 16-byte column p % 3, at byte 16 p of the window.  `origin` is
-// the frame address of window row 0, column 0 and off_i lane piece i's
-// offset from it (row * stride + 16 * column).  The data is in LDS once
+// A plane as a raw buffer: base in scalar registers, loads take a 32-bit
+// lane offset (plus an optional wave-uniform one) and read zero past
+// `bytes` instead of leaving the plane.  The DMA paths need every plane
+// below 2^31 bytes (the host clears uniform_strides otherwise).
+constexpr int kRawBufferWord3 = 0x00020000;  // 32-bit data format, no swizzle
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const uint8_t* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, kRawBufferWord3);
+}
+
+// DMA of one frame's luma window into `win` (buffer_load_dwordx4 ... lds:
+// lane l of an instruction lands at win + 16 l): piece p = lane + 64 i is
+// window row p / 3, 16-byte column p % 3, at byte 16 p of the window.
+// off_i is lane piece i's offset in the plane.  The data is in LDS once
 // vmcnt has drained (wait_vm).
 using LdsPtr = __attribute__((address_space(3))) void*;
 // MASKED: lane piece i is skipped when bit i of `outside` is set (a piece
 // wholly outside the frame's columns; the caller fills it).
 template <bool MASKED>
-__device__ __forceinline__ void dma_window(const uint8_t* origin, uint32_t off0, uint32_t off1, uint32_t off2,
+__device__ __forceinline__ void dma_window(__amdgpu_buffer_rsrc_t plane, uint32_t off0, uint32_t off1, uint32_t off2,
                                            uint8_t* win, int lane, uint32_t outside) {
-  if (!MASKED || !(outside & 1)) __builtin_amdgcn_global_load_lds(origin + off0, (LdsPtr)(win), 16, 0, 0);
-  if (!MASKED || !(outside & 2)) __builtin_amdgcn_global_load_lds(origin + off1, (LdsPtr)(win + 1024), 16, 0, 0);
+  if (!MASKED || !(outside & 1)) __builtin_amdgcn_raw_ptr_buffer_load_lds(plane, (LdsPtr)(win), 16, off0, 0, 0, 0);
+  if (!MASKED || !(outside & 2))
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(plane, (LdsPtr)(win + 1024), 16, off1, 0, 0, 0);
   if (lane < kLumaPieces - 128 && (!MASKED || !(outside & 4)))
-    __builtin_amdgcn_global_load_lds(origin + off2, (LdsPtr)(win + 2048), 16, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(plane, (LdsPtr)(win + 2048), 16, off2, 0, 0, 0);
 }
 
 // Every vector memory operation of the wave done (DMA writes included), and
@@ -231,12 +280,14 @@ __device__ __forceinline__ void stage_gather(uint8_t* win, const FrameDesc& f, c
 }
 
 // Row r of the anchor MB as 16 bytes (zero beyond (bw, bh)) from the lanes'
-// blend pixels: lane 4r + q holds dword q of row r in `ctr4`.  A crossbar
-// exchange (no LDS memory, whole wave active) instead of a second global
-// load of the centre, which every wave would issue at launch.
-__device__ __forceinline__ uint4 centre_row16(uint32_t ctr4, int r) {
+// blend pixels, in the search lane's dword order (pairs swapped when rh):
+// lane 4r + q holds dword q of row r in `ctr4`.  A crossbar exchange (no LDS
+// memory, whole wave active) instead of a second global load of the
+// centre, which every wave would issue at launch.
+__device__ __forceinline__ uint4 centre_row16(uint32_t ctr4, int r, int rh) {
   auto from = [&](int l) { return (uint32_t)__builtin_amdgcn_ds_bpermute(4 * l, (int)ctr4); };
-  return make_uint4(from(4 * r), from(4 * r + 1), from(4 * r + 2), from(4 * r + 3));
+  const int a = 4 * r + 2 * rh, b = 4 * r + 2 - 2 * rh;
+  return make_uint4(from(a), from(a + 1), from(b), from(b + 1));
 }
 
 // Issue priority of a wave at iteration `it` of `nf` frames: 3 down to 0 as
@@ -299,8 +350,9 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   const int lr = lane >> 2, lq = lane & 3;  // luma: row, dword
   const int cr = lane >> 3, cc = lane & 7;  // chroma: pixel
   const int nnb = a.filter_on ? a.n - 1 : 0;
-  // search: lane = probe group gq x row pair kk (rows kk, kk + 8)
-  const int gq = lane >> 3, kk = lane & 7;
+  // search: lane = probe group x row pair kk (rows kk, kk + 8); gq is the
+  // group's probe and rh its dword pair order
+  const int gq = lane_probe(lane), kk = lane & 7, rh = probe_pair_order(gq);
   constexpr bool kDma = PATH == kPathFast || PATH == kPathDma;
   // window pieces may lie outside the frame's columns (filled with the edge pixel)
   constexpr bool kMasked = PATH == kPathDma;
@@ -351,12 +403,12 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   FrameDesc F{};
   if (k < nnb) {
     F = frame_at(k < a.center ? k : k + 1);
-    if (kDma) dma_window<kMasked>(F.y, off0, off1, off2, win, lane, outside);
+    if (kDma) dma_window<kMasked>(plane_rsrc(F.y, (uint32_t)a.h * F.y_stride), off0, off1, off2, win, lane, outside);
     if (kMasked) fetch_edges(F.y);
   }
   lut_sync();
-  src0 = centre_row16(s.ctr4, kk);
-  src1 = centre_row16(s.ctr4, kk + 8);
+  src0 = centre_row16(s.ctr4, kk, rh);
+  src1 = centre_row16(s.ctr4, kk + 8, rh);
   bool pend = false;  // chroma bytes of the previous frame await their blend
   uint32_t ru = 0, rv = 0;
   const int nf = (nnb - split + K - 1) / K;
@@ -379,7 +431,8 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     }
     if (kDma) {
       if (kn < nnb) {
-        dma_window<kMasked>(Fn.y, off0, off1, off2, win + (cur ^ 1) * kLumaWin, lane, outside);
+        dma_window<kMasked>(plane_rsrc(Fn.y, (uint32_t)a.h * Fn.y_stride), off0, off1, off2, win + (cur ^ 1) * kLumaWin,
+                            lane, outside);
         if (kMasked) fetch_edges(Fn.y);
       }
     } else if (PATH == kPathGather) {
@@ -396,15 +449,15 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
       if constexpr (PATH == kPathFast) {
         // (0, 0) operands of the whole-wave SAD: row lane / 4, dword lane % 4
         const uint8_t* zrow = w + (lr + 15) * kLumaPitch + 16 + 4 * lq;
-        descend<kMaskNone, true, MbGeom>(row0, src0, src1, gd_off, gq, kk, 16, 16, bestx, besty, best_sad, zrow,
+        descend<kMaskNone, true, MbGeom>(row0, src0, src1, gd_off, gq, kk, rh, 16, 16, bestx, besty, best_sad, zrow,
                                          s.ctr4);
       } else if constexpr (PATH == kPathDma) {
         // a DMA MB is 16 pixels wide: a partial-width MB has a window piece
         // straddling the right edge, which sends it to the gather path
-        descend<kMaskRows, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, 16, s.bh, bestx, besty, best_sad,
+        descend<kMaskRows, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, rh, 16, s.bh, bestx, besty, best_sad,
                                           nullptr, 0);
       } else {
-        descend<kMaskBoth, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, s.bw, s.bh, bestx, besty, best_sad,
+        descend<kMaskBoth, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, rh, s.bw, s.bh, bestx, besty, best_sad,
                                           nullptr, 0);
       }
       if (a.mv_out && lane == 0) {
@@ -426,9 +479,15 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
       const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
       const uint32_t ref4 = funnel(rw[0], rw[1], rsh);
       if (PATH == kPathFast) {
-        const ptrdiff_t cb = (ptrdiff_t)(s.cby + cmvy) * F.uv_stride + s.cbx + cmvx;
-        ru = F.u[cb + coff];
-        rv = F.v[cb + coff];
+        // the block's chroma origin at the vector (inside the plane here) as
+        // the wave-uniform offset, the lane's pixel as the lane offset
+        // (the descriptor is read again here, so that its pointers stay
+        // wave-uniform scalars instead of being carried over the search)
+        const FrameDesc Fc = frame_at(fi);
+        const uint32_t cb = (uint32_t)(s.cby + cmvy) * Fc.uv_stride + (uint32_t)(s.cbx + cmvx);
+        const uint32_t cbytes = (uint32_t)a.ch * Fc.uv_stride;
+        ru = __builtin_amdgcn_raw_buffer_load_b8(plane_rsrc(Fc.u, cbytes), coff, cb, 0);
+        rv = __builtin_amdgcn_raw_buffer_load_b8(plane_rsrc(Fc.v, cbytes), coff, cb, 0);
       } else {
         const size_t o = (size_t)clampi(s.cby + cr + cmvy, 0, a.ch - 1) * F.uv_stride +
                          clampi(s.cbx + cc + cmvx, 0, a.cw - 1);
