@@ -94,28 +94,47 @@ constexpr uint32_t kProbeDy1 = (0u << 0) | (0u << 2) | (0u << 4) | (1u << 6) | (
 // which also reproduces the conflict counter of the plain order).
 constexpr uint32_t kLaneProbe = 0x75436210u;
 __device__ __forceinline__ int lane_probe(int lane) { return (kLaneProbe >> (4 * (lane >> 3))) & 7; }
+
+// Frame index of neighbour k (0 .. n - 2), skipping the centre, as plain
+// integer arithmetic (a select here becomes vector code).
+__device__ __forceinline__ int skip_centre(int k, int centre) { return k + (int)((uint32_t)(centre - 1 - k) >> 31); }
 __device__ __forceinline__ int probe_pair_order(int probe) { return probe < 5 ? 1 : 0; }
 
 // refine_search (src/arnr.rs:264-300) for one block against the staged
 // window.  Lanes are 8 probe groups x 8 row pairs (rows kk and kk + 8);
-// `row0` points at this lane's first source row's (0,0)-displaced reference
-// row; `rh` = 1 when the lane reads its dword pairs swapped (src0 / src1 are
-// held in the same order).  The probes of
+// `win` is the window's LDS address (wave-uniform) and `L` the lane's
+// offsets in it; `rh` = 1 when the lane reads its dword pairs swapped
+// (src0 / src1 are held in the same order).  The probes of
 // a step never leave +-15 (8+4+2+1), so the reference's range check never
 // fires and is not evaluated.  MASK: which of columns >= bw (kMaskCols) and
 // rows >= bh (kMaskRows) a partial block masks out of every SAD (the centre
 // strip holds zeros there; the reference bytes are masked to zero, rows on
 // the lane's row sums).  WAVE_ZERO (16x16 only): the (0, 0) SAD as one
-// wave-wide sum over `zrow` / `zsrc` (row lane / 4, dword lane % 4) instead
-// of the probe layout's eight group sums.
+// wave-wide sum over the window at `L.blend` / `zsrc` (row lane / 4, dword
+// lane % 4) instead of the probe layout's eight group sums.
 enum { kMaskNone = 0, kMaskRows = 1, kMaskCols = 2, kMaskBoth = 3 };
+
+// A lane's window byte offsets, computed once per wave and held opaque, so
+// that each probe address is one add of a wave-uniform term (window base
+// plus the best vector's offset) to a lane constant:
+//   at[s]  the lane's first row (kk) at the probe of step 2^s, pair A
+//          ((kk + 15) lp + 16 + 8 rh + gd_off 2^s); at0: the (0, 0) probe;
+//   dab    pair B's offset from pair A (8 - 16 rh);
+//   blend  the lane's blend dword: row lane / 4, dword lane % 4 of the
+//          block at (0, 0) ((lane / 4 + 15) lp + 16 + 4 (lane % 4)).
+struct SearchLanes {
+  uint32_t at8, at4, at2, at1, at0, dab, blend;
+};
+
+__device__ __forceinline__ LdsWords lds_words(uint32_t addr) { return (LdsWords)(uintptr_t)addr; }
+__device__ __forceinline__ uint32_t lds_addr(const void* p) { return (uint32_t)(uintptr_t)(LdsBytes)p; }
+
 template <int MASK, bool WAVE_ZERO, class G>
-__device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, const uint4 src1,
+__device__ __forceinline__ void descend(uint32_t win, const SearchLanes& L, const uint4 src0, const uint4 src1,
                                         int gd_off, int gq, int kk, int rh, int bw, int bh, int& bestx,
-                                        int& besty, uint32_t& best_sad, const uint8_t* zrow, uint32_t zsrc) {
+                                        int& besty, uint32_t& best_sad, uint32_t zsrc) {
   static_assert((G::lp & 3) == 0, "window rows are dword aligned");
   constexpr bool kCols = (MASK & kMaskCols) != 0, kRows = (MASK & kMaskRows) != 0;
-  const int dab = 8 - 16 * rh;  // pair B's offset from pair A
   uint32_t colmask[4] = {~0u, ~0u, ~0u, ~0u};
   uint32_t rowmask0 = ~0u, rowmask1 = ~0u;
   if (kCols) {
@@ -144,19 +163,21 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
   // puts a lane on two adjacent banks in one access, which spreads step 8's
   // probes (whose dword offsets otherwise share one bank class for every
   // horizontal direction) over more banks: fewer conflict cycles.
-  auto sad_at = [&](int off, auto align_tag) -> uint32_t {
+  // `at`: the lane's pair A offset at the probe, `off` the wave-uniform
+  // best-vector offset.
+  auto sad_at = [&](uint32_t at, int off, auto align_tag) -> uint32_t {
     constexpr int kAlign = decltype(align_tag)::value;
     constexpr bool ALIGNED = kAlign >= 4;
-    // Pair A starts at row0 + 8 rh + off rounded down to a dword (8 rh
-    // keeps the byte shift), pair B at a lane constant (8 - 16 rh) from it.
-    // The bases are opaque: all read offsets (at most 400 bytes) then fit
-    // the 8-bit offsets of ds_read2.
-    const uint8_t* p = row0 + (8 * rh + off);
-    const uint32_t sh = ALIGNED ? 0u : (uint32_t)(uintptr_t)p & 3;
-    LdsWords wa = (LdsWords)((LdsBytes)p - sh);
-    asm volatile("" : "+v"(wa));
-    LdsWords wb = (LdsWords)((LdsBytes)wa + dab);
-    asm volatile("" : "+v"(wb));
+    // Pair A starts at the probe rounded down to a dword (8 rh keeps the
+    // byte shift), pair B at dab from it.  The bases are opaque: all read
+    // offsets (at most 400 bytes) then fit the 8-bit offsets of ds_read2.
+    const uint32_t p = (win + (uint32_t)off) + at;
+    const uint32_t sh = ALIGNED ? 0u : p & 3;
+    uint32_t pa = p - sh;
+    asm volatile("" : "+v"(pa));
+    uint32_t pb = pa + L.dab;
+    asm volatile("" : "+v"(pb));
+    LdsWords wa = lds_words(pa), wb = lds_words(pb);
     constexpr int P = 8 * G::lp >> 2;  // row kk + 8
     uint4 q0, q1;
     if (kAlign >= 8) {
@@ -197,10 +218,10 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
   if constexpr (WAVE_ZERO) {
     // SAD at (0, 0) over the whole wave: lane = (row l / 4, dword l % 4),
     // both operands dword aligned.
-    uint32_t z = sad4(zsrc, *reinterpret_cast<const uint32_t*>(zrow), 0);
+    uint32_t z = sad4(zsrc, *lds_words(win + L.blend), 0);
     bsad = wave_sum_to_sgpr(sum16(z));
   } else {
-    bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(0, std::integral_constant<int, 4>{}));
+    bsad = (uint32_t)__builtin_amdgcn_readfirstlane((int)sad_at(L.at0, 0, std::integral_constant<int, 4>{}));
   }
   // Probe order is the group index and the incumbent keeps ties, so the key
   // sad << 11 | group << 7 | (probe offset + 64) against best << 11 yields
@@ -214,7 +235,8 @@ __device__ __forceinline__ void descend(const uint8_t* row0, const uint4 src0, c
     // alignment of the probe offsets: step 8 from the centre (boff = 0),
     // step 4 by whole dwords, steps 2 and 1 by bytes
     constexpr int kStepAlign = sh == 3 ? 8 : (sh == 2 ? 4 : 1);
-    const uint32_t s = sad_at(boff + gd_off * (1 << sh), std::integral_constant<int, kStepAlign>{});
+    const uint32_t at = sh == 3 ? L.at8 : (sh == 2 ? L.at4 : (sh == 1 ? L.at2 : L.at1));
+    const uint32_t s = sad_at(at, boff, std::integral_constant<int, kStepAlign>{});
     const uint32_t kmin = wave_min_to_sgpr((s << 11) | key_lo);
     if (kmin < (bsad << 11)) {
       bsad = kmin >> 11;
@@ -353,6 +375,19 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   // search: lane = probe group x row pair kk (rows kk, kk + 8); gq is the
   // group's probe and rh its dword pair order
   const int gq = lane_probe(lane), kk = lane & 7, rh = probe_pair_order(gq);
+  const int gd_off = (((kProbeDy1 >> (2 * gq)) & 3) - 1) * kLumaPitch + (((kProbeDx1 >> (2 * gq)) & 3) - 1);
+  SearchLanes L;
+  {
+    const uint32_t at0 = (uint32_t)((kk + 15) * kLumaPitch + 16 + 8 * rh);
+    L.at0 = at0;
+    L.at8 = at0 + (uint32_t)(gd_off * 8);
+    L.at4 = at0 + (uint32_t)(gd_off * 4);
+    L.at2 = at0 + (uint32_t)(gd_off * 2);
+    L.at1 = at0 + (uint32_t)gd_off;
+    L.dab = (uint32_t)(8 - 16 * rh);
+    L.blend = (uint32_t)((lr + 15) * kLumaPitch + 16 + 4 * lq);
+    asm volatile("" : "+v"(L.at8), "+v"(L.at4), "+v"(L.at2), "+v"(L.at1), "+v"(L.at0), "+v"(L.dab), "+v"(L.blend));
+  }
   constexpr bool kDma = PATH == kPathFast || PATH == kPathDma;
   // window pieces may lie outside the frame's columns (filled with the edge pixel)
   constexpr bool kMasked = PATH == kPathDma;
@@ -402,7 +437,7 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   int cur = 0;  // window buffer of the frame being searched
   FrameDesc F{};
   if (k < nnb) {
-    F = frame_at(k < a.center ? k : k + 1);
+    F = frame_at(skip_centre(k, a.center));
     if (kDma) dma_window<kMasked>(plane_rsrc(F.y, (uint32_t)a.h * F.y_stride), off0, off1, off2, win, lane, outside);
     if (kMasked) fetch_edges(F.y);
   }
@@ -413,10 +448,10 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
   uint32_t ru = 0, rv = 0;
   const int nf = (nnb - split + K - 1) / K;
   for (int it = 0; k < nnb; k += K, it++) {
-    const int fi = k < a.center ? k : k + 1;
+    const int fi = skip_centre(k, a.center);
     const int kn = k + K;
     FrameDesc Fn{};
-    if (kn < nnb) Fn = frame_at(kn < a.center ? kn : kn + 1);
+    if (kn < nnb) Fn = frame_at(skip_centre(kn, a.center));
     uint8_t* w = win + cur * kLumaWin;
     pace(it, nf);
     if (kDma) wait_vm();  // this frame's window and the previous frame's chroma have landed
@@ -444,21 +479,18 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     int bestx, besty;
     uint32_t best_sad;
     {
-      const int gd_off = (((kProbeDy1 >> (2 * gq)) & 3) - 1) * kLumaPitch + (((kProbeDx1 >> (2 * gq)) & 3) - 1);
-      const uint8_t* row0 = w + (kk + 15) * kLumaPitch + 16;
+      const uint32_t wl = lds_addr(w);
       if constexpr (PATH == kPathFast) {
-        // (0, 0) operands of the whole-wave SAD: row lane / 4, dword lane % 4
-        const uint8_t* zrow = w + (lr + 15) * kLumaPitch + 16 + 4 * lq;
-        descend<kMaskNone, true, MbGeom>(row0, src0, src1, gd_off, gq, kk, rh, 16, 16, bestx, besty, best_sad, zrow,
+        // (0, 0) operands of the whole-wave SAD: the lane's blend dword
+        descend<kMaskNone, true, MbGeom>(wl, L, src0, src1, gd_off, gq, kk, rh, 16, 16, bestx, besty, best_sad,
                                          s.ctr4);
       } else if constexpr (PATH == kPathDma) {
         // a DMA MB is 16 pixels wide: a partial-width MB has a window piece
         // straddling the right edge, which sends it to the gather path
-        descend<kMaskRows, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, rh, 16, s.bh, bestx, besty, best_sad,
-                                          nullptr, 0);
+        descend<kMaskRows, false, MbGeom>(wl, L, src0, src1, gd_off, gq, kk, rh, 16, s.bh, bestx, besty, best_sad, 0);
       } else {
-        descend<kMaskBoth, false, MbGeom>(row0, src0, src1, gd_off, gq, kk, rh, s.bw, s.bh, bestx, besty, best_sad,
-                                          nullptr, 0);
+        descend<kMaskBoth, false, MbGeom>(wl, L, src0, src1, gd_off, gq, kk, rh, s.bw, s.bh, bestx, besty, best_sad,
+                                          0);
       }
       if (a.mv_out && lane == 0) {
         int32_t* o = a.mv_out + 3 * (((size_t)fi * a.brows + s.brow) * a.bcols + (s.bx >> 4));
@@ -474,9 +506,11 @@ __device__ __forceinline__ void frame_loop(const ArnrGroup& a, const FrameAt& fr
     pend = best_sad < (kSadPerPixelCutoff + 1) * (uint32_t)(s.bw * s.bh);
     if (pend) {
       const int cmvx = bestx / 2, cmvy = besty / 2;  // truncation toward zero
-      const uint8_t* rp4 = w + (lr + besty + 15) * kLumaPitch + 16 + 4 * lq + bestx;
-      const uint32_t rsh = (uint32_t)(uintptr_t)rp4 & 3;
-      const uint32_t* rw = reinterpret_cast<const uint32_t*>(rp4 - rsh);
+      // the lane's 4 reference pixels at the vector: the lane's blend offset
+      // plus the wave-uniform window base and vector offset
+      const uint32_t rp4 = (lds_addr(w) + (uint32_t)(besty * kLumaPitch + bestx)) + L.blend;
+      const uint32_t rsh = rp4 & 3;
+      const LdsWords rw = lds_words(rp4 - rsh);
       const uint32_t ref4 = funnel(rw[0], rw[1], rsh);
       if (PATH == kPathFast) {
         // the block's chroma origin at the vector (inside the plane here) as
@@ -521,7 +555,7 @@ __device__ __forceinline__ TableEntry load_table_entry(const ArnrGroup& a, const
   TableEntry e;
   if (lane < count) {
     const int k = split + lane * K;
-    const int fi = k < a.center ? k : k + 1;
+    const int fi = skip_centre(k, a.center);
     const int32_t* t = table + 3 * (((size_t)fi * a.brows + s.brow) * a.bcols + (s.bx >> 4));
     e.x = t[0];
     e.y = t[1];
@@ -556,7 +590,7 @@ __device__ __forceinline__ void table_loop(const ArnrGroup& a, const FrameAt& fr
     const int j = __builtin_ctzll(todo);
     todo &= todo - 1;
     const int k = split + j * K;
-    const FrameDesc F = frame_at(k < a.center ? k : k + 1);
+    const FrameDesc F = frame_at(skip_centre(k, a.center));
     const int mvx = __builtin_amdgcn_readlane(e.x, j), mvy = __builtin_amdgcn_readlane(e.y, j);
     const uint8_t* rrow = F.y + (size_t)clampi(s.by + lr + mvy, 0, a.h - 1) * F.y_stride;
     const int xr = s.bx + 4 * lq + mvx;
