@@ -70,6 +70,18 @@ struct ConvArgs {
   int images, cw, ch;
 };
 
+// Inference form of a 3x3 conv (bias + leaky ReLU): persistent workgroups
+// that stage the weights once and overlap each tile's input loads with the
+// previous tile's products.  With `head_params` set (the last layer) the
+// activations are not stored: the head's vector goes to the alignment table.
+struct ConvInferArgs {
+  ConvArgs c;
+  const float* head_params;  // W5[2][32], b5[2], or null
+  int32_t* table;            // alignment table (head)
+  int frame_of_image[64];    // table frame index of each image (head)
+  int clamp;                 // |mv| bound (head)
+};
+
 struct WgradArgs {
   const uint16_t* dy;    // [images][ch][cw][cout]
   const uint16_t* x;     // [images][ch][cw][cin]
@@ -116,6 +128,7 @@ hipError_t launch_wgrad(const WgradArgs& a, int taps, int cin, int cout, int slo
 hipError_t launch_reduce_partials(const float* partials, int slots, int n, float* out, hipStream_t s);
 int head_blocks(long long cells);
 hipError_t launch_head(const HeadArgs& a, hipStream_t s);
+hipError_t launch_conv_infer(const ConvInferArgs& a, int cin, int cout, int cus, hipStream_t s);
 hipError_t launch_adam(const AdamArgs& a, hipStream_t s);
 hipError_t launch_pack_weights(const PackArgs& a, hipStream_t s);
 

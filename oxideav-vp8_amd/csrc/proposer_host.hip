@@ -301,19 +301,57 @@ int vp8g_proposer_propose_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
   if (st != VP8G_OK) return st;
   uint8_t* planes = p->arena + L.planes;
   st = vp8g_proposer_prepare_dev(p, frames, n, center, planes);
-  if (st == VP8G_OK) st = forward(p, planes, images, cw, ch, L, true);
   if (st != VP8G_OK) return st;
-  HeadArgs hd{};
-  hd.a4 = reinterpret_cast<const uint16_t*>(p->arena + L.a[kLayers - 1]);
-  hd.params = p->params + kHeadOffset;
-  hd.images = images;
-  hd.cells = cw * ch;
-  hd.table = table;
-  hd.clamp = clamp;
-  int k = 0;
-  for (size_t i = 0; i < n; i++)
-    if (i != center) hd.frame_of_image[k++] = (int)i;
-  if (!ok(launch_head(hd, p->ctx->stream))) return VP8G_E_DEVICE;
+  int frame_of_image[64];
+  {
+    int k = 0;
+    for (size_t i = 0; i < n; i++)
+      if (i != center) frame_of_image[k++] = (int)i;
+  }
+  // The inference convs address their input with 32-bit buffer offsets
+  // (64 channels of every cell below 2^31 bytes); larger batches take
+  // forward() and head_kernel, which give the same table.
+  if ((long long)images * cw * ch * 64 * 2 > 0x7FFFFFFFLL) {
+    st = forward(p, planes, images, cw, ch, L, true);
+    if (st != VP8G_OK) return st;
+    HeadArgs hd{};
+    hd.a4 = reinterpret_cast<const uint16_t*>(p->arena + L.a[kLayers - 1]);
+    hd.params = p->params + kHeadOffset;
+    hd.images = images;
+    hd.cells = cw * ch;
+    hd.table = table;
+    hd.clamp = clamp;
+    for (int k = 0; k < images; k++) hd.frame_of_image[k] = frame_of_image[k];
+    return ok(launch_head(hd, p->ctx->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+  }
+  // Inference: the features kernel applies the first layer, the 3x3 layers
+  // run in their inference form (every load of a workgroup issued at its
+  // start) and the last one applies the head, writing the table (the same
+  // activations and vectors as forward() followed by head_kernel).
+  uint8_t* b = p->arena;
+  hipStream_t s = p->ctx->stream;
+  FeatArgs f{planes, nullptr, images, cw, ch, p->wf[0], p->params + conv_offset(0) + conv_weights(0),
+             reinterpret_cast<uint16_t*>(b + L.a[0])};
+  if (!ok(launch_features(f, s))) return VP8G_E_DEVICE;
+  const uint16_t* in = f.out0;
+  for (int l = 1; l < kLayers; l++) {
+    ConvInferArgs a{};
+    a.c.in = in;
+    a.c.out = reinterpret_cast<uint16_t*>(b + L.a[l]);
+    a.c.w = p->wf[l];
+    a.c.bias = p->params + conv_offset(l) + conv_weights(l);
+    a.c.images = images;
+    a.c.cw = cw;
+    a.c.ch = ch;
+    if (l == kLayers - 1) {
+      a.head_params = p->params + kHeadOffset;
+      a.table = table;
+      a.clamp = clamp;
+      for (int k = 0; k < images; k++) a.frame_of_image[k] = frame_of_image[k];
+    }
+    if (!ok(launch_conv_infer(a, kConv[l].cin, kConv[l].cout, p->ctx->cus, s))) return VP8G_E_DEVICE;
+    in = a.c.out;
+  }
   return VP8G_OK;
 }
 

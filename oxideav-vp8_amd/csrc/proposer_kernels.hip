@@ -331,6 +331,160 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs a) {
   }
 }
 
+// Inference 3x3 conv, bias + leaky ReLU: the products, their order and the
+// epilogue are those of conv_kernel<9, CIN, COUT, EPI_BIAS_RELU>, so the
+// activations are identical.  A launch is short and its loads meet an idle
+// memory system all at once, so a load round costs about as much as the
+// whole layer's products: each workgroup therefore issues every load it will
+// need at its start.  Workgroup v (v spread so that consecutive ones share an
+// XCD and so its L2) owns tiles NT v .. NT v + NT - 1 (64-cell row segments
+// in row order, so its tiles share input rows) and, by buffer loads into LDS,
+// stages the layer's weights and each tile's 3-row input with its halo in
+// one round.  Each lane loads the global chunk that belongs at its LDS slot
+// under the 16-byte XOR swizzle (the swizzle is an involution), and chunks
+// outside the image are written as zeros.  After one wait and one barrier
+// each wave copies into registers the A fragment of every (tap, 32-channel
+// chunk, 16-output block) product it issues, then runs its tiles from LDS.
+// HEAD (the last layer): each wave writes its 16 cells' 32 activations to
+// LDS and 16 of its lanes apply the head to them with head_kernel's fp32
+// arithmetic and order, writing the alignment table.
+template <int CIN, int COUT, bool HEAD, int NT>
+__global__ void __launch_bounds__(256) conv3_infer_kernel(const ConvInferArgs A) {
+  const ConvArgs& a = A.c;
+  constexpr int CK = CIN / 8;   // 16-byte chunks per cell
+  constexpr int TW = 64;        // cells per tile
+  constexpr int IW = TW + 2;    // staged cells per row
+  constexpr int NIN = 3 * IW * CK;
+  constexpr int NW = 9 * COUT * CK;
+  constexpr int IITERS = (NIN + 255) / 256, WITERS = (NW + 255) / 256;
+  constexpr int NC = CIN / 32, NM = COUT / 16;
+  static_assert(!HEAD || COUT == 32, "the head reads 32 channels");
+  static_assert(WITERS + NT * IITERS <= 63, "loads in flight fit vmcnt");
+  __shared__ __align__(16) uint4 s_w[NW];
+  __shared__ __align__(16) uint4 s_in[NT][NIN];
+  __shared__ __align__(16) uint16_t s_a4[HEAD ? TW * 32 : 8];
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+  const int segs = (a.cw + TW - 1) / TW;
+  const int total = a.images * a.ch * segs;
+  const int v = xcd_band(blockIdx.x, gridDim.x);
+  const int tile0 = v * NT;
+  if (tile0 >= total) return;  // the whole workgroup
+  const int ntile = min(NT, total - tile0);
+  using LdsP = __attribute__((address_space(3))) void*;
+  {
+    const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)a.w, (short)0, NW * 16, 0x00020000);
+#pragma unroll
+    for (int k = 0; k < WITERS; k++) {
+      const int slot = tid + 256 * k;
+      if (slot < NW) {
+        const int row = slot / CK, ck = swz<CIN>(row, slot - row * CK);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(wr, (LdsP)(s_w + 256 * k + 64 * wave), 16, (uint32_t)(row * CK + ck) * 16u,
+                                                 0, 0, 0);
+      }
+    }
+    const uint32_t in_bytes = (uint32_t)a.images * a.ch * a.cw * CIN * 2;
+    const __amdgpu_buffer_rsrc_t ir = __builtin_amdgcn_make_buffer_rsrc((void*)a.in, (short)0, (int)in_bytes, 0x00020000);
+    for (int j = 0; j < ntile; j++) {
+      const int t = tile0 + j;
+      const int img = t / (a.ch * segs);
+      const int rem = t - img * a.ch * segs;
+      const int cy = rem / segs, cx0 = (rem - cy * segs) * TW;
+#pragma unroll
+      for (int k = 0; k < IITERS; k++) {
+        const int slot = tid + 256 * k;
+        if (slot < NIN) {
+          const int row = slot / CK, ck = swz<CIN>(row, slot - row * CK);  // row = r * IW + c
+          const int r = row / IW, c = row - r * IW;
+          const int y = cy + r - 1, x = cx0 + c - 1;
+          if (y >= 0 && y < a.ch && x >= 0 && x < a.cw) {
+            const uint32_t off = (uint32_t)((((img * a.ch + y) * a.cw + x) * CK + ck) * 16);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ir, (LdsP)(&s_in[j][256 * k + 64 * wave]), 16, off, 0, 0, 0);
+          } else {
+            s_in[j][slot] = make_uint4(0, 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  bf16x8 wr[9][NC][NM];
+#pragma unroll
+  for (int t = 0; t < 9; t++)
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int m = 0; m < NM; m++) {
+        const int arow = t * COUT + 16 * m + (l & 15), ck = 4 * c + (l >> 4);
+        wr[t][c][m] = as_bf16x8(s_w[arow * CK + swz<CIN>(arow, ck)]);
+      }
+  for (int j = 0; j < ntile; j++) {
+    const int tile = tile0 + j;
+    const int img = tile / (a.ch * segs);
+    const int rem = tile - img * a.ch * segs;
+    const int cy = rem / segs, cx0 = (rem - cy * segs) * TW;
+    const uint4* sb = s_in[j];
+    f32x4 acc[NM];
+#pragma unroll
+    for (int m = 0; m < NM; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+      const int dy = t / 3, dx = t - 3 * (t / 3);
+      const int brow = dy * IW + 16 * wave + (l & 15) + dx;
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int ck = 4 * c + (l >> 4);
+        const bf16x8 bv = as_bf16x8(sb[brow * CK + swz<CIN>(brow, ck)]);
+#pragma unroll
+        for (int m = 0; m < NM; m++) acc[m] = mfma16(wr[t][c][m], bv, acc[m]);
+      }
+    }
+    const int cell_x = cx0 + 16 * wave + (l & 15);
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const int co = 16 * m + 4 * (l >> 4);
+      float vv[4] = {acc[m][0], acc[m][1], acc[m][2], acc[m][3]};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float z = vv[i] + a.bias[co + i];
+        vv[i] = z > 0.f ? z : kLeak * z;
+      }
+      uint2 o;
+      o.x = (uint32_t)bf16_bits(vv[0]) | ((uint32_t)bf16_bits(vv[1]) << 16);
+      o.y = (uint32_t)bf16_bits(vv[2]) | ((uint32_t)bf16_bits(vv[3]) << 16);
+      if (HEAD) {
+        *reinterpret_cast<uint2*>(s_a4 + (16 * wave + (l & 15)) * 32 + co) = o;
+      } else if (cell_x < a.cw) {
+        const size_t cell = ((size_t)img * a.ch + cy) * a.cw + cell_x;
+        *reinterpret_cast<uint2*>(a.out + cell * COUT + co) = o;
+      }
+    }
+    if (HEAD) {
+      // a cell's 32 channels come from this wave's lanes only
+      wave_lds_sync();
+      if (l < 16 && cell_x < a.cw) {
+        float x[32];
+        const uint16_t* src = s_a4 + (16 * wave + l) * 32;
+#pragma unroll
+        for (int c = 0; c < 32; c++) x[c] = bf16_val(src[c]);
+        const int cells = a.cw * a.ch;
+        int32_t* tb = A.table + 3 * ((size_t)A.frame_of_image[img] * cells + (size_t)cy * a.cw + cell_x);
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+          float sum = A.head_params[64 + o];
+#pragma unroll
+          for (int c = 0; c < 32; c++) sum += A.head_params[32 * o + c] * x[c];
+          float mv = rintf(sum * kMvScale);
+          mv = fminf(fmaxf(mv, (float)-A.clamp), (float)A.clamp);
+          tb[o] = (int32_t)mv;
+        }
+        tb[2] = 0;
+      }
+      wave_lds_sync();  // the reads of s_a4 are done before the next tile writes it
+    }
+  }
+}
+
 // ---------------------------------------------------------------- wgrad
 // Partial weight and bias gradients of one conv over a slice of 32-cell
 // chunks: dW[t][co][ci] = sum_p dy[p][co] * x[p + d_t][ci], db[co] =
@@ -604,6 +758,25 @@ hipError_t launch_conv(const ConvArgs& a, int taps, int cin, int cout, int epi, 
   VP8G_CONV(9, 32, 64, EPI_MASK)
 #undef VP8G_CONV
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_conv_infer(const ConvInferArgs& a, int cin, int cout, int cus, hipStream_t s) {
+  const int total = a.c.images * a.c.ch * ((a.c.cw + 63) / 64);
+  if (total == 0) return hipSuccess;
+  // The input bytes must fit the 32-bit buffer offsets.
+  if ((long long)a.c.images * a.c.ch * a.c.cw * cin * 2 > 0x7FFFFFFFLL) return hipErrorInvalidValue;
+  const bool head = a.head_params != nullptr;
+  // NT tiles per workgroup: 4 with 64 input channels (weights and four
+  // inputs in 138 KB of LDS, one workgroup per CU), else 2 (two per CU).
+  const int nt = cin == 64 ? 4 : 2;
+  const dim3 grid((unsigned)((total + nt - 1) / nt)), block(256);
+  (void)cus;
+  if (cin == 32 && cout == 32 && !head) hipLaunchKernelGGL((conv3_infer_kernel<32, 32, false, 2>), grid, block, 0, s, a);
+  else if (cin == 32 && cout == 32 && head) hipLaunchKernelGGL((conv3_infer_kernel<32, 32, true, 2>), grid, block, 0, s, a);
+  else if (cin == 32 && cout == 64 && !head) hipLaunchKernelGGL((conv3_infer_kernel<32, 64, false, 2>), grid, block, 0, s, a);
+  else if (cin == 64 && cout == 32 && !head) hipLaunchKernelGGL((conv3_infer_kernel<64, 32, false, 4>), grid, block, 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 int wgrad_slots(const WgradArgs& a, int cus) {
