@@ -46,11 +46,14 @@ struct PrepArgs {
   const uint8_t* neighbours[64];
   uint32_t neighbour_stride[64];
   int pairs, w, h, cw, ch;
-  uint8_t* out;  // [pairs][2][4 ch][4 cw]
+  uint8_t* out;  // [pairs][2][4 ch][4 cw], or with shared_anchor [1 + pairs][4 ch][4 cw]
+  // The anchor downsampled once as plane 0, neighbour p as plane 1 + p
+  // (inference); otherwise each pair holds its own anchor copy (training).
+  int shared_anchor;
 };
 
 struct FeatArgs {
-  const uint8_t* planes;  // [images][2][4 ch][4 cw]
+  const uint8_t* planes;  // [images][2][4 ch][4 cw], or with shared_anchor [1 + images][4 ch][4 cw]
   uint16_t* out;          // [images][ch][cw][kFeat] bf16 (when w0 is null)
   int images, cw, ch;
   // Inference: the first layer (1x1 conv kFeat -> 32, bias, leaky ReLU)
@@ -59,6 +62,7 @@ struct FeatArgs {
   const uint16_t* w0;     // [32][kFeat] bf16, or null
   const float* b0;        // [32]
   uint16_t* out0;
+  int shared_anchor;      // planes hold one anchor plane and then the neighbours
 };
 
 struct ConvArgs {

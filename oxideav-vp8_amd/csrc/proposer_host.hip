@@ -244,14 +244,13 @@ int vp8g_proposer_get_grads(vp8g_proposer* p, float* host, size_t n, float* loss
   return good ? VP8G_OK : VP8G_E_DEVICE;
 }
 
-int vp8g_proposer_prepare_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t n, size_t center,
-                              uint8_t* planes) {
-  if (!p || !planes || ((uintptr_t)planes & 15)) return VP8G_E_ARG;
-  int st = vp8g_validate_group(frames, n, center, nullptr);
-  if (st != VP8G_OK) return st;
-  if (n < 2) return VP8G_OK;
-  if (!ok(hipSetDevice(p->ctx->device))) return VP8G_E_DEVICE;
+// The downsampled planes of a group: [pairs][2] (each pair's anchor and
+// neighbour), or with `shared_anchor` the anchor once and then the
+// neighbours (inference).
+static int prepare(vp8g_proposer* p, const vp8g_i420* frames, size_t n, size_t center, uint8_t* planes,
+                   bool shared_anchor) {
   PrepArgs a{};
+  a.shared_anchor = shared_anchor ? 1 : 0;
   const vp8g_i420& c = frames[center];
   a.anchor = c.y;
   a.anchor_stride = (uint32_t)c.y_stride;
@@ -269,6 +268,16 @@ int vp8g_proposer_prepare_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
   a.ch = (a.h + 15) / 16;
   a.out = planes;
   return ok(launch_prep_downsample(a, p->ctx->stream)) ? VP8G_OK : VP8G_E_DEVICE;
+}
+
+int vp8g_proposer_prepare_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t n, size_t center,
+                              uint8_t* planes) {
+  if (!p || !planes || ((uintptr_t)planes & 15)) return VP8G_E_ARG;
+  int st = vp8g_validate_group(frames, n, center, nullptr);
+  if (st != VP8G_OK) return st;
+  if (n < 2) return VP8G_OK;
+  if (!ok(hipSetDevice(p->ctx->device))) return VP8G_E_DEVICE;
+  return prepare(p, frames, n, center, planes, false);
 }
 
 int vp8g_proposer_forward_dev(vp8g_proposer* p, const uint8_t* planes, int images, int cw, int ch,
@@ -300,7 +309,11 @@ int vp8g_proposer_propose_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
   st = ensure(p, L.total);
   if (st != VP8G_OK) return st;
   uint8_t* planes = p->arena + L.planes;
-  st = vp8g_proposer_prepare_dev(p, frames, n, center, planes);
+  // The inference convs address their input with 32-bit buffer offsets
+  // (64 channels of every cell below 2^31 bytes); larger batches take
+  // forward() and head_kernel, which give the same table.
+  const bool big = (long long)images * cw * ch * 64 * 2 > 0x7FFFFFFFLL;
+  st = prepare(p, frames, n, center, planes, !big);
   if (st != VP8G_OK) return st;
   int frame_of_image[64];
   {
@@ -308,10 +321,7 @@ int vp8g_proposer_propose_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
     for (size_t i = 0; i < n; i++)
       if (i != center) frame_of_image[k++] = (int)i;
   }
-  // The inference convs address their input with 32-bit buffer offsets
-  // (64 channels of every cell below 2^31 bytes); larger batches take
-  // forward() and head_kernel, which give the same table.
-  if ((long long)images * cw * ch * 64 * 2 > 0x7FFFFFFFLL) {
+  if (big) {
     st = forward(p, planes, images, cw, ch, L, true);
     if (st != VP8G_OK) return st;
     HeadArgs hd{};
@@ -331,7 +341,7 @@ int vp8g_proposer_propose_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
   uint8_t* b = p->arena;
   hipStream_t s = p->ctx->stream;
   FeatArgs f{planes, nullptr, images, cw, ch, p->wf[0], p->params + conv_offset(0) + conv_weights(0),
-             reinterpret_cast<uint16_t*>(b + L.a[0])};
+             reinterpret_cast<uint16_t*>(b + L.a[0]), 1};
   if (!ok(launch_features(f, s))) return VP8G_E_DEVICE;
   const uint16_t* in = f.out0;
   for (int l = 1; l < kLayers; l++) {

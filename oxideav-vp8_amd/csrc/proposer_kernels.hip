@@ -37,11 +37,12 @@ __device__ __forceinline__ uint16_t bf16_bits(float f) { return __builtin_bit_ca
 __device__ __forceinline__ float bf16_val(uint16_t b) { return (float)__builtin_bit_cast(__bf16, b); }
 
 // ---------------------------------------------------------------- inputs
-// One thread per 4 horizontally adjacent downsampled pixels of a (pair,
-// frame) plane: 4 rows of 16 full-resolution bytes (dword loads when the
-// row segment is inside the frame and aligned).
-// Grid: x over the plane's quads, y over the 2 * pairs planes (so the
-// plane, its source frame and stride are uniform per workgroup).
+// One thread per 4 horizontally adjacent downsampled pixels of a plane: 4
+// rows of 16 full-resolution bytes (dword loads when the row segment is
+// inside the frame and aligned).
+// Grid: x over the plane's quads, y over the planes: 2 * pairs (anchor and
+// neighbour of each pair), or 1 + pairs with a shared anchor (so the plane,
+// its source frame and stride are uniform per workgroup).
 __global__ void prep_downsample_kernel(const PrepArgs a) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const int pf = blockIdx.y;
@@ -49,9 +50,10 @@ __global__ void prep_downsample_kernel(const PrepArgs a) {
   const uint32_t qw = (uint32_t)a.cw;  // quads per downsampled row (dw / 4)
   if (t >= qw * (uint32_t)dh) return;
   const int y = (int)(t / qw), xq = (int)(t - (uint32_t)y * qw);
-  const int p = pf >> 1, f = pf & 1;
-  const uint8_t* src = f == 0 ? a.anchor : a.neighbours[p];
-  const uint32_t stride = f == 0 ? a.anchor_stride : a.neighbour_stride[p];
+  const int p = a.shared_anchor ? pf - 1 : pf >> 1;
+  const bool anchor = a.shared_anchor ? pf == 0 : (pf & 1) == 0;
+  const uint8_t* src = anchor ? a.anchor : a.neighbours[p];
+  const uint32_t stride = anchor ? a.anchor_stride : a.neighbour_stride[p];
   const int x0 = 16 * xq;
   const bool fast = x0 + 16 <= a.w && ((((uintptr_t)src) | stride) & 3) == 0;
   uint32_t sum[4] = {0, 0, 0, 0};
@@ -92,12 +94,16 @@ __global__ void __launch_bounds__(320) features_kernel(const FeatArgs a) {
   constexpr int TPC = 2 * kSearch + 2;  // threads per cell
   __shared__ uint32_t sA[NA];
   __shared__ uint32_t sB[NB];
-  __shared__ __align__(16) uint16_t s_out[kFeatSeg * kFeat];
+  // a cell's kFeat channels at a row pitch of kFeat + 8 (272 bytes, 68
+  // dwords): the cells' rows start 4 banks apart, so the 2-byte writes of
+  // one displacement from different cells do not meet in one bank
+  constexpr int kRow = kFeat + 8;
+  __shared__ __align__(16) uint16_t s_out[kFeatSeg * kRow];
   const int cx0 = blockIdx.x * kFeatSeg, cy = blockIdx.y, img = blockIdx.z;
   const int dw = 4 * a.cw, dh = 4 * a.ch;
   const int ncell = min(kFeatSeg, a.cw - cx0);
-  const uint8_t* A = a.planes + (size_t)img * 2 * dw * dh;
-  const uint8_t* B = A + (size_t)dw * dh;
+  const uint8_t* A = a.shared_anchor ? a.planes : a.planes + (size_t)img * 2 * dw * dh;
+  const uint8_t* B = a.shared_anchor ? a.planes + (size_t)(1 + img) * dw * dh : A + (size_t)dw * dh;
   const int x0 = 4 * cx0, y0 = 4 * cy;
   const int tid = threadIdx.x;
   // staging dword i: anchor (row i / AD) or neighbour (row (i - NA) / BD,
@@ -126,7 +132,7 @@ __global__ void __launch_bounds__(320) features_kernel(const FeatArgs a) {
   __syncthreads();
   const int jc = tid / TPC, k = tid - jc * TPC;
   if (jc < ncell) {
-    uint16_t* out = s_out + jc * kFeat;
+    uint16_t* out = s_out + jc * kRow;
     uint32_t arow[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) arow[r] = sA[r * AD + jc];
@@ -181,7 +187,7 @@ __global__ void __launch_bounds__(320) features_kernel(const FeatArgs a) {
     if (wave < 4) {
       const int m = wave >> 1, cell = 16 * (wave & 1) + (l & 15);
       const uint4* wrow = reinterpret_cast<const uint4*>(a.w0 + (size_t)(16 * m + (l & 15)) * kFeat);
-      const uint4* xrow = reinterpret_cast<const uint4*>(s_out + cell * kFeat);
+      const uint4* xrow = reinterpret_cast<const uint4*>(s_out + cell * kRow);
       f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int c = 0; c < kFeat / 32; c++) {
@@ -207,7 +213,7 @@ __global__ void __launch_bounds__(320) features_kernel(const FeatArgs a) {
   // the segment's cells: consecutive rows of kFeat bf16 = 16 uint4 each
   uint4* dst = reinterpret_cast<uint4*>(a.out + (((size_t)img * a.ch + cy) * a.cw + cx0) * kFeat);
   const uint4* srcv = reinterpret_cast<const uint4*>(s_out);
-  for (int i = tid; i < ncell * (kFeat / 8); i += 320) dst[i] = srcv[i];
+  for (int i = tid; i < ncell * (kFeat / 8); i += 320) dst[i] = srcv[(i / (kFeat / 8)) * (kRow / 8) + i % (kFeat / 8)];
 }
 
 // ---------------------------------------------------------------- conv3x3
@@ -726,8 +732,8 @@ static int persistent_grid(int total, int per_cu, int cus) {
 hipError_t launch_prep_downsample(const PrepArgs& a, hipStream_t s) {
   const long long quads = (long long)a.cw * (4 * a.ch);  // per plane
   if (quads == 0 || a.pairs == 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_downsample_kernel, dim3((unsigned)((quads + 255) / 256), (unsigned)(2 * a.pairs)), dim3(256),
-                     0, s, a);
+  const unsigned planes = a.shared_anchor ? (unsigned)(1 + a.pairs) : (unsigned)(2 * a.pairs);
+  hipLaunchKernelGGL(prep_downsample_kernel, dim3((unsigned)((quads + 255) / 256), planes), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
