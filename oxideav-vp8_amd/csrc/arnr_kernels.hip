@@ -845,13 +845,20 @@ __device__ __forceinline__ T* uload(T* const* p) {
   return reinterpret_cast<T*>(lo | (hi << 32));
 }
 
+// A frame descriptor through the constant address space: one scalar load
+// into SGPRs (the batch's descriptors are written before the launch and
+// only read during it), instead of vector loads made uniform lane by lane.
 __device__ __forceinline__ FrameDesc load_frame(const FrameDesc* p) {
+  static_assert(sizeof(FrameDesc) == 32, "three pointers and two strides");
+  using C4 = const __attribute__((address_space(4))) uint32_t*;
+  const C4 q = (C4)p;
+  auto ptr = [&](int i) { return reinterpret_cast<const uint8_t*>(((uint64_t)q[i + 1] << 32) | q[i]); };
   FrameDesc f;
-  f.y = uload(&p->y);
-  f.u = uload(&p->u);
-  f.v = uload(&p->v);
-  f.y_stride = uload(&p->y_stride);
-  f.uv_stride = uload(&p->uv_stride);
+  f.y = ptr(0);
+  f.u = ptr(2);
+  f.v = ptr(4);
+  f.y_stride = q[6];
+  f.uv_stride = q[7];
   return f;
 }
 
