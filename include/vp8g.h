@@ -106,9 +106,10 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
  * VP8G_OPT_HOST_THREADS: host threads that pack the caller's planes into
  *   pinned staging and copy results out in vp8g_build_arnr_altref, 1..64
  *   (0 = default, min(8, half the hardware threads)).
- * VP8G_OPT_COPY_STREAMS: streams that carry vp8g_build_arnr_altref's
- *   host<->device copies, 1..4 (0 = default 2); the frames go out
- *   round-robin, each copy engine taking a share. */
+ * VP8G_OPT_COPY_STREAMS: accepted for callers that set it, 0..4, with no
+ *   effect: vp8g_build_arnr_altref's frames go out by the copy engines on
+ *   one stream of its own and the altref rows come back by a copy kernel
+ *   on the context's stream. */
 #define VP8G_OPT_ARNR_WAVES 1
 #define VP8G_OPT_XCD_REMAP 2
 #define VP8G_OPT_ARNR_MBS 3
@@ -123,8 +124,9 @@ int vp8g_ctx_synchronize(vp8g_ctx *ctx);
 /* Row bands of vp8g_build_arnr_altref (host buffers): the frames go out a
  * band of MB rows at a time (each band's rows plus the 15 luma / 7 chroma
  * rows its search reaches below), the kernel runs on each band as soon as
- * its rows are in, and each band's altref rows come back while later bands
- * are still in flight.  1..8; 0 restores the default (2). */
+ * its rows are in while later bands are still on the link, and each band's
+ * altref rows come back as soon as its kernel is done.  The last band is
+ * half as tall as the others.  1..8; 0 restores the default (3). */
 #define VP8G_OPT_HOST_BANDS 8
 int vp8g_ctx_set_option(vp8g_ctx *ctx, int option, int value);
 const char *vp8g_status_str(int status);

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <deque>
 
 #include <mutex>
 #include <string>
@@ -431,17 +432,14 @@ int finish(vp8g_ctx* c, int st) {
   return st;
 }
 
-int copy_stream_count(const vp8g_ctx* c) { return c->copy_streams > 0 ? c->copy_streams : 2; }
-
-// finish() after work on the copy streams: they are drained first, so no
-// copy still reads the staging or writes the pinned mirror on return.
+// finish() after work on the link stream: it is drained first, so no copy
+// still reads the staging on return.
 int finish_io(vp8g_ctx* c, int st) {
-  for (int k = 0; k < vp8g_ctx::kMaxCopyStreams; k++)
-    if (c->copy[k] && !hip_ok(hipStreamSynchronize(c->copy[k]))) st = VP8G_E_DEVICE;
+  if (c->link && !hip_ok(hipStreamSynchronize(c->link))) st = VP8G_E_DEVICE;
   return finish(c, st);
 }
 
-// The host threads, copy streams and events of the host-pointer ARNR entry
+// The host threads, link stream and events of the host-pointer ARNR entry
 // point, and `bytes` of write-combined pinned staging (grown, never shrunk).
 int ensure_host_io(vp8g_ctx* c, size_t bytes) {
   if (c->fault == VP8G_FAULT_ALLOC) return VP8G_E_DEVICE;
@@ -451,15 +449,13 @@ int ensure_host_io(vp8g_ctx* c, size_t bytes) {
     const int threads = c->host_threads > 0 ? c->host_threads : def;
     c->pool = new vp8g::HostPool(threads - 1);  // the calling thread works too
   }
-  for (int k = 0; k < copy_stream_count(c); k++) {
-    if (!c->copy[k] && !hip_ok(hipStreamCreateWithFlags(&c->copy[k], hipStreamNonBlocking))) return VP8G_E_DEVICE;
-    if (!c->copy_done[k] && !hip_ok(hipEventCreateWithFlags(&c->copy_done[k], hipEventDisableTiming)))
+  if (!c->link && !hip_ok(hipStreamCreateWithFlags(&c->link, hipStreamNonBlocking))) return VP8G_E_DEVICE;
+  for (int b = 0; b < vp8g_ctx::kMaxBands; b++) {
+    if (!c->band_in[b] && !hip_ok(hipEventCreateWithFlags(&c->band_in[b], hipEventDisableTiming)))
       return VP8G_E_DEVICE;
-  }
-  if (!c->ready && !hip_ok(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming))) return VP8G_E_DEVICE;
-  for (int b = 0; b < vp8g_ctx::kMaxBands; b++)
     if (!c->band_done[b] && !hip_ok(hipEventCreateWithFlags(&c->band_done[b], hipEventDisableTiming)))
       return VP8G_E_DEVICE;
+  }
   if (bytes <= c->staging_bytes) return VP8G_OK;
   if (c->staging) {
     (void)hipHostFree(c->staging);
@@ -480,6 +476,59 @@ void copy_rows(uint8_t* dst, size_t dp, const uint8_t* src, size_t ss, size_t w,
     return;
   }
   for (size_t r = r0; r < r1; r++) memcpy(dst + r * dp, src + r * ss, w);
+}
+
+// Copies of up to 3 byte ranges between pinned host memory and device
+// memory by a kernel, which reads or writes the host memory over PCIe.  The
+// source and destination of a range share their alignment mod 16: the
+// bytes before the first and after the last 16-byte boundary go one a
+// lane, the rest 16 at a time.
+struct LinkSeg {
+  const uint8_t* src;
+  uint8_t* dst;
+  size_t bytes;
+};
+struct LinkArgs {
+  LinkSeg seg[3];
+  int segs;
+};
+
+__global__ void __launch_bounds__(256) link_copy_kernel(const LinkArgs a) {
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, lanes = (size_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < a.segs; k++) {
+    const uint8_t* src = a.seg[k].src;
+    uint8_t* dst = a.seg[k].dst;
+    const size_t bytes = a.seg[k].bytes;
+    const size_t head = min(bytes, (size_t)((16 - ((uintptr_t)src & 15)) & 15));
+    const size_t body = (bytes - head) & ~(size_t)15, tail = bytes - head - body;
+    if (tid < head) dst[tid] = src[tid];
+    if (tid < tail) dst[head + body + tid] = src[head + body + tid];
+    const uint4* s4 = reinterpret_cast<const uint4*>(src + head);
+    uint4* d4 = reinterpret_cast<uint4*>(dst + head);
+    for (size_t i = tid; i < body / 16; i += lanes) d4[i] = s4[i];
+  }
+}
+
+// 256 workgroups keep enough transfers in flight to fill the link (about
+// 52-56 GB/s measured either way, as the copy engines).
+hipError_t launch_link_copy(const LinkArgs& a, hipStream_t s) {
+  for (int k = 0; k < a.segs; k++)
+    if ((((uintptr_t)a.seg[k].src ^ (uintptr_t)a.seg[k].dst) & 15) != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(link_copy_kernel, dim3(256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// The same ranges by the copy engines: they do not take CU time from the
+// kernels that run meanwhile, and their completion releases a waiting
+// stream within microseconds (a stream waiting on a copy kernel of another
+// stream was measured to start only when some kernel ended).
+hipError_t copy_engine_pull(const LinkArgs& a, hipStream_t s) {
+  for (int k = 0; k < a.segs; k++) {
+    if (!a.seg[k].bytes) continue;
+    const hipError_t e = hipMemcpyAsync(a.seg[k].dst, a.seg[k].src, a.seg[k].bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // A plane copy cut into row bands of about 256 KiB, for the host threads.
@@ -598,17 +647,15 @@ void vp8g_ctx_destroy(vp8g_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->link) (void)hipStreamSynchronize(c->link);
   if (c->arena) (void)hipFree(c->arena);
   if (c->pinned) (void)hipHostFree(c->pinned);  // allocated with the arena
   if (c->work) (void)hipFree(c->work);
-  for (int k = 0; k < vp8g_ctx::kMaxCopyStreams; k++) {
-    if (c->copy[k]) (void)hipStreamSynchronize(c->copy[k]);
-    if (c->copy[k]) (void)hipStreamDestroy(c->copy[k]);
-    if (c->copy_done[k]) (void)hipEventDestroy(c->copy_done[k]);
-  }
-  if (c->ready) (void)hipEventDestroy(c->ready);
-  for (int b = 0; b < vp8g_ctx::kMaxBands; b++)
+  if (c->link) (void)hipStreamDestroy(c->link);
+  for (int b = 0; b < vp8g_ctx::kMaxBands; b++) {
+    if (c->band_in[b]) (void)hipEventDestroy(c->band_in[b]);
     if (c->band_done[b]) (void)hipEventDestroy(c->band_done[b]);
+  }
   if (c->staging) (void)hipHostFree(c->staging);
   if (c->lf_work) (void)hipFree(c->lf_work);
   if (c->lf_io) (void)hipFree(c->lf_io);
@@ -698,132 +745,160 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
   if (!hip_ok(hipSetDevice(c->device))) return VP8G_E_DEVICE;
   const size_t w = frames[center].width, h = frames[center].height;
   const size_t cw = (w + 1) / 2, ch = (h + 1) / 2;
-  // Device layout: the frames' rows interleaved, one plane each of Y, U and
-  // V for the whole group: row r of frame i's luma at (r n + i) yp, so frame
-  // i's plane starts at i yp with stride n yp (pitches rounded to 64 bytes),
-  // and rows a..b-1 of every frame are one contiguous range.  Then the
-  // packed outputs.
+  // Device layout: the frames' rows interleaved, one plane of Y and one of
+  // chroma for the whole group: row r of frame i's luma at (r n + i) yp, so
+  // frame i's plane starts at i yp with stride n yp (pitches rounded to 64
+  // bytes); chroma row r holds every frame's U row r, then every frame's V
+  // row r (stride 2 n cp).  Rows a..b-1 of every frame are then one
+  // contiguous range of each.  Then the packed outputs.
   const size_t yp = align_up(w, 64), cp = align_up(cw, 64);
   const size_t yrow = yp * n, crow = cp * n;  // bytes of one row of every frame
-  const size_t ysz = align_up(yrow * h, 256), csz = align_up(crow * ch, 256);
-  const size_t in_bytes = ysz + 2 * csz;
+  const size_t ysz = align_up(yrow * h, 256), csz = align_up(2 * crow * ch, 256);
+  const size_t in_bytes = ysz + csz;
   const size_t oyb = align_up(w * h, 256), ocb = align_up(cw * ch, 256);
-  if (yrow > 0x7FFFFFFFu || ysz > 0x7FFFFFFFu) return VP8G_E_ARG;
+  if (yrow > 0x7FFFFFFFu || ysz > 0x7FFFFFFFu || csz > 0x7FFFFFFFu) return VP8G_E_ARG;
   st = ensure_arena(c, in_bytes + oyb + 2 * ocb);
   if (st == VP8G_OK) st = ensure_host_io(c, in_bytes);
   if (st != VP8G_OK) return st;
   if (c->fault == VP8G_FAULT_COPY) return VP8G_E_DEVICE;
-  const int S = copy_stream_count(c);
-  // The copies need not wait for the context's stream: the arena and the
-  // staging are only used by the host-pointer entry points, which all
+  // The link copies need not wait for the context's stream: the arena and
+  // the staging are only used by the host-pointer entry points, which all
   // return synchronised, so no queued work reads them.
   uint8_t* const dY = c->arena;
   uint8_t* const dU = dY + ysz;
-  uint8_t* const dV = dU + csz;
+  uint8_t* const dV = dU + crow;
   uint8_t* const pY = c->staging;
   uint8_t* const pU = pY + ysz;
-  uint8_t* const pV = pU + csz;
+  uint8_t* const pV = pU + crow;
   std::vector<vp8g_i420> dev(n);
   for (size_t i = 0; i < n; i++)
-    dev[i] = vp8g_i420{frames[i].width, frames[i].height, dY + i * yp, dU + i * cp, dV + i * cp, yrow, crow};
+    dev[i] = vp8g_i420{frames[i].width, frames[i].height, dY + i * yp, dU + i * cp, dV + i * cp, yrow, 2 * crow};
   uint8_t* dy = c->arena + in_bytes;
   uint8_t* du = dy + oyb;
   uint8_t* dv = du + ocb;
+  // the pinned mirror of the outputs (same offsets as in the arena, so a
+  // range and its mirror share their alignment)
   const size_t off = (size_t)(dy - c->arena);
+  uint8_t* const hy = c->pinned + off;
+  uint8_t* const hu = hy + oyb;
+  uint8_t* const hv = hu + ocb;
   // Row bands.  The search of an MB row reaches 15 luma and 7 chroma rows
   // below it (and as many above), clamped to the frame, so band b (MB rows
   // r0..r1-1) can run once every frame's rows up to 16 r1 + 15 and 8 r1 + 7
-  // are in.  Each band sends only the rows the previous ones did not: host
-  // threads pack them into write-combined staging, then one copy each for
-  // the Y, U and V rows, on the copy stream with the fewest bytes queued;
-  // the band's kernel waits for them on the context's stream, and the
-  // band's altref rows come back on that stream while the copy streams
-  // carry the next band.
+  // are in.  Each band sends only the rows the previous ones did not, as
+  // two pieces (its new luma rows, then its new chroma rows): host threads
+  // pack a piece into the pinned staging, then the copy engines pull it to
+  // the device planes over PCIe on the link stream.  The pulls run back to
+  // back; band b's kernel runs on the context's stream once band_in[b] has
+  // fired, writes the device outputs, and a copy kernel pushes its altref
+  // rows into the pinned mirror (the link carries both directions at once).
+  // The host threads pack piece p + 1 while the calling thread launches
+  // piece p's work (the pieces' staging rows are disjoint), then it joins
+  // them; after the last launch it copies each band's rows out as soon as
+  // band_done[b] has fired.  The last band is made half as tall as the
+  // others, since nothing overlaps its kernel, push and copy-out.
   const int brows = (int)((h + 15) / 16);
-  const int B = std::max(1, std::min(brows, c->host_bands > 0 ? c->host_bands : 2));
-  std::vector<PlaneJob> jobs;
-  jobs.reserve(3 * n);
+  const int B = std::max(1, std::min(brows, c->host_bands > 0 ? c->host_bands : 3));
+  std::vector<int> edge(B + 1);  // band b is MB rows edge[b] .. edge[b + 1] - 1
+  for (int b = 0; b <= B; b++) edge[b] = (int)((int64_t)brows * 2 * b / (2 * B - 1));
+  edge[B] = brows;
+  struct Piece {
+    size_t t0, t1;  // its packing tasks
+    LinkArgs pull;
+    int band;       // the band whose kernel this piece completes, else -1
+  };
+  std::deque<PlaneJob> jobs;  // stable addresses for the tasks
   std::vector<std::pair<const PlaneJob*, size_t>> tasks;
-  size_t queued[vp8g_ctx::kMaxCopyStreams] = {};
-  int failed = 0;
-  auto send = [&](uint8_t* dst, const uint8_t* src, size_t bytes) {
-    if (!bytes) return;
-    int k = 0;
-    for (int j = 1; j < S; j++)
-      if (queued[j] < queued[k]) k = j;
-    queued[k] += bytes;
-    if (!failed && !hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->copy[k]))) failed = 1;
+  std::vector<Piece> pieces;
+  // Rows [r0, r1) of every frame's luma or chroma as one piece (each copy
+  // costs the copy engines about 10 us beyond its bytes, more than a finer
+  // split saves); it completes `band` (-1: none).
+  const auto add_rows = [&](bool luma, size_t r0, size_t r1, int band) {
+    const size_t row = luma ? yrow : 2 * crow;
+    Piece pc{tasks.size(), 0, LinkArgs{}, band};
+    if (r1 > r0) {
+      for (size_t i = 0; i < n; i++) {
+        const vp8g_i420& f = frames[i];
+        if (luma) {
+          jobs.push_back(plane_job(pY + r0 * yrow + i * yp, yrow, f.y + r0 * f.y_stride, f.y_stride, w, r1 - r0));
+          add_plane(tasks, jobs.back());
+        } else {
+          jobs.push_back(plane_job(pU + r0 * row + i * cp, row, f.u + r0 * f.uv_stride, f.uv_stride, cw, r1 - r0));
+          add_plane(tasks, jobs.back());
+          jobs.push_back(plane_job(pV + r0 * row + i * cp, row, f.v + r0 * f.uv_stride, f.uv_stride, cw, r1 - r0));
+          add_plane(tasks, jobs.back());
+        }
+      }
+      pc.pull.seg[0] = {(luma ? pY : pU) + r0 * row, (luma ? dY : dU) + r0 * row, (r1 - r0) * row};
+      pc.pull.segs = 1;
+    }
+    pc.t1 = tasks.size();
+    if (pc.pull.segs || band >= 0) pieces.push_back(pc);
   };
-  auto recv = [&](size_t dev_off, size_t bytes) {
-    if (!failed && bytes &&
-        !hip_ok(hipMemcpyAsync(c->pinned + off + dev_off, dy + dev_off, bytes, hipMemcpyDeviceToHost, c->stream)))
-      failed = 1;
-  };
-  size_t sent_y = 0, sent_c = 0;  // rows of every frame already sent
-  int bands = 0;
-  for (int b = 0; b < B && !failed; b++) {
-    const int r0 = (int)((int64_t)brows * b / B), r1 = (int)((int64_t)brows * (b + 1) / B);
-    if (r1 <= r0) continue;
+  size_t sent_y = 0, sent_c = 0;  // rows of every frame already packed
+  for (int b = 0; b < B; b++) {
+    const int r1 = edge[b + 1];
+    if (r1 <= edge[b]) continue;
     const bool last = b == B - 1;
     const size_t need_y = last ? h : std::min(h, (size_t)16 * r1 + 15);
     const size_t need_c = last ? ch : std::min(ch, (size_t)8 * r1 + 7);
-    // luma first, sent while the chroma rows are packed
-    jobs.clear();
-    tasks.clear();
-    for (size_t i = 0; i < n && need_y > sent_y; i++)
-      jobs.push_back(plane_job(pY + sent_y * yrow + i * yp, yrow, frames[i].y + sent_y * frames[i].y_stride,
-                               frames[i].y_stride, w, need_y - sent_y));
-    for (const PlaneJob& j : jobs) add_plane(tasks, j);
-    run_plane_tasks(c, tasks);
-    send(dY + sent_y * yrow, pY + sent_y * yrow, (need_y - sent_y) * yrow);
-    jobs.clear();
-    tasks.clear();
-    for (size_t i = 0; i < n && need_c > sent_c; i++) {
-      const vp8g_i420& f = frames[i];
-      jobs.push_back(plane_job(pU + sent_c * crow + i * cp, crow, f.u + sent_c * f.uv_stride, f.uv_stride, cw,
-                               need_c - sent_c));
-      jobs.push_back(plane_job(pV + sent_c * crow + i * cp, crow, f.v + sent_c * f.uv_stride, f.uv_stride, cw,
-                               need_c - sent_c));
-    }
-    for (const PlaneJob& j : jobs) add_plane(tasks, j);
-    run_plane_tasks(c, tasks);
-    send(dU + sent_c * crow, pU + sent_c * crow, (need_c - sent_c) * crow);
-    send(dV + sent_c * crow, pV + sent_c * crow, (need_c - sent_c) * crow);
+    if (need_y > sent_y) add_rows(true, sent_y, need_y, -1);
+    add_rows(false, sent_c, need_c, b);
     sent_y = need_y, sent_c = need_c;
-    for (int k = 0; k < S; k++) {
-      if (!failed && (!hip_ok(hipEventRecord(c->copy_done[k], c->copy[k])) ||
-                      !hip_ok(hipStreamWaitEvent(c->stream, c->copy_done[k], 0))))
-        failed = 1;
-    }
-    if (failed) break;
-    st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr, r0, r1);
-    if (st != VP8G_OK) return finish_io(c, st);
-    const size_t y0 = (size_t)16 * r0, y1 = std::min(h, (size_t)16 * r1);
-    const size_t c0 = (size_t)8 * r0, c1 = std::min(ch, (size_t)8 * r1);
-    recv(y0 * w, (y1 - y0) * w);
-    recv(oyb + c0 * cw, (c1 - c0) * cw);
-    recv(oyb + ocb + c0 * cw, (c1 - c0) * cw);
-    if (!failed && !hip_ok(hipEventRecord(c->band_done[bands], c->stream))) failed = 1;
-    bands++;
   }
-  if (failed) return finish_io(c, VP8G_E_DEVICE);
+  size_t base = 0;
+  const std::function<void(size_t)> pack = [&](size_t i) {
+    const PlaneJob& j = *tasks[base + i].first;
+    const size_t q0 = tasks[base + i].second, q1 = std::min(j.h, q0 + j.rows_per_band);
+    copy_rows(j.dst, j.dp, j.src, j.ss, j.w, q0, q1);
+  };
+  base = pieces[0].t0;
+  c->pool->start(pieces[0].t1 - pieces[0].t0, pack);
+  for (size_t k = 0; k < pieces.size(); k++) {
+    c->pool->finish();  // piece k is packed
+    const Piece& pk = pieces[k];
+    if (k + 1 < pieces.size()) {
+      base = pieces[k + 1].t0;
+      c->pool->start(pieces[k + 1].t1 - pieces[k + 1].t0, pack);
+    }
+    if (pk.pull.segs) st = launch_status(c, [&] { return copy_engine_pull(pk.pull, c->link); });
+    if (st == VP8G_OK && pk.band >= 0) {
+      const int b = pk.band, r0 = edge[b], r1 = edge[b + 1];
+      const size_t y0 = (size_t)16 * r0, y1 = std::min(h, (size_t)16 * r1);
+      const size_t c0 = (size_t)8 * r0, c1 = std::min(ch, (size_t)8 * r1);
+      LinkArgs push{};
+      push.seg[0] = {dy + y0 * w, hy + y0 * w, (y1 - y0) * w};
+      push.seg[1] = {du + c0 * cw, hu + c0 * cw, (c1 - c0) * cw};
+      push.seg[2] = {dv + c0 * cw, hv + c0 * cw, (c1 - c0) * cw};
+      push.segs = 3;
+      if (!hip_ok(hipEventRecord(c->band_in[b], c->link)) || !hip_ok(hipStreamWaitEvent(c->stream, c->band_in[b], 0)))
+        st = VP8G_E_DEVICE;
+      if (st == VP8G_OK) st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr, r0, r1);
+      if (st == VP8G_OK) st = launch_status(c, [&] { return launch_link_copy(push, c->stream); });
+      if (st == VP8G_OK && !hip_ok(hipEventRecord(c->band_done[b], c->stream))) st = VP8G_E_DEVICE;
+    }
+    if (st != VP8G_OK) {
+      c->pool->finish();  // no packing task may outlive this call
+      return finish_io(c, st);
+    }
+  }
   // Each band's rows go to the caller as soon as they are back; if a later
   // band fails on the device, the rows of the bands before it are written.
-  for (int b = 0, done = 0; b < B; b++) {
-    const int r0 = (int)((int64_t)brows * b / B), r1 = (int)((int64_t)brows * (b + 1) / B);
+  for (int b = 0; b < B; b++) {
+    const int r0 = edge[b], r1 = edge[b + 1];
     if (r1 <= r0) continue;
-    if (!hip_ok(hipEventSynchronize(c->band_done[done++]))) return finish_io(c, VP8G_E_DEVICE);
+    if (!hip_ok(hipEventSynchronize(c->band_done[b]))) return finish_io(c, VP8G_E_DEVICE);
     const size_t y0 = (size_t)16 * r0, y1 = std::min(h, (size_t)16 * r1);
     const size_t c0 = (size_t)8 * r0, c1 = std::min(ch, (size_t)8 * r1);
-    const PlaneJob outs[3] = {plane_job(oy + y0 * w, w, c->pinned + off + y0 * w, w, w, y1 - y0),
-                              plane_job(ou + c0 * cw, cw, c->pinned + off + oyb + c0 * cw, cw, cw, c1 - c0),
-                              plane_job(ov + c0 * cw, cw, c->pinned + off + oyb + ocb + c0 * cw, cw, cw, c1 - c0)};
+    const PlaneJob outs[3] = {plane_job(oy + y0 * w, w, hy + y0 * w, w, w, y1 - y0),
+                              plane_job(ou + c0 * cw, cw, hu + c0 * cw, cw, cw, c1 - c0),
+                              plane_job(ov + c0 * cw, cw, hv + c0 * cw, cw, cw, c1 - c0)};
     tasks.clear();
     for (const PlaneJob& j : outs) add_plane(tasks, j);
     run_plane_tasks(c, tasks);
   }
-  (void)bands;
-  return finish_io(c, VP8G_OK);
+  // every pull was waited for by a band's kernel, so the link is idle
+  return finish(c, VP8G_OK);
 }
 
 int vp8g_block_search_dev(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center,

@@ -4,7 +4,8 @@
 // PCIe rate.
 //
 // run(tasks, fn) calls fn(0) .. fn(tasks - 1) across the workers and the
-// calling thread and returns when every call has returned.  Task indices
+// calling thread and returns when every call has returned; start() and
+// finish() split it so the caller can work between them.  Task indices
 // are claimed with a compare-and-swap on (generation, index), so a worker
 // that wakes late never takes an index of a later run.  Workers spin
 // on the generation counter for up to 2 ms before sleeping, since an entry
@@ -41,23 +42,43 @@ class HostPool {
   int workers() const { return (int)threads_.size(); }
 
   void run(size_t tasks, const std::function<void(size_t)>& fn) {
-    if (tasks == 0) return;
-    if (threads_.empty() || tasks == 1) {
-      for (size_t i = 0; i < tasks; i++) fn(i);
-      return;
-    }
-    uint32_t g;
+    start(tasks, fn);
+    finish();
+  }
+
+  // start() publishes a run and returns at once (the workers begin on it);
+  // finish() has the caller take part and returns when every index has
+  // run.  Between the two the caller may do other work; `fn` and what it
+  // captures must live until finish() returns, and runs do not nest.
+  void start(size_t tasks, const std::function<void(size_t)>& fn) {
+    pend_fn_ = &fn;
+    pend_tasks_ = tasks;
+    pend_live_ = false;
+    if (tasks <= 1 || threads_.empty()) return;  // finish() runs them on the caller
     {
       std::lock_guard<std::mutex> lk(m_);
       fn_ = &fn;
       tasks_ = tasks;
       finished_ = 0;
-      g = gen_.load(std::memory_order_relaxed) + 1;
-      next_.store((uint64_t)g << 32, std::memory_order_relaxed);
-      gen_.store(g, std::memory_order_release);
+      pend_g_ = gen_.load(std::memory_order_relaxed) + 1;
+      next_.store((uint64_t)pend_g_ << 32, std::memory_order_relaxed);
+      gen_.store(pend_g_, std::memory_order_release);
     }
+    pend_live_ = true;
     wake_.notify_all();
-    const size_t mine = work(g, &fn, tasks);
+  }
+
+  void finish() {
+    const std::function<void(size_t)>* fn = pend_fn_;
+    const size_t tasks = pend_tasks_;
+    pend_fn_ = nullptr;
+    if (!fn || tasks == 0) return;
+    if (!pend_live_) {
+      for (size_t i = 0; i < tasks; i++) (*fn)(i);
+      return;
+    }
+    pend_live_ = false;
+    const size_t mine = work(pend_g_, fn, tasks);
     std::unique_lock<std::mutex> lk(m_);
     finished_ += mine;
     done_.wait(lk, [&] { return finished_ == tasks_; });
@@ -117,6 +138,11 @@ class HostPool {
   const std::function<void(size_t)>* fn_ = nullptr;
   size_t tasks_ = 0, finished_ = 0;
   bool stop_ = false;
+  // the run between start() and finish() (caller side only)
+  const std::function<void(size_t)>* pend_fn_ = nullptr;
+  size_t pend_tasks_ = 0;
+  uint32_t pend_g_ = 0;
+  bool pend_live_ = false;
 };
 
 }  // namespace vp8g

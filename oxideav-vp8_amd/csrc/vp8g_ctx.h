@@ -38,19 +38,19 @@ struct vp8g_ctx {
   // Device address of this device's static no-alloc scratch (capi.hip).
   uint8_t* scratch = nullptr;
   // Host-pointer ARNR entry point: host threads that pack the caller's
-  // planes, copy streams that carry them (both created on first use), and
-  // the write-combined pinned staging the planes are packed into.
+  // planes into the write-combined pinned staging, and the link stream
+  // whose copy kernels pull the packed rows over PCIe (both created on
+  // first use).  Band b's rows are on the device when band_in[b] has fired
+  // on the link stream, and its altref rows are back in pinned memory when
+  // band_done[b] has fired on the context's stream.
   static constexpr int kMaxCopyStreams = 4;
-  int host_threads = 0;  // VP8G_OPT_HOST_THREADS (0 = default)
-  int copy_streams = 0;  // VP8G_OPT_COPY_STREAMS (0 = default)
-  vp8g::HostPool* pool = nullptr;
-  hipStream_t copy[kMaxCopyStreams] = {};
-  hipEvent_t copy_done[kMaxCopyStreams] = {};
-  hipEvent_t ready = nullptr;
-  // Row bands of the host-pointer ARNR entry point: band b's altref rows
-  // are back in pinned memory when band_done[b] has fired.
   static constexpr int kMaxBands = 8;
-  int host_bands = 0;  // VP8G_OPT_HOST_BANDS (0 = default)
+  int host_threads = 0;  // VP8G_OPT_HOST_THREADS (0 = default)
+  int copy_streams = 0;  // VP8G_OPT_COPY_STREAMS (range-checked, no effect)
+  int host_bands = 0;    // VP8G_OPT_HOST_BANDS (0 = default)
+  vp8g::HostPool* pool = nullptr;
+  hipStream_t link = nullptr;
+  hipEvent_t band_in[kMaxBands] = {};
   hipEvent_t band_done[kMaxBands] = {};
   uint8_t* staging = nullptr;
   size_t staging_bytes = 0;

@@ -395,11 +395,11 @@ class Context:
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_HOST_THREADS, threads), "set_host_threads")
 
     def set_copy_streams(self, streams: int) -> None:
-        """Copy streams of build_arnr_altref's transfers, 1..4 (0 = default 2)."""
+        """Accepted (0..4) and without effect: build_arnr_altref uses one copy stream."""
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_COPY_STREAMS, streams), "set_copy_streams")
 
     def set_host_bands(self, bands: int) -> None:
-        """Row bands of the host-buffer ARNR entry point (1..8; 0 = default 2)."""
+        """Row bands of the host-buffer ARNR entry point (1..8; 0 = default 3)."""
         self._raise(self.lib.vp8g_ctx_set_option(self.h, OPT_HOST_BANDS, int(bands)), "set_host_bands")
 
     def set_fault(self, mode: int) -> None:
