@@ -78,12 +78,16 @@ struct ConvArgs {
 // that stage the weights once and overlap each tile's input loads with the
 // previous tile's products.  With `head_params` set (the last layer) the
 // activations are not stored: the head's vector goes to the alignment table.
-struct ConvInferArgs {
-  ConvArgs c;
-  const float* head_params;  // W5[2][32], b5[2], or null
-  int32_t* table;            // alignment table (head)
-  int frame_of_image[64];    // table frame index of each image (head)
-  int clamp;                 // |mv| bound (head)
+// Inference of the four 3x3 layers and the head in one launch
+// (conv_stack_kernel): a0 in, the alignment table out.
+struct ConvStackArgs {
+  const uint16_t* in;        // a0 [images][ch][cw][32] bf16
+  const uint16_t* w[4];      // layers 1..4: [9][cout][cin] bf16
+  const float* bias[4];      // layers 1..4: [cout]
+  const float* head_params;  // W5[2][32], b5[2]
+  int32_t* table;            // alignment table
+  int frame_of_image[64];    // table frame index of each image
+  int images, cw, ch, clamp;
 };
 
 struct WgradArgs {
@@ -132,7 +136,7 @@ hipError_t launch_wgrad(const WgradArgs& a, int taps, int cin, int cout, int slo
 hipError_t launch_reduce_partials(const float* partials, int slots, int n, float* out, hipStream_t s);
 int head_blocks(long long cells);
 hipError_t launch_head(const HeadArgs& a, hipStream_t s);
-hipError_t launch_conv_infer(const ConvInferArgs& a, int cin, int cout, int cus, hipStream_t s);
+hipError_t launch_conv_stack(const ConvStackArgs& a, hipStream_t s);
 hipError_t launch_adam(const AdamArgs& a, hipStream_t s);
 hipError_t launch_pack_weights(const PackArgs& a, hipStream_t s);
 

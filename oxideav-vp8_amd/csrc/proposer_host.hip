@@ -334,35 +334,29 @@ int vp8g_proposer_propose_dev(vp8g_proposer* p, const vp8g_i420* frames, size_t 
     for (int k = 0; k < images; k++) hd.frame_of_image[k] = frame_of_image[k];
     return ok(launch_head(hd, p->ctx->stream)) ? VP8G_OK : VP8G_E_DEVICE;
   }
-  // Inference: the features kernel applies the first layer, the 3x3 layers
-  // run in their inference form (every load of a workgroup issued at its
-  // start) and the last one applies the head, writing the table (the same
-  // activations and vectors as forward() followed by head_kernel).
+  // Inference: the features kernel applies the first layer, then one launch
+  // runs the four 3x3 layers and the head per 16 x 16-cell tile, writing the
+  // table (the same activations and vectors as forward() followed by
+  // head_kernel).
   uint8_t* b = p->arena;
   hipStream_t s = p->ctx->stream;
   FeatArgs f{planes, nullptr, images, cw, ch, p->wf[0], p->params + conv_offset(0) + conv_weights(0),
              reinterpret_cast<uint16_t*>(b + L.a[0]), 1};
   if (!ok(launch_features(f, s))) return VP8G_E_DEVICE;
-  const uint16_t* in = f.out0;
+  ConvStackArgs a{};
+  a.in = f.out0;
   for (int l = 1; l < kLayers; l++) {
-    ConvInferArgs a{};
-    a.c.in = in;
-    a.c.out = reinterpret_cast<uint16_t*>(b + L.a[l]);
-    a.c.w = p->wf[l];
-    a.c.bias = p->params + conv_offset(l) + conv_weights(l);
-    a.c.images = images;
-    a.c.cw = cw;
-    a.c.ch = ch;
-    if (l == kLayers - 1) {
-      a.head_params = p->params + kHeadOffset;
-      a.table = table;
-      a.clamp = clamp;
-      for (int k = 0; k < images; k++) a.frame_of_image[k] = frame_of_image[k];
-    }
-    if (!ok(launch_conv_infer(a, kConv[l].cin, kConv[l].cout, p->ctx->cus, s))) return VP8G_E_DEVICE;
-    in = a.c.out;
+    a.w[l - 1] = p->wf[l];
+    a.bias[l - 1] = p->params + conv_offset(l) + conv_weights(l);
   }
-  return VP8G_OK;
+  a.head_params = p->params + kHeadOffset;
+  a.table = table;
+  for (int k = 0; k < images; k++) a.frame_of_image[k] = frame_of_image[k];
+  a.images = images;
+  a.cw = cw;
+  a.ch = ch;
+  a.clamp = clamp;
+  return ok(launch_conv_stack(a, s)) ? VP8G_OK : VP8G_E_DEVICE;
 }
 
 int vp8g_proposer_backward_dev(vp8g_proposer* p, const uint8_t* planes, const int32_t* labels, int images,

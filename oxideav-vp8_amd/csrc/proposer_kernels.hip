@@ -34,6 +34,13 @@ __device__ __forceinline__ f32x4 mfma16(const bf16x8& a, const bf16x8& b, const 
 __device__ __forceinline__ bf16x8 as_bf16x8(const uint4& v) { return __builtin_bit_cast(bf16x8, v); }
 
 __device__ __forceinline__ uint16_t bf16_bits(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// Two values rounded to bf16 (to nearest even, as bf16_bits) in one
+// instruction, lo in bits 0..15.
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2{lo, hi}), bf16x2));
+}
 __device__ __forceinline__ float bf16_val(uint16_t b) { return (float)__builtin_bit_cast(__bf16, b); }
 
 // ---------------------------------------------------------------- inputs
@@ -217,11 +224,13 @@ __global__ void __launch_bounds__(320) features_kernel(const FeatArgs a) {
 }
 
 // ---------------------------------------------------------------- conv3x3
-// LDS rows of 16-byte chunks are XOR-swizzled by the row index so that the
-// 16 lanes reading one chunk column of 16 consecutive rows hit distinct banks.
+// LDS rows of 16-byte chunks are XOR-swizzled by the row index so that any
+// 8 consecutive rows read at one chunk column (a 128-byte pass of a 16-byte
+// read) hit distinct banks: with 64-byte rows (C = 32) two rows share a
+// 128-byte line, so the key is the row pair's index.
 template <int C>
 __device__ __forceinline__ int swz(int row, int chunk) {
-  return chunk ^ (row & (C / 8 - 1));
+  return C >= 64 ? chunk ^ (row & (C / 8 - 1)) : chunk ^ ((row >> 1) & (C / 8 - 1));
 }
 
 // out = EPI_BIAS_RELU: lrelu(conv(in, w) + bias)
@@ -337,83 +346,78 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs a) {
   }
 }
 
-// Inference 3x3 conv, bias + leaky ReLU: the products, their order and the
-// epilogue are those of conv_kernel<9, CIN, COUT, EPI_BIAS_RELU>, so the
-// activations are identical.  A launch is short and its loads meet an idle
-// memory system all at once, so a load round costs about as much as the
-// whole layer's products: each workgroup therefore issues every load it will
-// need at its start.  Workgroup v (v spread so that consecutive ones share an
-// XCD and so its L2) owns tiles NT v .. NT v + NT - 1 (64-cell row segments
-// in row order, so its tiles share input rows) and, by buffer loads into LDS,
-// stages the layer's weights and each tile's 3-row input with its halo in
-// one round.  Each lane loads the global chunk that belongs at its LDS slot
-// under the 16-byte XOR swizzle (the swizzle is an involution), and chunks
-// outside the image are written as zeros.  After one wait and one barrier
-// each wave copies into registers the A fragment of every (tap, 32-channel
-// chunk, 16-output block) product it issues, then runs its tiles from LDS.
-// HEAD (the last layer): each wave writes its 16 cells' 32 activations to
-// LDS and 16 of its lanes apply the head to them with head_kernel's fp32
-// arithmetic and order, writing the alignment table.
-template <int CIN, int COUT, bool HEAD, int NT>
-__global__ void __launch_bounds__(256) conv3_infer_kernel(const ConvInferArgs A) {
-  const ConvArgs& a = A.c;
-  constexpr int CK = CIN / 8;   // 16-byte chunks per cell
-  constexpr int TW = 64;        // cells per tile
-  constexpr int IW = TW + 2;    // staged cells per row
-  constexpr int NIN = 3 * IW * CK;
-  constexpr int NW = 9 * COUT * CK;
-  constexpr int IITERS = (NIN + 255) / 256, WITERS = (NW + 255) / 256;
-  constexpr int NC = CIN / 32, NM = COUT / 16;
-  static_assert(!HEAD || COUT == 32, "the head reads 32 channels");
-  static_assert(WITERS + NT * IITERS <= 63, "loads in flight fit vmcnt");
-  __shared__ __align__(16) uint4 s_w[NW];
-  __shared__ __align__(16) uint4 s_in[NT][NIN];
-  __shared__ __align__(16) uint16_t s_a4[HEAD ? TW * 32 : 8];
-  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
-  const int segs = (a.cw + TW - 1) / TW;
-  const int total = a.images * a.ch * segs;
-  const int v = xcd_band(blockIdx.x, gridDim.x);
-  const int tile0 = v * NT;
-  if (tile0 >= total) return;  // the whole workgroup
-  const int ntile = min(NT, total - tile0);
+// ---------------------------------------------------------------- conv stack
+// The four inference 3x3 layers and the head for one tile of 16 x 16 cells
+// in one workgroup.  Each layer reaches one cell further, so the workgroup
+// stages a0 with a 4-cell halo (a 24 x 24 region) and computes layer k on
+// the region shrunk by k cells on each side: about 1.7x the products of the
+// tile alone, far cheaper than a launch per layer (a layer of a 1080p group
+// is under a microsecond of MFMA time, a launch several).  Cells of a layer
+// outside the image are stored as zeros, the zero padding the next layer
+// sees in the per-layer form.  The products of a cell, their order and its
+// epilogue are those of conv_kernel<9, CIN, COUT, EPI_BIAS_RELU>, and the
+// head's those of head_kernel, so the table equals forward() followed by
+// head_kernel.
+//
+// Region cell (r, c) is at flat index 24 r + c.  Layer k (1..3) computes
+// 16-cell blocks of consecutive flat indices from 25 k to past 25 (23 - k)
+// (rows and columns k .. 23 - k are the cells it must get right); a block's
+// products read flat index j + 24 (dy - 1) + dx - 1.  Cells of a block
+// outside that square (the row ends, and the last block's run past the
+// range) only feed cells of the next layer outside its own square, so their
+// values do not matter.  The last layer takes one block per tile row
+// (columns 4..19) and hands each cell's 32 activations to the head through
+// LDS.
+//
+// LDS (150 KB, one workgroup per CU): S1 holds a0 (32 channels, 608 cells:
+// the first layer's last block reads up to flat index 577), later a2 (64
+// channels); S2 holds a1, later a3; one layer's weights, 16-byte chunks
+// XOR-swizzled (swz).  A layer copies its weights into
+// registers, then the next layer's are fetched into LDS by buffer loads
+// while it runs.
+__device__ __forceinline__ uint32_t stack_lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x8 stack_lds_read(uint32_t a) {
+  return __builtin_bit_cast(bf16x8, *(const __attribute__((address_space(3))) u32x4v*)(uintptr_t)a);
+}
+__device__ __forceinline__ void stack_lds_write(uint32_t a, uint2 v) {
+  *(__attribute__((address_space(3))) u32x2v*)(uintptr_t)a = u32x2v{v.x, v.y};
+}
+
+namespace stack {
+constexpr int kWaves = 8;
+constexpr int kP = 24;              // region pitch and rows, in cells
+constexpr int kBufCells = 608;
+constexpr int kWChunks = 9 * 64 * 4;  // the largest layer: 9 x 64 x 32 or 9 x 32 x 64 bf16
+}  // namespace stack
+
+template <int CIN, int COUT>
+__device__ __forceinline__ void stack_fetch_weights(const uint16_t* w, uint4* s_w, int tid, int wave) {
+  constexpr int CK = CIN / 8, NW = 9 * COUT * CK, IT = (NW + 511) / 512;
   using LdsP = __attribute__((address_space(3))) void*;
-  {
-    const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)a.w, (short)0, NW * 16, 0x00020000);
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)w, (short)0, NW * 16, 0x00020000);
 #pragma unroll
-    for (int k = 0; k < WITERS; k++) {
-      const int slot = tid + 256 * k;
-      if (slot < NW) {
-        const int row = slot / CK, ck = swz<CIN>(row, slot - row * CK);
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(wr, (LdsP)(s_w + 256 * k + 64 * wave), 16, (uint32_t)(row * CK + ck) * 16u,
-                                                 0, 0, 0);
-      }
-    }
-    const uint32_t in_bytes = (uint32_t)a.images * a.ch * a.cw * CIN * 2;
-    const __amdgpu_buffer_rsrc_t ir = __builtin_amdgcn_make_buffer_rsrc((void*)a.in, (short)0, (int)in_bytes, 0x00020000);
-    for (int j = 0; j < ntile; j++) {
-      const int t = tile0 + j;
-      const int img = t / (a.ch * segs);
-      const int rem = t - img * a.ch * segs;
-      const int cy = rem / segs, cx0 = (rem - cy * segs) * TW;
-#pragma unroll
-      for (int k = 0; k < IITERS; k++) {
-        const int slot = tid + 256 * k;
-        if (slot < NIN) {
-          const int row = slot / CK, ck = swz<CIN>(row, slot - row * CK);  // row = r * IW + c
-          const int r = row / IW, c = row - r * IW;
-          const int y = cy + r - 1, x = cx0 + c - 1;
-          if (y >= 0 && y < a.ch && x >= 0 && x < a.cw) {
-            const uint32_t off = (uint32_t)((((img * a.ch + y) * a.cw + x) * CK + ck) * 16);
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(ir, (LdsP)(&s_in[j][256 * k + 64 * wave]), 16, off, 0, 0, 0);
-          } else {
-            s_in[j][slot] = make_uint4(0, 0, 0, 0);
-          }
-        }
-      }
+  for (int k = 0; k < IT; k++) {
+    const int slot = tid + 512 * k;
+    if (slot < NW) {
+      const int row = slot / CK, ck = swz<CIN>(row, slot - row * CK);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LdsP)(s_w + 512 * k + 64 * wave), 16, (uint32_t)(row * CK + ck) * 16u,
+                                               0, 0, 0);
     }
   }
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  __syncthreads();
+}
+
+// One layer: weights from LDS into registers, a barrier, the next layer's
+// weights requested (next != null), then this wave's blocks.
+template <int CIN, int COUT, int NCIN, int NCOUT, bool LAST>
+__device__ __forceinline__ void stack_layer(const ConvStackArgs& a, int k, const uint4* src, uint4* dst, uint4* s_w,
+                                            const uint16_t* next, int tid, int wave, int l, int img, int ty0,
+                                            int tx0) {
+  using namespace stack;
+  constexpr int CK = CIN / 8, OK = COUT / 8, NC = CIN / 32, NM = COUT / 16;
   bf16x8 wr[9][NC][NM];
 #pragma unroll
   for (int t = 0; t < 9; t++)
@@ -424,71 +428,150 @@ __global__ void __launch_bounds__(256) conv3_infer_kernel(const ConvInferArgs A)
         const int arow = t * COUT + 16 * m + (l & 15), ck = 4 * c + (l >> 4);
         wr[t][c][m] = as_bf16x8(s_w[arow * CK + swz<CIN>(arow, ck)]);
       }
-  for (int j = 0; j < ntile; j++) {
-    const int tile = tile0 + j;
-    const int img = tile / (a.ch * segs);
-    const int rem = tile - img * a.ch * segs;
-    const int cy = rem / segs, cx0 = (rem - cy * segs) * TW;
-    const uint4* sb = s_in[j];
+  float bias[NM][4];
+#pragma unroll
+  for (int m = 0; m < NM; m++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) bias[m][i] = a.bias[k - 1][16 * m + 4 * (l >> 4) + i];
+  __syncthreads();  // every wave holds its weights
+  if (next) stack_fetch_weights<NCIN, NCOUT>(next, s_w, tid, wave);
+  const int first = 25 * k;
+  const int nb = LAST ? 16 : (25 * (23 - k) - first) / 16 + 1;
+  const int jf = LAST ? 4 * kP + 4 : first;  // block 0's first cell
+  constexpr int kStep = LAST ? kP : 16;      // cells from one block to the next
+  uint16_t* s_a4 = reinterpret_cast<uint16_t*>(dst) + wave * 16 * 32;
+  // The lane's LDS byte addresses at block 0: its B chunk for tap row dy = 0
+  // of each dx and 32-channel chunk, and its output chunk of each 16-channel
+  // block.  The swizzle key of a row is the same 16 or 24 cells on and one
+  // tap row (24 cells) down, so a block adds a uniform step and a tap row an
+  // immediate offset.
+  uint32_t boff[3][NC], woff[NM];
+#pragma unroll
+  for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const int row = jf + (l & 15) - kP + dx - 1;
+      boff[dx][c] = stack_lds_addr(src) + (uint32_t)(row * CK + swz<CIN>(row, 4 * c + (l >> 4))) * 16u;
+    }
+#pragma unroll
+  for (int m = 0; m < NM; m++) {
+    const int co = 16 * m + 4 * (l >> 4), row = jf + (l & 15);
+    woff[m] = stack_lds_addr(dst) + (uint32_t)(row * OK + swz<COUT>(row, co >> 3)) * 16u + (uint32_t)((co >> 2) & 1) * 8u;
+  }
+  for (int b = wave; b < nb; b += kWaves) {
+    const int j0 = jf + kStep * b;
     f32x4 acc[NM];
 #pragma unroll
     for (int m = 0; m < NM; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // every B fragment of the block is read before the first product, so
+    // one LDS latency covers the block
+    bf16x8 bv[9][NC];
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
-      const int dy = t / 3, dx = t - 3 * (t / 3);
-      const int brow = dy * IW + 16 * wave + (l & 15) + dx;
+    for (int dx = 0; dx < 3; dx++)
 #pragma unroll
       for (int c = 0; c < NC; c++) {
-        const int ck = 4 * c + (l >> 4);
-        const bf16x8 bv = as_bf16x8(sb[brow * CK + swz<CIN>(brow, ck)]);
+        uint32_t p = boff[dx][c] + (uint32_t)(b * kStep * CK * 16);
+        asm volatile("" : "+v"(p));  // tap rows then fold into the read's offset
 #pragma unroll
-        for (int m = 0; m < NM; m++) acc[m] = mfma16(wr[t][c][m], bv, acc[m]);
+        for (int dy = 0; dy < 3; dy++) bv[3 * dy + dx][c] = stack_lds_read(p + (uint32_t)(dy * kP * CK * 16));
       }
-    }
-    const int cell_x = cx0 + 16 * wave + (l & 15);
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int m = 0; m < NM; m++) acc[m] = mfma16(wr[t][c][m], bv[t][c], acc[m]);
+    // keep that order: the scheduler would otherwise place each read just
+    // before its products and wait on it
+    __builtin_amdgcn_sched_group_barrier(0x100, 9 * NC, 0);        // DS reads
+    __builtin_amdgcn_sched_group_barrier(0x008, 9 * NC * NM, 0);   // MFMA
+    const int j = j0 + (l & 15);
+    const int rr = (int)((uint32_t)j / (uint32_t)kP), cc = j - rr * kP;
+    const int y = ty0 - 4 + rr, x = tx0 - 4 + cc;
+    const bool inside = y >= 0 && y < a.ch && x >= 0 && x < a.cw;
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       const int co = 16 * m + 4 * (l >> 4);
-      float vv[4] = {acc[m][0], acc[m][1], acc[m][2], acc[m][3]};
+      float vv[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const float z = vv[i] + a.bias[co + i];
-        vv[i] = z > 0.f ? z : kLeak * z;
+        const float z = acc[m][i] + bias[m][i];
+        vv[i] = fmaxf(z, kLeak * z);  // z > 0 ? z : z / 16, signed zeros included
       }
-      uint2 o;
-      o.x = (uint32_t)bf16_bits(vv[0]) | ((uint32_t)bf16_bits(vv[1]) << 16);
-      o.y = (uint32_t)bf16_bits(vv[2]) | ((uint32_t)bf16_bits(vv[3]) << 16);
-      if (HEAD) {
-        *reinterpret_cast<uint2*>(s_a4 + (16 * wave + (l & 15)) * 32 + co) = o;
-      } else if (cell_x < a.cw) {
-        const size_t cell = ((size_t)img * a.ch + cy) * a.cw + cell_x;
-        *reinterpret_cast<uint2*>(a.out + cell * COUT + co) = o;
+      uint2 o = make_uint2(bf16_pair(vv[0], vv[1]), bf16_pair(vv[2], vv[3]));
+      if (LAST) {
+        *reinterpret_cast<uint2*>(s_a4 + (l & 15) * 32 + co) = o;
+      } else {
+        if (!inside) o = make_uint2(0u, 0u);
+        stack_lds_write(woff[m] + (uint32_t)(b * kStep * OK * 16), o);
       }
     }
-    if (HEAD) {
-      // a cell's 32 channels come from this wave's lanes only
+    if (LAST) {
+      // a cell's 32 channels come from this wave's lanes only; lane l < 32
+      // forms output l / 16 of cell l % 16
       wave_lds_sync();
-      if (l < 16 && cell_x < a.cw) {
-        float x[32];
-        const uint16_t* src = s_a4 + (16 * wave + l) * 32;
+      const int hc = l & 15, ho = (l >> 4) & 1;
+      const int hy = ty0 + b, hx = tx0 + hc;
+      if (l < 32 && hy < a.ch && hx < a.cw) {
+        const uint16_t* s = s_a4 + hc * 32;
+        float sum = a.head_params[64 + ho];
 #pragma unroll
-        for (int c = 0; c < 32; c++) x[c] = bf16_val(src[c]);
-        const int cells = a.cw * a.ch;
-        int32_t* tb = A.table + 3 * ((size_t)A.frame_of_image[img] * cells + (size_t)cy * a.cw + cell_x);
-#pragma unroll
-        for (int o = 0; o < 2; o++) {
-          float sum = A.head_params[64 + o];
-#pragma unroll
-          for (int c = 0; c < 32; c++) sum += A.head_params[32 * o + c] * x[c];
-          float mv = rintf(sum * kMvScale);
-          mv = fminf(fmaxf(mv, (float)-A.clamp), (float)A.clamp);
-          tb[o] = (int32_t)mv;
-        }
-        tb[2] = 0;
+        for (int c = 0; c < 32; c++) sum += a.head_params[32 * ho + c] * bf16_val(s[c]);
+        float mv = rintf(sum * kMvScale);
+        mv = fminf(fmaxf(mv, (float)-a.clamp), (float)a.clamp);
+        int32_t* tb = a.table + 3 * ((size_t)a.frame_of_image[img] * a.cw * a.ch + (size_t)hy * a.cw + hx);
+        tb[ho] = (int32_t)mv;
+        if (ho == 0) tb[2] = 0;
       }
-      wave_lds_sync();  // the reads of s_a4 are done before the next tile writes it
+      wave_lds_sync();  // the reads of s_a4 are done before the next block writes it
     }
   }
+  if (!LAST) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next layer's weights are in
+    __syncthreads();                                  // and this layer's outputs
+  }
+}
+
+__global__ void __launch_bounds__(512) conv_stack_kernel(const ConvStackArgs a) {
+  using namespace stack;
+  __shared__ __align__(16) uint4 s1[kBufCells * 8];
+  __shared__ __align__(16) uint4 s2[kBufCells * 4];
+  __shared__ __align__(16) uint4 s_w[kWChunks];
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+  const int txs = (a.cw + 15) / 16, per = ((a.ch + 15) / 16) * txs;
+  const int t = xcd_band(blockIdx.x, gridDim.x);
+  const int img = t / per, rem = t - img * per;
+  const int ty0 = rem / txs * 16, tx0 = (rem - rem / txs * txs) * 16;
+  // a0's region (zeros outside the image and in the 32 cells past it) and
+  // the first layer's weights, every load issued before the one wait
+  stack_fetch_weights<32, 32>(a.w[0], s_w, tid, wave);
+  {
+    using LdsP = __attribute__((address_space(3))) void*;
+    const uint32_t in_bytes = (uint32_t)a.images * a.ch * a.cw * 64;
+    const __amdgpu_buffer_rsrc_t ir = __builtin_amdgcn_make_buffer_rsrc((void*)a.in, (short)0, (int)in_bytes, 0x00020000);
+    constexpr int NIN = kBufCells * 4, IT = (NIN + 511) / 512;
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+      const int slot = tid + 512 * k;
+      if (slot < NIN) {
+        const int cell = slot >> 2, ck = swz<32>(cell, slot & 3);
+        const int rr = (int)((uint32_t)cell / (uint32_t)kP), cc = cell - rr * kP;
+        const int y = ty0 - 4 + rr, x = tx0 - 4 + cc;
+        if (cell < kP * kP && y >= 0 && y < a.ch && x >= 0 && x < a.cw) {
+          const uint32_t off = (uint32_t)((((img * a.ch + y) * a.cw + x) * 4 + ck) * 16);
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(ir, (LdsP)(s1 + 512 * k + 64 * wave), 16, off, 0, 0, 0);
+        } else {
+          s1[slot] = make_uint4(0, 0, 0, 0);
+        }
+      }
+    }
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  stack_layer<32, 32, 32, 64, false>(a, 1, s1, s2, s_w, a.w[1], tid, wave, l, img, ty0, tx0);
+  stack_layer<32, 64, 64, 32, false>(a, 2, s2, s1, s_w, a.w[2], tid, wave, l, img, ty0, tx0);
+  stack_layer<64, 32, 32, 32, false>(a, 3, s1, s2, s_w, a.w[3], tid, wave, l, img, ty0, tx0);
+  stack_layer<32, 32, 32, 32, true>(a, 4, s2, s1, s_w, nullptr, tid, wave, l, img, ty0, tx0);
 }
 
 // ---------------------------------------------------------------- wgrad
@@ -766,22 +849,13 @@ hipError_t launch_conv(const ConvArgs& a, int taps, int cin, int cout, int epi, 
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_conv_infer(const ConvInferArgs& a, int cin, int cout, int cus, hipStream_t s) {
-  const int total = a.c.images * a.c.ch * ((a.c.cw + 63) / 64);
-  if (total == 0) return hipSuccess;
-  // The input bytes must fit the 32-bit buffer offsets.
-  if ((long long)a.c.images * a.c.ch * a.c.cw * cin * 2 > 0x7FFFFFFFLL) return hipErrorInvalidValue;
-  const bool head = a.head_params != nullptr;
-  // NT tiles per workgroup: 4 with 64 input channels (weights and four
-  // inputs in 138 KB of LDS, one workgroup per CU), else 2 (two per CU).
-  const int nt = cin == 64 ? 4 : 2;
-  const dim3 grid((unsigned)((total + nt - 1) / nt)), block(256);
-  (void)cus;
-  if (cin == 32 && cout == 32 && !head) hipLaunchKernelGGL((conv3_infer_kernel<32, 32, false, 2>), grid, block, 0, s, a);
-  else if (cin == 32 && cout == 32 && head) hipLaunchKernelGGL((conv3_infer_kernel<32, 32, true, 2>), grid, block, 0, s, a);
-  else if (cin == 32 && cout == 64 && !head) hipLaunchKernelGGL((conv3_infer_kernel<32, 64, false, 2>), grid, block, 0, s, a);
-  else if (cin == 64 && cout == 32 && !head) hipLaunchKernelGGL((conv3_infer_kernel<64, 32, false, 4>), grid, block, 0, s, a);
-  else return hipErrorInvalidValue;
+hipError_t launch_conv_stack(const ConvStackArgs& a, hipStream_t s) {
+  const long long tiles = (long long)a.images * ((a.ch + 15) / 16) * ((a.cw + 15) / 16);
+  if (tiles == 0) return hipSuccess;
+  // a0's bytes must fit the 32-bit buffer offsets; at most 64 images
+  if ((long long)a.images * a.ch * a.cw * 64 > 0x7FFFFFFFLL || a.images > 64 || tiles > 0x7FFFFFFF)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(conv_stack_kernel, dim3((unsigned)tiles), dim3(512), 0, s, a);
   return hipGetLastError();
 }
 

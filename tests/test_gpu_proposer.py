@@ -109,11 +109,15 @@ def test_forward_matches_bf16_reference(gpu, prop, shape):
     assert (err > tol).mean() < 2e-3, (err > tol).mean()
 
 
-def test_propose_fused_first_layer_matches_forward(gpu, prop):
-    """propose applies the first layer inside the features kernel; its
-    proposals must equal the unfused forward pass (features, then the 1x1
-    conv kernel) followed by the head, computed here on the host in fp32
-    (a value at a rounding tie may land one unit off)."""
+@pytest.mark.parametrize("w,h,n", [(200, 120, 5), (600, 344, 4), (1040, 280, 3)])
+def test_propose_fused_first_layer_matches_forward(gpu, prop, w, h, n):
+    """propose applies the first layer inside the features kernel and runs
+    the four 3x3 layers and the head as one launch per 16 x 16-cell tile
+    (recomputing a 4-cell halo, zeros outside the image); its proposals
+    must equal the per-layer forward pass followed by the head, computed
+    here on the host in fp32 (a value at a rounding tie may land one unit
+    off).  Shapes: one partial tile per image; 3 x 2 tiles with partial
+    last ones; a single tile row with 5 columns."""
     import torch
     import vp8g.synth as synth
     from test_gpu_search import Dev
@@ -121,7 +125,7 @@ def test_propose_fused_first_layer_matches_forward(gpu, prop):
     params = ref().he_init(17).astype(np.float32)
     params[4096:4128] = rng.normal(0, 0.05, 32)
     prop.set_params(params)
-    g = synth.arnr_group_regions(200, 120, 5, vmax=8, occluders=4, clip=2, config=3)
+    g = synth.arnr_group_regions(w, h, n, vmax=8, occluders=4, clip=2, config=3)
     dev = Dev(g.frames())
     n, c = len(dev.tuples), g.center
     bh, bw = dev.shape[1], dev.shape[2]
@@ -142,6 +146,7 @@ def test_propose_fused_first_layer_matches_forward(gpu, prop):
             s = (s.astype(np.float64) + head[32 * o + k].astype(np.float64) * x[..., k]).astype(np.float32)
         pred[..., o] = s
     want = np.clip(np.rint(pred * 8), -14, 14).astype(np.int32)
+    assert (np.abs(want) < 14).mean() > 0.2  # most vectors are not clamped
     got = table.cpu().numpy()[[i for i in range(n) if i != c]]
     assert (got[..., 2] == 0).all()
     diff = np.abs(got[..., :2] - want)
