@@ -388,7 +388,10 @@ __device__ __forceinline__ void stack_lds_write(uint32_t a, uint2 v) {
 }
 
 namespace stack {
+// 8 waves, two per SIMD: one wave's reads and epilogue overlap the other's
+// products (4 waves with two blocks per pass each were 15% slower).
 constexpr int kWaves = 8;
+constexpr int kThreads = 64 * kWaves;
 constexpr int kP = 24;              // region pitch and rows, in cells
 constexpr int kBufCells = 608;
 constexpr int kWChunks = 9 * 64 * 4;  // the largest layer: 9 x 64 x 32 or 9 x 32 x 64 bf16
@@ -396,15 +399,15 @@ constexpr int kWChunks = 9 * 64 * 4;  // the largest layer: 9 x 64 x 32 or 9 x 3
 
 template <int CIN, int COUT>
 __device__ __forceinline__ void stack_fetch_weights(const uint16_t* w, uint4* s_w, int tid, int wave) {
-  constexpr int CK = CIN / 8, NW = 9 * COUT * CK, IT = (NW + 511) / 512;
+  constexpr int CK = CIN / 8, NW = 9 * COUT * CK, IT = (NW + stack::kThreads - 1) / stack::kThreads;
   using LdsP = __attribute__((address_space(3))) void*;
   const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)w, (short)0, NW * 16, 0x00020000);
 #pragma unroll
   for (int k = 0; k < IT; k++) {
-    const int slot = tid + 512 * k;
+    const int slot = tid + stack::kThreads * k;
     if (slot < NW) {
       const int row = slot / CK, ck = swz<CIN>(row, slot - row * CK);
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LdsP)(s_w + 512 * k + 64 * wave), 16, (uint32_t)(row * CK + ck) * 16u,
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (LdsP)(s_w + stack::kThreads * k + 64 * wave), 16, (uint32_t)(row * CK + ck) * 16u,
                                                0, 0, 0);
     }
   }
@@ -458,34 +461,10 @@ __device__ __forceinline__ void stack_layer(const ConvStackArgs& a, int k, const
     const int co = 16 * m + 4 * (l >> 4), row = jf + (l & 15);
     woff[m] = stack_lds_addr(dst) + (uint32_t)(row * OK + swz<COUT>(row, co >> 3)) * 16u + (uint32_t)((co >> 2) & 1) * 8u;
   }
-  for (int b = wave; b < nb; b += kWaves) {
-    const int j0 = jf + kStep * b;
-    f32x4 acc[NM];
-#pragma unroll
-    for (int m = 0; m < NM; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // every B fragment of the block is read before the first product, so
-    // one LDS latency covers the block
-    bf16x8 bv[9][NC];
-#pragma unroll
-    for (int dx = 0; dx < 3; dx++)
-#pragma unroll
-      for (int c = 0; c < NC; c++) {
-        uint32_t p = boff[dx][c] + (uint32_t)(b * kStep * CK * 16);
-        asm volatile("" : "+v"(p));  // tap rows then fold into the read's offset
-#pragma unroll
-        for (int dy = 0; dy < 3; dy++) bv[3 * dy + dx][c] = stack_lds_read(p + (uint32_t)(dy * kP * CK * 16));
-      }
-#pragma unroll
-    for (int t = 0; t < 9; t++)
-#pragma unroll
-      for (int c = 0; c < NC; c++)
-#pragma unroll
-        for (int m = 0; m < NM; m++) acc[m] = mfma16(wr[t][c][m], bv[t][c], acc[m]);
-    // keep that order: the scheduler would otherwise place each read just
-    // before its products and wait on it
-    __builtin_amdgcn_sched_group_barrier(0x100, 9 * NC, 0);        // DS reads
-    __builtin_amdgcn_sched_group_barrier(0x008, 9 * NC * NM, 0);   // MFMA
-    const int j = j0 + (l & 15);
+  // A block's epilogue: bias, leaky ReLU, bf16; zeros outside the image;
+  // the last layer hands the cells to the head.
+  const auto finish = [&](int b, const f32x4 (&acc)[NM]) {
+    const int j = jf + kStep * b + (l & 15);
     const int rr = (int)((uint32_t)j / (uint32_t)kP), cc = j - rr * kP;
     const int y = ty0 - 4 + rr, x = tx0 - 4 + cc;
     const bool inside = y >= 0 && y < a.ch && x >= 0 && x < a.cw;
@@ -525,6 +504,34 @@ __device__ __forceinline__ void stack_layer(const ConvStackArgs& a, int k, const
       }
       wave_lds_sync();  // the reads of s_a4 are done before the next block writes it
     }
+  };
+  for (int b = wave; b < nb; b += kWaves) {
+    f32x4 acc[NM];
+#pragma unroll
+    for (int m = 0; m < NM; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // every B fragment of the block is read before the first product, so
+    // one LDS latency covers the block
+    bf16x8 bv[9][NC];
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        uint32_t p = boff[dx][c] + (uint32_t)(b * kStep * CK * 16);
+        asm volatile("" : "+v"(p));  // tap rows then fold into the read's offset
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) bv[3 * dy + dx][c] = stack_lds_read(p + (uint32_t)(dy * kP * CK * 16));
+      }
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int m = 0; m < NM; m++) acc[m] = mfma16(wr[t][c][m], bv[t][c], acc[m]);
+    // keep that order: the scheduler would otherwise place each read just
+    // before its products and wait on it
+    __builtin_amdgcn_sched_group_barrier(0x100, 9 * NC, 0);        // DS reads
+    __builtin_amdgcn_sched_group_barrier(0x008, 9 * NC * NM, 0);   // MFMA
+    finish(b, acc);
   }
   if (!LAST) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the next layer's weights are in
@@ -532,7 +539,7 @@ __device__ __forceinline__ void stack_layer(const ConvStackArgs& a, int k, const
   }
 }
 
-__global__ void __launch_bounds__(512) conv_stack_kernel(const ConvStackArgs a) {
+__global__ void __launch_bounds__(stack::kThreads) conv_stack_kernel(const ConvStackArgs a) {
   using namespace stack;
   __shared__ __align__(16) uint4 s1[kBufCells * 8];
   __shared__ __align__(16) uint4 s2[kBufCells * 4];
@@ -549,17 +556,17 @@ __global__ void __launch_bounds__(512) conv_stack_kernel(const ConvStackArgs a) 
     using LdsP = __attribute__((address_space(3))) void*;
     const uint32_t in_bytes = (uint32_t)a.images * a.ch * a.cw * 64;
     const __amdgpu_buffer_rsrc_t ir = __builtin_amdgcn_make_buffer_rsrc((void*)a.in, (short)0, (int)in_bytes, 0x00020000);
-    constexpr int NIN = kBufCells * 4, IT = (NIN + 511) / 512;
+    constexpr int NIN = kBufCells * 4, IT = (NIN + kThreads - 1) / kThreads;
 #pragma unroll
     for (int k = 0; k < IT; k++) {
-      const int slot = tid + 512 * k;
+      const int slot = tid + kThreads * k;
       if (slot < NIN) {
         const int cell = slot >> 2, ck = swz<32>(cell, slot & 3);
         const int rr = (int)((uint32_t)cell / (uint32_t)kP), cc = cell - rr * kP;
         const int y = ty0 - 4 + rr, x = tx0 - 4 + cc;
         if (cell < kP * kP && y >= 0 && y < a.ch && x >= 0 && x < a.cw) {
           const uint32_t off = (uint32_t)((((img * a.ch + y) * a.cw + x) * 4 + ck) * 16);
-          __builtin_amdgcn_raw_ptr_buffer_load_lds(ir, (LdsP)(s1 + 512 * k + 64 * wave), 16, off, 0, 0, 0);
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(ir, (LdsP)(s1 + kThreads * k + 64 * wave), 16, off, 0, 0, 0);
         } else {
           s1[slot] = make_uint4(0, 0, 0, 0);
         }
@@ -855,7 +862,7 @@ hipError_t launch_conv_stack(const ConvStackArgs& a, hipStream_t s) {
   // a0's bytes must fit the 32-bit buffer offsets; at most 64 images
   if ((long long)a.images * a.ch * a.cw * 64 > 0x7FFFFFFFLL || a.images > 64 || tiles > 0x7FFFFFFF)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(conv_stack_kernel, dim3((unsigned)tiles), dim3(512), 0, s, a);
+  hipLaunchKernelGGL(conv_stack_kernel, dim3((unsigned)tiles), dim3(stack::kThreads), 0, s, a);
   return hipGetLastError();
 }
 
