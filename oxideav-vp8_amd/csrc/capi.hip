@@ -322,7 +322,8 @@ uint32_t strip_divider(int d) {
 // Enqueue the fused ARNR kernel over device-resident frames.
 // MB rows [row0, row1) of the group (all of them by default).
 int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, uint8_t strength,
-                 uint8_t* oy, uint8_t* ou, uint8_t* ov, int32_t* mv_out, int row0 = 0, int row1 = -1) {
+                 uint8_t* oy, uint8_t* ou, uint8_t* ov, int32_t* mv_out, int row0 = 0, int row1 = -1,
+                 hipStream_t stream = nullptr) {
   ArnrArgs a;
   memset(&a, 0, sizeof(a));
   fill_group(frames, n, center, strength, oy, ou, ov, mv_out, a.frames, &a.g, a.wt4);
@@ -335,7 +336,7 @@ int enqueue_arnr(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_t center, 
   if (row1 < 0) row1 = a.g.brows;
   a.strip_base = row0 * a.g.strips_per_row;
   a.strip_count = (row1 - row0) * a.g.strips_per_row;
-  return launch_status(c, [&] { return launch_arnr_fused(a, k, c->stream); });
+  return launch_status(c, [&] { return launch_arnr_fused(a, k, stream ? stream : c->stream); });
 }
 
 // Caller tap tables whose taps keep every six-tap sum inside int32.
@@ -871,11 +872,17 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
       push.seg[1] = {du + c0 * cw, hu + c0 * cw, (c1 - c0) * cw};
       push.seg[2] = {dv + c0 * cw, hv + c0 * cw, (c1 - c0) * cw};
       push.segs = 3;
-      if (!hip_ok(hipEventRecord(c->band_in[b], c->link)) || !hip_ok(hipStreamWaitEvent(c->stream, c->band_in[b], 0)))
+      // The last band's work follows its pull on the link stream itself:
+      // nothing is left to overlap, and the wait on another stream's copy
+      // costs more than the kernel.
+      const bool tail = k + 1 == pieces.size();
+      hipStream_t ks = tail ? c->link : c->stream;
+      if (!tail && (!hip_ok(hipEventRecord(c->band_in[b], c->link)) ||
+                    !hip_ok(hipStreamWaitEvent(c->stream, c->band_in[b], 0))))
         st = VP8G_E_DEVICE;
-      if (st == VP8G_OK) st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr, r0, r1);
-      if (st == VP8G_OK) st = launch_status(c, [&] { return launch_link_copy(push, c->stream); });
-      if (st == VP8G_OK && !hip_ok(hipEventRecord(c->band_done[b], c->stream))) st = VP8G_E_DEVICE;
+      if (st == VP8G_OK) st = enqueue_arnr(c, dev.data(), n, center, strength, dy, du, dv, nullptr, r0, r1, ks);
+      if (st == VP8G_OK) st = launch_status(c, [&] { return launch_link_copy(push, ks); });
+      if (st == VP8G_OK && !hip_ok(hipEventRecord(c->band_done[b], ks))) st = VP8G_E_DEVICE;
     }
     if (st != VP8G_OK) {
       c->pool->finish();  // no packing task may outlive this call
@@ -897,7 +904,7 @@ int vp8g_build_arnr_altref(vp8g_ctx* c, const vp8g_i420* frames, size_t n, size_
     for (const PlaneJob& j : outs) add_plane(tasks, j);
     run_plane_tasks(c, tasks);
   }
-  // every pull was waited for by a band's kernel, so the link is idle
+  // the link's last work was the last band's, waited for above
   return finish(c, VP8G_OK);
 }
 
