@@ -230,10 +230,11 @@ def test_max_group_length(gpu, oracle):
 
 @pytest.mark.parametrize("threads,streams,bands", [(1, 1, 1), (3, 2, 2), (8, 4, 8), (0, 3, 0), (2, 2, 5)])
 def test_host_copy_shapes_leave_bytes_unchanged(gpu, oracle, threads, streams, bands):
-    """The host threads, copy streams and row bands of the host-pointer
-    entry change no output byte: strided planes of odd sizes (partial last
-    MB rows, bands of uneven MB-row counts, more bands than MB rows), a
-    group longer than the copy streams, and a call after a smaller one that
+    """The host threads and row bands of the host-pointer entry (and the
+    copy-stream option, accepted without effect) change no output byte:
+    strided planes of odd sizes (partial last MB rows, bands of uneven
+    MB-row counts, more bands than MB rows, rows whose chroma is odd-sized
+    and interleaved U|V on the device), and a call after a smaller one that
     reuses the staging."""
     import vp8g
     import vp8g.synth as synth
