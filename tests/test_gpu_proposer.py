@@ -109,7 +109,7 @@ def test_forward_matches_bf16_reference(gpu, prop, shape):
     assert (err > tol).mean() < 2e-3, (err > tol).mean()
 
 
-@pytest.mark.parametrize("w,h,n", [(200, 120, 5), (600, 344, 4), (1040, 280, 3)])
+@pytest.mark.parametrize("w,h,n", [(200, 120, 5), (600, 344, 4), (1040, 280, 3), (40, 24, 9), (16, 16, 2)])
 def test_propose_fused_first_layer_matches_forward(gpu, prop, w, h, n):
     """propose applies the first layer inside the features kernel and runs
     the four 3x3 layers and the head as one launch per 16 x 16-cell tile
@@ -117,7 +117,8 @@ def test_propose_fused_first_layer_matches_forward(gpu, prop, w, h, n):
     must equal the per-layer forward pass followed by the head, computed
     here on the host in fp32 (a value at a rounding tie may land one unit
     off).  Shapes: one partial tile per image; 3 x 2 tiles with partial
-    last ones; a single tile row with 5 columns."""
+    last ones; a single tile row with 5 columns; 3 x 2 cells in 8 images;
+    a single cell."""
     import torch
     import vp8g.synth as synth
     from test_gpu_search import Dev
@@ -146,7 +147,7 @@ def test_propose_fused_first_layer_matches_forward(gpu, prop, w, h, n):
             s = (s.astype(np.float64) + head[32 * o + k].astype(np.float64) * x[..., k]).astype(np.float32)
         pred[..., o] = s
     want = np.clip(np.rint(pred * 8), -14, 14).astype(np.int32)
-    assert (np.abs(want) < 14).mean() > 0.2  # most vectors are not clamped
+    assert want.size < 64 or (np.abs(want) < 14).mean() > 0.2  # most vectors are not clamped
     got = table.cpu().numpy()[[i for i in range(n) if i != c]]
     assert (got[..., 2] == 0).all()
     diff = np.abs(got[..., :2] - want)
