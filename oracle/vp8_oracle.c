@@ -552,24 +552,48 @@ void orc_fetch_block_halo(const uint8_t *plane, size_t stride, size_t w, size_t 
            9, 9, out);
 }
 
+/* One filter pass: dst row r, column c = interp(t, the six samples from
+ * src[r * sstride + c] on, `step` apart).  As the reference's per-block
+ * plan (TapPlan, src/motion_comp.rs:476-535, and the identity elision of
+ * its dispatcher, :919-1007) the row is classified once: an identity row
+ * {0,0,128,0,0,0} is a copy of sample 2, a row with taps 0 and 5 zero
+ * sums four products.  Both give interp's result exactly: dropped terms
+ * are zero, and (128 s + 64) >> 7 = s. */
+static void sixtap_pass(const uint8_t *src, int sstride, int step, int rows, int cols,
+                        const int32_t t[6], uint8_t *dst, int dstride)
+{
+    if (t[0] == 0 && t[1] == 0 && t[2] == 128 && t[3] == 0 && t[4] == 0 && t[5] == 0) {
+        for (int r = 0; r < rows; r++)
+            for (int c = 0; c < cols; c++) dst[r * dstride + c] = src[r * sstride + c + 2 * step];
+    } else if (t[0] == 0 && t[5] == 0) {
+        const uint32_t t1 = (uint32_t)t[1], t2 = (uint32_t)t[2], t3 = (uint32_t)t[3], t4 = (uint32_t)t[4];
+        for (int r = 0; r < rows; r++)
+            for (int c = 0; c < cols; c++) {
+                const uint8_t *q = src + r * sstride + c;
+                const uint32_t a = q[step] * t1 + q[2 * step] * t2 + q[3 * step] * t3 + q[4 * step] * t4;
+                dst[r * dstride + c] = clamp255(((int32_t)(a + 64u)) >> 7);
+            }
+    } else {
+        for (int r = 0; r < rows; r++)
+            for (int c = 0; c < cols; c++) {
+                uint8_t sv[6];
+                for (int k = 0; k < 6; k++) sv[k] = src[r * sstride + c + k * step];
+                dst[r * dstride + c] = orc_interp(t, sv);
+            }
+    }
+}
+
 /* Two-pass section 20.14 filter over an OW x OW block from its (OW+5)^2
  * halo: sixtap_mb_luma_scalar / sixtap_mb_chroma_scalar / sixtap_2d_scalar,
- * src/motion_comp.rs:1018-1057, 1311-1350, 680-736.  Running both passes
- * always is exact: an identity tap row {0,0,128,0,0,0} returns its input
- * (src/motion_comp.rs:919-1007). */
+ * src/motion_comp.rs:1018-1057, 1311-1350, 680-736, each pass planned as
+ * sixtap_pass says. */
 static void sixtap_block(const uint8_t *halo, int ow, int mx, int my,
                          const int32_t filters[8][6], uint8_t *out)
 {
     const int hw = ow + 5;
     uint8_t temp[21 * 16];
-    for (int r = 0; r < hw; r++)
-        for (int c = 0; c < ow; c++) temp[r * ow + c] = orc_interp(filters[mx], halo + r * hw + c);
-    for (int r = 0; r < ow; r++)
-        for (int c = 0; c < ow; c++) {
-            uint8_t s[6];
-            for (int k = 0; k < 6; k++) s[k] = temp[(r + k) * ow + c];
-            out[r * ow + c] = orc_interp(filters[my], s);
-        }
+    sixtap_pass(halo, hw, 1, hw, ow, filters[mx], temp, ow);
+    sixtap_pass(temp, ow, ow, ow, ow, filters[my], out, ow);
 }
 
 /* sixtap_2d: src/motion_comp.rs:680-736. */
