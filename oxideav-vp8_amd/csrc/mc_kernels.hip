@@ -387,17 +387,20 @@ __global__ void __launch_bounds__(256) predict_inter_mb_kernel(const McArgs a) {
     const uint32_t vsh = hv.store(chalo + 13 * 16, 16, lane);
     wave_lds_sync();
     hpass_dot<16>(lh, 24, lsh, lt, fh, lane);
-    if (lane < 26) {
-      hpass_dot<8>(chalo, 16, ush, ctmp, cfh, lane);  // lanes 0..25 -> U
-    } else if (lane < 52) {
-      hpass_dot<8>(chalo + 13 * 16, 16, vsh, ctmp + 13 * 8, cfh, lane - 26);  // V
+    // The chroma planes in one pass each, lanes picking the plane (U for
+    // lanes 0..25, V for 26..51; then U 0..15, V 16..31): the same
+    // instructions with per-lane addresses and shifts, not two branches.
+    {
+      const bool v = lane >= 26;
+      if (lane < 52)
+        hpass_dot<8>(chalo + (v ? 13 * 16 : 0), 16, v ? vsh : ush, ctmp + (v ? 13 * 8 : 0), cfh, lane - (v ? 26 : 0));
     }
     wave_lds_sync();
     *reinterpret_cast<uint32_t*>(out + 4 * lane) = vpass_dot<16>(lt, fv, lane);
-    if (lane < 16) {
-      *reinterpret_cast<uint32_t*>(out + 256 + 4 * lane) = vpass_dot<8>(ctmp, cfv, lane);
-    } else if (lane < 32) {
-      *reinterpret_cast<uint32_t*>(out + 320 + 4 * (lane - 16)) = vpass_dot<8>(ctmp + 13 * 8, cfv, lane - 16);
+    if (lane < 32) {
+      const bool v = lane >= 16;
+      *reinterpret_cast<uint32_t*>(out + (v ? 320 : 256) + 4 * (lane & 15)) =
+          vpass_dot<8>(ctmp + (v ? 13 * 8 : 0), cfv, lane & 15);
     }
     return;
   }
