@@ -292,7 +292,6 @@ __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::
 __device__ __forceinline__ void stage_gather(uint8_t* win, const FrameDesc& f, const ArnrGroup& a, int bx, int by,
                                              int lane) {
   constexpr int ldw = kLumaPitch / 4;  // dwords per window row
-#pragma unroll
   for (int s = lane; s < kWinRows * ldw; s += 64) {
     const int wy = s / ldw, dq = s - wy * ldw;
     const int y = clampi(by - 15 + wy, 0, a.h - 1);

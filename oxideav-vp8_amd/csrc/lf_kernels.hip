@@ -55,8 +55,6 @@ constexpr int kStripBytes = 640 + 2 * 192;  // luma 20 x 32, chroma 12 x 16 (per
 constexpr int kWaveLds = 2 * kStripBytes + kRingA * kSlotBytes;
 
 __device__ __forceinline__ int c8(int v) { return min(max(v, -128), 127); }
-__device__ __forceinline__ int s2u(int v) { return c8(v) + 128; }
-__device__ __forceinline__ int absd(int a, int b) { return a > b ? a - b : b - a; }
 
 __device__ __forceinline__ uint32_t ad(uint32_t a, uint32_t b) { return __builtin_amdgcn_sad_u16(a, b, 0u); }
 __device__ __forceinline__ uint32_t px(int v) { return (uint32_t)min(max(v, 0), 255); }
@@ -65,7 +63,8 @@ __device__ __forceinline__ uint32_t px(int v) { return (uint32_t)min(max(v, 0), 
 // p3 p2 p1 p0 | q0 q1 q2 q3 = bytes of lo then hi (src/loop_filter.rs:292-393),
 // both outcomes computed and selected.  In unsigned pixels: the signed
 // differences of the section 15.2 arithmetic are the unsigned ones, and
-// s2u(u2s(p) + a) = clamp(p + a, 0, 255).
+// converting u2s(p) + a back to unsigned (c8, then + 128) is
+// clamp(p + a, 0, 255).
 template <bool MB>
 __device__ __forceinline__ void normal_edge(uint32_t& lo, uint32_t& hi, int I, int E, int T, bool on) {
   const uint32_t p3 = lo & 255, p2 = (lo >> 8) & 255, p1 = (lo >> 16) & 255, p0 = lo >> 24;
